@@ -1,0 +1,296 @@
+"""MI355X SimPLIC alpha transport: Python front end of libsimplic_amd.so.
+
+``SolveVofEqu`` mirrors the reference's ``solveVofEqu`` facade
+(``src/SimPLIC/solveVofEqu/solveVofEqu.H:67-216``): construct it on a mesh with
+the fvSolution ``"alpha.*"`` dictionary, then call ``reconstruct()`` and
+``advect(dt, Sp, Su)`` once per time step.  All work runs in the HIP library;
+there is no CPU fallback: importing the library fails loudly when it has not
+been built.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from .mesh import PolyMesh, hex_box, tet_box  # noqa: F401
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsimplic_amd.so")
+HEADERS = [os.path.join(os.path.dirname(_HERE), "include", h) for h in ("simplic_amd.h", "simplic_harness.h")]
+
+SVP_HOST = 0
+SVP_DEVICE = 1
+
+_lib = None
+
+
+class SvpError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+class SvpDict(C.Structure):
+    _fields_ = [
+        ("orientation_method", C.c_char * 32),
+        ("mixed_cell_tol", C.c_double),
+        ("n_alpha_bounds", C.c_int32),
+        ("snap_tol", C.c_double),
+        ("clip", C.c_int32),
+        ("split_warped_face", C.c_int32),
+        ("write_plic_fields", C.c_int32),
+    ]
+
+
+class SvpMesh(C.Structure):
+    _fields_ = [
+        ("n_points", C.c_int32), ("n_faces", C.c_int32), ("n_internal_faces", C.c_int32),
+        ("n_cells", C.c_int32), ("n_patches", C.c_int32),
+        ("points", C.c_void_p), ("face_offsets", C.c_void_p), ("face_points", C.c_void_p),
+        ("owner", C.c_void_p), ("neighbour", C.c_void_p), ("patch_start", C.c_void_p),
+        ("patch_size", C.c_void_p), ("patch_type", C.c_void_p),
+    ]
+
+
+class SvpDeviceView(C.Structure):
+    _fields_ = [
+        ("alpha", C.c_void_p), ("phi", C.c_void_p), ("U", C.c_void_p), ("U_boundary", C.c_void_p),
+        ("dVf", C.c_void_p), ("Cf", C.c_void_p), ("Sf", C.c_void_p), ("C", C.c_void_p),
+        ("V", C.c_void_p), ("stream", C.c_void_p),
+    ]
+
+
+def lib():
+    """Load libsimplic_amd.so (raises if it is missing: no fallback path)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is not built; run __graft_entry__.build() or "
+                          "python geometricvofext_amd/build.py")
+    L = C.CDLL(LIB_PATH)
+    vp, i32, dbl = C.c_void_p, C.c_int32, C.c_double
+    L.svp_last_error.restype = C.c_char_p
+    L.svp_dict_defaults.argtypes = [C.POINTER(SvpDict)]
+    L.svp_dict_defaults.restype = None
+    L.svp_dict_parse.argtypes = [C.c_char_p, C.POINTER(SvpDict)]
+    L.svp_create.argtypes = [C.POINTER(SvpMesh), C.POINTER(SvpDict), C.c_int, C.POINTER(vp)]
+    L.svp_destroy.argtypes = [vp]
+    L.svp_set_alpha.argtypes = [vp, vp, C.c_int]
+    L.svp_set_flow.argtypes = [vp, vp, vp, vp, C.c_int]
+    L.svp_reconstruct.argtypes = [vp]
+    L.svp_advect.argtypes = [vp, dbl, vp, vp, C.c_int]
+    L.svp_get_alpha.argtypes = [vp, vp, C.c_int]
+    L.svp_get_alpha_phi.argtypes = [vp, vp, C.c_int]
+    L.svp_get_rho_phi.argtypes = [vp, dbl, dbl, vp, C.c_int]
+    L.svp_n_mixed.argtypes = [vp, C.POINTER(i32)]
+    L.svp_get_interface.argtypes = [vp, vp, vp, vp, vp, vp, vp, C.c_int]
+    L.svp_get_cut_class.argtypes = [vp, vp, vp, C.c_int]
+    L.svp_get_un0.argtypes = [vp, vp, C.c_int]
+    L.svp_get_geometry.argtypes = [vp, vp, vp, vp, vp, vp, vp]
+    L.svp_timers.argtypes = [vp, C.POINTER(dbl), C.POINTER(dbl), C.POINTER(C.c_uint64)]
+    L.svp_device_view_get.argtypes = [vp, C.POINTER(SvpDeviceView)]
+    L.svp_synchronize.argtypes = [vp]
+    L.svp_harness_deformation.argtypes = [vp, vp, vp]
+    L.svp_harness_rotation.argtypes = [vp, vp, vp, vp, vp]
+    L.svp_harness_upload_base_flow.argtypes = [vp, vp, vp, vp]
+    L.svp_harness_scale_flow.argtypes = [vp, dbl]
+    L.svp_harness_courant.argtypes = [vp, C.POINTER(dbl), C.POINTER(dbl)]
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc != 0:
+        raise SvpError(rc, lib().svp_last_error().decode())
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+def dict_defaults() -> SvpDict:
+    d = SvpDict()
+    lib().svp_dict_defaults(C.byref(d))
+    return d
+
+
+def parse_dict(text: str) -> SvpDict:
+    """Parse the body of an fvSolution ``"alpha.*"`` entry."""
+    d = SvpDict()
+    _check(lib().svp_dict_parse(text.encode(), C.byref(d)))
+    return d
+
+
+def make_dict(**kw) -> SvpDict:
+    d = dict_defaults()
+    names = {"orientationMethod": "orientation_method", "mixedCellTol": "mixed_cell_tol",
+             "nAlphaBounds": "n_alpha_bounds", "snapTol": "snap_tol", "clip": "clip",
+             "splitWarpedFace": "split_warped_face", "writePlicFields": "write_plic_fields"}
+    for k, v in kw.items():
+        k = names.get(k, k)
+        if k == "orientation_method":
+            v = v.encode()
+        elif isinstance(v, bool):
+            v = int(v)
+        setattr(d, k, v)
+    return d
+
+
+class SolveVofEqu:
+    """solveVofEqu(alpha1, phi, U) on a :class:`PolyMesh`, resident on one GPU."""
+
+    def __init__(self, mesh: PolyMesh, dictionary=None, device: int = 0):
+        L = lib()
+        self.mesh = mesh
+        if dictionary is None:
+            dictionary = dict_defaults()
+        elif isinstance(dictionary, str):
+            dictionary = parse_dict(dictionary)
+        elif isinstance(dictionary, dict):
+            dictionary = make_dict(**dictionary)
+        self.dict = dictionary
+        self._keep = [
+            np.ascontiguousarray(mesh.points, dtype=np.float64),
+            np.ascontiguousarray(mesh.face_offsets, dtype=np.int32),
+            np.ascontiguousarray(mesh.face_points, dtype=np.int32),
+            np.ascontiguousarray(mesh.owner, dtype=np.int32),
+            np.ascontiguousarray(mesh.neighbour if mesh.neighbour.size else np.zeros(1), dtype=np.int32),
+            np.ascontiguousarray(mesh.patch_start, dtype=np.int32),
+            np.ascontiguousarray(mesh.patch_size, dtype=np.int32),
+            np.ascontiguousarray(mesh.patch_type, dtype=np.int32),
+        ]
+        k = self._keep
+        sm = SvpMesh(mesh.n_points, mesh.n_faces, mesh.n_internal_faces, mesh.n_cells,
+                     len(mesh.patch_start), *[a.ctypes.data for a in k])
+        h = C.c_void_p()
+        _check(L.svp_create(C.byref(sm), C.byref(self.dict), device, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.svp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    # fields ---------------------------------------------------------------
+    def set_alpha(self, alpha):
+        a = np.ascontiguousarray(alpha, dtype=np.float64)
+        assert a.size == self.mesh.n_cells
+        _check(lib().svp_set_alpha(self.h, a.ctypes.data, SVP_HOST))
+
+    def set_flow(self, phi, U, U_boundary=None):
+        p = np.ascontiguousarray(phi, dtype=np.float64)
+        u = np.ascontiguousarray(U, dtype=np.float64).reshape(-1)
+        ub = None if U_boundary is None else np.ascontiguousarray(U_boundary, dtype=np.float64).reshape(-1)
+        assert p.size == self.mesh.n_faces and u.size == 3 * self.mesh.n_cells
+        _check(lib().svp_set_flow(self.h, p.ctypes.data, u.ctypes.data, _ptr(ub), SVP_HOST))
+
+    # the reference's per-step calls ----------------------------------------
+    def reconstruct(self):
+        _check(lib().svp_reconstruct(self.h))
+
+    def advect(self, dt, Sp=None, Su=None):
+        sp = None if Sp is None else np.ascontiguousarray(Sp, dtype=np.float64)
+        su = None if Su is None else np.ascontiguousarray(Su, dtype=np.float64)
+        _check(lib().svp_advect(self.h, float(dt), _ptr(sp), _ptr(su), SVP_HOST))
+
+    # outputs ----------------------------------------------------------------
+    def alpha(self):
+        a = np.zeros(self.mesh.n_cells)
+        _check(lib().svp_get_alpha(self.h, a.ctypes.data, SVP_HOST))
+        return a
+
+    def alpha_phi(self):
+        a = np.zeros(self.mesh.n_faces)
+        _check(lib().svp_get_alpha_phi(self.h, a.ctypes.data, SVP_HOST))
+        return a
+
+    def rho_phi(self, rho1, rho2):
+        a = np.zeros(self.mesh.n_faces)
+        _check(lib().svp_get_rho_phi(self.h, float(rho1), float(rho2), a.ctypes.data, SVP_HOST))
+        return a
+
+    def n_mixed(self) -> int:
+        n = C.c_int32()
+        _check(lib().svp_n_mixed(self.h, C.byref(n)))
+        return n.value
+
+    def interface(self):
+        n = self.n_mixed()
+        k = max(n, 1)
+        cells = np.zeros(k, np.int32)
+        status = np.zeros(k, np.int32)
+        nrm = np.zeros(3 * k)
+        d = np.zeros(k)
+        Cc = np.zeros(3 * k)
+        S = np.zeros(3 * k)
+        _check(lib().svp_get_interface(self.h, cells.ctypes.data, status.ctypes.data, nrm.ctypes.data,
+                                       d.ctypes.data, Cc.ctypes.data, S.ctypes.data, SVP_HOST))
+        fcl = np.zeros(k, np.uint64)
+        ncp = np.zeros(k, np.int32)
+        _check(lib().svp_get_cut_class(self.h, fcl.ctypes.data, ncp.ctypes.data, SVP_HOST))
+        return dict(cells=cells[:n], status=status[:n], n=nrm.reshape(-1, 3)[:n], d=d[:n],
+                    C=Cc.reshape(-1, 3)[:n], S=S.reshape(-1, 3)[:n], faceclass=fcl[:n], ncutpts=ncp[:n])
+
+    def un0(self):
+        n = self.n_mixed()
+        u = np.zeros(max(n, 1))
+        _check(lib().svp_get_un0(self.h, u.ctypes.data, SVP_HOST))
+        return u[:n]
+
+    def geometry(self):
+        nf, nc = self.mesh.n_faces, self.mesh.n_cells
+        g = dict(Cf=np.zeros((nf, 3)), Sf=np.zeros((nf, 3)), magSf=np.zeros(nf), flatness=np.zeros(nf),
+                 C=np.zeros((nc, 3)), V=np.zeros(nc))
+        _check(lib().svp_get_geometry(self.h, g["Cf"].ctypes.data, g["Sf"].ctypes.data, g["magSf"].ctypes.data,
+                                      g["flatness"].ctypes.data, g["C"].ctypes.data, g["V"].ctypes.data))
+        return g
+
+    def timers(self):
+        r, a, n = C.c_double(), C.c_double(), C.c_uint64()
+        _check(lib().svp_timers(self.h, C.byref(r), C.byref(a), C.byref(n)))
+        return r.value, a.value, n.value
+
+    def synchronize(self):
+        _check(lib().svp_synchronize(self.h))
+
+    def device_view(self) -> SvpDeviceView:
+        v = SvpDeviceView()
+        _check(lib().svp_device_view_get(self.h, C.byref(v)))
+        return v
+
+    # harness ---------------------------------------------------------------
+    def deformation_field(self):
+        nc, nf = self.mesh.n_cells, self.mesh.n_faces
+        U = np.zeros(3 * nc)
+        phi = np.zeros(nf)
+        _check(lib().svp_harness_deformation(self.h, U.ctypes.data, phi.ctypes.data))
+        return U.reshape(nc, 3), phi
+
+    def rotation_field(self, omega, origin):
+        nc, nf = self.mesh.n_cells, self.mesh.n_faces
+        U = np.zeros(3 * nc)
+        phi = np.zeros(nf)
+        w = np.ascontiguousarray(omega, dtype=np.float64)
+        o = np.ascontiguousarray(origin, dtype=np.float64)
+        _check(lib().svp_harness_rotation(self.h, w.ctypes.data, o.ctypes.data, U.ctypes.data, phi.ctypes.data))
+        return U.reshape(nc, 3), phi
+
+    def upload_base_flow(self, U0, phi0, U0_boundary=None):
+        u = np.ascontiguousarray(U0, dtype=np.float64).reshape(-1)
+        p = np.ascontiguousarray(phi0, dtype=np.float64)
+        ub = None if U0_boundary is None else np.ascontiguousarray(U0_boundary, dtype=np.float64).reshape(-1)
+        _check(lib().svp_harness_upload_base_flow(self.h, u.ctypes.data, p.ctypes.data, _ptr(ub)))
+
+    def scale_flow(self, factor):
+        _check(lib().svp_harness_scale_flow(self.h, float(factor)))
+
+    def courant(self):
+        a, b = C.c_double(), C.c_double()
+        _check(lib().svp_harness_courant(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value

@@ -1,0 +1,833 @@
+// Host side of libsimplic_amd.so: the solveVofEqu facade over the HIP kernels
+// and its C-ABI (include/simplic_amd.h, include/simplic_harness.h).
+//
+// Reference interface mirrored: src/SimPLIC/solveVofEqu/solveVofEqu.H:67-216
+// (constructor, reconstruct(), advect(Sp, Su), alphaPhi(), getRhoPhi(),
+// reconstructionTime(), advectionTime()).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "simplic_amd.h"
+#include "simplic_harness.h"
+#include "simplic_device.hpp"
+#include "simplic_types.hpp"
+
+namespace svp {
+void launch_scale_flow(int nC, int nF, int nB, const double* U0, const double* phi0, const double* U0b,
+                       double factor, double* U, double* phi, double* Ub, hipStream_t s);
+void launch_courant(const DevMesh& m, const double* phi, const double* alpha, unsigned long long* out,
+                    hipStream_t s);
+void launch_count_nonfinite(const double* x, long n, int* count, hipStream_t s);
+}  // namespace svp
+
+namespace {
+
+thread_local std::string g_err;
+
+struct SvpError : std::runtime_error {
+    int code;
+    SvpError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPCHECK(x)                                                                            \
+    do {                                                                                       \
+        hipError_t e_ = (x);                                                                   \
+        if (e_ != hipSuccess)                                                                  \
+            throw SvpError(SVP_E_HIP, std::string(#x) + ": " + hipGetErrorString(e_));         \
+    } while (0)
+
+template <class F>
+int guarded(F&& f) {
+    try {
+        f();
+        return SVP_OK;
+    } catch (const SvpError& e) {
+        g_err = e.what();
+        return e.code;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return SVP_E_ARG;
+    }
+}
+
+// Owning device buffer
+struct DBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DBuf() = default;
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    ~DBuf() {
+        if (p) (void)hipFree(p);
+    }
+    template <class T>
+    T* alloc(size_t n) {
+        bytes = std::max<size_t>(n, 1) * sizeof(T);
+        HIPCHECK(hipMalloc(&p, bytes));
+        HIPCHECK(hipMemset(p, 0, bytes));
+        return (T*)p;
+    }
+    template <class T>
+    T* upload(const std::vector<T>& v) {
+        T* d = alloc<T>(v.size());
+        if (!v.empty()) HIPCHECK(hipMemcpy(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+        return d;
+    }
+};
+
+}  // namespace
+
+namespace gve {
+namespace SimPLIC {
+
+using namespace svp;
+
+// Host mirror of the reference's solveVofEqu: owns the device mesh, the
+// per-step state and the stream all kernels of this alpha field run on.
+class solveVofEqu {
+public:
+    solveVofEqu(const svp_mesh& mesh, const svp_dict& dict, int device) : dict_(dict), device_(device) {
+        HIPCHECK(hipSetDevice(device));
+        HIPCHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+        params_.mixedCellTol = dict.mixed_cell_tol;
+        params_.nAlphaBounds = dict.n_alpha_bounds;
+        params_.snapTol = dict.snap_tol;
+        params_.clip = dict.clip;
+        params_.splitWarpedFace = dict.split_warped_face;
+        buildMesh(mesh);
+        allocState();
+        launch_geometry(m_, stream_);
+        HIPCHECK(hipGetLastError());
+        HIPCHECK(hipStreamSynchronize(stream_));
+    }
+
+    ~solveVofEqu() {
+        (void)hipSetDevice(device_);
+        if (stream_) (void)hipStreamSynchronize(stream_);
+        for (auto& e : events_) {
+            (void)hipEventDestroy(e.first);
+            (void)hipEventDestroy(e.second);
+        }
+        if (stream_) (void)hipStreamDestroy(stream_);
+    }
+
+    // reconstruction::reconstruct (reconstruction.C:680-722)
+    void reconstruct() {
+        hipEvent_t a = event(), b = event();
+        HIPCHECK(hipEventRecord(a, stream_));
+        launch_detect(m_, st_, params_, stream_);
+        launch_ls_normal(m_, st_, stream_);
+        launch_find_signed_distance(m_, st_, params_, stream_);
+        HIPCHECK(hipEventRecord(b, stream_));
+        HIPCHECK(hipGetLastError());
+        pendingRecon_.push_back({a, b});
+        reconstructed_ = true;
+    }
+
+    // advection::advect (advectionTemplates.C:352-418); alpha1.oldTime() is
+    // the field at the call, i.e. the double buffer swaps here.
+    void advect(double dt, const double* Sp, const double* Su) {
+        if (!(dt > 0.0) || !std::isfinite(dt)) throw SvpError(SVP_E_ARG, "advect: dt must be positive");
+        if (!reconstructed_) reconstruct();
+        hipEvent_t a = event(), b = event();
+        HIPCHECK(hipEventRecord(a, stream_));
+        std::swap(st_.alpha, st_.alphaOld);
+        launch_advect(m_, st_, params_, dt, Sp, Su, stream_);
+        HIPCHECK(hipEventRecord(b, stream_));
+        HIPCHECK(hipGetLastError());
+        pendingAdvect_.push_back({a, b});
+        lastDt_ = dt;
+        alphaPhiValid_ = false;
+        reconstructed_ = false;
+    }
+
+    void checkDeviceErrors() {
+        int err = 0;
+        HIPCHECK(hipMemcpyAsync(&err, st_.errFlag, sizeof(int), hipMemcpyDeviceToHost, stream_));
+        HIPCHECK(hipStreamSynchronize(stream_));
+        if (err) {
+            HIPCHECK(hipMemsetAsync(st_.errFlag, 0, sizeof(int), stream_));
+            char buf[256];
+            std::snprintf(buf, sizeof buf,
+                          "device error bits 0x%x (1: cell exceeds cut limits, 2: more than %d plane "
+                          "crossings on a face, 4: bounding candidate overflow, 8: bounding work overflow)",
+                          err, MAX_IF_POINTS);
+            throw SvpError(SVP_E_DEVICE, buf);
+        }
+    }
+
+    void setAlpha(const double* a, int where) {
+        copyIn(st_.alpha, a, m_.nC, where);
+        checkFinite(st_.alpha, m_.nC, "alpha");
+        reconstructed_ = false;
+    }
+    void setFlow(const double* phi, const double* U, const double* Ub, int where) {
+        copyIn(st_.phi, phi, m_.nF, where);
+        copyIn(st_.U, U, 3L * m_.nC, where);
+        if (Ub) copyIn(st_.Ub, Ub, 3L * nB_, where);
+        else HIPCHECK(hipMemsetAsync(st_.Ub, 0, sizeof(double) * 3 * std::max(nB_, 1), stream_));
+        checkFinite(st_.phi, m_.nF, "phi");
+        checkFinite(st_.U, 3L * m_.nC, "U");
+    }
+
+    void getAlpha(double* out, int where) { copyOut(out, st_.alpha, m_.nC, where); }
+
+    const double* alphaPhi() {
+        if (!alphaPhiValid_) {
+            if (lastDt_ <= 0.0) throw SvpError(SVP_E_ARG, "alphaPhi: advect has not run");
+            launch_alpha_phi(m_, st_, lastDt_, stream_);
+            alphaPhiValid_ = true;
+        }
+        return st_.alphaPhi;
+    }
+
+    int nMixed() {
+        int n = 0;
+        HIPCHECK(hipMemcpyAsync(&n, st_.nMixed, sizeof(int), hipMemcpyDeviceToHost, stream_));
+        HIPCHECK(hipStreamSynchronize(stream_));
+        return n;
+    }
+
+    void timers(double& rec, double& adv, uint64_t& nic) {
+        HIPCHECK(hipStreamSynchronize(stream_));
+        auto drain = [&](std::vector<std::pair<hipEvent_t, hipEvent_t>>& v, double& acc) {
+            for (auto& e : v) {
+                float ms = 0.f;
+                HIPCHECK(hipEventElapsedTime(&ms, e.first, e.second));
+                acc += 1e-3 * ms;
+                freeEvents_.push_back(e.first);
+                freeEvents_.push_back(e.second);
+            }
+            v.clear();
+        };
+        drain(pendingRecon_, reconTime_);
+        drain(pendingAdvect_, advectTime_);
+        rec = reconTime_;
+        adv = advectTime_;
+        unsigned long long n = 0;
+        HIPCHECK(hipMemcpy(&n, st_.nIcTotal, sizeof(n), hipMemcpyDeviceToHost));
+        nic = n;
+    }
+
+    // accessors used by the C-ABI
+    DevMesh m_{};
+    DevState st_{};
+    Params params_{};
+    svp_dict dict_;
+    int device_ = 0;
+    int nB_ = 0;
+    hipStream_t stream_ = nullptr;
+    double lastDt_ = 0.0;
+    // harness base flow
+    double* U0_ = nullptr;
+    double* phi0_ = nullptr;
+    double* U0b_ = nullptr;
+    unsigned long long* courant_ = nullptr;
+    int* nonfinite_ = nullptr;
+    std::vector<double> hostC_, hostCf_, hostSf_;
+
+    void copyIn(double* dst, const double* src, long n, int where) {
+        if (!src) throw SvpError(SVP_E_ARG, "null input pointer");
+        HIPCHECK(hipMemcpyAsync(dst, src, sizeof(double) * n,
+                                where == SVP_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                                stream_));
+        if (where != SVP_DEVICE) HIPCHECK(hipStreamSynchronize(stream_));
+    }
+    void copyOut(double* dst, const double* src, long n, int where) {
+        if (!dst) return;
+        HIPCHECK(hipMemcpyAsync(dst, src, sizeof(double) * n,
+                                where == SVP_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
+                                stream_));
+        if (where != SVP_DEVICE) HIPCHECK(hipStreamSynchronize(stream_));
+    }
+    template <class T>
+    void copyOutT(T* dst, const T* src, long n, int where) {
+        if (!dst || n <= 0) return;
+        HIPCHECK(hipMemcpyAsync(dst, src, sizeof(T) * n,
+                                where == SVP_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
+                                stream_));
+        if (where != SVP_DEVICE) HIPCHECK(hipStreamSynchronize(stream_));
+    }
+    void checkFinite(const double* x, long n, const char* what) {
+        HIPCHECK(hipMemsetAsync(nonfinite_, 0, sizeof(int), stream_));
+        launch_count_nonfinite(x, n, nonfinite_, stream_);
+        int bad = 0;
+        HIPCHECK(hipMemcpyAsync(&bad, nonfinite_, sizeof(int), hipMemcpyDeviceToHost, stream_));
+        HIPCHECK(hipStreamSynchronize(stream_));
+        if (bad) throw SvpError(SVP_E_NAN, std::string(what) + ": " + std::to_string(bad) + " non-finite values");
+    }
+    void ensureHostGeometry() {
+        if (!hostC_.empty()) return;
+        hostC_.resize(3L * m_.nC);
+        hostCf_.resize(3L * m_.nF);
+        hostSf_.resize(3L * m_.nF);
+        HIPCHECK(hipMemcpy(hostC_.data(), m_.C, sizeof(double) * hostC_.size(), hipMemcpyDeviceToHost));
+        HIPCHECK(hipMemcpy(hostCf_.data(), m_.Cf, sizeof(double) * hostCf_.size(), hipMemcpyDeviceToHost));
+        HIPCHECK(hipMemcpy(hostSf_.data(), m_.Sf, sizeof(double) * hostSf_.size(), hipMemcpyDeviceToHost));
+    }
+    double* allocHarness(DBuf& b, long n) { return b.alloc<double>(n); }
+    DBuf harnessBufs_[3];
+
+private:
+    std::vector<std::unique_ptr<DBuf>> bufs_;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pendingRecon_, pendingAdvect_;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> events_;
+    std::vector<hipEvent_t> freeEvents_;
+    double reconTime_ = 0.0, advectTime_ = 0.0;
+    bool reconstructed_ = false;
+    bool alphaPhiValid_ = false;
+
+    template <class T>
+    T* dalloc(size_t n) {
+        bufs_.emplace_back(new DBuf());
+        return bufs_.back()->alloc<T>(n);
+    }
+    template <class T>
+    T* dupload(const std::vector<T>& v) {
+        bufs_.emplace_back(new DBuf());
+        return bufs_.back()->upload<T>(v);
+    }
+    hipEvent_t event() {
+        if (!freeEvents_.empty()) {
+            hipEvent_t e = freeEvents_.back();
+            freeEvents_.pop_back();
+            return e;
+        }
+        hipEvent_t e;
+        HIPCHECK(hipEventCreate(&e));
+        events_.push_back({e, e});
+        return e;
+    }
+
+    // Host-built addressing of the raw polyMesh (primitiveMesh::cells(),
+    // cell-point-cell stencil, volPointInterpolation sources).
+    void buildMesh(const svp_mesh& M) {
+        const int nP = M.n_points, nF = M.n_faces, nIF = M.n_internal_faces, nC = M.n_cells;
+        if (nP <= 0 || nF <= 0 || nC <= 0 || nIF < 0 || nIF > nF || !M.points || !M.face_offsets ||
+            !M.face_points || !M.owner || (nIF > 0 && !M.neighbour))
+            throw SvpError(SVP_E_ARG, "svp_create: incomplete mesh");
+        nB_ = nF - nIF;
+        m_.nP = nP; m_.nF = nF; m_.nIF = nIF; m_.nC = nC;
+        std::vector<int> fOff(M.face_offsets, M.face_offsets + nF + 1);
+        if (fOff[0] != 0) throw SvpError(SVP_E_ARG, "face_offsets[0] != 0");
+        for (int f = 0; f < nF; ++f) {
+            const int n = fOff[f + 1] - fOff[f];
+            if (n < 3 || n > MAX_FACE_POINTS)
+                throw SvpError(SVP_E_MESH, "face " + std::to_string(f) + " has " + std::to_string(n) +
+                                               " points (supported 3.." + std::to_string(MAX_FACE_POINTS) + ")");
+        }
+        std::vector<int> fPts(M.face_points, M.face_points + fOff[nF]);
+        for (int p : fPts)
+            if (p < 0 || p >= nP) throw SvpError(SVP_E_ARG, "face point label out of range");
+        std::vector<int> own(M.owner, M.owner + nF), nbr(nF, -1);
+        for (int f = 0; f < nIF; ++f) nbr[f] = M.neighbour[f];
+        for (int f = 0; f < nF; ++f) {
+            if (own[f] < 0 || own[f] >= nC) throw SvpError(SVP_E_ARG, "owner out of range");
+            if (f < nIF && (nbr[f] < 0 || nbr[f] >= nC || nbr[f] <= own[f]))
+                throw SvpError(SVP_E_ARG, "neighbour must exceed owner on every internal face");
+        }
+        std::vector<unsigned char> fEmpty(nF, 0);
+        for (int p = 0; p < M.n_patches; ++p) {
+            const int s = M.patch_start[p], n = M.patch_size[p];
+            if (s < nIF || s + n > nF) throw SvpError(SVP_E_ARG, "patch range outside boundary faces");
+            if (M.patch_type && M.patch_type[p] == SVP_PATCH_EMPTY)
+                for (int k = 0; k < n; ++k) fEmpty[s + k] = 1;
+        }
+        // cells(): owned faces ascending then neighbour faces ascending
+        std::vector<int> cnt(nC + 1, 0);
+        for (int f = 0; f < nF; ++f) cnt[own[f] + 1]++;
+        for (int f = 0; f < nIF; ++f) cnt[nbr[f] + 1]++;
+        std::vector<int> cOff(nC + 1, 0);
+        for (int c = 0; c < nC; ++c) cOff[c + 1] = cOff[c] + cnt[c + 1];
+        std::vector<int> fill(cOff.begin(), cOff.end() - 1), cFaces(cOff[nC]), cFacesAsc(cOff[nC]);
+        for (int f = 0; f < nF; ++f) cFaces[fill[own[f]]++] = f;
+        for (int f = 0; f < nIF; ++f) cFaces[fill[nbr[f]]++] = f;
+        int maxCellFaces = 0;
+        for (int c = 0; c < nC; ++c) {
+            const int b = cOff[c], e = cOff[c + 1];
+            maxCellFaces = std::max(maxCellFaces, e - b);
+            std::vector<int> srt(cFaces.begin() + b, cFaces.begin() + e);
+            std::sort(srt.begin(), srt.end());
+            for (int k = 0; k < e - b; ++k) cFacesAsc[b + k] = (own[srt[k]] == c) ? srt[k] : ~srt[k];
+        }
+        const int faceLimit = dict_.split_warped_face ? MAX_CELL_FACES : NS_CELL_FACES;
+        if (maxCellFaces > faceLimit)
+            throw SvpError(SVP_E_MESH, "a cell has " + std::to_string(maxCellFaces) + " faces (supported " +
+                                           std::to_string(faceLimit) + ")");
+        // point -> cells (ascending) and cell points
+        std::vector<std::vector<int>> cellPts(nC);
+        std::vector<int> pcCnt(nP + 1, 0);
+        int maxCellPts = 0;
+        for (int c = 0; c < nC; ++c) {
+            auto& v = cellPts[c];
+            for (int k = cOff[c]; k < cOff[c + 1]; ++k)
+                for (int q = fOff[cFaces[k]]; q < fOff[cFaces[k] + 1]; ++q) v.push_back(fPts[q]);
+            std::sort(v.begin(), v.end());
+            v.erase(std::unique(v.begin(), v.end()), v.end());
+            maxCellPts = std::max(maxCellPts, (int)v.size());
+            for (int p : v) pcCnt[p + 1]++;
+        }
+        if (maxCellPts > (dict_.split_warped_face ? MAX_CELL_POINTS - 1 : NS_CELL_POINTS))
+            throw SvpError(SVP_E_MESH, "a cell has " + std::to_string(maxCellPts) + " points");
+        std::vector<int> pcOff(nP + 1, 0);
+        for (int p = 0; p < nP; ++p) pcOff[p + 1] = pcOff[p] + pcCnt[p + 1];
+        std::vector<int> pcFill(pcOff.begin(), pcOff.end() - 1), pCells(pcOff[nP]);
+        for (int c = 0; c < nC; ++c)
+            for (int p : cellPts[c]) pCells[pcFill[p]++] = c;
+        // cell-point-cell stencil: cell first, then the others ascending
+        std::vector<int> stOff(nC + 1, 0), st;
+        st.reserve((size_t)nC * 27);
+        std::vector<int> tmp;
+        for (int c = 0; c < nC; ++c) {
+            tmp.clear();
+            for (int p : cellPts[c])
+                for (int k = pcOff[p]; k < pcOff[p + 1]; ++k)
+                    if (pCells[k] != c) tmp.push_back(pCells[k]);
+            std::sort(tmp.begin(), tmp.end());
+            tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
+            st.push_back(c);
+            st.insert(st.end(), tmp.begin(), tmp.end());
+            stOff[c + 1] = (int)st.size();
+        }
+        cellPts.clear();
+        cellPts.shrink_to_fit();
+        // point interpolation sources: boundary points -> their non-empty
+        // boundary faces (ascending); other points -> their cells (ascending)
+        std::vector<std::vector<int>> bFaces(nP);
+        std::vector<unsigned char> isPatch(nP, 0);
+        for (int f = nIF; f < nF; ++f) {
+            if (fEmpty[f]) continue;
+            for (int q = fOff[f]; q < fOff[f + 1]; ++q) {
+                isPatch[fPts[q]] = 1;
+                bFaces[fPts[q]].push_back(f);
+            }
+        }
+        std::vector<int> psOff(nP + 1, 0), psSrc;
+        psSrc.reserve(pCells.size());
+        for (int p = 0; p < nP; ++p) {
+            if (isPatch[p]) {
+                for (int f : bFaces[p]) psSrc.push_back(~f);
+            } else {
+                for (int k = pcOff[p]; k < pcOff[p + 1]; ++k) psSrc.push_back(pCells[k]);
+            }
+            psOff[p + 1] = (int)psSrc.size();
+        }
+        std::vector<double> pts(M.points, M.points + 3L * nP);
+        m_.pts = dupload(pts);
+        m_.fOff = dupload(fOff);
+        m_.fPts = dupload(fPts);
+        m_.own = dupload(own);
+        m_.nbr = dupload(nbr);
+        m_.fEmpty = dupload(fEmpty);
+        m_.cOff = dupload(cOff);
+        m_.cFaces = dupload(cFaces);
+        m_.cFacesAsc = dupload(cFacesAsc);
+        m_.stOff = dupload(stOff);
+        m_.st = dupload(st);
+        m_.psOff = dupload(psOff);
+        m_.psSrc = dupload(psSrc);
+        m_.Cf = dalloc<double>(3L * nF);
+        m_.Sf = dalloc<double>(3L * nF);
+        m_.magSf = dalloc<double>(nF);
+        m_.flat = dalloc<double>(nF);
+        m_.C = dalloc<double>(3L * nC);
+        m_.V = dalloc<double>(nC);
+        m_.psW = dalloc<double>(psSrc.size());
+    }
+
+    void allocState() {
+        const int nC = m_.nC, nF = m_.nF;
+        st_.alpha = dalloc<double>(nC);
+        st_.alphaOld = dalloc<double>(nC);
+        st_.phi = dalloc<double>(nF);
+        st_.U = dalloc<double>(3L * nC);
+        st_.Ub = dalloc<double>(3L * std::max(nB_, 1));
+        st_.dVf = dalloc<double>(nF);
+        st_.alphaPhi = dalloc<double>(nF);
+        st_.corr = dalloc<double>(nF);
+        st_.mixedFlag = dalloc<unsigned char>(nC);
+        st_.candFlag = dalloc<unsigned char>(nC);
+        st_.blockCount = dalloc<int>(detect_blocks(nC));
+        st_.nMixed = dalloc<int>(1);
+        st_.mixed = dalloc<int>(nC);
+        st_.status = dalloc<int>(nC);
+        st_.nCutPts = dalloc<int>(nC);
+        st_.faceClass = dalloc<unsigned long long>(nC);
+        st_.ifN = dalloc<double>(3L * nC);
+        st_.ifD = dalloc<double>(nC);
+        st_.ifC = dalloc<double>(3L * nC);
+        st_.ifS = dalloc<double>(3L * nC);
+        st_.un0 = dalloc<double>(nC);
+        st_.oobCount = dalloc<int>(1);
+        st_.candCount = dalloc<int>(1);
+        st_.candCap = nC;
+        st_.cand = dalloc<int>(nC);
+        st_.workCap = 2 * std::max(1 << 16, std::min(nF, 1 << 22));
+        st_.work = dalloc<int>(st_.workCap);
+        st_.minmax = dalloc<double>(4);
+        st_.errFlag = dalloc<int>(1);
+        st_.nIcTotal = dalloc<unsigned long long>(1);
+        courant_ = dalloc<unsigned long long>(2);
+        nonfinite_ = dalloc<int>(1);
+    }
+};
+
+}  // namespace SimPLIC
+}  // namespace gve
+
+struct svp_handle {
+    std::unique_ptr<gve::SimPLIC::solveVofEqu> s;
+};
+
+using gve::SimPLIC::solveVofEqu;
+
+static solveVofEqu& H(svp_handle* h) {
+    if (!h || !h->s) throw SvpError(SVP_E_ARG, "null handle");
+    HIPCHECK(hipSetDevice(h->s->device_));
+    return *h->s;
+}
+
+extern "C" {
+
+void svp_dict_defaults(svp_dict* d) {
+    if (!d) return;
+    std::memset(d, 0, sizeof(*d));
+    std::snprintf(d->orientation_method, sizeof d->orientation_method, "isoAlphaGrad");
+    d->mixed_cell_tol = 1e-8;
+    d->n_alpha_bounds = 10;
+    d->snap_tol = 0.0;
+    d->clip = 1;
+    d->split_warped_face = 0;
+    d->write_plic_fields = 0;
+}
+
+// OpenFOAM dictionary entries "key value;" with comments; sub-dictionaries
+// are skipped.
+int svp_dict_parse(const char* text, svp_dict* out) {
+    return guarded([&] {
+        if (!text || !out) throw SvpError(SVP_E_ARG, "svp_dict_parse: null argument");
+        svp_dict d;
+        svp_dict_defaults(&d);
+        std::string s(text), clean;
+        for (size_t i = 0; i < s.size(); ++i) {
+            if (s[i] == '/' && i + 1 < s.size() && s[i + 1] == '/') {
+                while (i < s.size() && s[i] != '\n') ++i;
+                clean += '\n';
+            } else if (s[i] == '/' && i + 1 < s.size() && s[i + 1] == '*') {
+                i += 2;
+                while (i + 1 < s.size() && !(s[i] == '*' && s[i + 1] == '/')) ++i;
+                ++i;
+                clean += ' ';
+            } else {
+                clean += s[i];
+            }
+        }
+        std::string method;
+        size_t pos = 0;
+        int depth = 0;
+        while (pos < clean.size()) {
+            const size_t semi = clean.find_first_of(";{}", pos);
+            if (semi == std::string::npos) break;
+            const char delim = clean[semi];
+            std::string stmt = clean.substr(pos, semi - pos);
+            pos = semi + 1;
+            if (delim == '{') { depth++; continue; }
+            if (delim == '}') { depth--; continue; }
+            if (depth != 0) continue;
+            size_t a = stmt.find_first_not_of(" \t\r\n");
+            if (a == std::string::npos) continue;
+            size_t b = stmt.find_first_of(" \t\r\n", a);
+            if (b == std::string::npos) continue;
+            std::string key = stmt.substr(a, b - a);
+            size_t c = stmt.find_first_not_of(" \t\r\n", b);
+            size_t e = stmt.find_last_not_of(" \t\r\n");
+            if (c == std::string::npos) continue;
+            std::string val = stmt.substr(c, e - c + 1);
+            auto toBool = [&](const std::string& v) {
+                if (v == "true" || v == "on" || v == "yes" || v == "1") return 1;
+                if (v == "false" || v == "off" || v == "no" || v == "0" || v == "none") return 0;
+                throw SvpError(SVP_E_DICT, "bad bool '" + v + "' for " + key);
+            };
+            if (key == "orientationMethod" || (key == "reconstructionScheme" && method.empty())) method = val;
+            else if (key == "mixedCellTol") d.mixed_cell_tol = std::stod(val);
+            else if (key == "nAlphaBounds") d.n_alpha_bounds = std::stoi(val);
+            else if (key == "snapTol") d.snap_tol = std::stod(val);
+            else if (key == "clip") d.clip = toBool(val);
+            else if (key == "splitWarpedFace") d.split_warped_face = toBool(val);
+            else if (key == "writePlicFields") d.write_plic_fields = toBool(val);
+        }
+        if (!method.empty()) {
+            if (method == "LS" || method == "isoAlphaGrad") {
+                std::snprintf(d.orientation_method, sizeof d.orientation_method, "%s", method.c_str());
+            } else if (method == "alphaGrad" || method == "isoRDF" || method == "RDF") {
+                throw SvpError(SVP_E_UNSUPPORTED, "orientationMethod " + method + " is not provided (LS/isoAlphaGrad are)");
+            } else {
+                throw SvpError(SVP_E_DICT, "Invalid orientationMethod " + method +
+                                               ". Valid methods are: alphaGrad isoAlphaGrad isoRDF LS RDF");
+            }
+        }
+        *out = d;
+    });
+}
+
+int svp_create(const svp_mesh* mesh, const svp_dict* dict, int device, svp_handle** out) {
+    return guarded([&] {
+        if (!mesh || !out) throw SvpError(SVP_E_ARG, "svp_create: null argument");
+        svp_dict d;
+        if (dict) d = *dict;
+        else svp_dict_defaults(&d);
+        const std::string m(d.orientation_method);
+        if (m != "LS" && m != "isoAlphaGrad") {
+            if (m == "alphaGrad" || m == "isoRDF" || m == "RDF")
+                throw SvpError(SVP_E_UNSUPPORTED, "orientationMethod " + m + " is not provided");
+            throw SvpError(SVP_E_DICT, "Invalid orientationMethod " + m);
+        }
+        if (d.n_alpha_bounds < 0 || !(d.mixed_cell_tol >= 0.0) || d.snap_tol < 0.0)
+            throw SvpError(SVP_E_DICT, "invalid tolerance or bound count");
+        auto h = std::make_unique<svp_handle>();
+        h->s = std::make_unique<solveVofEqu>(*mesh, d, device);
+        *out = h.release();
+    });
+}
+
+int svp_destroy(svp_handle* h) {
+    return guarded([&] { delete h; });
+}
+
+int svp_set_alpha(svp_handle* h, const double* alpha, int where) {
+    return guarded([&] { H(h).setAlpha(alpha, where); });
+}
+
+int svp_set_flow(svp_handle* h, const double* phi, const double* U, const double* Ub, int where) {
+    return guarded([&] { H(h).setFlow(phi, U, Ub, where); });
+}
+
+int svp_reconstruct(svp_handle* h) {
+    return guarded([&] { H(h).reconstruct(); });
+}
+
+int svp_advect(svp_handle* h, double dt, const double* Sp, const double* Su, int where) {
+    return guarded([&] {
+        solveVofEqu& s = H(h);
+        if (where == SVP_HOST && (Sp || Su)) {
+            // stage host sources in the harness buffers
+            double* dSp = nullptr;
+            double* dSu = nullptr;
+            if (Sp) {
+                if (!s.harnessBufs_[1].p) s.allocHarness(s.harnessBufs_[1], s.m_.nC);
+                dSp = (double*)s.harnessBufs_[1].p;
+                s.copyIn(dSp, Sp, s.m_.nC, SVP_HOST);
+            }
+            if (Su) {
+                if (!s.harnessBufs_[2].p) s.allocHarness(s.harnessBufs_[2], s.m_.nC);
+                dSu = (double*)s.harnessBufs_[2].p;
+                s.copyIn(dSu, Su, s.m_.nC, SVP_HOST);
+            }
+            s.advect(dt, dSp, dSu);
+        } else {
+            s.advect(dt, Sp, Su);
+        }
+    });
+}
+
+int svp_get_alpha(svp_handle* h, double* alpha, int where) {
+    return guarded([&] { H(h).getAlpha(alpha, where); });
+}
+
+int svp_get_alpha_phi(svp_handle* h, double* ap, int where) {
+    return guarded([&] {
+        solveVofEqu& s = H(h);
+        s.copyOut(ap, s.alphaPhi(), s.m_.nF, where);
+    });
+}
+
+int svp_get_rho_phi(svp_handle* h, double rho1, double rho2, double* out, int where) {
+    return guarded([&] {
+        solveVofEqu& s = H(h);
+        std::vector<double> ap(s.m_.nF), ph(s.m_.nF);
+        s.copyOut(ap.data(), s.alphaPhi(), s.m_.nF, SVP_HOST);
+        s.copyOut(ph.data(), s.st_.phi, s.m_.nF, SVP_HOST);
+        for (int f = 0; f < s.m_.nF; ++f) ap[f] = (rho1 - rho2) * ap[f] + rho2 * ph[f];
+        if (where == SVP_DEVICE) HIPCHECK(hipMemcpy(out, ap.data(), sizeof(double) * ap.size(), hipMemcpyHostToDevice));
+        else std::memcpy(out, ap.data(), sizeof(double) * ap.size());
+    });
+}
+
+int svp_n_mixed(svp_handle* h, int32_t* n) {
+    return guarded([&] {
+        if (!n) throw SvpError(SVP_E_ARG, "null output");
+        solveVofEqu& s = H(h);
+        s.checkDeviceErrors();
+        *n = s.nMixed();
+    });
+}
+
+int svp_get_interface(svp_handle* h, int32_t* cells, int32_t* status, double* n, double* d, double* C,
+                      double* S, int where) {
+    return guarded([&] {
+        solveVofEqu& s = H(h);
+        s.checkDeviceErrors();
+        const int k = s.nMixed();
+        s.copyOutT(cells, s.st_.mixed, k, where);
+        s.copyOutT(status, s.st_.status, k, where);
+        s.copyOut(n, s.st_.ifN, 3L * k, where);
+        s.copyOut(d, s.st_.ifD, k, where);
+        s.copyOut(C, s.st_.ifC, 3L * k, where);
+        s.copyOut(S, s.st_.ifS, 3L * k, where);
+    });
+}
+
+int svp_get_cut_class(svp_handle* h, uint64_t* fc, int32_t* ncp, int where) {
+    return guarded([&] {
+        solveVofEqu& s = H(h);
+        const int k = s.nMixed();
+        s.copyOutT((unsigned long long*)fc, s.st_.faceClass, k, where);
+        s.copyOutT(ncp, s.st_.nCutPts, k, where);
+    });
+}
+
+int svp_get_un0(svp_handle* h, double* un0, int where) {
+    return guarded([&] {
+        solveVofEqu& s = H(h);
+        s.copyOut(un0, s.st_.un0, s.nMixed(), where);
+    });
+}
+
+int svp_get_geometry(svp_handle* h, double* Cf, double* Sf, double* magSf, double* flat, double* C, double* V) {
+    return guarded([&] {
+        solveVofEqu& s = H(h);
+        s.copyOut(Cf, s.m_.Cf, 3L * s.m_.nF, SVP_HOST);
+        s.copyOut(Sf, s.m_.Sf, 3L * s.m_.nF, SVP_HOST);
+        s.copyOut(magSf, s.m_.magSf, s.m_.nF, SVP_HOST);
+        s.copyOut(flat, s.m_.flat, s.m_.nF, SVP_HOST);
+        s.copyOut(C, s.m_.C, 3L * s.m_.nC, SVP_HOST);
+        s.copyOut(V, s.m_.V, s.m_.nC, SVP_HOST);
+    });
+}
+
+int svp_timers(svp_handle* h, double* rec, double* adv, uint64_t* nic) {
+    return guarded([&] {
+        double r, a;
+        uint64_t n;
+        H(h).timers(r, a, n);
+        if (rec) *rec = r;
+        if (adv) *adv = a;
+        if (nic) *nic = n;
+    });
+}
+
+int svp_device_view_get(svp_handle* h, svp_device_view* out) {
+    return guarded([&] {
+        if (!out) throw SvpError(SVP_E_ARG, "null output");
+        solveVofEqu& s = H(h);
+        out->alpha = s.st_.alpha;
+        out->phi = s.st_.phi;
+        out->U = s.st_.U;
+        out->U_boundary = s.st_.Ub;
+        out->dVf = s.st_.dVf;
+        out->Cf = s.m_.Cf;
+        out->Sf = s.m_.Sf;
+        out->C = s.m_.C;
+        out->V = s.m_.V;
+        out->stream = (void*)s.stream_;
+    });
+}
+
+int svp_synchronize(svp_handle* h) {
+    return guarded([&] {
+        solveVofEqu& s = H(h);
+        HIPCHECK(hipStreamSynchronize(s.stream_));
+        s.checkDeviceErrors();
+    });
+}
+
+const char* svp_last_error(void) { return g_err.c_str(); }
+
+// ------------------------------- harness -----------------------------------
+
+int svp_harness_deformation(svp_handle* h, double* U0, double* phi0) {
+    return guarded([&] {
+        solveVofEqu& s = H(h);
+        s.ensureHostGeometry();
+        const double PI = M_PI;
+        auto U = [&](double x, double y, double z, double* u) {
+            u[0] = 2 * ((std::sin(PI * x)) * (std::sin(PI * x))) * std::sin(2 * PI * y) * std::sin(2 * PI * z);
+            u[1] = -std::sin(2 * PI * x) * ((std::sin(PI * y)) * (std::sin(PI * y))) * std::sin(2 * PI * z);
+            u[2] = -std::sin(2 * PI * x) * std::sin(2 * PI * y) * ((std::sin(PI * z)) * (std::sin(PI * z)));
+        };
+        for (int c = 0; c < s.m_.nC; ++c) U(s.hostC_[3 * c], s.hostC_[3 * c + 1], s.hostC_[3 * c + 2], U0 + 3 * c);
+        for (int f = 0; f < s.m_.nF; ++f) {
+            double u[3];
+            U(s.hostCf_[3 * f], s.hostCf_[3 * f + 1], s.hostCf_[3 * f + 2], u);
+            phi0[f] = u[0] * s.hostSf_[3 * f] + u[1] * s.hostSf_[3 * f + 1] + u[2] * s.hostSf_[3 * f + 2];
+        }
+    });
+}
+
+int svp_harness_rotation(svp_handle* h, const double* w, const double* o, double* U0, double* phi0) {
+    return guarded([&] {
+        solveVofEqu& s = H(h);
+        s.ensureHostGeometry();
+        auto U = [&](const double* x, double* u) {
+            const double r0 = x[0] - o[0], r1 = x[1] - o[1], r2 = x[2] - o[2];
+            u[0] = w[1] * r2 - w[2] * r1;
+            u[1] = w[2] * r0 - w[0] * r2;
+            u[2] = w[0] * r1 - w[1] * r0;
+        };
+        for (int c = 0; c < s.m_.nC; ++c) U(&s.hostC_[3 * c], U0 + 3 * c);
+        for (int f = 0; f < s.m_.nF; ++f) {
+            double u[3];
+            U(&s.hostCf_[3 * f], u);
+            phi0[f] = u[0] * s.hostSf_[3 * f] + u[1] * s.hostSf_[3 * f + 1] + u[2] * s.hostSf_[3 * f + 2];
+        }
+    });
+}
+
+int svp_harness_upload_base_flow(svp_handle* h, const double* U0, const double* phi0, const double* U0b) {
+    return guarded([&] {
+        solveVofEqu& s = H(h);
+        if (!s.U0_) {
+            s.U0_ = s.allocHarness(s.harnessBufs_[0], 3L * s.m_.nC + s.m_.nF + 3L * std::max(s.nB_, 1));
+            s.phi0_ = s.U0_ + 3L * s.m_.nC;
+            s.U0b_ = s.phi0_ + s.m_.nF;
+        }
+        s.copyIn(s.U0_, U0, 3L * s.m_.nC, SVP_HOST);
+        s.copyIn(s.phi0_, phi0, s.m_.nF, SVP_HOST);
+        if (U0b) s.copyIn(s.U0b_, U0b, 3L * s.nB_, SVP_HOST);
+        else HIPCHECK(hipMemset(s.U0b_, 0, sizeof(double) * 3 * std::max(s.nB_, 1)));
+    });
+}
+
+int svp_harness_scale_flow(svp_handle* h, double factor) {
+    return guarded([&] {
+        solveVofEqu& s = H(h);
+        if (!s.U0_) throw SvpError(SVP_E_ARG, "base flow not uploaded");
+        svp::launch_scale_flow(s.m_.nC, s.m_.nF, s.nB_, s.U0_, s.phi0_, s.U0b_, factor, s.st_.U, s.st_.phi,
+                               s.st_.Ub, s.stream_);
+        HIPCHECK(hipGetLastError());
+    });
+}
+
+int svp_harness_courant(svp_handle* h, double* co, double* aco) {
+    return guarded([&] {
+        solveVofEqu& s = H(h);
+        svp::launch_courant(s.m_, s.st_.phi, s.st_.alpha, s.courant_, s.stream_);
+        unsigned long long r[2];
+        HIPCHECK(hipMemcpyAsync(r, s.courant_, sizeof r, hipMemcpyDeviceToHost, s.stream_));
+        HIPCHECK(hipStreamSynchronize(s.stream_));
+        double v[2];
+        std::memcpy(v, r, sizeof v);
+        if (co) *co = v[0];
+        if (aco) *aco = v[1];
+    });
+}
+
+}  // extern "C"

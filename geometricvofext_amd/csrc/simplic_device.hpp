@@ -1,0 +1,382 @@
+// Device-side FP64 geometry of the SimPLIC alpha transport (gfx950).
+//
+// Every routine reproduces the operation order of the reference so that the
+// results are bit-identical to an IEEE-754 x86-64 build of the reference:
+// the translation unit is compiled with -ffp-contract=off (no FMA fusion) and
+// uses the correctly rounded double '/' and sqrt.
+//
+// Face cuts are evaluated one face at a time into a FaceCut record; a cell cut
+// combines the records of its faces in cell-face order.  The records are the
+// unit of parallelism of the kernels (one lane per face and cut level) and the
+// combination order is what keeps the sums bit-exact.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace svp {
+
+// OpenFOAM double-precision constants
+#define SVP_SMALL 1e-15
+#define SVP_VSMALL 1e-300
+#define SVP_ROOTVSMALL 1e-150
+#define SVP_TSMALL 1e-14
+
+// Size limits of the device cut routines (checked on the host at create time).
+constexpr int MAX_FACE_POINTS = 8;    // points of one (local) face
+constexpr int MAX_IF_POINTS = 4;      // plane crossings recorded per face
+constexpr int MAX_CELL_FACES = 24;    // local faces of one cell (split-warped hex: 24)
+constexpr int MAX_CELL_POINTS = 24;   // local points of one cell
+// cells without warped-face splitting (mesh faces as they are)
+constexpr int NS_CELL_FACES = 8;
+constexpr int NS_CELL_POINTS = 16;
+constexpr int NS_FACE_IDX = 48;
+// cells with warped faces split into triangles around the face centre
+constexpr int SP_FACE_IDX = 96;
+
+struct V3 {
+    double x, y, z;
+};
+__host__ __device__ inline V3 v3(double a, double b, double c) { return V3{a, b, c}; }
+__host__ __device__ inline V3 operator+(V3 a, V3 b) { return v3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__host__ __device__ inline V3 operator-(V3 a, V3 b) { return v3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__host__ __device__ inline V3 operator-(V3 a) { return v3(-a.x, -a.y, -a.z); }
+__host__ __device__ inline V3 operator*(double s, V3 a) { return v3(s * a.x, s * a.y, s * a.z); }
+__host__ __device__ inline V3 operator/(V3 a, double s) { return v3(a.x / s, a.y / s, a.z / s); }
+__host__ __device__ inline double dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__host__ __device__ inline V3 cross(V3 a, V3 b) {
+    return v3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+__host__ __device__ inline double magSqr(V3 a) { return a.x * a.x + a.y * a.y + a.z * a.z; }
+__device__ inline double mag(V3 a) { return sqrt(magSqr(a)); }
+__host__ __device__ inline double sgn(double s) { return (s >= 0.0) ? 1.0 : -1.0; }
+__host__ __device__ inline double pos0(double s) { return (s >= 0.0) ? 1.0 : 0.0; }
+__host__ __device__ inline double neg0(double s) { return (s <= 0.0) ? 1.0 : 0.0; }
+__device__ inline V3 ld3(const double* p, int i) { return v3(p[3 * i], p[3 * i + 1], p[3 * i + 2]); }
+__device__ inline void st3(double* p, int i, V3 v) { p[3 * i] = v.x; p[3 * i + 1] = v.y; p[3 * i + 2] = v.z; }
+constexpr V3 VZERO = {0.0, 0.0, 0.0};
+
+// face::centre of an ordered point list (OpenFOAM face.C)
+__device__ inline V3 faceCentre(const V3* p, int n) {
+    if (n == 3) return (1.0 / 3.0) * (p[0] + p[1] + p[2]);
+    V3 c = VZERO;
+    for (int i = 0; i < n; ++i) c = c + p[i];
+    c = c / (double)n;
+    double sumA = 0.0;
+    V3 sumAc = VZERO;
+    for (int i = 0; i < n; ++i) {
+        const V3 nx = p[(i + 1) % n];
+        const V3 ttc = p[i] + nx + c;
+        const double ta = mag(cross(p[i] - c, nx - c));
+        sumA += ta;
+        sumAc = sumAc + ta * ttc;
+    }
+    if (sumA > SVP_VSMALL) return sumAc / (3.0 * sumA);
+    return c;
+}
+
+__device__ inline V3 triAreaNormal(V3 a, V3 b, V3 c) { return 0.5 * cross(b - a, c - a); }
+
+// face::areaNormal (OpenFOAM face.C)
+__device__ inline V3 faceAreaNormal(const V3* p, int n) {
+    if (n == 3) return triAreaNormal(p[0], p[1], p[2]);
+    V3 c = VZERO;
+    for (int i = 0; i < n; ++i) c = c + p[i];
+    c = c / (double)n;
+    V3 nrm = VZERO;
+    for (int i = 0; i < n; ++i) {
+        const V3 nx = (i < n - 1) ? p[i + 1] : p[0];
+        nrm = nrm + triAreaNormal(p[i], nx, c);
+    }
+    return nrm;
+}
+
+// primitiveMeshTools::makeFaceCentresAndAreas for one face
+__device__ inline void primitiveFaceGeom(const V3* p, int n, V3& fc, V3& fa) {
+    if (n == 3) {
+        fc = (1.0 / 3.0) * (p[0] + p[1] + p[2]);
+        fa = 0.5 * cross(p[1] - p[0], p[2] - p[0]);
+        return;
+    }
+    V3 sumN = VZERO, sumAc = VZERO;
+    double sumA = 0.0;
+    V3 c = p[0];
+    for (int i = 1; i < n; ++i) c = c + p[i];
+    c = c / (double)n;
+    for (int i = 0; i < n; ++i) {
+        const V3 nx = p[i == n - 1 ? 0 : i + 1];
+        const V3 th = p[i];
+        const V3 cc = th + nx + c;
+        const V3 nn = cross(nx - th, c - th);
+        const double a = mag(nn);
+        sumN = sumN + nn;
+        sumA += a;
+        sumAc = sumAc + a * cc;
+    }
+    if (sumA < SVP_ROOTVSMALL) {
+        fc = c;
+        fa = VZERO;
+    } else {
+        fc = (1.0 / 3.0) * sumAc / sumA;
+        fa = 0.5 * sumN;
+    }
+}
+
+// Result of cutting one face by the plane n.x + d = 0 (cutFace::calcSubFace)
+struct FaceCut {
+    int status;          // -1 submerged, 0 cut, 1 empty
+    int nIf;             // number of recorded plane crossings
+    V3 centre, area;     // sub-face centre and area vector
+    V3 ifPts[MAX_IF_POINTS];
+};
+
+// cutFace::calcSubFace (cutFace.C:136-259) + calcSubFaceCentreAndArea
+// (cutFace.C:37-96).  The sub-face polygon is walked twice instead of being
+// stored: once for the mean point, once for the triangle fan.  Returns false
+// when more than MAX_IF_POINTS crossings occur.
+__device__ inline bool calcSubFace(const V3* fPts, int n, V3 normal, double distance, FaceCut& out,
+                                   bool wantArea = true) {
+    double dist[MAX_FACE_POINTS];
+    int nSub = 0, first = -1;
+    for (int i = 0; i < n; ++i) {
+        double di = dot(fPts[i], normal) + distance;
+        if (fabs(di) < SVP_TSMALL) di += sgn(di) * SVP_TSMALL;
+        dist[i] = di;
+        if (di < 0.0) {
+            nSub++;
+            if (first == -1) first = i;
+        }
+    }
+    out.nIf = 0;
+    if (nSub == n) {
+        out.status = -1;
+        out.centre = faceCentre(fPts, n);
+        out.area = faceAreaNormal(fPts, n);
+        return true;
+    }
+    if (nSub == 0) {
+        out.status = 1;
+        out.centre = VZERO;
+        out.area = VZERO;
+        return true;
+    }
+    // pass 1: count points, mean, crossings
+    bool ok = true;
+    int nPts = 0;
+    V3 p0 = VZERO, p1 = VZERO, p2 = VZERO;
+    V3 fCentre = VZERO;
+    for (int i = first; i < first + n; ++i) {
+        const int cur = i % n, nxt = (i + 1) % n;
+        if (dist[cur] < 0) {
+            const V3 q = fPts[cur];
+            if (nPts == 0) fCentre = q; else fCentre = fCentre + q;
+            if (nPts == 0) p0 = q; else if (nPts == 1) p1 = q; else if (nPts == 2) p2 = q;
+            nPts++;
+        }
+        if ((dist[cur] * dist[nxt]) < 0) {
+            const double w = dist[cur] / (dist[cur] - dist[nxt]);
+            const V3 q = fPts[cur] + w * (fPts[nxt] - fPts[cur]);
+            if (nPts == 0) fCentre = q; else fCentre = fCentre + q;
+            if (nPts == 0) p0 = q; else if (nPts == 1) p1 = q; else if (nPts == 2) p2 = q;
+            nPts++;
+            if (out.nIf < MAX_IF_POINTS) out.ifPts[out.nIf] = q; else ok = false;
+            out.nIf++;
+        }
+    }
+    if (out.nIf > MAX_IF_POINTS) out.nIf = MAX_IF_POINTS;
+    if (nPts < 3) {
+        out.status = -1;
+        out.centre = faceCentre(fPts, n);
+        out.area = faceAreaNormal(fPts, n);
+        return ok;
+    }
+    out.status = 0;
+    if (!wantArea) return ok;
+    if (nPts == 3) {
+        out.centre = (1.0 / 3.0) * (p0 + p1 + p2);
+        out.area = 0.5 * cross(p1 - p0, p2 - p0);
+        return ok;
+    }
+    fCentre = fCentre / (double)nPts;
+    // pass 2: triangle fan (p_k, p_k+1, fCentre) in emission order
+    V3 sumN = VZERO, sumAc = VZERO;
+    double sumA = 0.0;
+    V3 prev = VZERO, firstP = VZERO;
+    int k = 0;
+    auto tri = [&](V3 a, V3 b) {
+        const V3 c = a + b + fCentre;
+        const V3 nn = cross(b - a, fCentre - a);
+        const double ar = mag(nn);
+        sumN = sumN + nn;
+        sumA += ar;
+        sumAc = sumAc + ar * c;
+    };
+    for (int i = first; i < first + n; ++i) {
+        const int cur = i % n, nxt = (i + 1) % n;
+        if (dist[cur] < 0) {
+            const V3 q = fPts[cur];
+            if (k == 0) firstP = q; else tri(prev, q);
+            prev = q;
+            k++;
+        }
+        if ((dist[cur] * dist[nxt]) < 0) {
+            const double w = dist[cur] / (dist[cur] - dist[nxt]);
+            const V3 q = fPts[cur] + w * (fPts[nxt] - fPts[cur]);
+            if (k == 0) firstP = q; else tri(prev, q);
+            prev = q;
+            k++;
+        }
+    }
+    tri(prev, firstP);
+    if (sumA < SVP_ROOTVSMALL) {
+        out.centre = fCentre;
+        out.area = VZERO;
+    } else {
+        out.centre = (1.0 / 3.0) * sumAc / sumA;
+        out.area = 0.5 * sumN;
+    }
+    return ok;
+}
+
+// Local description of one cell: its points and faces as point-index lists
+// (cutCell::getLocalPointFieldAndFaceList, cutCell.C:140-236).  Without
+// splitWarpedFace the faces are the mesh faces in their stored orientation.
+template <int MF, int MP, int MI>
+struct CellPolyT {
+    static constexpr int maxFaces = MF, maxPts = MP, maxIdx = MI;
+    int nPts, nFaces;
+    V3 pts[MP];
+    int fStart[MF + 1];
+    unsigned char fIdx[MI];
+};
+using CellPolyNS = CellPolyT<NS_CELL_FACES, NS_CELL_POINTS, NS_FACE_IDX>;
+using CellPolySP = CellPolyT<MAX_CELL_FACES, MAX_CELL_POINTS, SP_FACE_IDX>;
+
+// Result of cutting a cell (cutCell::calcSubCell)
+struct CellCut {
+    int status;
+    double VOF;
+    V3 ifCentre, ifArea;
+};
+
+template <class Poly>
+__device__ inline void gatherFace(const Poly& P, int f, V3* buf, int& n) {
+    n = P.fStart[f + 1] - P.fStart[f];
+    for (int k = 0; k < n; ++k) buf[k] = P.pts[P.fIdx[P.fStart[f] + k]];
+}
+
+// cutCell::calcInterfaceCentreAndArea (cutCell.C:37-100) over face records
+// in cell-face order; the orientation test is against the zero vector, which
+// is what subCellCentre_ holds at that point of the reference.
+__device__ inline void interfaceCentreAndArea(const FaceCut* fc, int nF, V3& ifC, V3& ifS) {
+    V3 fCentre = VZERO;
+    int nEdgePoints = 0;
+    for (int f = 0; f < nF; ++f) {
+        if (fc[f].status != 0) continue;
+        for (int k = 0; k < fc[f].nIf; ++k) {
+            fCentre = fCentre + fc[f].ifPts[k];
+            nEdgePoints++;
+        }
+    }
+    if (nEdgePoints > 0) fCentre = fCentre / (double)nEdgePoints;
+    V3 sumN = VZERO, sumAc = VZERO;
+    double sumA = 0.0;
+    for (int f = 0; f < nF; ++f) {
+        if (fc[f].status != 0) continue;
+        for (int k = 0; k < fc[f].nIf - 1; ++k) {
+            const V3 a = fc[f].ifPts[k], b = fc[f].ifPts[k + 1];
+            const V3 c = a + b + fCentre;
+            const V3 nn = cross(b - a, fCentre - a);
+            const double ar = mag(nn);
+            sumN = sumN + sgn(dot(nn, sumN)) * nn;
+            sumA += ar;
+            sumAc = sumAc + ar * c;
+        }
+    }
+    if (sumA < SVP_ROOTVSMALL) {
+        ifC = fCentre;
+        ifS = VZERO;
+    } else {
+        ifC = (1.0 / 3.0) * sumAc / sumA;
+        ifS = 0.5 * sumN;
+    }
+    if (dot(ifS, ifC - VZERO) < 0.0) ifS = (-1.0) * ifS;
+}
+
+// cutCell::calcSubCell (cutCell.C:343-542) combining face records fc[0..nF)
+// produced in cell-face order.  Cc, Vc: cell centre and volume.
+__device__ inline void combineSubCell(const FaceCut* fc, int nF, V3 Cc, double Vc, CellCut& out) {
+    bool fullySubmerged = true, fullyEmpty = true;
+    int nSubmergedFaces = 0;
+    for (int f = 0; f < nF; ++f) {
+        if (fc[f].status == 0) {
+            fullySubmerged = false;
+            fullyEmpty = false;
+        } else if (fc[f].status == -1) {
+            fullyEmpty = false;
+            nSubmergedFaces++;
+        } else {
+            fullySubmerged = false;
+        }
+    }
+    out.ifCentre = VZERO;
+    out.ifArea = VZERO;
+    if (!fullySubmerged && !fullyEmpty) {
+        out.status = 0;
+        interfaceCentreAndArea(fc, nF, out.ifCentre, out.ifArea);
+        if (mag(out.ifArea) < SVP_TSMALL) {
+            if (nSubmergedFaces == 0) {
+                out.status = 1;
+                out.VOF = 0.0;
+            } else {
+                out.status = -1;
+                out.VOF = 1.0;
+            }
+            return;
+        }
+        // calcSubCellCentreAndVolume (cutCell.C:103-137): cut/submerged faces
+        // in order, then the interface polygon
+        V3 sum = VZERO;
+        int cnt = 0;
+        for (int f = 0; f < nF; ++f)
+            if (fc[f].status != 1) { sum = sum + fc[f].centre; cnt++; }
+        sum = sum + out.ifCentre;
+        cnt++;
+        const V3 cEst = sum / (double)cnt;
+        double vol = 0.0;
+        for (int f = 0; f < nF; ++f) {
+            if (fc[f].status == 1) continue;
+            const double pyr3Vol = fmax(fabs(dot(fc[f].area, fc[f].centre - cEst)), SVP_VSMALL);
+            vol += pyr3Vol;
+        }
+        {
+            const double pyr3Vol = fmax(fabs(dot(out.ifArea, out.ifCentre - cEst)), SVP_VSMALL);
+            vol += pyr3Vol;
+        }
+        vol /= 3.0;
+        out.VOF = vol / Vc;
+    } else if (fullyEmpty) {
+        out.status = 1;
+        out.VOF = 0.0;
+    } else {
+        out.status = -1;
+        out.VOF = 1.0;
+    }
+    (void)Cc;
+}
+
+// Cut the whole cell sequentially (one lane)
+template <class Poly>
+__device__ inline bool calcSubCell(const Poly& P, V3 Cc, double Vc, V3 normal, double distance,
+                                   FaceCut* fc, CellCut& out) {
+    bool ok = true;
+    V3 buf[MAX_FACE_POINTS];
+    for (int f = 0; f < P.nFaces; ++f) {
+        int n;
+        gatherFace(P, f, buf, n);
+        ok &= calcSubFace(buf, n, normal, distance, fc[f]);
+    }
+    combineSubCell(fc, P.nFaces, Cc, Vc, out);
+    return ok;
+}
+
+}  // namespace svp

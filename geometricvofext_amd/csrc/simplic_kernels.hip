@@ -1,0 +1,930 @@
+// HIP kernels of the SimPLIC per-time-step alpha transport on gfx950.
+//
+// Step = reconstruct (detection -> LS normal -> plane placement) + advect
+// (upwind face volumes -> interface face volumes -> conservative update ->
+// bounding).  All arithmetic is FP64 in the reference's operation order
+// (compiled with -ffp-contract=off); each kernel cites the reference lines it
+// follows.  Control flow that the reference runs on the host (mixed-cell count,
+// bounding passes) stays on the device, so a step never waits for the host.
+#include "simplic_device.hpp"
+#include "simplic_types.hpp"
+
+namespace svp {
+
+static inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
+
+// ---------------------------------------------------------------------------
+// Mesh geometry (OpenFOAM primitiveMeshTools), computed once at create time
+// ---------------------------------------------------------------------------
+__global__ void k_face_geom(DevMesh m) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= m.nF) return;
+    const int s = m.fOff[f];
+    const int n = m.fOff[f + 1] - s;
+    V3 p[MAX_FACE_POINTS];
+    for (int k = 0; k < n; ++k) p[k] = ld3(m.pts, m.fPts[s + k]);
+    V3 fc, fa;
+    primitiveFaceGeom(p, n, fc, fa);
+    st3(m.Cf, f, fc);
+    st3(m.Sf, f, fa);
+    const double ms = mag(fa);
+    m.magSf[f] = ms;
+    // reconstruction::updateFaceFlatness (reconstruction.C:408-440)
+    double fl = 1.0;
+    if (n > 3 && ms > SVP_ROOTVSMALL) {
+        double sumA = 0.0;
+        for (int i = 0; i < n; ++i) {
+            const V3 th = p[i], nx = p[(i + 1) % n];
+            sumA += mag(0.5 * cross(nx - th, fc - th));
+        }
+        fl = ms / (sumA + SVP_ROOTVSMALL);
+    }
+    m.flat[f] = fl;
+}
+
+// primitiveMeshTools::makeCellCentresAndVols; cells() face order equals the
+// owner-loop-then-neighbour-loop order of the reference implementation.
+__global__ void k_cell_geom(DevMesh m) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= m.nC) return;
+    const int b = m.cOff[c], e = m.cOff[c + 1];
+    V3 cEst = VZERO;
+    for (int k = b; k < e; ++k) cEst = cEst + ld3(m.Cf, m.cFaces[k]);
+    cEst = cEst / (double)(e - b);
+    V3 ctr = VZERO;
+    double vol = 0.0;
+    for (int k = b; k < e; ++k) {
+        const int f = m.cFaces[k];
+        const V3 fc = ld3(m.Cf, f), fa = ld3(m.Sf, f);
+        const double pyr3Vol = (m.own[f] == c) ? dot(fa, fc - cEst) : dot(fa, cEst - fc);
+        const V3 pc = (3.0 / 4.0) * fc + (1.0 / 4.0) * cEst;
+        ctr = ctr + pyr3Vol * pc;
+        vol += pyr3Vol;
+    }
+    if (fabs(vol) > SVP_VSMALL) ctr = ctr / vol;
+    else ctr = cEst;
+    st3(m.C, c, ctr);
+    m.V[c] = vol * (1.0 / 3.0);
+}
+
+// volPointInterpolation weights: inverse distance to the point's cells, or for
+// boundary points to its boundary faces, normalised by their sum.
+__global__ void k_point_weights(DevMesh m) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= m.nP) return;
+    const V3 x = ld3(m.pts, p);
+    double sumW = 0.0;
+    for (int k = m.psOff[p]; k < m.psOff[p + 1]; ++k) {
+        const int s = m.psSrc[k];
+        const V3 y = (s >= 0) ? ld3(m.C, s) : ld3(m.Cf, ~s);
+        const double w = 1.0 / mag(x - y);
+        m.psW[k] = w;
+        sumW += w;
+    }
+    for (int k = m.psOff[p]; k < m.psOff[p + 1]; ++k) m.psW[k] = m.psW[k] / sumW;
+}
+
+void launch_geometry(const DevMesh& m, hipStream_t s) {
+    k_face_geom<<<cdiv(m.nF, 256), 256, 0, s>>>(m);
+    k_cell_geom<<<cdiv(m.nC, 256), 256, 0, s>>>(m);
+    k_point_weights<<<cdiv(m.nP, 256), 256, 0, s>>>(m);
+}
+
+// ---------------------------------------------------------------------------
+// Ordered compaction helpers (block-count / scan / write)
+// ---------------------------------------------------------------------------
+int detect_blocks(int nC) { return cdiv(nC, DETECT_BLOCK); }
+
+// exclusive prefix of a per-lane flag within a DETECT_BLOCK block; returns the
+// block total in *total (LDS-resident)
+__device__ inline int block_prefix(bool flag, int* lds, int& total) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const unsigned long long bal = __ballot(flag);
+    const int inWave = __popcll(bal & ((1ull << lane) - 1ull));
+    if (lane == 0) lds[wave] = __popcll(bal);
+    __syncthreads();
+    int off = 0, tot = 0;
+    for (int w = 0; w < DETECT_BLOCK / 64; ++w) {
+        if (w < wave) off += lds[w];
+        tot += lds[w];
+    }
+    total = tot;
+    return off + inWave;
+}
+
+// single-workgroup exclusive scan of counts[0..nb) in place; total -> *out
+__global__ void k_scan_counts(int* counts, int nb, int* out, unsigned long long* accum) {
+    __shared__ int lds[1024];
+    __shared__ int carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (int base = 0; base < nb; base += 1024) {
+        const int i = base + threadIdx.x;
+        const int v = (i < nb) ? counts[i] : 0;
+        lds[threadIdx.x] = v;
+        __syncthreads();
+        for (int o = 1; o < 1024; o <<= 1) {
+            const int t = (threadIdx.x >= o) ? lds[threadIdx.x - o] : 0;
+            __syncthreads();
+            lds[threadIdx.x] += t;
+            __syncthreads();
+        }
+        if (i < nb) counts[i] = carry + lds[threadIdx.x] - v;
+        __syncthreads();
+        if (threadIdx.x == 1023) carry += lds[1023];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        *out = carry;
+        if (accum) *accum += (unsigned long long)carry;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// reconstruction::initialize mixed-cell detection (reconstruction.C:634-677,
+// isAMixedCell reconstruction.H:281-288): ascending list of mixed cells
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(DETECT_BLOCK) k_detect_flags(DevMesh m, DevState st, double tol) {
+    __shared__ int lds[DETECT_BLOCK / 64];
+    const int c = blockIdx.x * DETECT_BLOCK + threadIdx.x;
+    bool flag = false;
+    if (c < m.nC) {
+        const double a = st.alpha[c];
+        flag = (tol < a) && (a < 1.0 - tol);
+        st.mixedFlag[c] = flag ? 1 : 0;
+    }
+    int total;
+    block_prefix(flag, lds, total);
+    if (threadIdx.x == 0) st.blockCount[blockIdx.x] = total;
+}
+
+__global__ void __launch_bounds__(DETECT_BLOCK) k_compact_flags(int n, const unsigned char* flag,
+                                                                const int* blockOff, int* out) {
+    __shared__ int lds[DETECT_BLOCK / 64];
+    const int c = blockIdx.x * DETECT_BLOCK + threadIdx.x;
+    const bool f = (c < n) && flag[c];
+    int total;
+    const int pre = block_prefix(f, lds, total);
+    if (f) out[blockOff[blockIdx.x] + pre] = c;
+}
+
+void launch_detect(const DevMesh& m, const DevState& st, const Params& p, hipStream_t s) {
+    const int nb = detect_blocks(m.nC);
+    k_detect_flags<<<nb, DETECT_BLOCK, 0, s>>>(m, st, p.mixedCellTol);
+    k_scan_counts<<<1, 1024, 0, s>>>(st.blockCount, nb, st.nMixed, st.nIcTotal);
+    k_compact_flags<<<nb, DETECT_BLOCK, 0, s>>>(m.nC, st.mixedFlag, st.blockCount, st.mixed);
+}
+
+// ---------------------------------------------------------------------------
+// LS normal (reconstruction.C:85-141): n = -grad(alpha)/(|grad| + SMALL) with
+// the polyDegree1 least-squares gradient over the cell-point-cell stencil
+// (normal equations, Cholesky decomposition and inverse of OpenFOAM's
+// scalarMatrices, x = inv(A) B).
+// ---------------------------------------------------------------------------
+__device__ inline V3 ls_gradient(const DevMesh& m, const double* alpha, int c) {
+    double A[4][4] = {};
+    double B[4] = {};
+    const V3 cc = ld3(m.C, c);
+    for (int k = m.stOff[c]; k < m.stOff[c + 1]; ++k) {
+        const int g = m.st[k];
+        const V3 d = ld3(m.C, g) - cc;
+        const double t[4] = {1.0, d.x, d.y, d.z};
+        const double v = alpha[g];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) A[r][q] += t[r] * t[q];
+            B[r] += t[r] * v;
+        }
+    }
+    // LUDecompose(scalarSymmetricSquareMatrix)
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+#pragma unroll
+        for (int k = j + 1; k < 4; k++) A[j][k] = 0.0;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        double d = 0.0;
+#pragma unroll
+        for (int k = 0; k < j; k++) {
+            double s = 0.0;
+#pragma unroll
+            for (int i = 0; i < k; i++) s += A[i][k] * A[i][j];
+            s = (A[j][k] - s) / A[k][k];
+            A[k][j] = s;
+            A[j][k] = s;
+            d += s * s;
+        }
+        d = A[j][j] - d;
+        A[j][j] = sqrt(d);
+    }
+    // invDecomposed
+    double inv[4][4] = {};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        inv[i][i] = 1.0 / A[i][i];
+#pragma unroll
+        for (int j = 0; j < i; ++j) {
+            double sum = 0.0;
+#pragma unroll
+            for (int k = j; k < i; k++) sum -= A[i][k] * inv[k][j];
+            inv[i][j] = sum / A[i][i];
+        }
+    }
+    double R[4][4] = {};
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+#pragma unroll
+        for (int i = 0; i <= k; i++)
+#pragma unroll
+            for (int j = 0; j <= k; j++) R[i][j] += inv[k][i] * inv[k][j];
+    double x[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) x[i] += R[i][j] * B[j];
+    return v3(x[1], x[2], x[3]);
+}
+
+__global__ void k_ls_normal(DevMesh m, DevState st) {
+    const int nMixed = *st.nMixed;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nMixed; i += gridDim.x * blockDim.x) {
+        const int c = st.mixed[i];
+        V3 n = -ls_gradient(m, st.alpha, c);
+        n = n / (mag(n) + SVP_SMALL);
+        st3(st.ifN, i, n);
+    }
+}
+
+void launch_ls_normal(const DevMesh& m, const DevState& st, hipStream_t s) {
+    k_ls_normal<<<1024, 64, 0, s>>>(m, st);
+}
+
+// ---------------------------------------------------------------------------
+// Local polyhedron of a cell (cutCell.C:140-236)
+// ---------------------------------------------------------------------------
+template <class Poly>
+__device__ inline bool build_cell_poly(const DevMesh& m, int c, bool split, Poly& P) {
+    constexpr int MP = Poly::maxPts, MF = Poly::maxFaces, MI = Poly::maxIdx;
+    P.nPts = 0;
+    P.nFaces = 0;
+    P.fStart[0] = 0;
+    int gl[MP];    // global labels of P.pts in first-appearance order
+    int nIdx = 0;
+    bool ok = true;
+    const int b = m.cOff[c], e = m.cOff[c + 1];
+    // c.labels(faces): unique points in order of first appearance
+    for (int k = b; k < e; ++k) {
+        const int f = m.cFaces[k];
+        for (int q = m.fOff[f]; q < m.fOff[f + 1]; ++q) {
+            const int p = m.fPts[q];
+            bool found = false;
+            for (int r = 0; r < P.nPts; ++r) found |= (gl[r] == p);
+            if (!found) {
+                if (P.nPts < MP) {
+                    gl[P.nPts] = p;
+                    P.pts[P.nPts] = ld3(m.pts, p);
+                }
+                P.nPts++;
+            }
+        }
+    }
+    if (P.nPts > MP) return false;
+    auto local = [&](int p) {
+        int r = 0;
+        while (gl[r] != p) ++r;
+        return r;
+    };
+    for (int k = b; k < e; ++k) {
+        const int f = m.cFaces[k];
+        const int fs = m.fOff[f], n = m.fOff[f + 1] - fs;
+        const bool rev = split && (m.own[f] != c);
+        if (!split || m.flat[f] > (1.0 - SVP_TSMALL)) {
+            if (P.nFaces >= MF || nIdx + n > MI) { ok = false; break; }
+            for (int q = 0; q < n; ++q) {
+                // face::reverseFace keeps the first point
+                const int qq = rev ? (q == 0 ? 0 : n - q) : q;
+                P.fIdx[nIdx++] = (unsigned char)local(m.fPts[fs + qq]);
+            }
+            P.fStart[++P.nFaces] = nIdx;
+        } else {
+            if (P.nPts >= MP) { ok = false; break; }
+            P.pts[P.nPts] = ld3(m.Cf, f);
+            const int centre = P.nPts++;
+            for (int q = 0; q < n; ++q) {
+                if (P.nFaces >= MF || nIdx + 3 > MI) { ok = false; break; }
+                const int a0 = centre, a1 = local(m.fPts[fs + q]), a2 = local(m.fPts[fs + (q + 1) % n]);
+                P.fIdx[nIdx++] = (unsigned char)a0;
+                P.fIdx[nIdx++] = (unsigned char)(rev ? a2 : a1);
+                P.fIdx[nIdx++] = (unsigned char)(rev ? a1 : a2);
+                P.fStart[++P.nFaces] = nIdx;
+            }
+            if (!ok) break;
+        }
+    }
+    return ok;
+}
+
+// ---------------------------------------------------------------------------
+// cutCell::findSignedDistance (cutCell.C:611-799), one lane per mixed cell
+// ---------------------------------------------------------------------------
+struct FsdOut {
+    int status, nCutPts;
+    unsigned long long faceClass;
+    double d;
+    V3 C, S;
+    bool wrote;
+};
+
+__device__ inline void classify(const FaceCut* fc, int nF, FsdOut& o) {
+    o.faceClass = 0;
+    o.nCutPts = 0;
+    for (int f = 0; f < nF; ++f) {
+        o.faceClass |= (unsigned long long)(fc[f].status + 1) << (2 * f);
+        if (fc[f].status == 0) o.nCutPts += fc[f].nIf;
+    }
+}
+
+template <class Poly>
+__device__ inline void find_signed_distance(const Poly& P, V3 Cc, double Vc, double alphaI,
+                                            V3 normalI, FaceCut* fc, FsdOut& o, bool& ok) {
+    o.wrote = false;
+    o.faceClass = 0;
+    o.nCutPts = 0;
+    if (mag(normalI) < SVP_TSMALL) {
+        o.status = (int)sgn(0.5 - alphaI);
+        return;
+    }
+    double vd[Poly::maxPts];
+    const int n = P.nPts;
+    for (int k = 0; k < n; ++k) vd[k] = -dot(normalI, P.pts[k]);
+    // descending sort of the vertex levels (only the sorted values are used)
+    for (int k = 1; k < n; ++k) {
+        const double v = vd[k];
+        int j = k - 1;
+        while (j >= 0 && vd[j] < v) {
+            vd[j + 1] = vd[j];
+            --j;
+        }
+        vd[j + 1] = v;
+    }
+    double lowDistance = vd[0], upDistance = vd[n - 1];
+    int lowLabel = 0, upLabel = n - 1;
+    double lowAlpha = 0.0, upAlpha = 1.0;
+    CellCut cut;
+    while ((upLabel - lowLabel) > 1) {
+        const int midLabel = (int)round(0.5 * (double)(upLabel + lowLabel));
+        const double midDistance = vd[midLabel];
+        ok &= calcSubCell(P, Cc, Vc, normalI, midDistance, fc, cut);
+        const double midAlpha = cut.VOF;
+        if (fabs(midAlpha - alphaI) < SVP_TSMALL) {
+            o.d = midDistance;
+            o.C = cut.ifCentre;
+            o.S = cut.ifArea;
+            o.status = cut.status;
+            o.wrote = true;
+            classify(fc, P.nFaces, o);
+            return;
+        }
+        if (midAlpha > alphaI) {
+            upLabel = midLabel;
+            upDistance = midDistance;
+            upAlpha = midAlpha;
+        } else {
+            lowLabel = midLabel;
+            lowDistance = midDistance;
+            lowAlpha = midAlpha;
+        }
+    }
+    if (fabs(lowDistance - upDistance) < SVP_TSMALL) {
+        const double midDistance = 0.5 * (lowDistance + upDistance);
+        ok &= calcSubCell(P, Cc, Vc, normalI, midDistance, fc, cut);
+        o.d = midDistance;
+        o.C = cut.ifCentre;
+        o.S = cut.ifArea;
+        o.status = cut.status;
+        o.wrote = true;
+        classify(fc, P.nFaces, o);
+        return;
+    }
+    const double alphaPrismatoid = upAlpha - lowAlpha;
+    const double deltaDistance = (upDistance - lowDistance) / 3.0;
+    const double distanceOneThird = lowDistance + deltaDistance;
+    ok &= calcSubCell(P, Cc, Vc, normalI, distanceOneThird, fc, cut);
+    const double alphaOneThird = cut.VOF - lowAlpha;
+    const double distanceTwoThirds = lowDistance + 2.0 * deltaDistance;
+    ok &= calcSubCell(P, Cc, Vc, normalI, distanceTwoThirds, fc, cut);
+    const double alphaTwoThirds = cut.VOF - lowAlpha;
+    const double a = 13.5 * alphaOneThird - 13.5 * alphaTwoThirds + 4.5 * alphaPrismatoid;
+    const double b = -22.5 * alphaOneThird + 18.0 * alphaTwoThirds - 4.5 * alphaPrismatoid;
+    const double c = 9.0 * alphaOneThird - 4.5 * alphaTwoThirds + 1.0 * alphaPrismatoid;
+    const double d = lowAlpha - alphaI;
+    double lambda = 0.5;
+    for (int iter = 0; iter < 100; iter++) {
+        const double func = a * (lambda * lambda * lambda) + b * (lambda * lambda) + c * lambda + d;
+        const double funcPrime = 3.0 * a * (lambda * lambda) + 2.0 * b * lambda + c;
+        const double lambdaNew = lambda - (func / funcPrime);
+        if (fabs(lambdaNew - lambda) < SVP_TSMALL) break;
+        lambda = lambdaNew;
+    }
+    const double distance0 = lowDistance - lambda * (lowDistance - upDistance);
+    ok &= calcSubCell(P, Cc, Vc, normalI, distance0, fc, cut);
+    o.d = distance0;
+    o.C = cut.ifCentre;
+    o.S = cut.ifArea;
+    o.status = cut.status;
+    o.wrote = true;
+    classify(fc, P.nFaces, o);
+}
+
+template <class Poly>
+__global__ void __launch_bounds__(64) k_find_signed_distance(DevMesh m, DevState st, int split) {
+    const int nMixed = *st.nMixed;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nMixed; i += gridDim.x * blockDim.x) {
+        const int c = st.mixed[i];
+        Poly P;
+        FaceCut fc[Poly::maxFaces];
+        FsdOut o;
+        bool ok = build_cell_poly(m, c, split != 0, P);
+        if (!ok) {
+            atomicOr(st.errFlag, ERR_CELL_TOO_LARGE);
+            st.status[i] = -100;
+            continue;
+        }
+        find_signed_distance(P, ld3(m.C, c), m.V[c], st.alpha[c], ld3(st.ifN, i), fc, o, ok);
+        if (!ok) atomicOr(st.errFlag, ERR_TOO_MANY_CROSSINGS);
+        st.status[i] = o.status;
+        st.nCutPts[i] = o.nCutPts;
+        st.faceClass[i] = o.faceClass;
+        // interfaceD/C/S stay zero where the reference returns before writing
+        st.ifD[i] = o.wrote ? o.d : 0.0;
+        st3(st.ifC, i, o.wrote ? o.C : VZERO);
+        st3(st.ifS, i, o.wrote ? o.S : VZERO);
+    }
+}
+
+void launch_find_signed_distance(const DevMesh& m, const DevState& st, const Params& p, hipStream_t s) {
+    if (p.splitWarpedFace) k_find_signed_distance<CellPolySP><<<1024, 64, 0, s>>>(m, st, 1);
+    else k_find_signed_distance<CellPolyNS><<<1024, 64, 0, s>>>(m, st, 0);
+}
+
+// ---------------------------------------------------------------------------
+// Advection
+// ---------------------------------------------------------------------------
+
+// dVf = upwind(phi).flux(alpha)*dt (advectionTemplates.C:371): face value
+// lambda*(aP - aN) + aN with lambda = pos0(phi); zeroGradient boundary value
+// is the owner value; empty patches carry nothing.
+__global__ void k_upwind(DevMesh m, DevState st, double dt) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= m.nF) return;
+    const double ph = st.phi[f];
+    double v;
+    if (f < m.nIF) {
+        const double aP = st.alphaOld[m.own[f]], aN = st.alphaOld[m.nbr[f]];
+        const double af = pos0(ph) * (aP - aN) + aN;
+        v = (ph * af) * dt;
+    } else if (m.fEmpty[f]) {
+        v = 0.0;
+    } else {
+        v = (ph * st.alphaOld[m.own[f]]) * dt;
+    }
+    st.dVf[f] = v;
+}
+
+// volPointInterpolation value at a point
+__device__ inline V3 point_u(const DevMesh& m, const DevState& st, int p) {
+    V3 t = VZERO;
+    for (int k = m.psOff[p]; k < m.psOff[p + 1]; ++k) {
+        const int s = m.psSrc[k];
+        const V3 u = (s >= 0) ? ld3(st.U, s) : ld3(st.Ub, (~s) - m.nIF);
+        t = t + m.psW[k] * u;
+    }
+    return t;
+}
+
+// interpolationCellPoint<vector>::interpolate(position, cell) (advection.C:91,126):
+// barycentric weights in the tets (cell centre, face point 0, fan triangle).
+__device__ inline V3 interpolate_u(const DevMesh& m, const DevState& st, V3 pos, int c) {
+    const double tol = SVP_SMALL;
+    const V3 cc = ld3(m.C, c);
+    double bestMin = -1e300;
+    double bw[4] = {0.0, 0.0, 0.0, 0.0};
+    int bp[3] = {0, 0, 0};
+    for (int k = m.cOff[c]; k < m.cOff[c + 1]; ++k) {
+        const int f = m.cFaces[k];
+        const int fs = m.fOff[f], n = m.fOff[f + 1] - fs;
+        for (int t = 1; t < n - 1; ++t) {
+            int p0 = m.fPts[fs], p1 = m.fPts[fs + t], p2 = m.fPts[fs + t + 1];
+            if (m.own[f] != c) {
+                const int tmp = p1;
+                p1 = p2;
+                p2 = tmp;
+            }
+            const V3 b = ld3(m.pts, p0), cq = ld3(m.pts, p1), d = ld3(m.pts, p2);
+            const V3 e0 = cc - d, e1 = b - d, e2 = cq - d;
+            const double detT = dot(e0, cross(e1, e2));
+            double w[4];
+            if (fabs(detT) < SVP_SMALL) {
+                w[0] = w[1] = w[2] = w[3] = -1.0;
+            } else {
+                const V3 r = pos - d;
+                w[0] = dot(r, cross(e1, e2)) / detT;
+                w[1] = dot(e0, cross(r, e2)) / detT;
+                w[2] = dot(e0, cross(e1, r)) / detT;
+                w[3] = 1.0 - w[0] - w[1] - w[2];
+            }
+            const double mn = fmin(fmin(w[0], w[1]), fmin(w[2], w[3]));
+            if (mn + tol > 0.0) {
+                return w[0] * ld3(st.U, c) + w[1] * point_u(m, st, p0) + w[2] * point_u(m, st, p1) +
+                       w[3] * point_u(m, st, p2);
+            }
+            if (mn > bestMin) {
+                bestMin = mn;
+                bw[0] = w[0]; bw[1] = w[1]; bw[2] = w[2]; bw[3] = w[3];
+                bp[0] = p0; bp[1] = p1; bp[2] = p2;
+            }
+        }
+    }
+    return bw[0] * ld3(st.U, c) + bw[1] * point_u(m, st, bp[0]) + bw[2] * point_u(m, st, bp[1]) +
+           bw[3] * point_u(m, st, bp[2]);
+}
+
+// cutFace::timeIntegratedArea (cutFace.C:392-506)
+__device__ inline double time_integrated_area(const V3* fPts, int n, V3 normal, double distance,
+                                              const double* pTimesIn, double Un0, double dt,
+                                              double magSf, bool& ok) {
+    double pt[MAX_FACE_POINTS];
+    for (int k = 0; k < n; ++k) pt[k] = pTimesIn[k];
+    for (int k = 1; k < n; ++k) {
+        const double v = pt[k];
+        int j = k - 1;
+        while (j >= 0 && pt[j] > v) {
+            pt[j + 1] = pt[j];
+            --j;
+        }
+        pt[j + 1] = v;
+    }
+    const double firstTime = pt[0], lastTime = pt[n - 1];
+    if (lastTime <= 0.0) return magSf * dt * pos0(Un0);
+    if (firstTime >= dt) return magSf * dt * (1.0 - pos0(Un0));
+    double tIntArea = 0.0;
+    double sortedTimes[MAX_FACE_POINTS + 2];
+    int nSorted = 0;
+    double prevTime;
+    double subAreaOld;
+    FaceCut fc;
+    if (firstTime > 0.0) {
+        subAreaOld = magSf * (1.0 - pos0(Un0));
+        tIntArea = subAreaOld * firstTime;
+        sortedTimes[nSorted++] = firstTime;
+        prevTime = firstTime;
+    } else {
+        sortedTimes[nSorted++] = 0.0;
+        prevTime = 0.0;
+        ok &= calcSubFace(fPts, n, normal, distance, fc);
+        subAreaOld = mag(fc.area);
+    }
+    const double smallTime = fmax(SVP_TSMALL / fabs(Un0), SVP_TSMALL);
+    for (int ti = 0; ti < n; ++ti) {
+        const double timeI = pt[ti];
+        if (timeI > (prevTime + smallTime) && timeI < dt) {
+            sortedTimes[nSorted++] = timeI;
+            prevTime = timeI;
+        }
+    }
+    if (lastTime > dt) sortedTimes[nSorted++] = dt;
+    else tIntArea += magSf * (dt - lastTime) * pos0(Un0);
+    for (int k = 0; k < nSorted - 1; k++) {
+        const double tauOld = sortedTimes[k], tauNew = sortedTimes[k + 1];
+        const double deltaTau = 0.5 * (tauNew - tauOld);
+        ok &= calcSubFace(fPts, n, normal, distance - tauNew * Un0, fc);
+        const double subAreaNew = mag(fc.area);
+        ok &= calcSubFace(fPts, n, normal, distance - (tauOld + deltaTau) * Un0, fc);
+        const double subAreaMid = mag(fc.area);
+        tIntArea += (deltaTau / 3.0) * (subAreaOld + 4.0 * subAreaMid + subAreaNew);
+        subAreaOld = subAreaNew;
+    }
+    return tIntArea;
+}
+
+// cutFace::timeIntegratedFaceFlux (cutFace.C:262-389)
+__device__ inline double time_integrated_face_flux(const DevMesh& m, int f, V3 normal, double distance,
+                                                   double Un0, double dt, double phi, double magSf,
+                                                   bool& ok) {
+    if (fabs(phi) <= SVP_TSMALL) return 0.0;
+    const int fs = m.fOff[f], n = m.fOff[f + 1] - fs;
+    V3 p[MAX_FACE_POINTS];
+    for (int k = 0; k < n; ++k) p[k] = ld3(m.pts, m.fPts[fs + k]);
+    const bool flat = m.flat[f] > (1.0 - SVP_TSMALL);
+    if (fabs(Un0 * dt) > SVP_TSMALL) {
+        double pTimes[MAX_FACE_POINTS];
+        for (int i = 0; i < n; ++i) {
+            const double t = (dot(p[i], normal) + distance) / Un0;
+            pTimes[i] = fabs(t) < SVP_TSMALL ? 0.0 : t;
+        }
+        if (flat) {
+            return phi / magSf * time_integrated_area(p, n, normal, distance, pTimes, Un0, dt, magSf, ok);
+        }
+        double dVf = 0.0;
+        V3 tri[3];
+        double tt[3];
+        tri[0] = ld3(m.Cf, f);
+        const double t0 = (dot(tri[0], normal) + distance) / Un0;
+        tt[0] = fabs(t0) < SVP_TSMALL ? 0.0 : t0;
+        for (int pi = 0; pi < n; ++pi) {
+            tri[1] = p[pi];
+            tt[1] = pTimes[pi];
+            tri[2] = p[(pi + 1) % n];
+            tt[2] = pTimes[(pi + 1) % n];
+            const double magSfTri = mag(triAreaNormal(tri[0], tri[1], tri[2]));
+            const double phiTri = phi * magSfTri / magSf;
+            dVf += phiTri / magSfTri *
+                   time_integrated_area(tri, 3, normal, distance, tt, Un0, dt, magSfTri, ok);
+        }
+        return dVf;
+    }
+    FaceCut fc;
+    if (flat) {
+        ok &= calcSubFace(p, n, normal, distance, fc);
+        const double alphaf = mag(fc.area) / magSf;
+        return (phi * dt * alphaf);
+    }
+    V3 tri[3];
+    tri[0] = ld3(m.Cf, f);
+    double dVf = 0.0;
+    for (int pi = 0; pi < n; ++pi) {
+        tri[1] = p[pi];
+        tri[2] = p[(pi + 1) % n];
+        const double magSfTri = mag(triAreaNormal(tri[0], tri[1], tri[2]));
+        const double phiTri = phi * magSfTri / magSf;
+        ok &= calcSubFace(tri, 3, normal, distance, fc);
+        const double alphafTri = mag(fc.area) / magSfTri;
+        dVf += (phiTri * dt * alphafTri);
+    }
+    return dVf;
+}
+
+// advection::timeIntegratedFlux (advection.C:85-221): each downwind face of a
+// cut cell gets its time-integrated volume; a face is downwind of exactly one
+// cell, so the writes never collide.
+__global__ void __launch_bounds__(64) k_interface_flux(DevMesh m, DevState st, double dt) {
+    const int nMixed = *st.nMixed;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nMixed; i += gridDim.x * blockDim.x) {
+        if (st.status[i] != 0) {
+            st.un0[i] = 0.0;
+            continue;
+        }
+        const int c = st.mixed[i];
+        const V3 n = ld3(st.ifN, i);
+        const double d = st.ifD[i];
+        const double Un0 = dot(interpolate_u(m, st, ld3(st.ifC, i), c), n);
+        st.un0[i] = Un0;
+        bool ok = true;
+        for (int k = m.cOff[c]; k < m.cOff[c + 1]; ++k) {
+            const int f = m.cFaces[k];
+            const double ph = st.phi[f];
+            bool down;
+            if (f < m.nIF) down = (m.own[f] == c) ? (ph >= 0.0) : (ph < 0.0);
+            else down = !m.fEmpty[f] && (ph >= 0.0);
+            if (down) st.dVf[f] = time_integrated_face_flux(m, f, n, d, Un0, dt, ph, m.magSf[f], ok);
+        }
+        if (!ok) atomicOr(st.errFlag, ERR_TOO_MANY_CROSSINGS);
+    }
+}
+
+__device__ inline bool loop_oob(double a, double aTol) { return (a - 1.0 > aTol) || (a < -aTol); }
+__device__ inline bool cell_oob(double a, double aTol) { return (a < -aTol) || (a > 1.0 + aTol); }
+
+// needBounding = mixed cells and their face neighbours (advection.C:54-82)
+__device__ inline bool need_bounding(const DevMesh& m, const unsigned char* mixedFlag, int c) {
+    if (mixedFlag[c]) return true;
+    for (int k = m.cOff[c]; k < m.cOff[c + 1]; ++k) {
+        const int f = m.cFaces[k];
+        if (f < m.nIF) {
+            const int o = (m.own[f] == c) ? m.nbr[f] : m.own[f];
+            if (mixedFlag[o]) return true;
+        }
+    }
+    return false;
+}
+
+// Conservative update (advectionTemplates.C:399-404) with fvc::surfaceIntegrate
+// summing the cell's faces in ascending face order.  Also counts cells outside
+// [-aTol, 1+aTol] for the bounding loop and flags bounding candidates.
+__global__ void __launch_bounds__(DETECT_BLOCK) k_update(DevMesh m, DevState st, double dt,
+                                                         const double* Sp, const double* Su,
+                                                         unsigned char* candFlag) {
+    __shared__ int lds[DETECT_BLOCK / 64];
+    const double aTol = 100.0 * SVP_SMALL;
+    const double rDeltaT = 1.0 / dt;
+    const int c = blockIdx.x * DETECT_BLOCK + threadIdx.x;
+    bool cand = false;
+    if (c < m.nC) {
+        double sI = 0.0;
+        for (int k = m.cOff[c]; k < m.cOff[c + 1]; ++k) {
+            const int e = m.cFacesAsc[k];
+            if (e >= 0) {
+                if (e >= m.nIF && m.fEmpty[e]) continue;
+                sI = sI + st.dVf[e];
+            } else {
+                sI = sI - st.dVf[~e];
+            }
+        }
+        sI = sI / m.V[c];
+        double num = st.alphaOld[c] * rDeltaT;
+        if (Su) num = num + Su[c];
+        num = num - sI * rDeltaT;
+        const double den = Sp ? (rDeltaT - Sp[c]) : rDeltaT;
+        const double a = num / den;
+        st.alpha[c] = a;
+        if (loop_oob(a, aTol)) atomicAdd(st.oobCount, 1);
+        cand = cell_oob(a, aTol) && need_bounding(m, st.mixedFlag, c);
+        candFlag[c] = cand ? 1 : 0;
+    }
+    int total;
+    block_prefix(cand, lds, total);
+    if (threadIdx.x == 0) st.blockCount[blockIdx.x] = total;
+}
+
+__device__ inline double face_value(const DevMesh& m, const double* f, int facei) {
+    if (facei < m.nIF) return f[facei];
+    if (m.fEmpty[facei]) return 0.0;
+    return f[facei];
+}
+
+__device__ inline double net_flux(const DevMesh& m, const double* f, int c) {
+    double dV = 0.0;
+    for (int k = m.cOff[c]; k < m.cOff[c + 1]; ++k) {
+        const int facei = m.cFaces[k];
+        const double v = face_value(m, f, facei);
+        if (m.own[facei] == c) dV += v;
+        else dV -= v;
+    }
+    return dV;
+}
+
+// limitFlux + boundFlux (advectionTemplates.C:118-349).  One lane walks the
+// ascending candidate list exactly as the reference walks its cells, since a
+// cell reads the corrections written by earlier cells of the same pass.
+__global__ void k_bound(DevMesh m, DevState st, Params p, double dt, const double* Sp, const double* Su) {
+    if (threadIdx.x != 0) return;
+    const double aTol = 100.0 * SVP_SMALL;
+    const double rDeltaT = 1.0 / dt;
+    int nCand = *st.candCount;
+    int* cand = st.cand;
+    int oob = *st.oobCount;
+    // work layout: correctedFaces | touched faces
+    const int half = st.workCap / 2;
+    int* corrected = st.work;
+    int* touched = st.work + half;
+    for (int pass = 0; pass < p.nAlphaBounds; ++pass) {
+        if (oob == 0) break;
+        int nCorrected = 0, nTouched = 0;
+        for (int ci = 0; ci < nCand; ++ci) {
+            const int celli = cand[ci];
+            const double ac = st.alpha[celli];
+            if (!cell_oob(ac, aTol)) continue;
+            const double Vi = m.V[celli];
+            double alphaOvershoot = pos0(ac - 1.0) * (ac - 1.0) + neg0(ac) * ac;
+            double fluidToPassOn = alphaOvershoot * Vi;
+            int nFaces = 1;
+            bool firstLoop = true;
+            for (int iter = 0; iter < 10; iter++) {
+                if (fabs(alphaOvershoot) < aTol || nFaces == 0) break;
+                int pf[MAX_CELL_FACES];
+                double pphi[MAX_CELL_FACES], pmax[MAX_CELL_FACES];
+                int np = 0;
+                double dVftot = 0.0;
+                nFaces = 0;
+                for (int k = m.cOff[celli]; k < m.cOff[celli + 1]; ++k) {
+                    const int facei = m.cFaces[k];
+                    const double ph = face_value(m, st.phi, facei);
+                    const bool down = (m.own[facei] == celli) ? (ph >= 0) : (ph < 0);
+                    if (!down) continue;
+                    const double dVff = face_value(m, st.dVf, facei) + face_value(m, st.corr, facei);
+                    const double maxExtra = fabs(pos0(fluidToPassOn) * ph * dt - dVff);
+                    if (maxExtra / Vi > aTol && np < MAX_CELL_FACES) {
+                        pf[np] = facei;
+                        pphi[np] = ph;
+                        pmax[np] = maxExtra;
+                        np++;
+                        dVftot += fabs(ph * dt);
+                    }
+                }
+                for (int fi = 0; fi < np; ++fi) {
+                    const int facei = pf[fi];
+                    double fl = fabs(fluidToPassOn) * fabs(pphi[fi] * dt) / dVftot;
+                    nFaces += (int)pos0(pmax[fi] - fl);
+                    fl = fmin(fl, pmax[fi]);
+                    double dVff = face_value(m, st.corr, facei);
+                    dVff += sgn(pphi[fi]) * sgn(fluidToPassOn) * fl;
+                    if (facei < m.nIF || !m.fEmpty[facei]) {
+                        st.corr[facei] = dVff;
+                        if (nTouched < half) touched[nTouched++] = facei;
+                        else atomicOr(st.errFlag, ERR_WORK_OVERFLOW);
+                    }
+                    if (firstLoop) {
+                        if (nCorrected < half) corrected[nCorrected++] = facei;
+                        else atomicOr(st.errFlag, ERR_WORK_OVERFLOW);
+                    }
+                }
+                firstLoop = false;
+                double num = st.alphaOld[celli] * rDeltaT;
+                if (Su) num = num + Su[celli];
+                num = num - net_flux(m, st.dVf, celli) / Vi * rDeltaT - net_flux(m, st.corr, celli) / Vi * rDeltaT;
+                const double den = Sp ? (rDeltaT - Sp[celli]) : rDeltaT;
+                const double alpha1New = num / den;
+                alphaOvershoot = pos0(alpha1New - 1.0) * (alpha1New - 1.0) + neg0(alpha1New) * alpha1New;
+                fluidToPassOn = alphaOvershoot * Vi;
+            }
+        }
+        // apply corrections in correctedFaces order, first occurrence only
+        for (int q = 0; q < nCorrected; ++q) {
+            const int facei = corrected[q];
+            bool dup = false;
+            for (int r = 0; r < q; ++r) dup |= (corrected[r] == facei);
+            if (dup) continue;
+            const double cv = face_value(m, st.corr, facei);
+            const int o = m.own[facei];
+            {
+                const double before = st.alpha[o];
+                const double after = before - cv / m.V[o];
+                st.alpha[o] = after;
+                oob += (int)loop_oob(after, aTol) - (int)loop_oob(before, aTol);
+            }
+            if (facei < m.nIF) {
+                const int nb = m.nbr[facei];
+                const double before = st.alpha[nb];
+                const double after = before + cv / m.V[nb];
+                st.alpha[nb] = after;
+                oob += (int)loop_oob(after, aTol) - (int)loop_oob(before, aTol);
+            }
+            const double corrVf = face_value(m, st.dVf, facei) + cv;
+            if (facei < m.nIF || !m.fEmpty[facei]) st.dVf[facei] = corrVf;
+        }
+        // candidates of the next pass: keep failing cells, add touched cells
+        int nNew = 0;
+        for (int ci = 0; ci < nCand; ++ci)
+            if (cell_oob(st.alpha[cand[ci]], aTol)) cand[nNew++] = cand[ci];
+        nCand = nNew;
+        for (int q = 0; q < nCorrected; ++q) {
+            const int facei = corrected[q];
+            for (int side = 0; side < 2; ++side) {
+                int cc = side == 0 ? m.own[facei] : (facei < m.nIF ? m.nbr[facei] : -1);
+                if (cc < 0) continue;
+                if (!cell_oob(st.alpha[cc], aTol) || !need_bounding(m, st.mixedFlag, cc)) continue;
+                int pos = 0;
+                while (pos < nCand && cand[pos] < cc) ++pos;
+                if (pos < nCand && cand[pos] == cc) continue;
+                if (nCand >= st.candCap) { atomicOr(st.errFlag, ERR_CAND_OVERFLOW); continue; }
+                for (int r = nCand; r > pos; --r) cand[r] = cand[r - 1];
+                cand[pos] = cc;
+                nCand++;
+            }
+        }
+        // dVfCorrectionValues = 0 for the next pass
+        for (int q = 0; q < nTouched; ++q) st.corr[touched[q]] = 0.0;
+    }
+    *st.candCount = nCand;
+    *st.oobCount = oob;
+}
+
+// applyBruteForceBounding (advection.C:291-308)
+__global__ void k_snap_clip(int nC, double* alpha, double snapTol, int clip) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nC) return;
+    double a = alpha[c];
+    if (snapTol > 0.0) a = a * pos0(a - snapTol) * neg0(a - (1.0 - snapTol)) + pos0(a - (1.0 - snapTol));
+    if (clip) a = fmin(1.0, fmax(0.0, a));
+    alpha[c] = a;
+}
+
+__global__ void k_zero_int(int* p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+void launch_advect(const DevMesh& m, const DevState& st, const Params& p, double dt, const double* Sp,
+                   const double* Su, hipStream_t s) {
+    const int nb = detect_blocks(m.nC);
+    k_zero_int<<<1, 64, 0, s>>>(st.oobCount, 1);
+    k_upwind<<<cdiv(m.nF, 256), 256, 0, s>>>(m, st, dt);
+    k_interface_flux<<<1024, 64, 0, s>>>(m, st, dt);
+    k_update<<<nb, DETECT_BLOCK, 0, s>>>(m, st, dt, Sp, Su, st.candFlag);
+    k_scan_counts<<<1, 1024, 0, s>>>(st.blockCount, nb, st.candCount, nullptr);
+    k_compact_flags<<<nb, DETECT_BLOCK, 0, s>>>(m.nC, st.candFlag, st.blockCount, st.cand);
+    k_bound<<<1, 64, 0, s>>>(m, st, p, dt, Sp, Su);
+    if (p.snapTol > 0.0 || p.clip) k_snap_clip<<<cdiv(m.nC, 256), 256, 0, s>>>(m.nC, st.alpha, p.snapTol, p.clip);
+}
+
+__global__ void k_alpha_phi(int nF, const double* dVf, double* alphaPhi, double dt) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f < nF) alphaPhi[f] = dVf[f] / dt;
+}
+
+void launch_alpha_phi(const DevMesh& m, const DevState& st, double dt, hipStream_t s) {
+    k_alpha_phi<<<cdiv(m.nF, 256), 256, 0, s>>>(m.nF, st.dVf, st.alphaPhi, dt);
+}
+
+}  // namespace svp

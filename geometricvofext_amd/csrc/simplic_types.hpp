@@ -1,0 +1,105 @@
+// Device-resident data of one SimPLIC alpha-transport instance and the
+// launchers of its kernels.  Shared by the kernels and the host facade.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace svp {
+
+// Static mesh in HBM.  Index arrays are int32, values FP64, 3-vectors AoS.
+struct DevMesh {
+    int nP, nF, nIF, nC;
+    const double* pts;      // 3*nP
+    const int* fOff;        // nF+1, face -> point CSR
+    const int* fPts;
+    const int* own;         // nF
+    const int* nbr;         // nF, -1 on boundary faces
+    const unsigned char* fEmpty;  // nF, 1 for faces of empty patches
+    // cell -> face CSR in primitiveMesh::cells() order (owned faces ascending,
+    // then neighbour faces ascending)
+    const int* cOff;
+    const int* cFaces;
+    // same cells, faces in ascending face order (fvc::surfaceIntegrate order),
+    // neighbour faces stored as ~f
+    const int* cFacesAsc;
+    // cell-point-cell stencil: the cell first, then the other cells ascending
+    const int* stOff;
+    const int* st;
+    // volPointInterpolation sources per point: cell c >= 0 or boundary face ~f
+    const int* psOff;
+    const int* psSrc;
+    // geometry (computed on the device at create time)
+    double* Cf;      // 3*nF
+    double* Sf;      // 3*nF
+    double* magSf;   // nF
+    double* flat;    // nF
+    double* C;       // 3*nC
+    double* V;       // nC
+    double* psW;     // point weights, psOff CSR
+};
+
+// Per-step state.  Interface data are compact arrays indexed by the position
+// of a cell in the ascending mixed-cell list.
+struct DevState {
+    double* alpha;      // nC, current alpha
+    double* alphaOld;   // nC
+    double* phi;        // nF
+    double* U;          // 3*nC
+    double* Ub;         // 3*(nF-nIF), boundary values of U
+    double* dVf;        // nF
+    double* alphaPhi;   // nF
+    double* corr;       // nF, bounding corrections (kept zero between passes)
+    unsigned char* mixedFlag;   // nC
+    unsigned char* candFlag;    // nC, bounding candidates after the update
+    int* blockCount;    // per detection block
+    int* nMixed;        // [1]
+    int* mixed;         // capacity nC
+    int* status;
+    int* nCutPts;
+    unsigned long long* faceClass;
+    double* ifN;        // 3*cap
+    double* ifD;
+    double* ifC;        // 3*cap
+    double* ifS;        // 3*cap
+    double* un0;
+    // bounding bookkeeping
+    int* oobCount;      // [1] cells with alpha-1 > aTol or alpha < -aTol
+    int* candCount;     // [1]
+    int* cand;          // capacity candCap
+    int candCap;
+    int* work;          // scratch ints for the bounding kernel (workCap)
+    int workCap;
+    double* minmax;     // [4]: min, max before bounding, min, max after
+    int* errFlag;       // [1] device error bits
+    unsigned long long* nIcTotal;  // [1] running sum of mixed-cell counts
+};
+
+struct Params {
+    double mixedCellTol;
+    int nAlphaBounds;
+    double snapTol;
+    int clip;
+    int splitWarpedFace;
+};
+
+// error bits written to DevState::errFlag
+enum : int {
+    ERR_CELL_TOO_LARGE = 1,
+    ERR_TOO_MANY_CROSSINGS = 2,
+    ERR_CAND_OVERFLOW = 4,
+    ERR_WORK_OVERFLOW = 8,
+};
+
+constexpr int DETECT_BLOCK = 256;
+
+// launchers (simplic_kernels.hip)
+void launch_geometry(const DevMesh& m, hipStream_t s);
+void launch_detect(const DevMesh& m, const DevState& st, const Params& p, hipStream_t s);
+void launch_ls_normal(const DevMesh& m, const DevState& st, hipStream_t s);
+void launch_find_signed_distance(const DevMesh& m, const DevState& st, const Params& p, hipStream_t s);
+void launch_advect(const DevMesh& m, const DevState& st, const Params& p, double dt, const double* Sp,
+                   const double* Su, hipStream_t s);
+void launch_alpha_phi(const DevMesh& m, const DevState& st, double dt, hipStream_t s);
+int detect_blocks(int nC);
+
+}  // namespace svp

@@ -1,0 +1,152 @@
+/*
+ * C-ABI of the MI355X SimPLIC alpha-transport library (libsimplic_amd.so).
+ *
+ * One handle = one alpha field on one mesh = one reference `solveVofEqu`
+ * object (src/SimPLIC/solveVofEqu/solveVofEqu.H:67-216): a reconstruction
+ * (src/SimPLIC/reconstruction/reconstruction.C) and an advection
+ * (src/SimPLIC/advection/advection.C) sharing the fvSolution "alpha.*" dict.
+ * Fields live in GPU memory; the caller hands over host or device pointers
+ * (`where` = SVP_HOST or SVP_DEVICE).  Every entry point returns 0 on success
+ * or a negative SVP_E_* code; svp_last_error() gives the message.  The
+ * reference aborts the process (FatalError) where this ABI returns an error.
+ *
+ * Threading: a handle is not reentrant (the reference objects hold mutable
+ * scratch state as well); distinct handles may be used from distinct threads.
+ * All work of a handle is ordered on its HIP stream.
+ */
+#ifndef SIMPLIC_AMD_H
+#define SIMPLIC_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SVP_HOST 0
+#define SVP_DEVICE 1
+
+#define SVP_OK 0
+#define SVP_E_ARG (-1)          /* invalid argument or size */
+#define SVP_E_DICT (-2)         /* invalid dictionary entry (reference: FatalError) */
+#define SVP_E_UNSUPPORTED (-3)  /* valid reference option not provided by this library */
+#define SVP_E_HIP (-4)          /* HIP runtime failure */
+#define SVP_E_MESH (-5)         /* mesh exceeds the device cut limits */
+#define SVP_E_DEVICE (-6)       /* a kernel reported an error (see svp_last_error) */
+#define SVP_E_NAN (-7)          /* non-finite input */
+
+#define SVP_PATCH_WALL 0
+#define SVP_PATCH_EMPTY 1
+
+typedef struct svp_handle svp_handle;
+
+/* Raw OpenFOAM polyMesh: owner-outward face point lists, internal faces in
+ * upper-triangular order followed by boundary faces patch by patch. */
+typedef struct {
+    int32_t n_points, n_faces, n_internal_faces, n_cells, n_patches;
+    const double* points;          /* 3*n_points */
+    const int32_t* face_offsets;   /* n_faces+1 */
+    const int32_t* face_points;    /* face_offsets[n_faces] */
+    const int32_t* owner;          /* n_faces */
+    const int32_t* neighbour;      /* n_internal_faces */
+    const int32_t* patch_start;    /* n_patches */
+    const int32_t* patch_size;     /* n_patches */
+    const int32_t* patch_type;     /* n_patches: SVP_PATCH_* */
+} svp_mesh;
+
+/* The fvSolution solvers{"alpha.*"{...}} keys the reference reads
+ * (reconstruction.C:502-516, advection.C:455-457). */
+typedef struct {
+    char orientation_method[32];   /* "isoAlphaGrad" (default) or "LS" */
+    double mixed_cell_tol;         /* 1e-8 */
+    int32_t n_alpha_bounds;        /* 10 */
+    double snap_tol;               /* 0 */
+    int32_t clip;                  /* 1 */
+    int32_t split_warped_face;     /* 0 */
+    int32_t write_plic_fields;     /* 0 */
+} svp_dict;
+
+/* Defaults of every key (reconstruction.C:502-516, advection.C:455-457). */
+void svp_dict_defaults(svp_dict* d);
+
+/* Parse the body of an fvSolution "alpha.*" dictionary ("key value;" lines,
+ * C/C++ comments allowed).  Unknown keys are ignored, as OpenFOAM does.
+ * `reconstructionScheme` is accepted as an alias of `orientationMethod`.
+ * Returns SVP_E_DICT for an unknown orientation method
+ * (reconstruction.C:610-625) and SVP_E_UNSUPPORTED for alphaGrad / isoRDF / RDF. */
+int svp_dict_parse(const char* text, svp_dict* out);
+
+/* solveVofEqu constructor (solveVofEqu.C:55-91): uploads the mesh, computes
+ * its geometry and face flatness on the GPU (updateFaceFlatness,
+ * reconstruction.C:408-473).  `device` selects the HIP device. */
+int svp_create(const svp_mesh* mesh, const svp_dict* dict, int device, svp_handle** out);
+int svp_destroy(svp_handle* h);
+
+/* Fields.  alpha: n_cells.  phi: n_faces (internal then boundary faces).
+ * U: 3*n_cells cell values.  U_boundary: 3*(n_faces-n_internal_faces) boundary
+ * face values of U (NULL = zero), used by the cell-point interpolation. */
+int svp_set_alpha(svp_handle* h, const double* alpha, int where);
+int svp_set_flow(svp_handle* h, const double* phi, const double* U, const double* U_boundary,
+                 int where);
+
+/* solveVofEqu::reconstruct (solveVofEqu.C:96-99 -> reconstruction.C:680-722). */
+int svp_reconstruct(svp_handle* h);
+
+/* solveVofEqu::advect<Sp,Su> (solveVofEquTemplates.C:35-43 ->
+ * advectionTemplates.C:352-418) over time step dt.  Sp/Su: n_cells implicit /
+ * explicit sources or NULL for zeroField. */
+int svp_advect(svp_handle* h, double dt, const double* Sp, const double* Su, int where);
+
+/* Outputs. */
+int svp_get_alpha(svp_handle* h, double* alpha, int where);
+/* alphaPhi = dVf/dt of the last advect (advectionTemplates.C:417). */
+int svp_get_alpha_phi(svp_handle* h, double* alpha_phi, int where);
+/* getRhoPhi = (rho1 - rho2) alphaPhi + rho2 phi (advection.H:329-343). */
+int svp_get_rho_phi(svp_handle* h, double rho1, double rho2, double* rho_phi, int where);
+/* Number of mixed cells of the last reconstruct (reconstruction.C:675-676). */
+int svp_n_mixed(svp_handle* h, int32_t* n_mixed);
+/* Interface data of the last reconstruct per mixed cell (ascending cells):
+ * cell label, status (-1/0/1), normal n (3), distance d, centre C (3), area
+ * vector S (3).  Any output pointer may be NULL. */
+int svp_get_interface(svp_handle* h, int32_t* cells, int32_t* status, double* n, double* d,
+                      double* C, double* S, int where);
+/* Face-cut classification of the final plane cut per mixed cell: 2 bits per
+ * cell face ((status+1) << 2*k, cells() face order) and the number of plane
+ * crossings (cut vertices) on its cut faces. */
+int svp_get_cut_class(svp_handle* h, uint64_t* face_class, int32_t* n_cut_points, int where);
+/* Interface normal velocity Un0 of each mixed cell of the last advect. */
+int svp_get_un0(svp_handle* h, double* un0, int where);
+/* Mesh geometry computed at create time: face centres/areas (3*n_faces),
+ * |Sf|, flatness (n_faces), cell centres (3*n_cells), volumes (n_cells). */
+int svp_get_geometry(svp_handle* h, double* Cf, double* Sf, double* magSf, double* flatness,
+                     double* C, double* V);
+/* Cumulative GPU seconds spent in reconstruct and advect (the reference's
+ * reconstructionTime/advectionTime, solveVofEqu.H:195-211) and the running sum
+ * of mixed-cell counts over all reconstruct calls. */
+int svp_timers(svp_handle* h, double* reconstruction_s, double* advection_s, uint64_t* n_ic_total);
+
+/* Device pointers of the resident fields, for zero-copy drivers. */
+typedef struct {
+    double* alpha;
+    double* phi;
+    double* U;
+    double* U_boundary;
+    double* dVf;
+    const double* Cf;
+    const double* Sf;
+    const double* C;
+    const double* V;
+    void* stream;   /* hipStream_t */
+} svp_device_view;
+int svp_device_view_get(svp_handle* h, svp_device_view* out);
+
+/* Wait for all queued work of the handle. */
+int svp_synchronize(svp_handle* h);
+
+/* Message of the last failing call on this thread. */
+const char* svp_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SIMPLIC_AMD_H */

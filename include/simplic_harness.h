@@ -1,0 +1,42 @@
+/*
+ * Harness entry points of libsimplic_amd.so: the deformation test driver of
+ * the reference (applications/test/plicVofAdvectionFoam) on the GPU.  Not part
+ * of the solveVofEqu drop-in; they set the flow that a solver would provide.
+ */
+#ifndef SIMPLIC_HARNESS_H
+#define SIMPLIC_HARNESS_H
+
+#include "simplic_amd.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Unscaled deformation velocity at cell centres (3*n_cells) and face flux
+ * U(face centre) . Sf (n_faces) on the handle's geometry, evaluated on the
+ * host with the C library sin (updateU.H:66-112).  Host output arrays. */
+int svp_harness_deformation(svp_handle* h, double* U0, double* phi0);
+
+/* Solid-body rotation U = omega x (x - origin) at cell centres and faces
+ * (linear, so face-centre fluxes are exactly divergence free). */
+int svp_harness_rotation(svp_handle* h, const double* omega, const double* origin, double* U0,
+                         double* phi0);
+
+/* Upload unscaled fields once (host arrays); the handle keeps them on the GPU. */
+int svp_harness_upload_base_flow(svp_handle* h, const double* U0, const double* phi0,
+                                 const double* U0_boundary);
+
+/* phi = phi0*factor, U = U0*factor, U_boundary = U0_boundary*factor on the
+ * GPU (updateU.H:117-127). */
+int svp_harness_scale_flow(svp_handle* h, double factor);
+
+/* max over cells of sum|phi|/V, and of the same restricted to
+ * 0.01 <= alpha <= 0.99 (CourantNo.H, alphaCourantNo.H:95-103), from the
+ * current phi and alpha; multiply by 0.5*dt for the Courant numbers.
+ * Synchronises with the handle's stream. */
+int svp_harness_courant(svp_handle* h, double* max_sum_phi_by_v, double* max_alpha_sum_phi_by_v);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,36 @@
+"""Shared set-up of the parity and accuracy tests."""
+import numpy as np
+
+from geometricvofext_amd import harness
+
+
+def ramp_alpha(C, shapes, width):
+    """alpha = clip(0.5 - sdist/width) of a union of spheres (centre, radius):
+    a sharp field with a one-to-two cell mixed band, for parity runs."""
+    sd = np.full(C.shape[0], np.inf)
+    for c, r in shapes:
+        sd = np.minimum(sd, np.linalg.norm(C - np.asarray(c), axis=1) - r)
+    return np.clip(0.5 - sd / width, 0.0, 1.0)
+
+
+def deformation_steps(n_steps, co_fn, period=6.0, rt=None):
+    """Yield (t, dt, uFactor) of the reference harness loop; co_fn() returns
+    (max sum|phi|/V, alpha-restricted max) of the current flow."""
+    rt = rt or harness.RunTime()
+    for _ in range(n_steps):
+        co, aco = co_fn()
+        rt.set_delta_t(0.5 * co * rt.delta_t, 0.5 * aco * rt.delta_t)
+        rt.increment()
+        yield rt.value, rt.delta_t, harness.u_factor(rt.value, rt.delta_t, period)
+
+
+def bits_equal(a, b):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    return a.shape == b.shape and np.array_equal(a.view(np.int64), b.view(np.int64))
+
+
+def max_abs(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    return 0.0 if a.size == 0 else float(np.max(np.abs(a - b)))
