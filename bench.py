@@ -1,0 +1,279 @@
+#!/usr/bin/env python3
+"""Benchmark of the SimPLIC alpha transport on one MI355X.
+
+Metric: interface-cell updates/s = sum over steps of the mixed
+cell count N_ic / sum over steps of (t_reconstruct + t_advect), both times
+measured with HIP events on the library's stream.  A step is one
+reconstruct() + advect() of the deformation test (applications/test/
+plicVofAdvectionFoam) on the workload below; the unscaled velocity and flux
+are resident in HBM, and each step rescales them on the GPU by the harness's
+time factor before the timed hot path runs.
+
+Default workload: 128^3 hex mesh of the unit cube with interior vertices
+displaced by U(-0.2h, 0.2h) per axis (seed 1), sphere r=0.15 at
+(0.35, 0.35, 0.35) initialised by adaptive exact-cut volumes, deformation
+field with period 6 run to t = 3 at Courant 0.5, FP64, the reference test
+case's alpha dictionary (LS normals, nAlphaBounds 3, snapTol 0, clip off).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+  --steps 0 (default) times the full run to t = T.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "interface-cell advect updates/s; plain vs learned normal at equal L1 error"
+HBM_PEAK_GBS = 8000.0
+
+WORKLOADS = {
+    "hex128p": dict(kind="hex", N=128, perturb=0.2, seed=1, shapes=[((0.35, 0.35, 0.35), 0.15)],
+                    flow="deformation", end_time=3.0,
+                    desc="128^3 hex, interior vertices +-20% h (seed 1), sphere r=0.15 at 0.35, "
+                         "deformation T=3, Co 0.5"),
+    "hex64": dict(kind="hex", N=64, perturb=0.0, seed=0, shapes=[((0.35, 0.35, 0.35), 0.15)],
+                  flow="deformation", end_time=3.0,
+                  desc="64^3 uniform hex, sphere r=0.15 at 0.35, deformation T=3, Co 0.5"),
+}
+
+ALPHA_DICT = dict(orientationMethod="LS", nAlphaBounds=3, snapTol=0.0, clip=False, mixedCellTol=1e-8)
+
+
+def algorithmic_bytes(stage, n_cells, n_faces, n_ic, cell_kind):
+    """Algorithmic HBM bytes of one launch of a stage (FP64 values, int32 labels).
+
+    detect      : alpha read per cell, mixed label written per interface cell
+    normal      : per interface cell the 27 stencil alphas and centres, normal out
+    fsd         : per interface cell its points, face->point labels, alpha, n in; d, C, S out
+    upwind      : phi, owner, neighbour read and dVf written per face; alpha read per cell
+    iflux       : per interface cell points, face labels, point velocities, n, d, C, and
+                  phi in / dVf out on its downwind faces
+    update      : alpha_old, V read and alpha written per cell, dVf read per face
+    """
+    npts, nfc, fpts = (8, 6, 4) if cell_kind == "hex" else (4, 4, 3)
+    per_ic_fsd = npts * 24 + nfc * fpts * 4 + 8 + 24 + 8 + 24 + 24
+    per_ic_iflux = npts * 24 + nfc * fpts * 4 + npts * 24 + 24 + 8 + 24 + (nfc // 2) * 16
+    table = {
+        "detect": 8 * n_cells + 4 * n_ic,
+        "normal": n_ic * (27 * 32 + 24),
+        "findSignedDistance": n_ic * per_ic_fsd,
+        "upwind": 24 * n_faces + 8 * n_cells,
+        "interfaceFlux": n_ic * per_ic_iflux,
+        "update": 24 * n_cells + 8 * n_faces,
+    }
+    return table.get(stage)
+
+
+def step_bytes(n_cells, n_faces, n_ic, cell_kind):
+    """Whole-step algorithmic bytes 24 N_c + 24 N_f + B_ic N_ic (B_ic 512 hex / 272 tet)."""
+    return 24 * n_cells + 24 * n_faces + (512 if cell_kind == "hex" else 272) * n_ic
+
+
+def build_case(wl):
+    import geometricvofext_amd as g
+    if wl["kind"] == "hex":
+        mesh = g.hex_box(wl["N"], perturb=wl["perturb"], seed=wl["seed"])
+    else:
+        mesh = g.tet_box(wl["N"], perturb=wl["perturb"], seed=wl["seed"])
+    solver = g.SolveVofEqu(mesh, ALPHA_DICT)
+    solver.init_alpha(wl["shapes"], max_depth=5)
+    alpha0 = solver.alpha()
+    U0, phi0 = solver.deformation_field()
+    solver.upload_base_flow(U0, phi0)
+    return mesh, solver, alpha0, U0, phi0
+
+
+def time_sequence(solver, alpha0, end_time, max_steps=None):
+    """(t, dt, uFactor) of every step to end_time.  With maxCo == maxAlphaCo
+    the alpha-restricted Courant number never exceeds the full one, so the
+    sequence does not depend on alpha and is computed from the flow alone."""
+    from geometricvofext_amd import harness
+    rt = harness.RunTime(end_time=end_time)
+    assert rt.max_co == rt.max_alpha_co
+    solver.set_alpha(alpha0)
+    solver.scale_flow(0.0)
+    seq = []
+    while rt.run():
+        co, aco = solver.courant()
+        rt.set_delta_t(0.5 * co * rt.delta_t, 0.5 * aco * rt.delta_t)
+        rt.increment()
+        uf = harness.u_factor(rt.value, rt.delta_t, 6.0)
+        seq.append((rt.value, rt.delta_t, uf))
+        solver.scale_flow(uf)
+        if max_steps and len(seq) >= max_steps:
+            break
+    return seq
+
+
+def run_steps(solver, seq):
+    for (_, dt, uf) in seq:
+        solver.scale_flow(uf)
+        solver.reconstruct()
+        solver.advect(dt)
+
+
+def device_sync(solver):
+    solver.synchronize()
+    try:
+        import torch
+        if torch.cuda.is_available():
+            torch.cuda.synchronize()
+    except Exception:
+        pass
+
+
+def cpu_baseline(mesh, alpha0, seq, U0, phi0, n_steps, threads):
+    """The oracle (CPU restatement of the reference path) on the same mesh,
+    initial field and time steps, first n_steps steps."""
+    from oracle import oracle as O
+    om = O.OracleMesh(mesh)
+    s = O.OracleSolver(om, mixedCellTol=ALPHA_DICT["mixedCellTol"], nAlphaBounds=ALPHA_DICT["nAlphaBounds"],
+                       snapTol=ALPHA_DICT["snapTol"], clip=ALPHA_DICT["clip"], threads=threads)
+    s.set_alpha(alpha0)
+    n_ic = 0
+    for (_, dt, uf) in seq[:n_steps]:
+        s.set_flow(phi0 * uf, U0 * uf)
+        s.reconstruct()
+        n_ic += len(s.mixed()["cells"])
+        s.advect(dt)
+    rec, adv = s.timers()
+    return n_ic, rec + adv
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=0, help="timed steps (0 = run to t=T)")
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--workload", default="hex128p", choices=sorted(WORKLOADS))
+    ap.add_argument("--cpu-steps", type=int, default=8, help="steps of the CPU baseline sample")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--stage", default="auto", help="stage timed for the roofline (auto = slowest)")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build measures one GPU; the alpha transport shards by spatial domain "
+              "decomposition (see DESIGN.md) but no multi-GPU path is built, so --gpus must be 1.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import geometricvofext_amd as g
+    wl = WORKLOADS[args.workload]
+    t_setup = time.time()
+    mesh, solver, alpha0, U0, phi0 = build_case(wl)
+    seq = time_sequence(solver, alpha0, wl["end_time"])
+    n_total = len(seq)
+    K = args.steps if args.steps > 0 else n_total
+    K = min(K, n_total)
+    W = min(args.warmup, n_total)
+    setup_s = time.time() - t_setup
+    print(f"[bench] {args.workload}: {mesh.n_cells} cells, {mesh.n_faces} faces, {n_total} steps to "
+          f"t={wl['end_time']}, setup {setup_s:.1f}s", file=sys.stderr, flush=True)
+
+    # warm-up: W steps from t=0, then rewind the field
+    solver.set_alpha(alpha0)
+    run_steps(solver, seq[:W])
+    device_sync(solver)
+
+    # stage breakdown (untimed pass over the first steps) to pick the roofline stage
+    solver.set_alpha(alpha0)
+    solver.profile_set(0xFF)
+    run_steps(solver, seq[:min(60, K)])
+    prof_all = solver.profile_get(reset=True)
+    solver.profile_set(0)
+    stages = {k: v for k, v in prof_all.items() if v[1] > 0}
+    stage = args.stage
+    if stage == "auto":
+        stage = max((k for k in stages if algorithmic_bytes(k, 1, 1, 1, wl["kind"]) is not None),
+                    key=lambda k: stages[k][0])
+
+    # timed region
+    solver.set_alpha(alpha0)
+    stage_bit = 1 << g.SolveVofEqu.STAGES.index(stage)
+    solver.profile_set(stage_bit)
+    device_sync(solver)
+    rec0, adv0, nic0 = solver.timers()
+    t0 = time.perf_counter()
+    run_steps(solver, seq[:K])
+    device_sync(solver)
+    wall = time.perf_counter() - t0
+    rec1, adv1, nic1 = solver.timers()
+    prof = solver.profile_get(reset=True)
+    solver.profile_set(0)
+    solver.synchronize()
+
+    n_ic = nic1 - nic0
+    hot_s = (rec1 - rec0) + (adv1 - adv0)
+    value = n_ic / hot_s
+    ms_per_step = 1e3 * hot_s / K
+    n_ic_mean = n_ic / K
+    st_sec, st_cnt = prof[stage]
+    st_avg = st_sec / max(st_cnt, 1)
+    st_bytes = algorithmic_bytes(stage, mesh.n_cells, mesh.n_faces, n_ic_mean, wl["kind"])
+    achieved = st_bytes / st_avg / 1e9
+    step_gbs = step_bytes(mesh.n_cells, mesh.n_faces, n_ic_mean, wl["kind"]) / (hot_s / K) / 1e9
+
+    alpha_end = solver.alpha()
+    V = solver.geometry()["V"]
+    from geometricvofext_amd import harness
+    res = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "interface-cell updates/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": wl["desc"], "cells": mesh.n_cells, "faces": mesh.n_faces,
+                   "alpha_dict": ALPHA_DICT, "path": "plain LS"},
+        "roofline": {"bound": "hbm", "kernel": stage, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "algorithmic_bytes_per_launch": st_bytes, "avg_launch_us": 1e6 * st_avg},
+        "step_roofline": {"achieved": step_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                          "frac": step_gbs / HBM_PEAK_GBS},
+        "n_ic_mean": n_ic_mean,
+        "wall_s_timed_region": wall,
+        "stage_ms_per_step": {k: 1e3 * v[0] / max(v[1], 1) for k, v in stages.items()},
+        "t_end": seq[K - 1][0],
+    }
+    if K == n_total:
+        res["accuracy"] = {"E_s_vs_t0": harness.shape_error(alpha_end, alpha0, V),
+                           "E_v": harness.volume_error(alpha_end, V, float(np.sum(alpha0 * V)))}
+
+    if not args.no_cpu_baseline:
+        ncpu = os.cpu_count() or 1
+        try:
+            ncpu = len(os.sched_getaffinity(0))
+        except Exception:
+            pass
+        threads = min(ncpu, 16)
+        t_c = time.time()
+        nic_c, sec_c = cpu_baseline(mesh, alpha0, seq, U0, phi0, args.cpu_steps, threads)
+        res["cpu_baseline"] = {"value": nic_c / sec_c, "unit": "interface-cell updates/s", "cores": threads,
+                               "kind": "port",
+                               "sample": f"oracle on the same mesh, field and time steps, first "
+                                         f"{args.cpu_steps} steps, {threads} OpenMP threads "
+                                         f"({platform.processor() or platform.machine()})"}
+        nic_1, sec_1 = cpu_baseline(mesh, alpha0, seq, U0, phi0, max(2, args.cpu_steps // 2), 1)
+        res["cpu_baseline_serial"] = {"value": nic_1 / sec_1, "unit": "interface-cell updates/s", "cores": 1,
+                                      "kind": "port", "sample": f"first {max(2, args.cpu_steps // 2)} steps"}
+        print(f"[bench] cpu baseline {time.time() - t_c:.1f}s", file=sys.stderr, flush=True)
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -98,6 +98,9 @@ def lib():
     L.svp_harness_upload_base_flow.argtypes = [vp, vp, vp, vp]
     L.svp_harness_scale_flow.argtypes = [vp, dbl]
     L.svp_harness_courant.argtypes = [vp, C.POINTER(dbl), C.POINTER(dbl)]
+    L.svp_harness_init_alpha.argtypes = [vp, vp, C.c_int, C.c_int]
+    L.svp_profile_set.argtypes = [vp, C.c_int]
+    L.svp_profile_get.argtypes = [vp, vp, vp, C.c_int]
     _lib = L
     return L
 
@@ -294,3 +297,30 @@ class SolveVofEqu:
         a, b = C.c_double(), C.c_double()
         _check(lib().svp_harness_courant(self.h, C.byref(a), C.byref(b)))
         return a.value, b.value
+
+    def init_alpha(self, shapes, max_depth=4):
+        """alpha of a union of shapes; each shape is (centre, semi_axes, R) or
+        (centre, radius) for a sphere."""
+        rows = []
+        for s in shapes:
+            if len(s) == 2:
+                c, r = s
+                rows.append(np.concatenate([np.asarray(c, float), [r, r, r], np.eye(3).ravel()]))
+            else:
+                c, a, R = s
+                rows.append(np.concatenate([np.asarray(c, float), np.asarray(a, float),
+                                            np.asarray(R, float).reshape(9)]))
+        arr = np.ascontiguousarray(np.stack(rows), dtype=np.float64)
+        _check(lib().svp_harness_init_alpha(self.h, arr.ctypes.data, len(rows), int(max_depth)))
+
+    STAGES = ("detect", "normal", "findSignedDistance", "upwind", "interfaceFlux", "update",
+              "bounding", "snapClip")
+
+    def profile_set(self, mask):
+        _check(lib().svp_profile_set(self.h, int(mask)))
+
+    def profile_get(self, reset=False):
+        sec = np.zeros(len(self.STAGES))
+        cnt = np.zeros(len(self.STAGES), np.int64)
+        _check(lib().svp_profile_get(self.h, sec.ctypes.data, cnt.ctypes.data, int(reset)))
+        return {n: (s, int(c)) for n, s, c in zip(self.STAGES, sec, cnt)}

@@ -64,6 +64,144 @@ void launch_courant(const DevMesh& m, const double* phi, const double* alpha, un
     k_courant<<<cdiv(m.nC, 256), 256, 0, s>>>(m, phi, alpha, out);
 }
 
+// ---------------------------------------------------------------------------
+// Initial alpha of a union of spheres / ellipsoids by adaptive tetrahedral
+// subdivision: each cell is split into tets (cell centre, face centre, face
+// edge); a tet whose ball misses every shape is empty, a tet with all vertices
+// inside one (convex) shape is full, any other tet is refined 1:8 (edge
+// midpoints) down to maxDepth, where the linear interpolant of the shape
+// function is integrated exactly.  Shape k: g_k(x) = |R_k (x - c_k) / a_k|^2 - 1,
+// union f = min_k g_k.
+// ---------------------------------------------------------------------------
+struct Shape {
+    double c[3], a[3], R[9];   // centre, semi-axes, rows of the rotation
+};
+
+__device__ inline double shape_g(const Shape& s, V3 x) {
+    const double d0 = x.x - s.c[0], d1 = x.y - s.c[1], d2 = x.z - s.c[2];
+    const double q0 = (s.R[0] * d0 + s.R[1] * d1 + s.R[2] * d2) / s.a[0];
+    const double q1 = (s.R[3] * d0 + s.R[4] * d1 + s.R[5] * d2) / s.a[1];
+    const double q2 = (s.R[6] * d0 + s.R[7] * d1 + s.R[8] * d2) / s.a[2];
+    return q0 * q0 + q1 * q1 + q2 * q2 - 1.0;
+}
+
+// volume fraction of {f < 0} for f linear on a tet with vertex values v[4]:
+// CDF of a linear function over a simplex, sum_i (-v_i)_+^3 / prod_j (v_j - v_i)
+__device__ inline double linear_tet_fraction(double v0, double v1, double v2, double v3) {
+    double v[4] = {v0, v1, v2, v3};
+    int nneg = 0;
+    for (int i = 0; i < 4; ++i) nneg += v[i] < 0.0;
+    if (nneg == 0) return 0.0;
+    if (nneg == 4) return 1.0;
+    // separate nearly equal values so the divided differences stay finite
+    double scale = 0.0;
+    for (int i = 0; i < 4; ++i) scale = fmax(scale, fabs(v[i]));
+    const double eps = 1e-9 * scale;
+    for (int i = 1; i < 4; ++i)
+        for (int j = 0; j < i; ++j)
+            if (fabs(v[i] - v[j]) < eps) v[i] += (v[i] >= 0.0 ? 1.0 : -1.0) * eps * (1 + i);
+    const bool useNeg = nneg <= 2;
+    double sum = 0.0;
+    for (int i = 0; i < 4; ++i) {
+        const bool neg = v[i] < 0.0;
+        if (neg != useNeg) continue;
+        double den = 1.0;
+        for (int j = 0; j < 4; ++j)
+            if (j != i) den *= (v[j] - v[i]);
+        const double t = -v[i];
+        sum += (t * t * t) / den;
+    }
+    const double frac = useNeg ? sum : 1.0 + sum;
+    return fmin(1.0, fmax(0.0, frac));
+}
+
+__device__ inline double tet_vol(V3 a, V3 b, V3 c, V3 d) {
+    return fabs(dot(b - a, cross(c - a, d - a))) / 6.0;
+}
+
+__global__ void k_init_alpha(DevMesh m, const Shape* shapes, int nShapes, int maxDepth, double* alpha) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= m.nC) return;
+    const V3 cc = ld3(m.C, c);
+    // quick reject/accept of the whole cell by its bounding ball
+    double R = 0.0;
+    for (int k = m.cOff[c]; k < m.cOff[c + 1]; ++k) {
+        const int f = m.cFaces[k];
+        for (int q = m.fOff[f]; q < m.fOff[f + 1]; ++q) R = fmax(R, mag(ld3(m.pts, m.fPts[q]) - cc));
+    }
+    bool touch = false;
+    for (int s = 0; s < nShapes; ++s) {
+        const Shape& S = shapes[s];
+        const double big = fmax(S.a[0], fmax(S.a[1], S.a[2]));
+        if (mag(cc - v3(S.c[0], S.c[1], S.c[2])) - R <= big) touch = true;
+    }
+    if (!touch) {
+        alpha[c] = 0.0;
+        return;
+    }
+    struct Tet { V3 p[4]; int depth; };
+    Tet stack[64];
+    double volIn = 0.0, volAll = 0.0;
+    for (int k = m.cOff[c]; k < m.cOff[c + 1]; ++k) {
+        const int f = m.cFaces[k];
+        const int fs = m.fOff[f], n = m.fOff[f + 1] - fs;
+        const V3 fc = ld3(m.Cf, f);
+        for (int q = 0; q < n; ++q) {
+            int sp = 0;
+            stack[sp++] = Tet{{cc, fc, ld3(m.pts, m.fPts[fs + q]), ld3(m.pts, m.fPts[fs + (q + 1) % n])}, 0};
+            while (sp > 0) {
+                const Tet t = stack[--sp];
+                const double vol = tet_vol(t.p[0], t.p[1], t.p[2], t.p[3]);
+                volAll += vol;
+                const V3 g = 0.25 * (t.p[0] + t.p[1] + t.p[2] + t.p[3]);
+                double rad = 0.0;
+                for (int i = 0; i < 4; ++i) rad = fmax(rad, mag(t.p[i] - g));
+                bool full = false, maybe = false;
+                for (int s = 0; s < nShapes && !full; ++s) {
+                    const Shape& S = shapes[s];
+                    bool allIn = true;
+                    for (int i = 0; i < 4; ++i) allIn &= shape_g(S, t.p[i]) < 0.0;
+                    if (allIn) full = true;
+                    const double big = fmax(S.a[0], fmax(S.a[1], S.a[2]));
+                    if (mag(g - v3(S.c[0], S.c[1], S.c[2])) - rad <= big) maybe = true;
+                }
+                if (full) { volIn += vol; continue; }
+                if (!maybe) continue;
+                if (t.depth >= maxDepth || sp + 8 > 64) {
+                    double fv[4];
+                    for (int i = 0; i < 4; ++i) {
+                        double mn = 1e300;
+                        for (int s = 0; s < nShapes; ++s) mn = fmin(mn, shape_g(shapes[s], t.p[i]));
+                        fv[i] = mn;
+                    }
+                    volIn += vol * linear_tet_fraction(fv[0], fv[1], fv[2], fv[3]);
+                    continue;
+                }
+                // not counted twice: remove this tet's volume, children add theirs
+                volAll -= vol;
+                const V3 a = t.p[0], b = t.p[1], cq = t.p[2], d = t.p[3];
+                const V3 ab = 0.5 * (a + b), ac = 0.5 * (a + cq), ad = 0.5 * (a + d);
+                const V3 bc = 0.5 * (b + cq), bd = 0.5 * (b + d), cd = 0.5 * (cq + d);
+                const int nd = t.depth + 1;
+                stack[sp++] = Tet{{a, ab, ac, ad}, nd};
+                stack[sp++] = Tet{{ab, b, bc, bd}, nd};
+                stack[sp++] = Tet{{ac, bc, cq, cd}, nd};
+                stack[sp++] = Tet{{ad, bd, cd, d}, nd};
+                stack[sp++] = Tet{{ab, ac, ad, bd}, nd};
+                stack[sp++] = Tet{{ab, ac, bc, bd}, nd};
+                stack[sp++] = Tet{{ac, ad, bd, cd}, nd};
+                stack[sp++] = Tet{{ac, bc, bd, cd}, nd};
+            }
+        }
+    }
+    alpha[c] = volAll > 0.0 ? fmin(1.0, fmax(0.0, volIn / volAll)) : 0.0;
+}
+
+void launch_init_alpha(const DevMesh& m, const void* shapes, int nShapes, int maxDepth, double* alpha,
+                       hipStream_t s) {
+    k_init_alpha<<<cdiv(m.nC, 128), 128, 0, s>>>(m, (const Shape*)shapes, nShapes, maxDepth, alpha);
+}
+
 __global__ void k_count_nonfinite(const double* x, long n, int* count) {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const bool bad = (i < n) && !isfinite(x[i]);

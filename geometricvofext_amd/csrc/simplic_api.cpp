@@ -26,6 +26,8 @@ void launch_scale_flow(int nC, int nF, int nB, const double* U0, const double* p
 void launch_courant(const DevMesh& m, const double* phi, const double* alpha, unsigned long long* out,
                     hipStream_t s);
 void launch_count_nonfinite(const double* x, long n, int* count, hipStream_t s);
+void launch_init_alpha(const DevMesh& m, const void* shapes, int nShapes, int maxDepth, double* alpha,
+                       hipStream_t s);
 }  // namespace svp
 
 namespace {
@@ -123,9 +125,9 @@ public:
     void reconstruct() {
         hipEvent_t a = event(), b = event();
         HIPCHECK(hipEventRecord(a, stream_));
-        launch_detect(m_, st_, params_, stream_);
-        launch_ls_normal(m_, st_, stream_);
-        launch_find_signed_distance(m_, st_, params_, stream_);
+        stage(STAGE_DETECT, [&] { launch_detect(m_, st_, params_, stream_); });
+        stage(STAGE_NORMAL, [&] { launch_ls_normal(m_, st_, stream_); });
+        stage(STAGE_FSD, [&] { launch_find_signed_distance(m_, st_, params_, stream_); });
         HIPCHECK(hipEventRecord(b, stream_));
         HIPCHECK(hipGetLastError());
         pendingRecon_.push_back({a, b});
@@ -140,13 +142,34 @@ public:
         hipEvent_t a = event(), b = event();
         HIPCHECK(hipEventRecord(a, stream_));
         std::swap(st_.alpha, st_.alphaOld);
-        launch_advect(m_, st_, params_, dt, Sp, Su, stream_);
+        for (int sg = STAGE_UPWIND; sg <= STAGE_SNAPCLIP; ++sg)
+            stage(sg, [&] { launch_advect_stage(sg, m_, st_, params_, dt, Sp, Su, stream_); });
         HIPCHECK(hipEventRecord(b, stream_));
         HIPCHECK(hipGetLastError());
         pendingAdvect_.push_back({a, b});
         lastDt_ = dt;
         alphaPhiValid_ = false;
         reconstructed_ = false;
+    }
+
+    // Per-stage HIP-event timing on this handle's stream (bit i of mask =
+    // stage i); accumulated on readout.
+    void setProfile(int mask) {
+        drainProfile();
+        profMask_ = mask;
+    }
+    void profile(double* seconds, int64_t* counts) {
+        HIPCHECK(hipStreamSynchronize(stream_));
+        drainProfile();
+        for (int i = 0; i < N_STAGES; ++i) {
+            if (seconds) seconds[i] = profSec_[i];
+            if (counts) counts[i] = profCnt_[i];
+        }
+    }
+    void resetProfile() {
+        HIPCHECK(hipStreamSynchronize(stream_));
+        drainProfile();
+        for (int i = 0; i < N_STAGES; ++i) { profSec_[i] = 0.0; profCnt_[i] = 0; }
     }
 
     void checkDeviceErrors() {
@@ -284,6 +307,37 @@ private:
     double reconTime_ = 0.0, advectTime_ = 0.0;
     bool reconstructed_ = false;
     bool alphaPhiValid_ = false;
+    int maxCellFaces_ = 0;
+    int profMask_ = 0;
+    double profSec_[N_STAGES] = {};
+    int64_t profCnt_[N_STAGES] = {};
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> profPending_;
+
+    template <class F>
+    void stage(int sg, F&& launch) {
+        if (!(profMask_ & (1 << sg))) {
+            launch();
+            return;
+        }
+        hipEvent_t a = event(), b = event();
+        HIPCHECK(hipEventRecord(a, stream_));
+        launch();
+        HIPCHECK(hipEventRecord(b, stream_));
+        profPending_.push_back({sg, {a, b}});
+    }
+    void drainProfile() {
+        if (profPending_.empty()) return;
+        HIPCHECK(hipStreamSynchronize(stream_));
+        for (auto& e : profPending_) {
+            float ms = 0.f;
+            HIPCHECK(hipEventElapsedTime(&ms, e.second.first, e.second.second));
+            profSec_[e.first] += 1e-3 * ms;
+            profCnt_[e.first] += 1;
+            freeEvents_.push_back(e.second.first);
+            freeEvents_.push_back(e.second.second);
+        }
+        profPending_.clear();
+    }
 
     template <class T>
     T* dalloc(size_t n) {
@@ -358,6 +412,7 @@ private:
             std::sort(srt.begin(), srt.end());
             for (int k = 0; k < e - b; ++k) cFacesAsc[b + k] = (own[srt[k]] == c) ? srt[k] : ~srt[k];
         }
+        maxCellFaces_ = maxCellFaces;
         const int faceLimit = dict_.split_warped_face ? MAX_CELL_FACES : NS_CELL_FACES;
         if (maxCellFaces > faceLimit)
             throw SvpError(SVP_E_MESH, "a cell has " + std::to_string(maxCellFaces) + " faces (supported " +
@@ -454,7 +509,6 @@ private:
         st_.alphaPhi = dalloc<double>(nF);
         st_.corr = dalloc<double>(nF);
         st_.mixedFlag = dalloc<unsigned char>(nC);
-        st_.candFlag = dalloc<unsigned char>(nC);
         st_.blockCount = dalloc<int>(detect_blocks(nC));
         st_.nMixed = dalloc<int>(1);
         st_.mixed = dalloc<int>(nC);
@@ -468,10 +522,21 @@ private:
         st_.un0 = dalloc<double>(nC);
         st_.oobCount = dalloc<int>(1);
         st_.candCount = dalloc<int>(1);
-        st_.candCap = nC;
+        st_.candBits = dalloc<unsigned long long>((nC + 63) / 64);
         st_.cand = dalloc<int>(nC);
-        st_.workCap = 2 * std::max(1 << 16, std::min(nF, 1 << 22));
-        st_.work = dalloc<int>(st_.workCap);
+        st_.nFirst = dalloc<int>(nC);
+        st_.active = dalloc<unsigned char>(nC);
+        st_.pend = dalloc<int>(nC);
+        st_.queue = dalloc<int>(2L * nC);
+        st_.actIdx = dalloc<int>(nC);
+        HIPCHECK(hipMemset(st_.actIdx, 0xFF, sizeof(int) * nC));
+        st_.corrPos = dalloc<int>(nF);
+        HIPCHECK(hipMemset(st_.corrPos, 0xFF, sizeof(int) * nF));
+        st_.corrList = dalloc<int>(nF);
+        st_.touched = dalloc<int>(nC);
+        st_.touchedFlag = dalloc<int>(nC);
+        st_.boundStats = dalloc<int>(4 * MAX_BOUND_PASSES);
+        st_.maxCellFaces = maxCellFaces_;
         st_.minmax = dalloc<double>(4);
         st_.errFlag = dalloc<int>(1);
         st_.nIcTotal = dalloc<unsigned long long>(1);
@@ -741,6 +806,28 @@ int svp_device_view_get(svp_handle* h, svp_device_view* out) {
     });
 }
 
+int svp_bound_stats(svp_handle* h, int32_t* out, int cap) {
+    return guarded([&] {
+        solveVofEqu& s = H(h);
+        std::vector<int> v(4 * svp::MAX_BOUND_PASSES);
+        s.copyOutT(v.data(), s.st_.boundStats, (long)v.size(), SVP_HOST);
+        for (int i = 0; i < cap && i < (int)v.size(); ++i) out[i] = v[i];
+        HIPCHECK(hipMemsetAsync(s.st_.boundStats, 0, sizeof(int) * v.size(), s.stream_));
+    });
+}
+
+int svp_profile_set(svp_handle* h, int mask) {
+    return guarded([&] { H(h).setProfile(mask); });
+}
+
+int svp_profile_get(svp_handle* h, double* seconds, int64_t* counts, int reset) {
+    return guarded([&] {
+        solveVofEqu& s = H(h);
+        s.profile(seconds, counts);
+        if (reset) s.resetProfile();
+    });
+}
+
 int svp_synchronize(svp_handle* h) {
     return guarded([&] {
         solveVofEqu& s = H(h);
@@ -813,6 +900,20 @@ int svp_harness_scale_flow(svp_handle* h, double factor) {
         svp::launch_scale_flow(s.m_.nC, s.m_.nF, s.nB_, s.U0_, s.phi0_, s.U0b_, factor, s.st_.U, s.st_.phi,
                                s.st_.Ub, s.stream_);
         HIPCHECK(hipGetLastError());
+    });
+}
+
+int svp_harness_init_alpha(svp_handle* h, const double* shapes, int nShapes, int maxDepth) {
+    return guarded([&] {
+        solveVofEqu& s = H(h);
+        if (!shapes || nShapes <= 0 || maxDepth < 0 || maxDepth > 8)
+            throw SvpError(SVP_E_ARG, "init_alpha: need shapes and 0 <= max_depth <= 8");
+        DBuf d;
+        double* ds = d.alloc<double>(15L * nShapes);
+        HIPCHECK(hipMemcpy(ds, shapes, sizeof(double) * 15 * nShapes, hipMemcpyHostToDevice));
+        svp::launch_init_alpha(s.m_, ds, nShapes, maxDepth, s.st_.alpha, s.stream_);
+        HIPCHECK(hipGetLastError());
+        HIPCHECK(hipStreamSynchronize(s.stream_));
     });
 }
 

@@ -711,15 +711,15 @@ __device__ inline bool need_bounding(const DevMesh& m, const unsigned char* mixe
 
 // Conservative update (advectionTemplates.C:399-404) with fvc::surfaceIntegrate
 // summing the cell's faces in ascending face order.  Also counts cells outside
-// [-aTol, 1+aTol] for the bounding loop and flags bounding candidates.
+// [-aTol, 1+aTol] for the bounding loop and writes the bounding candidates
+// (outside the bounds and next to the interface) as a bitmap, one 64-bit word
+// per wave.
 __global__ void __launch_bounds__(DETECT_BLOCK) k_update(DevMesh m, DevState st, double dt,
-                                                         const double* Sp, const double* Su,
-                                                         unsigned char* candFlag) {
-    __shared__ int lds[DETECT_BLOCK / 64];
+                                                         const double* Sp, const double* Su) {
     const double aTol = 100.0 * SVP_SMALL;
     const double rDeltaT = 1.0 / dt;
     const int c = blockIdx.x * DETECT_BLOCK + threadIdx.x;
-    bool cand = false;
+    bool cand = false, oob = false;
     if (c < m.nC) {
         double sI = 0.0;
         for (int k = m.cOff[c]; k < m.cOff[c + 1]; ++k) {
@@ -738,13 +738,17 @@ __global__ void __launch_bounds__(DETECT_BLOCK) k_update(DevMesh m, DevState st,
         const double den = Sp ? (rDeltaT - Sp[c]) : rDeltaT;
         const double a = num / den;
         st.alpha[c] = a;
-        if (loop_oob(a, aTol)) atomicAdd(st.oobCount, 1);
+        oob = loop_oob(a, aTol);
         cand = cell_oob(a, aTol) && need_bounding(m, st.mixedFlag, c);
-        candFlag[c] = cand ? 1 : 0;
     }
-    int total;
-    block_prefix(cand, lds, total);
-    if (threadIdx.x == 0) st.blockCount[blockIdx.x] = total;
+    const unsigned long long bits = __ballot(cand);
+    const unsigned long long oobBits = __ballot(oob);
+    const int lane = threadIdx.x & 63;
+    if (lane == 0) {
+        const int word = c >> 6;
+        if (word < (m.nC + 63) / 64) st.candBits[word] = bits;
+        if (oobBits) atomicAdd(st.oobCount, __popcll(oobBits));
+    }
 }
 
 __device__ inline double face_value(const DevMesh& m, const double* f, int facei) {
@@ -753,141 +757,319 @@ __device__ inline double face_value(const DevMesh& m, const double* f, int facei
     return f[facei];
 }
 
-__device__ inline double net_flux(const DevMesh& m, const double* f, int c) {
-    double dV = 0.0;
-    for (int k = m.cOff[c]; k < m.cOff[c + 1]; ++k) {
-        const int facei = m.cFaces[k];
-        const double v = face_value(m, f, facei);
-        if (m.own[facei] == c) dV += v;
-        else dV -= v;
-    }
-    return dV;
+__device__ inline bool face_downwind_of(const DevMesh& m, const double* phi, int facei, int c) {
+    const double ph = face_value(m, phi, facei);
+    return (m.own[facei] == c) ? (ph >= 0) : (ph < 0);
 }
 
-// limitFlux + boundFlux (advectionTemplates.C:118-349).  One lane walks the
-// ascending candidate list exactly as the reference walks its cells, since a
-// cell reads the corrections written by earlier cells of the same pass.
-__global__ void k_bound(DevMesh m, DevState st, Params p, double dt, const double* Sp, const double* Su) {
-    if (threadIdx.x != 0) return;
+// workgroup exclusive scan of one int per thread (blockDim.x == 1024)
+__device__ inline int wg_exclusive_scan(int v, int* lds, int& total) {
+    const int t = threadIdx.x;
+    lds[t] = v;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+        const int x = (t >= o) ? lds[t - o] : 0;
+        __syncthreads();
+        lds[t] += x;
+        __syncthreads();
+    }
+    total = lds[1023];
+    const int r = lds[t] - v;
+    __syncthreads();
+    return r;
+}
+
+// First redistribution loop of boundFlux (advectionTemplates.C:260-330 with
+// iter = 0): it only reads the cell's own data, because the corrections of a
+// face are written solely by the cell that face is downwind of.  Returns the
+// number of faces the loop passes fluid through (they open correctedFaces in
+// this order) or 0 when the cell writes no corrections in the whole pass.
+__device__ inline int bound_first_faces(const DevMesh& m, const DevState& st, int c, double dt,
+                                        int* facesOut) {
+    const double aTol = 100.0 * SVP_SMALL;
+    const double ac = st.alpha[c];
+    const double Vi = m.V[c];
+    const double alphaOvershoot = pos0(ac - 1.0) * (ac - 1.0) + neg0(ac) * ac;
+    if (fabs(alphaOvershoot) < aTol) return 0;
+    const double fluidToPassOn = alphaOvershoot * Vi;
+    int n = 0;
+    for (int k = m.cOff[c]; k < m.cOff[c + 1]; ++k) {
+        const int f = m.cFaces[k];
+        if (!face_downwind_of(m, st.phi, f, c)) continue;
+        const double ph = face_value(m, st.phi, f);
+        const double dVff = face_value(m, st.dVf, f) + 0.0;
+        const double maxExtra = fabs(pos0(fluidToPassOn) * ph * dt - dVff);
+        if (maxExtra / Vi > aTol) {
+            if (facesOut) facesOut[n] = f;
+            n++;
+        }
+    }
+    return n;
+}
+
+// Whole boundFlux of one cell (advectionTemplates.C:243-347).  Corrections of
+// the cell's downwind faces live in registers; those of its upwind faces come
+// from lower-numbered cells that have completed (the reference processes cells
+// in ascending order, so a higher-numbered writer contributes zero).
+template <int MF>
+__device__ inline void bound_cell(const DevMesh& m, const DevState& st, int c, double dt, double rDeltaT,
+                                  const double* Sp, const double* Su) {
+    const double aTol = 100.0 * SVP_SMALL;
+    const int b = m.cOff[c], e = m.cOff[c + 1];
+    const int nf = e - b;
+    int fid[MF];
+    bool isOwn[MF], down[MF], writable[MF];
+    double phf[MF], dvf[MF], cor[MF];
+    for (int k = 0; k < nf; ++k) {
+        const int f = m.cFaces[b + k];
+        fid[k] = f;
+        isOwn[k] = (m.own[f] == c);
+        phf[k] = face_value(m, st.phi, f);
+        dvf[k] = face_value(m, st.dVf, f);
+        down[k] = isOwn[k] ? (phf[k] >= 0) : (phf[k] < 0);
+        writable[k] = (f < m.nIF) || !m.fEmpty[f];
+        double cv = 0.0;
+        if (!down[k] && f < m.nIF) {
+            const int j = isOwn[k] ? m.nbr[f] : m.own[f];
+            if (j < c) cv = st.corr[f];
+        }
+        cor[k] = cv;
+    }
+    double nfd = 0.0;
+    for (int k = 0; k < nf; ++k) nfd = isOwn[k] ? nfd + dvf[k] : nfd - dvf[k];
+    const double Vi = m.V[c];
+    const double ac = st.alpha[c];
+    double alphaOvershoot = pos0(ac - 1.0) * (ac - 1.0) + neg0(ac) * ac;
+    double fluidToPassOn = alphaOvershoot * Vi;
+    int nFaces = 1;
+    for (int iter = 0; iter < 10; iter++) {
+        if (fabs(alphaOvershoot) < aTol || nFaces == 0) break;
+        int sel[MF];
+        double smax[MF];
+        int ns = 0;
+        double dVftot = 0.0;
+        nFaces = 0;
+        for (int k = 0; k < nf; ++k) {
+            if (!down[k]) continue;
+            const double dVff = dvf[k] + cor[k];
+            const double maxExtra = fabs(pos0(fluidToPassOn) * phf[k] * dt - dVff);
+            if (maxExtra / Vi > aTol) {
+                sel[ns] = k;
+                smax[ns] = maxExtra;
+                ns++;
+                dVftot += fabs(phf[k] * dt);
+            }
+        }
+        for (int q = 0; q < ns; ++q) {
+            const int k = sel[q];
+            double fl = fabs(fluidToPassOn) * fabs(phf[k] * dt) / dVftot;
+            nFaces += (int)pos0(smax[q] - fl);
+            fl = fmin(fl, smax[q]);
+            double dVff = cor[k];
+            dVff += sgn(phf[k]) * sgn(fluidToPassOn) * fl;
+            if (writable[k]) cor[k] = dVff;
+        }
+        double nfc = 0.0;
+        for (int k = 0; k < nf; ++k) nfc = isOwn[k] ? nfc + cor[k] : nfc - cor[k];
+        double num = st.alphaOld[c] * rDeltaT;
+        if (Su) num = num + Su[c];
+        num = num - nfd / Vi * rDeltaT - nfc / Vi * rDeltaT;
+        const double den = Sp ? (rDeltaT - Sp[c]) : rDeltaT;
+        const double alpha1New = num / den;
+        alphaOvershoot = pos0(alpha1New - 1.0) * (alpha1New - 1.0) + neg0(alpha1New) * alpha1New;
+        fluidToPassOn = alphaOvershoot * Vi;
+    }
+    for (int k = 0; k < nf; ++k)
+        if (down[k] && writable[k]) st.corr[fid[k]] = cor[k];
+}
+
+// limitFlux (advectionTemplates.C:118-215) for all passes in one workgroup.
+// Per pass: ascending candidates from the bitmap; the first redistribution
+// loop of every candidate in parallel (it fixes correctedFaces and its order);
+// the full boundFlux of the cells that write, in rounds over the DAG "lower
+// numbered upwind neighbour that writes"; corrections applied per cell in
+// correctedFaces order; bitmap and bound count updated for the touched cells.
+template <int MF>
+__global__ void __launch_bounds__(1024) k_bound_all(DevMesh m, DevState st, Params p, double dt,
+                                                    const double* Sp, const double* Su) {
+    __shared__ int lds[1024];
+    __shared__ int sN, sQ, sNext, sRound, sTouched, sOob, sActive;
+    const int t = threadIdx.x;
     const double aTol = 100.0 * SVP_SMALL;
     const double rDeltaT = 1.0 / dt;
-    int nCand = *st.candCount;
-    int* cand = st.cand;
-    int oob = *st.oobCount;
-    // work layout: correctedFaces | touched faces
-    const int half = st.workCap / 2;
-    int* corrected = st.work;
-    int* touched = st.work + half;
-    for (int pass = 0; pass < p.nAlphaBounds; ++pass) {
-        if (oob == 0) break;
-        int nCorrected = 0, nTouched = 0;
-        for (int ci = 0; ci < nCand; ++ci) {
-            const int celli = cand[ci];
-            const double ac = st.alpha[celli];
-            if (!cell_oob(ac, aTol)) continue;
-            const double Vi = m.V[celli];
-            double alphaOvershoot = pos0(ac - 1.0) * (ac - 1.0) + neg0(ac) * ac;
-            double fluidToPassOn = alphaOvershoot * Vi;
-            int nFaces = 1;
-            bool firstLoop = true;
-            for (int iter = 0; iter < 10; iter++) {
-                if (fabs(alphaOvershoot) < aTol || nFaces == 0) break;
-                int pf[MAX_CELL_FACES];
-                double pphi[MAX_CELL_FACES], pmax[MAX_CELL_FACES];
-                int np = 0;
-                double dVftot = 0.0;
-                nFaces = 0;
-                for (int k = m.cOff[celli]; k < m.cOff[celli + 1]; ++k) {
-                    const int facei = m.cFaces[k];
-                    const double ph = face_value(m, st.phi, facei);
-                    const bool down = (m.own[facei] == celli) ? (ph >= 0) : (ph < 0);
-                    if (!down) continue;
-                    const double dVff = face_value(m, st.dVf, facei) + face_value(m, st.corr, facei);
-                    const double maxExtra = fabs(pos0(fluidToPassOn) * ph * dt - dVff);
-                    if (maxExtra / Vi > aTol && np < MAX_CELL_FACES) {
-                        pf[np] = facei;
-                        pphi[np] = ph;
-                        pmax[np] = maxExtra;
-                        np++;
-                        dVftot += fabs(ph * dt);
-                    }
+    const int nWords = (m.nC + 63) / 64;
+    if (t == 0) sOob = *st.oobCount;
+    __syncthreads();
+    const int nPasses = p.nAlphaBounds < MAX_BOUND_PASSES ? p.nAlphaBounds : MAX_BOUND_PASSES;
+    for (int pass = 0; pass < nPasses; ++pass) {
+        if (sOob == 0) break;
+        // ascending candidate list from the bitmap
+        const int w0 = (int)((long)nWords * t / 1024), w1 = (int)((long)nWords * (t + 1) / 1024);
+        int cnt = 0;
+        for (int w = w0; w < w1; ++w) cnt += __popcll(st.candBits[w]);
+        int nCand;
+        int off = wg_exclusive_scan(cnt, lds, nCand);
+        for (int w = w0; w < w1; ++w) {
+            unsigned long long bits = st.candBits[w];
+            while (bits) {
+                const int bit = __ffsll((long long)bits) - 1;
+                bits &= bits - 1;
+                st.cand[off++] = w * 64 + bit;
+            }
+        }
+        __syncthreads();
+        // first loop of every candidate; active cells get their index
+        int faces[MF];
+        for (int i = t; i < nCand; i += 1024) {
+            const int c = st.cand[i];
+            const int n = bound_first_faces(m, st, c, dt, nullptr);
+            st.nFirst[i] = n;
+            st.active[i] = n > 0 ? 1 : 0;
+            if (n > 0) st.actIdx[c] = i;
+        }
+        __syncthreads();
+        // correctedFaces positions: exclusive scan of nFirst in candidate order
+        const int i0 = (int)((long)nCand * t / 1024), i1 = (int)((long)nCand * (t + 1) / 1024);
+        int part = 0;
+        for (int i = i0; i < i1; ++i) part += st.nFirst[i];
+        int nCorr;
+        int pos = wg_exclusive_scan(part, lds, nCorr);
+        for (int i = i0; i < i1; ++i) {
+            if (st.nFirst[i] == 0) continue;
+            const int n = bound_first_faces(m, st, st.cand[i], dt, faces);
+            for (int k = 0; k < n; ++k) {
+                st.corrPos[faces[k]] = pos + k;
+                st.corrList[pos + k] = faces[k];
+            }
+            pos += n;
+        }
+        // predecessor counts and the first frontier
+        if (t == 0) { sQ = 0; sNext = 0; sRound = 0; sTouched = 0; sActive = 0; }
+        __syncthreads();
+        for (int i = t; i < nCand; i += 1024) {
+            if (!st.active[i]) continue;
+            atomicAdd(&sActive, 1);
+            const int c = st.cand[i];
+            int np = 0;
+            for (int k = m.cOff[c]; k < m.cOff[c + 1]; ++k) {
+                const int f = m.cFaces[k];
+                if (f >= m.nIF || face_downwind_of(m, st.phi, f, c)) continue;
+                const int j = (m.own[f] == c) ? m.nbr[f] : m.own[f];
+                if (j < c && st.actIdx[j] >= 0) np++;
+            }
+            st.pend[i] = np;
+            if (np == 0) st.queue[atomicAdd(&sQ, 1)] = i;
+        }
+        __syncthreads();
+        // rounds over the dependency DAG (edges point to higher cell numbers,
+        // so every round completes at least one cell until none is left)
+        int cur = 0;
+        while (true) {
+            const int qn = sQ;
+            if (qn == 0) break;
+            int* q = st.queue + (long)cur * m.nC;
+            int* qnext = st.queue + (long)(cur ^ 1) * m.nC;
+            for (int k = t; k < qn; k += 1024) {
+                const int i = q[k];
+                const int c = st.cand[i];
+                bound_cell<MF>(m, st, c, dt, rDeltaT, Sp, Su);
+                for (int kk = m.cOff[c]; kk < m.cOff[c + 1]; ++kk) {
+                    const int f = m.cFaces[kk];
+                    if (f >= m.nIF || !face_downwind_of(m, st.phi, f, c)) continue;
+                    const int d = (m.own[f] == c) ? m.nbr[f] : m.own[f];
+                    if (d < c) continue;
+                    const int a = st.actIdx[d];
+                    if (a < 0) continue;
+                    if (atomicSub(&st.pend[a], 1) == 1) qnext[atomicAdd(&sNext, 1)] = a;
                 }
-                for (int fi = 0; fi < np; ++fi) {
-                    const int facei = pf[fi];
-                    double fl = fabs(fluidToPassOn) * fabs(pphi[fi] * dt) / dVftot;
-                    nFaces += (int)pos0(pmax[fi] - fl);
-                    fl = fmin(fl, pmax[fi]);
-                    double dVff = face_value(m, st.corr, facei);
-                    dVff += sgn(pphi[fi]) * sgn(fluidToPassOn) * fl;
-                    if (facei < m.nIF || !m.fEmpty[facei]) {
-                        st.corr[facei] = dVff;
-                        if (nTouched < half) touched[nTouched++] = facei;
-                        else atomicOr(st.errFlag, ERR_WORK_OVERFLOW);
-                    }
-                    if (firstLoop) {
-                        if (nCorrected < half) corrected[nCorrected++] = facei;
-                        else atomicOr(st.errFlag, ERR_WORK_OVERFLOW);
-                    }
+            }
+            __threadfence_block();
+            __syncthreads();
+            if (t == 0) {
+                sQ = sNext;
+                sNext = 0;
+                sRound++;
+            }
+            cur ^= 1;
+            __syncthreads();
+        }
+        // cells that receive corrections
+        for (int q2 = t; q2 < nCorr; q2 += 1024) {
+            const int f = st.corrList[q2];
+            const int cs[2] = {m.own[f], f < m.nIF ? m.nbr[f] : -1};
+            for (int s2 = 0; s2 < 2; ++s2) {
+                if (cs[s2] < 0) continue;
+                if (atomicCAS(&st.touchedFlag[cs[s2]], 0, 1) == 0) st.touched[atomicAdd(&sTouched, 1)] = cs[s2];
+            }
+        }
+        __syncthreads();
+        // apply in correctedFaces order per cell; refresh bound count and bitmap
+        const int nT = sTouched;
+        for (int k = t; k < nT; k += 1024) {
+            const int c = st.touched[k];
+            int kpos[MF];
+            double kval[MF];
+            bool kown[MF];
+            int nk = 0;
+            for (int kk = m.cOff[c]; kk < m.cOff[c + 1]; ++kk) {
+                const int f = m.cFaces[kk];
+                const int ps = st.corrPos[f];
+                if (ps < 0) continue;
+                // insertion by position
+                int j = nk - 1;
+                while (j >= 0 && kpos[j] > ps) {
+                    kpos[j + 1] = kpos[j];
+                    kval[j + 1] = kval[j];
+                    kown[j + 1] = kown[j];
+                    --j;
                 }
-                firstLoop = false;
-                double num = st.alphaOld[celli] * rDeltaT;
-                if (Su) num = num + Su[celli];
-                num = num - net_flux(m, st.dVf, celli) / Vi * rDeltaT - net_flux(m, st.corr, celli) / Vi * rDeltaT;
-                const double den = Sp ? (rDeltaT - Sp[celli]) : rDeltaT;
-                const double alpha1New = num / den;
-                alphaOvershoot = pos0(alpha1New - 1.0) * (alpha1New - 1.0) + neg0(alpha1New) * alpha1New;
-                fluidToPassOn = alphaOvershoot * Vi;
+                kpos[j + 1] = ps;
+                kval[j + 1] = face_value(m, st.corr, f);
+                kown[j + 1] = (m.own[f] == c);
+                nk++;
             }
+            const double before = st.alpha[c];
+            double a = before;
+            for (int q2 = 0; q2 < nk; ++q2) a = kown[q2] ? a - kval[q2] / m.V[c] : a + kval[q2] / m.V[c];
+            st.alpha[c] = a;
+            const int dOob = (int)loop_oob(a, aTol) - (int)loop_oob(before, aTol);
+            if (dOob) atomicAdd(&sOob, dOob);
+            const bool candNew = cell_oob(a, aTol) && need_bounding(m, st.mixedFlag, c);
+            const unsigned long long bit = 1ull << (c & 63);
+            if (candNew) atomicOr(&st.candBits[c >> 6], bit);
+            else atomicAnd(&st.candBits[c >> 6], ~bit);
+            st.touchedFlag[c] = 0;
         }
-        // apply corrections in correctedFaces order, first occurrence only
-        for (int q = 0; q < nCorrected; ++q) {
-            const int facei = corrected[q];
-            bool dup = false;
-            for (int r = 0; r < q; ++r) dup |= (corrected[r] == facei);
-            if (dup) continue;
-            const double cv = face_value(m, st.corr, facei);
-            const int o = m.own[facei];
-            {
-                const double before = st.alpha[o];
-                const double after = before - cv / m.V[o];
-                st.alpha[o] = after;
-                oob += (int)loop_oob(after, aTol) - (int)loop_oob(before, aTol);
-            }
-            if (facei < m.nIF) {
-                const int nb = m.nbr[facei];
-                const double before = st.alpha[nb];
-                const double after = before + cv / m.V[nb];
-                st.alpha[nb] = after;
-                oob += (int)loop_oob(after, aTol) - (int)loop_oob(before, aTol);
-            }
-            const double corrVf = face_value(m, st.dVf, facei) + cv;
-            if (facei < m.nIF || !m.fEmpty[facei]) st.dVf[facei] = corrVf;
+        // dVf of the corrected faces (each face appears once: only the cell it
+        // is downwind of corrects it)
+        for (int q2 = t; q2 < nCorr; q2 += 1024) {
+            const int f = st.corrList[q2];
+            if (f < m.nIF || !m.fEmpty[f]) st.dVf[f] = face_value(m, st.dVf, f) + face_value(m, st.corr, f);
         }
-        // candidates of the next pass: keep failing cells, add touched cells
-        int nNew = 0;
-        for (int ci = 0; ci < nCand; ++ci)
-            if (cell_oob(st.alpha[cand[ci]], aTol)) cand[nNew++] = cand[ci];
-        nCand = nNew;
-        for (int q = 0; q < nCorrected; ++q) {
-            const int facei = corrected[q];
-            for (int side = 0; side < 2; ++side) {
-                int cc = side == 0 ? m.own[facei] : (facei < m.nIF ? m.nbr[facei] : -1);
-                if (cc < 0) continue;
-                if (!cell_oob(st.alpha[cc], aTol) || !need_bounding(m, st.mixedFlag, cc)) continue;
-                int pos = 0;
-                while (pos < nCand && cand[pos] < cc) ++pos;
-                if (pos < nCand && cand[pos] == cc) continue;
-                if (nCand >= st.candCap) { atomicOr(st.errFlag, ERR_CAND_OVERFLOW); continue; }
-                for (int r = nCand; r > pos; --r) cand[r] = cand[r - 1];
-                cand[pos] = cc;
-                nCand++;
+        __syncthreads();
+        // dVfCorrectionValues = 0 and bookkeeping reset for the next pass
+        for (int i = t; i < nCand; i += 1024) {
+            if (!st.active[i]) continue;
+            const int c = st.cand[i];
+            for (int k = m.cOff[c]; k < m.cOff[c + 1]; ++k) {
+                const int f = m.cFaces[k];
+                if (face_downwind_of(m, st.phi, f, c)) {
+                    st.corr[f] = 0.0;
+                    st.corrPos[f] = -1;
+                }
             }
+            st.actIdx[c] = -1;
         }
-        // dVfCorrectionValues = 0 for the next pass
-        for (int q = 0; q < nTouched; ++q) st.corr[touched[q]] = 0.0;
+        if (t == 0 && pass < MAX_BOUND_PASSES) {
+            st.boundStats[4 * pass + 0] = nCand;
+            st.boundStats[4 * pass + 1] = sActive;
+            st.boundStats[4 * pass + 2] = sRound;
+            st.boundStats[4 * pass + 3] = nCorr;
+            *st.candCount = nCand;
+        }
+        __syncthreads();
     }
-    *st.candCount = nCand;
-    *st.oobCount = oob;
+    if (t == 0) *st.oobCount = sOob;
 }
 
 // applyBruteForceBounding (advection.C:291-308)
@@ -905,17 +1087,31 @@ __global__ void k_zero_int(int* p, int n) {
     if (i < n) p[i] = 0;
 }
 
-void launch_advect(const DevMesh& m, const DevState& st, const Params& p, double dt, const double* Sp,
-                   const double* Su, hipStream_t s) {
+void launch_advect_stage(int stage, const DevMesh& m, const DevState& st, const Params& p, double dt,
+                         const double* Sp, const double* Su, hipStream_t s) {
     const int nb = detect_blocks(m.nC);
-    k_zero_int<<<1, 64, 0, s>>>(st.oobCount, 1);
-    k_upwind<<<cdiv(m.nF, 256), 256, 0, s>>>(m, st, dt);
-    k_interface_flux<<<1024, 64, 0, s>>>(m, st, dt);
-    k_update<<<nb, DETECT_BLOCK, 0, s>>>(m, st, dt, Sp, Su, st.candFlag);
-    k_scan_counts<<<1, 1024, 0, s>>>(st.blockCount, nb, st.candCount, nullptr);
-    k_compact_flags<<<nb, DETECT_BLOCK, 0, s>>>(m.nC, st.candFlag, st.blockCount, st.cand);
-    k_bound<<<1, 64, 0, s>>>(m, st, p, dt, Sp, Su);
-    if (p.snapTol > 0.0 || p.clip) k_snap_clip<<<cdiv(m.nC, 256), 256, 0, s>>>(m.nC, st.alpha, p.snapTol, p.clip);
+    switch (stage) {
+        case STAGE_UPWIND:
+            k_zero_int<<<1, 64, 0, s>>>(st.oobCount, 1);
+            k_upwind<<<cdiv(m.nF, 256), 256, 0, s>>>(m, st, dt);
+            break;
+        case STAGE_IFLUX:
+            k_interface_flux<<<1024, 64, 0, s>>>(m, st, dt);
+            break;
+        case STAGE_UPDATE:
+            k_update<<<nb, DETECT_BLOCK, 0, s>>>(m, st, dt, Sp, Su);
+            break;
+        case STAGE_BOUND:
+            if (st.maxCellFaces <= NS_CELL_FACES) k_bound_all<NS_CELL_FACES><<<1, 1024, 0, s>>>(m, st, p, dt, Sp, Su);
+            else k_bound_all<MAX_CELL_FACES><<<1, 1024, 0, s>>>(m, st, p, dt, Sp, Su);
+            break;
+        case STAGE_SNAPCLIP:
+            if (p.snapTol > 0.0 || p.clip)
+                k_snap_clip<<<cdiv(m.nC, 256), 256, 0, s>>>(m.nC, st.alpha, p.snapTol, p.clip);
+            break;
+        default:
+            break;
+    }
 }
 
 __global__ void k_alpha_phi(int nF, const double* dVf, double* alphaPhi, double dt) {

@@ -50,7 +50,6 @@ struct DevState {
     double* alphaPhi;   // nF
     double* corr;       // nF, bounding corrections (kept zero between passes)
     unsigned char* mixedFlag;   // nC
-    unsigned char* candFlag;    // nC, bounding candidates after the update
     int* blockCount;    // per detection block
     int* nMixed;        // [1]
     int* mixed;         // capacity nC
@@ -62,13 +61,22 @@ struct DevState {
     double* ifC;        // 3*cap
     double* ifS;        // 3*cap
     double* un0;
-    // bounding bookkeeping
+    // bounding bookkeeping (limitFlux/boundFlux)
     int* oobCount;      // [1] cells with alpha-1 > aTol or alpha < -aTol
-    int* candCount;     // [1]
-    int* cand;          // capacity candCap
-    int candCap;
-    int* work;          // scratch ints for the bounding kernel (workCap)
-    int workCap;
+    int* candCount;     // [1] candidates of the last pass (diagnostics)
+    unsigned long long* candBits;  // nC bits: alpha outside [-aTol,1+aTol] and needBounding
+    int* cand;          // nC, ascending candidates of the current pass
+    int* nFirst;        // nC, per candidate: faces of its first redistribution loop
+    unsigned char* active;  // nC, per candidate: writes corrections this pass
+    int* pend;          // nC, per candidate: active upwind predecessors not yet done
+    int* queue;         // 2*nC, Kahn frontiers
+    int* actIdx;        // nC cells -> candidate index of active cells, else -1
+    int* corrPos;       // nF faces -> position in correctedFaces, else -1
+    int* corrList;      // nF, correctedFaces of the current pass
+    int* touched;       // nC, cells whose alpha gets corrections
+    int* touchedFlag;   // nC
+    int* boundStats;    // [4*maxPasses]: candidates, active, rounds, corrected faces
+    int maxCellFaces;
     double* minmax;     // [4]: min, max before bounding, min, max after
     int* errFlag;       // [1] device error bits
     unsigned long long* nIcTotal;  // [1] running sum of mixed-cell counts
@@ -86,9 +94,9 @@ struct Params {
 enum : int {
     ERR_CELL_TOO_LARGE = 1,
     ERR_TOO_MANY_CROSSINGS = 2,
-    ERR_CAND_OVERFLOW = 4,
-    ERR_WORK_OVERFLOW = 8,
+    ERR_BOUND_SCHEDULE = 4,
 };
+constexpr int MAX_BOUND_PASSES = 64;
 
 constexpr int DETECT_BLOCK = 256;
 
@@ -97,8 +105,20 @@ void launch_geometry(const DevMesh& m, hipStream_t s);
 void launch_detect(const DevMesh& m, const DevState& st, const Params& p, hipStream_t s);
 void launch_ls_normal(const DevMesh& m, const DevState& st, hipStream_t s);
 void launch_find_signed_distance(const DevMesh& m, const DevState& st, const Params& p, hipStream_t s);
-void launch_advect(const DevMesh& m, const DevState& st, const Params& p, double dt, const double* Sp,
-                   const double* Su, hipStream_t s);
+// pipeline stages of one step, in launch order (also the profiler's slots)
+enum Stage : int {
+    STAGE_DETECT = 0,
+    STAGE_NORMAL = 1,
+    STAGE_FSD = 2,
+    STAGE_UPWIND = 3,
+    STAGE_IFLUX = 4,
+    STAGE_UPDATE = 5,
+    STAGE_BOUND = 6,
+    STAGE_SNAPCLIP = 7,
+    N_STAGES = 8,
+};
+void launch_advect_stage(int stage, const DevMesh& m, const DevState& st, const Params& p, double dt,
+                         const double* Sp, const double* Su, hipStream_t s);
 void launch_alpha_phi(const DevMesh& m, const DevState& st, double dt, hipStream_t s);
 int detect_blocks(int nC);
 

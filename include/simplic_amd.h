@@ -140,6 +140,20 @@ typedef struct {
 } svp_device_view;
 int svp_device_view_get(svp_handle* h, svp_device_view* out);
 
+/* Per-stage GPU timing with HIP events on the handle's stream (no
+ * counterpart in the reference, which has only the two cumulative CPU timers).
+ * Stages: 0 detect, 1 normal, 2 findSignedDistance, 3 upwind, 4 interface
+ * flux, 5 update, 6 bounding, 7 snap/clip; bit i of mask enables stage i.
+ * svp_profile_get fills seconds[8] and counts[8] (either may be NULL) and
+ * optionally resets them. */
+#define SVP_N_STAGES 8
+/* Statistics of the bounding passes since the last call, 4 ints per pass:
+ * candidates, cells that wrote corrections, dependency rounds, corrected
+ * faces.  Zero entries after the last pass that ran. */
+int svp_bound_stats(svp_handle* h, int32_t* out, int cap);
+int svp_profile_set(svp_handle* h, int mask);
+int svp_profile_get(svp_handle* h, double* seconds, int64_t* counts, int reset);
+
 /* Wait for all queued work of the handle. */
 int svp_synchronize(svp_handle* h);
 

@@ -30,6 +30,13 @@ int svp_harness_upload_base_flow(svp_handle* h, const double* U0, const double* 
  * GPU (updateU.H:117-127). */
 int svp_harness_scale_flow(svp_handle* h, double factor);
 
+/* Initial alpha of a union of n_shapes ellipsoids (spheres: equal semi-axes,
+ * identity rotation), each given by 15 doubles: centre (3), semi-axes (3),
+ * rotation rows (9).  Cells are split into tets that are refined 1:8 near the
+ * surface down to max_depth levels, where the linear interpolant of the shape
+ * function is integrated exactly.  Writes the handle's alpha. */
+int svp_harness_init_alpha(svp_handle* h, const double* shapes, int n_shapes, int max_depth);
+
 /* max over cells of sum|phi|/V, and of the same restricted to
  * 0.01 <= alpha <= 0.99 (CourantNo.H, alphaCourantNo.H:95-103), from the
  * current phi and alpha; multiply by 0.5*dt for the Courant numbers.

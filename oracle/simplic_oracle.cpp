@@ -1164,9 +1164,15 @@ struct Solver {
         correctedFaces.clear();
         std::vector<int> downwindFaces, facesToPass;
         std::vector<double> dVfmax, phiL;
+        // schedule statistics (test diagnostics): candidates, cells that write
+        // corrections, and the depth of the write-dependency chain
+        std::vector<int> depth(m->nC, 0);
+        int nCandStat = 0, nWriters = 0, maxDepth = 0;
         for (int celli = 0; celli < m->nC; ++celli) {
             if (!nextToInterface[celli]) continue;
             if (!(alpha[celli] < -aTol || alpha[celli] > 1.0 + aTol)) continue;
+            nCandStat++;
+            bool wrote = false;
             const double Vi = m->V[celli];
             double alphaOvershoot = pos0(alpha[celli] - 1.0) * (alpha[celli] - 1.0) +
                                     neg0(alpha[celli]) * alpha[celli];
@@ -1202,6 +1208,7 @@ struct Solver {
                     double dVff = faceValue(corr, facei);
                     dVff += sgn(phiL[fi]) * sgn(fluidToPassOn) * fluidToPassThroughFace;
                     if (m->isInternal(facei) || !m->boundaryEmpty(facei)) corr[facei] = dVff;
+                    wrote = true;
                     if (firstLoop) correctedFaces.push_back(facei);
                 }
                 firstLoop = false;
@@ -1213,8 +1220,25 @@ struct Solver {
                 alphaOvershoot = pos0(alpha1New - 1.0) * (alpha1New - 1.0) + neg0(alpha1New) * alpha1New;
                 fluidToPassOn = alphaOvershoot * Vi;
             }
+            if (wrote) {
+                // longest chain of writers j < celli upwind of celli
+                int dmax = 0;
+                for (int f : m->cells[celli]) {
+                    if (!m->isInternal(f)) continue;
+                    const int o = m->own[f] == celli ? m->nei[f] : m->own[f];
+                    const bool upwindOfCell = (m->own[f] == celli) ? (phi[f] < 0) : (phi[f] >= 0);
+                    if (o < celli && upwindOfCell) dmax = std::max(dmax, depth[o]);
+                }
+                depth[celli] = dmax + 1;
+                nWriters++;
+                maxDepth = std::max(maxDepth, depth[celli]);
+            }
         }
+        schedStats.push_back(nCandStat);
+        schedStats.push_back(nWriters);
+        schedStats.push_back(maxDepth);
     }
+    std::vector<int> schedStats;
 
     // advection.C:54-82
     void extendMarkedCells(std::vector<char>& marked) const {
@@ -1238,6 +1262,7 @@ struct Solver {
         double maxAlphaMinus1 = gMax() - 1.0;
         double minAlpha = gMin();
         boundLog.clear();
+        schedStats.clear();
         std::vector<double> corr(m->nF, 0.0);
         std::vector<char> needBounding(m->nC, 0);
         for (int c : mixedCells) needBounding[c] = 1;
@@ -1512,6 +1537,13 @@ int or_bound_log(void* h, double* out, int cap) {
     const int n = std::min(cap, (int)s->boundLog.size());
     for (int i = 0; i < n; ++i) out[i] = s->boundLog[i];
     return (int)s->boundLog.size();
+}
+
+int or_sched_stats(void* h, int* out, int cap) {
+    Solver* s = (Solver*)h;
+    const int n = std::min(cap, (int)s->schedStats.size());
+    for (int i = 0; i < n; ++i) out[i] = s->schedStats[i];
+    return (int)s->schedStats.size();
 }
 
 // Point value of U by volPointInterpolation (exposed for tests)
