@@ -140,11 +140,12 @@ def cpu_baseline(mesh, alpha0, seq, U0, phi0, n_steps, threads):
                        snapTol=ALPHA_DICT["snapTol"], clip=ALPHA_DICT["clip"], threads=threads)
     s.set_alpha(alpha0)
     n_ic = 0
-    for (_, dt, uf) in seq[:n_steps]:
+    for k, (_, dt, uf) in enumerate(seq[:n_steps]):
         s.set_flow(phi0 * uf, U0 * uf)
         s.reconstruct()
         n_ic += len(s.mixed()["cells"])
         s.advect(dt)
+        print(f"[bench] cpu baseline ({threads} threads) step {k + 1}/{n_steps}", file=sys.stderr, flush=True)
     rec, adv = s.timers()
     return n_ic, rec + adv
 
@@ -179,10 +180,14 @@ def main():
     print(f"[bench] {args.workload}: {mesh.n_cells} cells, {mesh.n_faces} faces, {n_total} steps to "
           f"t={wl['end_time']}, setup {setup_s:.1f}s", file=sys.stderr, flush=True)
 
+    def say(msg):
+        print(f"[bench] {msg} ({time.time() - t_setup:.1f}s)", file=sys.stderr, flush=True)
+
     # warm-up: W steps from t=0, then rewind the field
     solver.set_alpha(alpha0)
     run_steps(solver, seq[:W])
     device_sync(solver)
+    say("warm-up done")
 
     # stage breakdown (untimed pass over the first steps) to pick the roofline stage
     solver.set_alpha(alpha0)
@@ -196,6 +201,7 @@ def main():
         stage = max((k for k in stages if algorithmic_bytes(k, 1, 1, 1, wl["kind"]) is not None),
                     key=lambda k: stages[k][0])
 
+    say(f"stage breakdown done, roofline stage {stage}")
     # timed region
     solver.set_alpha(alpha0)
     stage_bit = 1 << g.SolveVofEqu.STAGES.index(stage)
@@ -211,6 +217,7 @@ def main():
     solver.profile_set(0)
     solver.synchronize()
 
+    say(f"timed region done: {K} steps in {wall:.2f}s wall")
     n_ic = nic1 - nic0
     hot_s = (rec1 - rec0) + (adv1 - adv0)
     value = n_ic / hot_s

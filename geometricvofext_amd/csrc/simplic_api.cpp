@@ -452,8 +452,20 @@ private:
             st.insert(st.end(), tmp.begin(), tmp.end());
             stOff[c + 1] = (int)st.size();
         }
+        std::vector<int> cpOff(nC + 1, 0), cPtsFlat;
+        cPtsFlat.reserve((size_t)nC * 8);
+        for (int c = 0; c < nC; ++c) {
+            cPtsFlat.insert(cPtsFlat.end(), cellPts[c].begin(), cellPts[c].end());
+            cpOff[c + 1] = (int)cPtsFlat.size();
+        }
         cellPts.clear();
         cellPts.shrink_to_fit();
+        m_.cpOff = dupload(cpOff);
+        m_.cPts = dupload(cPtsFlat);
+        m_.maxCellPoints = maxCellPts;
+        m_.triQuadFaces = 1;
+        for (int f = 0; f < nF; ++f)
+            if (fOff[f + 1] - fOff[f] > 4) m_.triQuadFaces = 0;
         // point interpolation sources: boundary points -> their non-empty
         // boundary faces (ascending); other points -> their cells (ascending)
         std::vector<std::vector<int>> bFaces(nP);
@@ -536,6 +548,7 @@ private:
         st_.touched = dalloc<int>(nC);
         st_.touchedFlag = dalloc<int>(nC);
         st_.boundStats = dalloc<int>(4 * MAX_BOUND_PASSES);
+        st_.bcount = dalloc<int>(16);
         st_.maxCellFaces = maxCellFaces_;
         st_.minmax = dalloc<double>(4);
         st_.errFlag = dalloc<int>(1);

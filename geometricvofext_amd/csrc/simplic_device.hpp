@@ -237,6 +237,136 @@ __device__ inline bool calcSubFace(const V3* fPts, int n, V3 normal, double dist
     return ok;
 }
 
+// Element j of a register array of N values, as a select chain (keeps the
+// array in registers where a dynamic index would spill it to scratch).
+template <int N, class T>
+__device__ inline T pick(const T (&a)[N], int j) {
+    T r = a[0];
+#pragma unroll
+    for (int k = 1; k < N; ++k)
+        if (j == k) r = a[k];
+    return r;
+}
+
+// calcSubFace for a face of exactly N points held in registers; the same
+// operations in the same order as calcSubFace above.
+template <int N>
+__device__ inline bool calcSubFaceN(const V3 (&fPts)[N], V3 normal, double distance, FaceCut& out) {
+    double dist[N];
+    int nSub = 0, first = -1;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        double di = dot(fPts[i], normal) + distance;
+        if (fabs(di) < SVP_TSMALL) di += sgn(di) * SVP_TSMALL;
+        dist[i] = di;
+        if (di < 0.0) {
+            nSub++;
+            if (first == -1) first = i;
+        }
+    }
+    out.nIf = 0;
+    if (nSub == N || nSub == 0) {
+        if (nSub == N) {
+            out.status = -1;
+            out.centre = faceCentre(fPts, N);
+            out.area = faceAreaNormal(fPts, N);
+        } else {
+            out.status = 1;
+            out.centre = VZERO;
+            out.area = VZERO;
+        }
+        return true;
+    }
+    bool ok = true;
+    int nPts = 0;
+    V3 p0 = VZERO, p1 = VZERO, p2 = VZERO, fCentre = VZERO;
+    auto emit1 = [&](V3 q) {
+        if (nPts == 0) fCentre = q; else fCentre = fCentre + q;
+        if (nPts == 0) p0 = q; else if (nPts == 1) p1 = q; else if (nPts == 2) p2 = q;
+        nPts++;
+    };
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        int cur = first + k;
+        if (cur >= N) cur -= N;
+        const int nxt = (cur + 1 == N) ? 0 : cur + 1;
+        const double dc = pick(dist, cur), dn = pick(dist, nxt);
+        const V3 pc = pick(fPts, cur), pn = pick(fPts, nxt);
+        if (dc < 0) emit1(pc);
+        if ((dc * dn) < 0) {
+            const double w = dc / (dc - dn);
+            const V3 q = pc + w * (pn - pc);
+            emit1(q);
+#pragma unroll
+            for (int s = 0; s < MAX_IF_POINTS; ++s)
+                if (out.nIf == s) out.ifPts[s] = q;
+            if (out.nIf >= MAX_IF_POINTS) ok = false;
+            out.nIf++;
+        }
+    }
+    if (out.nIf > MAX_IF_POINTS) out.nIf = MAX_IF_POINTS;
+    if (nPts < 3) {
+        out.status = -1;
+        out.centre = faceCentre(fPts, N);
+        out.area = faceAreaNormal(fPts, N);
+        return ok;
+    }
+    out.status = 0;
+    if (nPts == 3) {
+        out.centre = (1.0 / 3.0) * (p0 + p1 + p2);
+        out.area = 0.5 * cross(p1 - p0, p2 - p0);
+        return ok;
+    }
+    fCentre = fCentre / (double)nPts;
+    V3 sumN = VZERO, sumAc = VZERO;
+    double sumA = 0.0;
+    V3 prev = VZERO, firstP = VZERO;
+    int kk = 0;
+    auto emit2 = [&](V3 q) {
+        if (kk == 0) {
+            firstP = q;
+        } else {
+            const V3 c = prev + q + fCentre;
+            const V3 nn = cross(q - prev, fCentre - prev);
+            const double ar = mag(nn);
+            sumN = sumN + nn;
+            sumA += ar;
+            sumAc = sumAc + ar * c;
+        }
+        prev = q;
+        kk++;
+    };
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        int cur = first + k;
+        if (cur >= N) cur -= N;
+        const int nxt = (cur + 1 == N) ? 0 : cur + 1;
+        const double dc = pick(dist, cur), dn = pick(dist, nxt);
+        const V3 pc = pick(fPts, cur), pn = pick(fPts, nxt);
+        if (dc < 0) emit2(pc);
+        if ((dc * dn) < 0) {
+            const double w = dc / (dc - dn);
+            emit2(pc + w * (pn - pc));
+        }
+    }
+    {
+        const V3 c = prev + firstP + fCentre;
+        const V3 nn = cross(firstP - prev, fCentre - prev);
+        const double ar = mag(nn);
+        sumN = sumN + nn;
+        sumA += ar;
+        sumAc = sumAc + ar * c;
+    }
+    if (sumA < SVP_ROOTVSMALL) {
+        out.centre = fCentre;
+        out.area = VZERO;
+    } else {
+        out.centre = (1.0 / 3.0) * sumAc / sumA;
+        out.area = 0.5 * sumN;
+    }
+    return ok;
+}
+
 // Local description of one cell: its points and faces as point-index lists
 // (cutCell::getLocalPointFieldAndFaceList, cutCell.C:140-236).  Without
 // splitWarpedFace the faces are the mesh faces in their stored orientation.

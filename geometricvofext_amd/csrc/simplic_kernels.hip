@@ -463,8 +463,168 @@ __global__ void __launch_bounds__(64) k_find_signed_distance(DevMesh m, DevState
     }
 }
 
+// findSignedDistance with one 8-lane group per mixed cell and one lane per
+// cell face (cells of at most 8 faces and 16 points, faces as stored).  A
+// plane cut = every lane cuts its face (points in registers), writes the face
+// record to LDS, and every lane of the group combines the records in face
+// order, so all lanes hold the same VOF and follow the same bracketing and
+// Newton steps as the one-lane version.  The vertex levels are sorted by rank
+// in LDS.  One wave per workgroup: LDS traffic of a wave is in order.
+constexpr int FSD_G = 8;
+
+__device__ inline void fsd_face_cut(int fsz, const V3 (&q4)[4], const V3 (&q3)[3], V3 n, double dist,
+                                    FaceCut& fc, bool& ok) {
+    if (fsz == 4) ok &= calcSubFaceN<4>(q4, n, dist, fc);
+    else ok &= calcSubFaceN<3>(q3, n, dist, fc);
+}
+
+__global__ void __launch_bounds__(64) k_fsd_group(DevMesh m, DevState st) {
+    __shared__ FaceCut rec[64 / FSD_G][FSD_G];
+    __shared__ double vraw[64 / FSD_G][NS_CELL_POINTS];
+    __shared__ double vsrt[64 / FSD_G][NS_CELL_POINTS];
+    const int g = threadIdx.x / FSD_G, l = threadIdx.x % FSD_G;
+    const int nMixed = *st.nMixed;
+    for (int i = blockIdx.x * (64 / FSD_G) + g; i < nMixed; i += gridDim.x * (64 / FSD_G)) {
+        const int c = st.mixed[i];
+        const int fb = m.cOff[c], nF = m.cOff[c + 1] - fb;
+        const V3 normalI = ld3(st.ifN, i);
+        const double alphaI = st.alpha[c];
+        const V3 Cc = ld3(m.C, c);
+        const double Vc = m.V[c];
+        if (mag(normalI) < SVP_TSMALL) {
+            if (l == 0) {
+                st.status[i] = (int)sgn(0.5 - alphaI);
+                st.nCutPts[i] = 0;
+                st.faceClass[i] = 0ull;
+                st.ifD[i] = 0.0;
+                st3(st.ifC, i, VZERO);
+                st3(st.ifS, i, VZERO);
+            }
+            continue;
+        }
+        // this lane's face in registers (faces of 3 or 4 points)
+        int fsz = 0;
+        V3 q4[4], q3[3];
+        if (l < nF) {
+            const int f = m.cFaces[fb + l];
+            const int s0 = m.fOff[f];
+            fsz = m.fOff[f + 1] - s0;
+            if (fsz == 4) {
+#pragma unroll
+                for (int k = 0; k < 4; ++k) q4[k] = ld3(m.pts, m.fPts[s0 + k]);
+            } else {
+#pragma unroll
+                for (int k = 0; k < 3; ++k) q3[k] = ld3(m.pts, m.fPts[s0 + k]);
+            }
+        }
+        // vertex levels -n.x of the cell points, sorted descending by rank
+        const int pb = m.cpOff[c], np = m.cpOff[c + 1] - pb;
+        double va = 0.0, vb = 0.0;
+        if (l < np) va = -dot(normalI, ld3(m.pts, m.cPts[pb + l]));
+        if (l + FSD_G < np) vb = -dot(normalI, ld3(m.pts, m.cPts[pb + l + FSD_G]));
+        if (l < np) vraw[g][l] = va;
+        if (l + FSD_G < np) vraw[g][l + FSD_G] = vb;
+        __builtin_amdgcn_wave_barrier();
+        {
+            int ra = 0, rb = 0;
+            for (int j = 0; j < np; ++j) {
+                const double v = vraw[g][j];
+                ra += (v > va) || (v == va && j < l);
+                rb += (v > vb) || (v == vb && j < l + FSD_G);
+            }
+            if (l < np) vsrt[g][ra] = va;
+            if (l + FSD_G < np) vsrt[g][rb] = vb;
+        }
+        __builtin_amdgcn_wave_barrier();
+        bool ok = true;
+        auto cut = [&](double dist, CellCut& cc) {
+            if (l < nF) {
+                FaceCut fc;
+                fsd_face_cut(fsz, q4, q3, normalI, dist, fc, ok);
+                rec[g][l] = fc;
+            }
+            __builtin_amdgcn_wave_barrier();
+            combineSubCell(&rec[g][0], nF, Cc, Vc, cc);
+            __builtin_amdgcn_wave_barrier();
+        };
+        // cutCell::findSignedDistance (cutCell.C:611-799)
+        double lowDistance = vsrt[g][0], upDistance = vsrt[g][np - 1];
+        int lowLabel = 0, upLabel = np - 1;
+        double lowAlpha = 0.0, upAlpha = 1.0;
+        CellCut cc;
+        bool done = false;
+        double dOut = 0.0;
+        while ((upLabel - lowLabel) > 1) {
+            const int midLabel = (int)round(0.5 * (double)(upLabel + lowLabel));
+            const double midDistance = vsrt[g][midLabel];
+            cut(midDistance, cc);
+            const double midAlpha = cc.VOF;
+            if (fabs(midAlpha - alphaI) < SVP_TSMALL) {
+                dOut = midDistance;
+                done = true;
+                break;
+            }
+            if (midAlpha > alphaI) {
+                upLabel = midLabel;
+                upDistance = midDistance;
+                upAlpha = midAlpha;
+            } else {
+                lowLabel = midLabel;
+                lowDistance = midDistance;
+                lowAlpha = midAlpha;
+            }
+        }
+        if (!done && fabs(lowDistance - upDistance) < SVP_TSMALL) {
+            dOut = 0.5 * (lowDistance + upDistance);
+            cut(dOut, cc);
+            done = true;
+        }
+        if (!done) {
+            const double alphaPrismatoid = upAlpha - lowAlpha;
+            const double deltaDistance = (upDistance - lowDistance) / 3.0;
+            cut(lowDistance + deltaDistance, cc);
+            const double alphaOneThird = cc.VOF - lowAlpha;
+            cut(lowDistance + 2.0 * deltaDistance, cc);
+            const double alphaTwoThirds = cc.VOF - lowAlpha;
+            const double a = 13.5 * alphaOneThird - 13.5 * alphaTwoThirds + 4.5 * alphaPrismatoid;
+            const double b = -22.5 * alphaOneThird + 18.0 * alphaTwoThirds - 4.5 * alphaPrismatoid;
+            const double cq = 9.0 * alphaOneThird - 4.5 * alphaTwoThirds + 1.0 * alphaPrismatoid;
+            const double d = lowAlpha - alphaI;
+            double lambda = 0.5;
+            for (int iter = 0; iter < 100; iter++) {
+                const double func = a * (lambda * lambda * lambda) + b * (lambda * lambda) + cq * lambda + d;
+                const double funcPrime = 3.0 * a * (lambda * lambda) + 2.0 * b * lambda + cq;
+                const double lambdaNew = lambda - (func / funcPrime);
+                if (fabs(lambdaNew - lambda) < SVP_TSMALL) break;
+                lambda = lambdaNew;
+            }
+            dOut = lowDistance - lambda * (lowDistance - upDistance);
+            cut(dOut, cc);
+        }
+        if (l == 0) {
+            unsigned long long fcls = 0ull;
+            int ncp = 0;
+            for (int k = 0; k < nF; ++k) {
+                const int s2 = rec[g][k].status;
+                fcls |= (unsigned long long)(s2 + 1) << (2 * k);
+                if (s2 == 0) ncp += rec[g][k].nIf;
+            }
+            st.status[i] = cc.status;
+            st.nCutPts[i] = ncp;
+            st.faceClass[i] = fcls;
+            st.ifD[i] = dOut;
+            st3(st.ifC, i, cc.ifCentre);
+            st3(st.ifS, i, cc.ifArea);
+        }
+        if (!ok) atomicOr(st.errFlag, ERR_TOO_MANY_CROSSINGS);
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
 void launch_find_signed_distance(const DevMesh& m, const DevState& st, const Params& p, hipStream_t s) {
     if (p.splitWarpedFace) k_find_signed_distance<CellPolySP><<<1024, 64, 0, s>>>(m, st, 1);
+    else if (st.maxCellFaces <= FSD_G && m.maxCellPoints <= NS_CELL_POINTS && m.triQuadFaces)
+        k_fsd_group<<<2048, 64, 0, s>>>(m, st);
     else k_find_signed_distance<CellPolyNS><<<1024, 64, 0, s>>>(m, st, 0);
 }
 
@@ -883,193 +1043,313 @@ __device__ inline void bound_cell(const DevMesh& m, const DevState& st, int c, d
         if (down[k] && writable[k]) st.corr[fid[k]] = cor[k];
 }
 
-// limitFlux (advectionTemplates.C:118-215) for all passes in one workgroup.
-// Per pass: ascending candidates from the bitmap; the first redistribution
-// loop of every candidate in parallel (it fixes correctedFaces and its order);
-// the full boundFlux of the cells that write, in rounds over the DAG "lower
-// numbered upwind neighbour that writes"; corrections applied per cell in
-// correctedFaces order; bitmap and bound count updated for the touched cells.
-template <int MF>
-__global__ void __launch_bounds__(1024) k_bound_all(DevMesh m, DevState st, Params p, double dt,
-                                                    const double* Sp, const double* Su) {
-    __shared__ int lds[1024];
-    __shared__ int sN, sQ, sNext, sRound, sTouched, sOob, sActive;
-    const int t = threadIdx.x;
-    const double aTol = 100.0 * SVP_SMALL;
-    const double rDeltaT = 1.0 / dt;
-    const int nWords = (m.nC + 63) / 64;
-    if (t == 0) sOob = *st.oobCount;
+// limitFlux (advectionTemplates.C:118-215), one pass = the kernels below.
+// The pass runs only while cells lie outside [-aTol, 1+aTol] (bound count of
+// the update kernel, kept current by the apply kernel):
+//   count/scan/write  ascending candidates from the bitmap
+//   prep              first redistribution loop of every candidate (it
+//                     decides which cells write and opens correctedFaces)
+//   scan2/scatter     correctedFaces positions, predecessor counts, frontier
+//   round r           full boundFlux of the ready cells; successors whose
+//                     last predecessor completed join the next frontier
+//   tail              the remaining rounds inside one workgroup
+//   touch/apply       corrections applied per cell in correctedFaces order,
+//                     dVf of the corrected faces, bitmap and bound count
+//   reset             dVfCorrectionValues = 0 for the next pass
+// bcount: [0] candidates, [1] corrected faces, [2] touched cells, [3+r]
+// frontier size of round r.
+constexpr int BND_GRID_ROUNDS = 4;
+constexpr int BND_CTRL = 3 + BND_GRID_ROUNDS + 2;
+
+__global__ void __launch_bounds__(256) k_bnd_count(DevState st, int nWords) {
+    __shared__ int lds[4];
+    if (*st.oobCount == 0) return;
+    const int w = blockIdx.x * 256 + threadIdx.x;
+    const int c = (w < nWords) ? __popcll(st.candBits[w]) : 0;
+    int s = c;
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = s;
     __syncthreads();
-    const int nPasses = p.nAlphaBounds < MAX_BOUND_PASSES ? p.nAlphaBounds : MAX_BOUND_PASSES;
-    for (int pass = 0; pass < nPasses; ++pass) {
-        if (sOob == 0) break;
-        // ascending candidate list from the bitmap
-        const int w0 = (int)((long)nWords * t / 1024), w1 = (int)((long)nWords * (t + 1) / 1024);
-        int cnt = 0;
-        for (int w = w0; w < w1; ++w) cnt += __popcll(st.candBits[w]);
-        int nCand;
-        int off = wg_exclusive_scan(cnt, lds, nCand);
-        for (int w = w0; w < w1; ++w) {
-            unsigned long long bits = st.candBits[w];
-            while (bits) {
-                const int bit = __ffsll((long long)bits) - 1;
-                bits &= bits - 1;
-                st.cand[off++] = w * 64 + bit;
-            }
-        }
+    if (threadIdx.x == 0) st.blockCount[blockIdx.x] = lds[0] + lds[1] + lds[2] + lds[3];
+}
+
+// exclusive scan of counts[0..n) in one workgroup; total -> *out.  With a
+// zero bound count the pass is off: total 0.
+__global__ void __launch_bounds__(1024) k_bnd_scan(int* counts, int n, int* out, const int* gate, int* zero,
+                                                   int nZero) {
+    __shared__ int lds[1024];
+    __shared__ int carry;
+    const int t = threadIdx.x;
+    for (int i = t; i < nZero; i += 1024) zero[i] = 0;
+    if (gate && *gate == 0) {
+        if (t == 0) *out = 0;
+        return;
+    }
+    if (t == 0) carry = 0;
+    __syncthreads();
+    for (int base = 0; base < n; base += 1024) {
+        const int i = base + t;
+        const int v = (i < n) ? counts[i] : 0;
+        int tot;
+        const int ex = wg_exclusive_scan(v, lds, tot);
+        if (i < n) counts[i] = carry + ex;
         __syncthreads();
-        // first loop of every candidate; active cells get their index
-        int faces[MF];
-        for (int i = t; i < nCand; i += 1024) {
-            const int c = st.cand[i];
-            const int n = bound_first_faces(m, st, c, dt, nullptr);
-            st.nFirst[i] = n;
-            st.active[i] = n > 0 ? 1 : 0;
-            if (n > 0) st.actIdx[c] = i;
-        }
-        __syncthreads();
-        // correctedFaces positions: exclusive scan of nFirst in candidate order
-        const int i0 = (int)((long)nCand * t / 1024), i1 = (int)((long)nCand * (t + 1) / 1024);
-        int part = 0;
-        for (int i = i0; i < i1; ++i) part += st.nFirst[i];
-        int nCorr;
-        int pos = wg_exclusive_scan(part, lds, nCorr);
-        for (int i = i0; i < i1; ++i) {
-            if (st.nFirst[i] == 0) continue;
-            const int n = bound_first_faces(m, st, st.cand[i], dt, faces);
-            for (int k = 0; k < n; ++k) {
-                st.corrPos[faces[k]] = pos + k;
-                st.corrList[pos + k] = faces[k];
-            }
-            pos += n;
-        }
-        // predecessor counts and the first frontier
-        if (t == 0) { sQ = 0; sNext = 0; sRound = 0; sTouched = 0; sActive = 0; }
-        __syncthreads();
-        for (int i = t; i < nCand; i += 1024) {
-            if (!st.active[i]) continue;
-            atomicAdd(&sActive, 1);
-            const int c = st.cand[i];
-            int np = 0;
-            for (int k = m.cOff[c]; k < m.cOff[c + 1]; ++k) {
-                const int f = m.cFaces[k];
-                if (f >= m.nIF || face_downwind_of(m, st.phi, f, c)) continue;
-                const int j = (m.own[f] == c) ? m.nbr[f] : m.own[f];
-                if (j < c && st.actIdx[j] >= 0) np++;
-            }
-            st.pend[i] = np;
-            if (np == 0) st.queue[atomicAdd(&sQ, 1)] = i;
-        }
-        __syncthreads();
-        // rounds over the dependency DAG (edges point to higher cell numbers,
-        // so every round completes at least one cell until none is left)
-        int cur = 0;
-        while (true) {
-            const int qn = sQ;
-            if (qn == 0) break;
-            int* q = st.queue + (long)cur * m.nC;
-            int* qnext = st.queue + (long)(cur ^ 1) * m.nC;
-            for (int k = t; k < qn; k += 1024) {
-                const int i = q[k];
-                const int c = st.cand[i];
-                bound_cell<MF>(m, st, c, dt, rDeltaT, Sp, Su);
-                for (int kk = m.cOff[c]; kk < m.cOff[c + 1]; ++kk) {
-                    const int f = m.cFaces[kk];
-                    if (f >= m.nIF || !face_downwind_of(m, st.phi, f, c)) continue;
-                    const int d = (m.own[f] == c) ? m.nbr[f] : m.own[f];
-                    if (d < c) continue;
-                    const int a = st.actIdx[d];
-                    if (a < 0) continue;
-                    if (atomicSub(&st.pend[a], 1) == 1) qnext[atomicAdd(&sNext, 1)] = a;
-                }
-            }
-            __threadfence_block();
-            __syncthreads();
-            if (t == 0) {
-                sQ = sNext;
-                sNext = 0;
-                sRound++;
-            }
-            cur ^= 1;
-            __syncthreads();
-        }
-        // cells that receive corrections
-        for (int q2 = t; q2 < nCorr; q2 += 1024) {
-            const int f = st.corrList[q2];
-            const int cs[2] = {m.own[f], f < m.nIF ? m.nbr[f] : -1};
-            for (int s2 = 0; s2 < 2; ++s2) {
-                if (cs[s2] < 0) continue;
-                if (atomicCAS(&st.touchedFlag[cs[s2]], 0, 1) == 0) st.touched[atomicAdd(&sTouched, 1)] = cs[s2];
-            }
-        }
-        __syncthreads();
-        // apply in correctedFaces order per cell; refresh bound count and bitmap
-        const int nT = sTouched;
-        for (int k = t; k < nT; k += 1024) {
-            const int c = st.touched[k];
-            int kpos[MF];
-            double kval[MF];
-            bool kown[MF];
-            int nk = 0;
-            for (int kk = m.cOff[c]; kk < m.cOff[c + 1]; ++kk) {
-                const int f = m.cFaces[kk];
-                const int ps = st.corrPos[f];
-                if (ps < 0) continue;
-                // insertion by position
-                int j = nk - 1;
-                while (j >= 0 && kpos[j] > ps) {
-                    kpos[j + 1] = kpos[j];
-                    kval[j + 1] = kval[j];
-                    kown[j + 1] = kown[j];
-                    --j;
-                }
-                kpos[j + 1] = ps;
-                kval[j + 1] = face_value(m, st.corr, f);
-                kown[j + 1] = (m.own[f] == c);
-                nk++;
-            }
-            const double before = st.alpha[c];
-            double a = before;
-            for (int q2 = 0; q2 < nk; ++q2) a = kown[q2] ? a - kval[q2] / m.V[c] : a + kval[q2] / m.V[c];
-            st.alpha[c] = a;
-            const int dOob = (int)loop_oob(a, aTol) - (int)loop_oob(before, aTol);
-            if (dOob) atomicAdd(&sOob, dOob);
-            const bool candNew = cell_oob(a, aTol) && need_bounding(m, st.mixedFlag, c);
-            const unsigned long long bit = 1ull << (c & 63);
-            if (candNew) atomicOr(&st.candBits[c >> 6], bit);
-            else atomicAnd(&st.candBits[c >> 6], ~bit);
-            st.touchedFlag[c] = 0;
-        }
-        // dVf of the corrected faces (each face appears once: only the cell it
-        // is downwind of corrects it)
-        for (int q2 = t; q2 < nCorr; q2 += 1024) {
-            const int f = st.corrList[q2];
-            if (f < m.nIF || !m.fEmpty[f]) st.dVf[f] = face_value(m, st.dVf, f) + face_value(m, st.corr, f);
-        }
-        __syncthreads();
-        // dVfCorrectionValues = 0 and bookkeeping reset for the next pass
-        for (int i = t; i < nCand; i += 1024) {
-            if (!st.active[i]) continue;
-            const int c = st.cand[i];
-            for (int k = m.cOff[c]; k < m.cOff[c + 1]; ++k) {
-                const int f = m.cFaces[k];
-                if (face_downwind_of(m, st.phi, f, c)) {
-                    st.corr[f] = 0.0;
-                    st.corrPos[f] = -1;
-                }
-            }
-            st.actIdx[c] = -1;
-        }
-        if (t == 0 && pass < MAX_BOUND_PASSES) {
-            st.boundStats[4 * pass + 0] = nCand;
-            st.boundStats[4 * pass + 1] = sActive;
-            st.boundStats[4 * pass + 2] = sRound;
-            st.boundStats[4 * pass + 3] = nCorr;
-            *st.candCount = nCand;
-        }
+        if (t == 0) carry += tot;
         __syncthreads();
     }
-    if (t == 0) *st.oobCount = sOob;
+    if (t == 0) *out = carry;
+}
+
+__global__ void __launch_bounds__(256) k_bnd_write(DevState st, int nWords) {
+    __shared__ int lds[256];
+    if (*st.bcount == 0) return;
+    const int w = blockIdx.x * 256 + threadIdx.x;
+    unsigned long long bits = (w < nWords) ? st.candBits[w] : 0ull;
+    // block exclusive scan of popcounts
+    const int c = __popcll(bits);
+    lds[threadIdx.x] = c;
+    __syncthreads();
+    for (int o = 1; o < 256; o <<= 1) {
+        const int x = (threadIdx.x >= o) ? lds[threadIdx.x - o] : 0;
+        __syncthreads();
+        lds[threadIdx.x] += x;
+        __syncthreads();
+    }
+    int off = st.blockCount[blockIdx.x] + lds[threadIdx.x] - c;
+    while (bits) {
+        const int bit = __ffsll((long long)bits) - 1;
+        bits &= bits - 1;
+        st.cand[off++] = w * 64 + bit;
+    }
+}
+
+__global__ void k_bnd_prep(DevMesh m, DevState st, double dt) {
+    const int nCand = st.bcount[0];
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nCand; i += gridDim.x * blockDim.x) {
+        const int c = st.cand[i];
+        const int n = bound_first_faces(m, st, c, dt, nullptr);
+        st.nFirst[i] = n;
+        st.active[i] = n > 0 ? 1 : 0;
+        if (n > 0) st.actIdx[c] = i;
+    }
+}
+
+__global__ void __launch_bounds__(1024) k_bnd_scan2(DevState st) {
+    __shared__ int lds[1024];
+    __shared__ int carry;
+    const int n = st.bcount[0];
+    const int t = threadIdx.x;
+    if (t == 0) carry = 0;
+    __syncthreads();
+    for (int base = 0; base < n; base += 1024) {
+        const int i = base + t;
+        const int v = (i < n) ? st.nFirst[i] : 0;
+        int tot;
+        const int ex = wg_exclusive_scan(v, lds, tot);
+        if (i < n) st.nFirst[i] = carry + ex;
+        __syncthreads();
+        if (t == 0) carry += tot;
+        __syncthreads();
+    }
+    if (t == 0) st.bcount[1] = carry;
+}
+
+template <int MF>
+__global__ void k_bnd_scatter(DevMesh m, DevState st, double dt) {
+    const int nCand = st.bcount[0];
+    int faces[MF];
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nCand; i += gridDim.x * blockDim.x) {
+        if (!st.active[i]) continue;
+        const int c = st.cand[i];
+        const int n = bound_first_faces(m, st, c, dt, faces);
+        const int pos = st.nFirst[i];
+        for (int k = 0; k < n; ++k) {
+            st.corrPos[faces[k]] = pos + k;
+            st.corrList[pos + k] = faces[k];
+        }
+        int np = 0;
+        for (int k = m.cOff[c]; k < m.cOff[c + 1]; ++k) {
+            const int f = m.cFaces[k];
+            if (f >= m.nIF || face_downwind_of(m, st.phi, f, c)) continue;
+            const int j = (m.own[f] == c) ? m.nbr[f] : m.own[f];
+            if (j < c && st.actIdx[j] >= 0) np++;
+        }
+        st.pend[i] = np;
+        if (np == 0) st.queue[atomicAdd(&st.bcount[3], 1)] = i;
+    }
+}
+
+// completes cell i of frontier r and releases its successors into frontier r+1
+template <int MF>
+__device__ inline void bnd_complete(const DevMesh& m, const DevState& st, int i, double dt, double rDeltaT,
+                                    const double* Sp, const double* Su, int* qnext, int* nNext) {
+    const int c = st.cand[i];
+    bound_cell<MF>(m, st, c, dt, rDeltaT, Sp, Su);
+    __threadfence();
+    for (int kk = m.cOff[c]; kk < m.cOff[c + 1]; ++kk) {
+        const int f = m.cFaces[kk];
+        if (f >= m.nIF || !face_downwind_of(m, st.phi, f, c)) continue;
+        const int d = (m.own[f] == c) ? m.nbr[f] : m.own[f];
+        if (d < c) continue;
+        const int a = st.actIdx[d];
+        if (a < 0) continue;
+        if (atomicSub(&st.pend[a], 1) == 1) qnext[atomicAdd(nNext, 1)] = a;
+    }
+}
+
+template <int MF>
+__global__ void k_bnd_round(DevMesh m, DevState st, int r, double dt, const double* Sp, const double* Su) {
+    const int n = st.bcount[3 + r];
+    const int* q = st.queue + (long)(r & 1) * m.nC;
+    int* qnext = st.queue + (long)((r + 1) & 1) * m.nC;
+    const double rDeltaT = 1.0 / dt;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x)
+        bnd_complete<MF>(m, st, q[k], dt, rDeltaT, Sp, Su, qnext, &st.bcount[3 + r + 1]);
+}
+
+// remaining rounds in one workgroup (frontiers are small by now); every round
+// completes at least one cell because edges point to higher cell numbers
+template <int MF>
+__global__ void __launch_bounds__(1024) k_bnd_tail(DevMesh m, DevState st, double dt, const double* Sp,
+                                                   const double* Su) {
+    __shared__ int sQ, sNext, sRounds;
+    const int t = threadIdx.x;
+    const double rDeltaT = 1.0 / dt;
+    if (t == 0) {
+        sQ = st.bcount[3 + BND_GRID_ROUNDS];
+        sNext = 0;
+        sRounds = BND_GRID_ROUNDS;
+    }
+    __syncthreads();
+    int cur = BND_GRID_ROUNDS & 1;
+    while (true) {
+        const int qn = sQ;
+        if (qn == 0) break;
+        const int* q = st.queue + (long)cur * m.nC;
+        int* qnext = st.queue + (long)(cur ^ 1) * m.nC;
+        for (int k = t; k < qn; k += 1024) bnd_complete<MF>(m, st, q[k], dt, rDeltaT, Sp, Su, qnext, &sNext);
+        __syncthreads();
+        if (t == 0) {
+            sQ = sNext;
+            sNext = 0;
+            sRounds++;
+        }
+        cur ^= 1;
+        __syncthreads();
+    }
+    if (t == 0) st.bcount[BND_CTRL - 1] = sRounds;
+}
+
+__global__ void k_bnd_touch(DevMesh m, DevState st) {
+    const int nCorr = st.bcount[1];
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < nCorr; q += gridDim.x * blockDim.x) {
+        const int f = st.corrList[q];
+        const int cs[2] = {m.own[f], f < m.nIF ? m.nbr[f] : -1};
+        for (int s = 0; s < 2; ++s) {
+            if (cs[s] < 0) continue;
+            if (atomicCAS(&st.touchedFlag[cs[s]], 0, 1) == 0) st.touched[atomicAdd(&st.bcount[2], 1)] = cs[s];
+        }
+    }
+}
+
+// alpha of every touched cell in correctedFaces order (advectionTemplates.C:164-192)
+// and the dVf of every corrected face
+template <int MF>
+__global__ void k_bnd_apply(DevMesh m, DevState st) {
+    const double aTol = 100.0 * SVP_SMALL;
+    const int nT = st.bcount[2], nCorr = st.bcount[1];
+    for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < nT + nCorr; idx += gridDim.x * blockDim.x) {
+        if (idx >= nT) {
+            const int f = st.corrList[idx - nT];
+            if (f < m.nIF || !m.fEmpty[f]) st.dVf[f] = face_value(m, st.dVf, f) + face_value(m, st.corr, f);
+            continue;
+        }
+        const int c = st.touched[idx];
+        int kpos[MF];
+        double kval[MF];
+        bool kown[MF];
+        int nk = 0;
+        for (int kk = m.cOff[c]; kk < m.cOff[c + 1]; ++kk) {
+            const int f = m.cFaces[kk];
+            const int ps = st.corrPos[f];
+            if (ps < 0) continue;
+            int j = nk - 1;
+            while (j >= 0 && kpos[j] > ps) {
+                kpos[j + 1] = kpos[j];
+                kval[j + 1] = kval[j];
+                kown[j + 1] = kown[j];
+                --j;
+            }
+            kpos[j + 1] = ps;
+            kval[j + 1] = face_value(m, st.corr, f);
+            kown[j + 1] = (m.own[f] == c);
+            nk++;
+        }
+        const double before = st.alpha[c];
+        double a = before;
+        for (int q = 0; q < nk; ++q) a = kown[q] ? a - kval[q] / m.V[c] : a + kval[q] / m.V[c];
+        st.alpha[c] = a;
+        const int dOob = (int)loop_oob(a, aTol) - (int)loop_oob(before, aTol);
+        if (dOob) atomicAdd(st.oobCount, dOob);
+        const bool candNew = cell_oob(a, aTol) && need_bounding(m, st.mixedFlag, c);
+        const unsigned long long bit = 1ull << (c & 63);
+        if (candNew) atomicOr(&st.candBits[c >> 6], bit);
+        else atomicAnd(&st.candBits[c >> 6], ~bit);
+        st.touchedFlag[c] = 0;
+    }
+}
+
+__global__ void k_bnd_reset(DevMesh m, DevState st, int pass) {
+    const int nCand = st.bcount[0];
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nCand; i += gridDim.x * blockDim.x) {
+        if (!st.active[i]) continue;
+        const int c = st.cand[i];
+        for (int k = m.cOff[c]; k < m.cOff[c + 1]; ++k) {
+            const int f = m.cFaces[k];
+            if (face_downwind_of(m, st.phi, f, c)) {
+                st.corr[f] = 0.0;
+                st.corrPos[f] = -1;
+            }
+        }
+        st.actIdx[c] = -1;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0 && pass < MAX_BOUND_PASSES && nCand > 0) {
+        int nAct = 0;
+        for (int r = 0; r <= BND_GRID_ROUNDS; ++r) nAct += st.bcount[3 + r];
+        st.boundStats[4 * pass + 0] = nCand;
+        st.boundStats[4 * pass + 1] = nAct;
+        st.boundStats[4 * pass + 2] = st.bcount[BND_CTRL - 1];
+        st.boundStats[4 * pass + 3] = st.bcount[1];
+        *st.candCount = nCand;
+    }
+}
+
+template <int MF>
+void launch_bound_pass(const DevMesh& m, const DevState& st, int pass, double dt, const double* Sp,
+                       const double* Su, hipStream_t s) {
+    const int nWords = (m.nC + 63) / 64;
+    const int nbw = cdiv(nWords, 256);
+    const int grid = std::min(1024, cdiv(m.nC, 256));
+    k_bnd_count<<<nbw, 256, 0, s>>>(st, nWords);
+    k_bnd_scan<<<1, 1024, 0, s>>>(st.blockCount, nbw, st.bcount, st.oobCount, st.bcount + 1, BND_CTRL - 1);
+    k_bnd_write<<<nbw, 256, 0, s>>>(st, nWords);
+    k_bnd_prep<<<grid, 256, 0, s>>>(m, st, dt);
+    k_bnd_scan2<<<1, 1024, 0, s>>>(st);
+    k_bnd_scatter<MF><<<grid, 256, 0, s>>>(m, st, dt);
+    for (int r = 0; r < BND_GRID_ROUNDS; ++r) k_bnd_round<MF><<<grid, 256, 0, s>>>(m, st, r, dt, Sp, Su);
+    k_bnd_tail<MF><<<1, 1024, 0, s>>>(m, st, dt, Sp, Su);
+    k_bnd_touch<<<grid, 256, 0, s>>>(m, st);
+    k_bnd_apply<MF><<<grid, 256, 0, s>>>(m, st);
+    k_bnd_reset<<<grid, 256, 0, s>>>(m, st, pass);
+}
+
+void launch_bound(const DevMesh& m, const DevState& st, const Params& p, double dt, const double* Sp,
+                  const double* Su, hipStream_t s) {
+    const int nPasses = std::min(p.nAlphaBounds, MAX_BOUND_PASSES);
+    for (int pass = 0; pass < nPasses; ++pass) {
+        if (st.maxCellFaces <= NS_CELL_FACES) launch_bound_pass<NS_CELL_FACES>(m, st, pass, dt, Sp, Su, s);
+        else launch_bound_pass<MAX_CELL_FACES>(m, st, pass, dt, Sp, Su, s);
+    }
 }
 
 // applyBruteForceBounding (advection.C:291-308)
@@ -1102,8 +1382,7 @@ void launch_advect_stage(int stage, const DevMesh& m, const DevState& st, const 
             k_update<<<nb, DETECT_BLOCK, 0, s>>>(m, st, dt, Sp, Su);
             break;
         case STAGE_BOUND:
-            if (st.maxCellFaces <= NS_CELL_FACES) k_bound_all<NS_CELL_FACES><<<1, 1024, 0, s>>>(m, st, p, dt, Sp, Su);
-            else k_bound_all<MAX_CELL_FACES><<<1, 1024, 0, s>>>(m, st, p, dt, Sp, Su);
+            launch_bound(m, st, p, dt, Sp, Su, s);
             break;
         case STAGE_SNAPCLIP:
             if (p.snapTol > 0.0 || p.clip)

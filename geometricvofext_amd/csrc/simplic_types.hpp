@@ -22,6 +22,11 @@ struct DevMesh {
     // same cells, faces in ascending face order (fvc::surfaceIntegrate order),
     // neighbour faces stored as ~f
     const int* cFacesAsc;
+    // cell -> point CSR (unique points, ascending labels)
+    const int* cpOff;
+    const int* cPts;
+    int maxCellPoints;
+    int triQuadFaces;       // 1 when every face has 3 or 4 points
     // cell-point-cell stencil: the cell first, then the other cells ascending
     const int* stOff;
     const int* st;
@@ -76,6 +81,7 @@ struct DevState {
     int* touched;       // nC, cells whose alpha gets corrections
     int* touchedFlag;   // nC
     int* boundStats;    // [4*maxPasses]: candidates, active, rounds, corrected faces
+    int* bcount;        // [16] per-pass counters of the bounding kernels
     int maxCellFaces;
     double* minmax;     // [4]: min, max before bounding, min, max after
     int* errFlag;       // [1] device error bits
