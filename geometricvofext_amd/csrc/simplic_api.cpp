@@ -550,6 +550,8 @@ private:
         st_.boundStats = dalloc<int>(4 * MAX_BOUND_PASSES);
         st_.bcount = dalloc<int>(16);
         st_.maxCellFaces = maxCellFaces_;
+        st_.un0Fused = (!dict_.split_warped_face && maxCellFaces_ <= 8 && m_.maxCellPoints <= NS_CELL_POINTS &&
+                        m_.triQuadFaces) ? 1 : 0;
         st_.minmax = dalloc<double>(4);
         st_.errFlag = dalloc<int>(1);
         st_.nIcTotal = dalloc<unsigned long long>(1);

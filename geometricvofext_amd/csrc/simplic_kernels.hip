@@ -246,7 +246,7 @@ __device__ inline V3 ls_gradient(const DevMesh& m, const double* alpha, int c) {
     return v3(x[1], x[2], x[3]);
 }
 
-__global__ void k_ls_normal(DevMesh m, DevState st) {
+__global__ void __launch_bounds__(64) k_ls_normal(DevMesh m, DevState st) {
     const int nMixed = *st.nMixed;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nMixed; i += gridDim.x * blockDim.x) {
         const int c = st.mixed[i];
@@ -499,22 +499,28 @@ __global__ void __launch_bounds__(64) k_fsd_group(DevMesh m, DevState st) {
                 st.ifD[i] = 0.0;
                 st3(st.ifC, i, VZERO);
                 st3(st.ifS, i, VZERO);
+                st.un0[i] = 0.0;
             }
             continue;
         }
         // this lane's face in registers (faces of 3 or 4 points)
-        int fsz = 0;
+        int fsz = 0, myFace = -1;
         V3 q4[4], q3[3];
+        int pl[4] = {0, 0, 0, 0};
         if (l < nF) {
             const int f = m.cFaces[fb + l];
+            myFace = f;
             const int s0 = m.fOff[f];
             fsz = m.fOff[f + 1] - s0;
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                if (k < fsz) pl[k] = m.fPts[s0 + k];
             if (fsz == 4) {
 #pragma unroll
-                for (int k = 0; k < 4; ++k) q4[k] = ld3(m.pts, m.fPts[s0 + k]);
+                for (int k = 0; k < 4; ++k) q4[k] = ld3(m.pts, pl[k]);
             } else {
 #pragma unroll
-                for (int k = 0; k < 3; ++k) q3[k] = ld3(m.pts, m.fPts[s0 + k]);
+                for (int k = 0; k < 3; ++k) q3[k] = ld3(m.pts, pl[k]);
             }
         }
         // vertex levels -n.x of the cell points, sorted descending by rank
@@ -618,13 +624,100 @@ __global__ void __launch_bounds__(64) k_fsd_group(DevMesh m, DevState st) {
         }
         if (!ok) atomicOr(st.errFlag, ERR_TOO_MANY_CROSSINGS);
         __builtin_amdgcn_wave_barrier();
+        // Un0 = U(interface centre) . n for a cut cell (advection.C:126):
+        // interpolationCellPoint's tet search with one lane per face
+        double un0 = 0.0;
+        if (cc.status == 0) {
+            const double tol = SVP_SMALL;
+            const V3 pos = cc.ifCentre;
+            bool found = false;
+            double fw[4] = {0, 0, 0, 0}, bw[4] = {0, 0, 0, 0}, bestMin = -1e300;
+            int fp[3] = {0, 0, 0}, bp[3] = {0, 0, 0};
+            if (l < nF) {
+                const bool ownFace = (m.own[myFace] == c);
+#pragma unroll
+                for (int t = 1; t < 3; ++t) {
+                    if (t >= fsz - 1 || found) continue;
+                    int a0 = pl[0], a1 = (t == 1) ? pl[1] : pl[2], a2 = (t == 1) ? pl[2] : pl[3];
+                    V3 b0 = (fsz == 4) ? q4[0] : q3[0];
+                    V3 b1 = (fsz == 4) ? ((t == 1) ? q4[1] : q4[2]) : q3[1];
+                    V3 b2 = (fsz == 4) ? ((t == 1) ? q4[2] : q4[3]) : q3[2];
+                    if (!ownFace) {
+                        const int ti = a1; a1 = a2; a2 = ti;
+                        const V3 tv = b1; b1 = b2; b2 = tv;
+                    }
+                    const V3 e0 = Cc - b2, e1 = b0 - b2, e2 = b1 - b2;
+                    const double detT = dot(e0, cross(e1, e2));
+                    double w[4];
+                    if (fabs(detT) < SVP_SMALL) {
+                        w[0] = w[1] = w[2] = w[3] = -1.0;
+                    } else {
+                        const V3 r = pos - b2;
+                        w[0] = dot(r, cross(e1, e2)) / detT;
+                        w[1] = dot(e0, cross(r, e2)) / detT;
+                        w[2] = dot(e0, cross(e1, r)) / detT;
+                        w[3] = 1.0 - w[0] - w[1] - w[2];
+                    }
+                    const double mn = fmin(fmin(w[0], w[1]), fmin(w[2], w[3]));
+                    if (mn + tol > 0.0) {
+                        found = true;
+                        fw[0] = w[0]; fw[1] = w[1]; fw[2] = w[2]; fw[3] = w[3];
+                        fp[0] = a0; fp[1] = a1; fp[2] = a2;
+                    } else if (mn > bestMin) {
+                        bestMin = mn;
+                        bw[0] = w[0]; bw[1] = w[1]; bw[2] = w[2]; bw[3] = w[3];
+                        bp[0] = a0; bp[1] = a1; bp[2] = a2;
+                    }
+                }
+            }
+            // first face (in cell-face order) with a containing tet, else the
+            // tet of largest minimum weight, earliest on ties
+            const unsigned long long grpMask = 0xFFull << (g * FSD_G);
+            const unsigned long long foundMask = __ballot(found && l < nF) & grpMask;
+            int src = -1;
+            if (foundMask) {
+                src = __ffsll((long long)foundMask) - 1 - g * FSD_G;
+            } else {
+                double best = -1e300;
+                for (int k = 0; k < nF; ++k) {
+                    const double v = __shfl(bestMin, g * FSD_G + k);
+                    if (v > best) {
+                        best = v;
+                        src = k;
+                    }
+                }
+            }
+            double w[4];
+            int pp[3];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) w[q] = __shfl(foundMask ? fw[q] : bw[q], g * FSD_G + src);
+#pragma unroll
+            for (int q = 0; q < 3; ++q) pp[q] = __shfl(foundMask ? fp[q] : bp[q], g * FSD_G + src);
+            // point values: lane q of the group evaluates point q
+            V3 pu = VZERO;
+            if (l < 3) {
+                const int p = (l == 0) ? pp[0] : (l == 1 ? pp[1] : pp[2]);
+                for (int k = m.psOff[p]; k < m.psOff[p + 1]; ++k) {
+                    const int s2 = m.psSrc[k];
+                    const V3 u = (s2 >= 0) ? ld3(st.U, s2) : ld3(st.Ub, (~s2) - m.nIF);
+                    pu = pu + m.psW[k] * u;
+                }
+            }
+            V3 pv[3];
+#pragma unroll
+            for (int q = 0; q < 3; ++q)
+                pv[q] = v3(__shfl(pu.x, g * FSD_G + q), __shfl(pu.y, g * FSD_G + q), __shfl(pu.z, g * FSD_G + q));
+            const V3 Ui = w[0] * ld3(st.U, c) + w[1] * pv[0] + w[2] * pv[1] + w[3] * pv[2];
+            un0 = dot(Ui, normalI);
+        }
+        if (l == 0) st.un0[i] = un0;
+        __builtin_amdgcn_wave_barrier();
     }
 }
 
 void launch_find_signed_distance(const DevMesh& m, const DevState& st, const Params& p, hipStream_t s) {
     if (p.splitWarpedFace) k_find_signed_distance<CellPolySP><<<1024, 64, 0, s>>>(m, st, 1);
-    else if (st.maxCellFaces <= FSD_G && m.maxCellPoints <= NS_CELL_POINTS && m.triQuadFaces)
-        k_fsd_group<<<2048, 64, 0, s>>>(m, st);
+    else if (st.un0Fused) k_fsd_group<<<2048, 64, 0, s>>>(m, st);
     else k_find_signed_distance<CellPolyNS><<<1024, 64, 0, s>>>(m, st, 0);
 }
 
@@ -635,7 +728,7 @@ void launch_find_signed_distance(const DevMesh& m, const DevState& st, const Par
 // dVf = upwind(phi).flux(alpha)*dt (advectionTemplates.C:371): face value
 // lambda*(aP - aN) + aN with lambda = pos0(phi); zeroGradient boundary value
 // is the owner value; empty patches carry nothing.
-__global__ void k_upwind(DevMesh m, DevState st, double dt) {
+__global__ void __launch_bounds__(256) k_upwind(DevMesh m, DevState st, double dt) {
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= m.nF) return;
     const double ph = st.phi[f];
@@ -825,6 +918,166 @@ __device__ inline double time_integrated_face_flux(const DevMesh& m, int f, V3 n
     return dVf;
 }
 
+// timeIntegratedArea for a face of N points in registers: the sorted arrival
+// times are walked once and every accepted sub-interval gets its Simpson term
+// right away; the additions to tIntArea keep the reference's order (leading
+// uncut segment, trailing uncut segment, then the Simpson terms in time order).
+template <int N>
+__device__ inline double tia_regs(const V3 (&P)[N], const double (&ptIn)[N], V3 normal, double distance,
+                                  double Un0, double dt, double magSf, bool& ok) {
+    double pt[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) pt[k] = ptIn[k];
+    // ascending sort (values only): odd-even transposition, static indices
+#pragma unroll
+    for (int r = 0; r < N; ++r)
+#pragma unroll
+        for (int k = r & 1; k + 1 < N; k += 2) {
+            const double a = pt[k], b = pt[k + 1];
+            pt[k] = fmin(a, b);
+            pt[k + 1] = fmax(a, b);
+        }
+    const double firstTime = pt[0], lastTime = pt[N - 1];
+    if (lastTime <= 0.0) return magSf * dt * pos0(Un0);
+    if (firstTime >= dt) return magSf * dt * (1.0 - pos0(Un0));
+    double tIntArea = 0.0;
+    double prevTime, tauOld, subAreaOld;
+    FaceCut fc;
+    if (firstTime > 0.0) {
+        subAreaOld = magSf * (1.0 - pos0(Un0));
+        tIntArea = subAreaOld * firstTime;
+        prevTime = firstTime;
+        tauOld = firstTime;
+    } else {
+        prevTime = 0.0;
+        tauOld = 0.0;
+        ok &= calcSubFaceN<N>(P, normal, distance, fc);
+        subAreaOld = mag(fc.area);
+    }
+    if (!(lastTime > dt)) tIntArea += magSf * (dt - lastTime) * pos0(Un0);
+    const double smallTime = fmax(SVP_TSMALL / fabs(Un0), SVP_TSMALL);
+    auto simpson = [&](double tauNew) {
+        const double deltaTau = 0.5 * (tauNew - tauOld);
+        ok &= calcSubFaceN<N>(P, normal, distance - tauNew * Un0, fc);
+        const double subAreaNew = mag(fc.area);
+        ok &= calcSubFaceN<N>(P, normal, distance - (tauOld + deltaTau) * Un0, fc);
+        const double subAreaMid = mag(fc.area);
+        tIntArea += (deltaTau / 3.0) * (subAreaOld + 4.0 * subAreaMid + subAreaNew);
+        subAreaOld = subAreaNew;
+        tauOld = tauNew;
+    };
+#pragma unroll
+    for (int ti = 0; ti < N; ++ti) {
+        const double timeI = pt[ti];
+        if (timeI > (prevTime + smallTime) && timeI < dt) {
+            simpson(timeI);
+            prevTime = timeI;
+        }
+    }
+    if (lastTime > dt) simpson(dt);
+    return tIntArea;
+}
+
+// timeIntegratedFaceFlux (cutFace.C:262-389) for a face of N points in registers
+template <int N>
+__device__ inline double tiff_regs(const DevMesh& m, int f, const V3 (&p)[N], V3 normal, double distance,
+                                   double Un0, double dt, double phi, double magSf, bool& ok) {
+    if (fabs(phi) <= SVP_TSMALL) return 0.0;
+    const bool flat = m.flat[f] > (1.0 - SVP_TSMALL);
+    if (fabs(Un0 * dt) > SVP_TSMALL) {
+        double pTimes[N];
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            const double t = (dot(p[i], normal) + distance) / Un0;
+            pTimes[i] = fabs(t) < SVP_TSMALL ? 0.0 : t;
+        }
+        if (flat) return phi / magSf * tia_regs<N>(p, pTimes, normal, distance, Un0, dt, magSf, ok);
+        double dVf = 0.0;
+        const V3 fc = ld3(m.Cf, f);
+        const double t0 = (dot(fc, normal) + distance) / Un0;
+        const double tc = fabs(t0) < SVP_TSMALL ? 0.0 : t0;
+#pragma unroll 1
+        for (int pi = 0; pi < N; ++pi) {
+            const int pn = (pi + 1 == N) ? 0 : pi + 1;
+            const V3 tri[3] = {fc, pick(p, pi), pick(p, pn)};
+            const double tt[3] = {tc, pick(pTimes, pi), pick(pTimes, pn)};
+            const double magSfTri = mag(triAreaNormal(tri[0], tri[1], tri[2]));
+            const double phiTri = phi * magSfTri / magSf;
+            dVf += phiTri / magSfTri * tia_regs<3>(tri, tt, normal, distance, Un0, dt, magSfTri, ok);
+        }
+        return dVf;
+    }
+    FaceCut fcut;
+    if (flat) {
+        ok &= calcSubFaceN<N>(p, normal, distance, fcut);
+        const double alphaf = mag(fcut.area) / magSf;
+        return (phi * dt * alphaf);
+    }
+    const V3 fc = ld3(m.Cf, f);
+    double dVf = 0.0;
+#pragma unroll
+    for (int pi = 0; pi < N; ++pi) {
+        const V3 tri[3] = {fc, p[pi], p[(pi + 1) % N]};
+        const double magSfTri = mag(triAreaNormal(tri[0], tri[1], tri[2]));
+        const double phiTri = phi * magSfTri / magSf;
+        ok &= calcSubFaceN<3>(tri, normal, distance, fcut);
+        const double alphafTri = mag(fcut.area) / magSfTri;
+        dVf += (phiTri * dt * alphafTri);
+    }
+    return dVf;
+}
+
+// Un0 = U(interface centre) . n of every cut cell (advection.C:126)
+__global__ void __launch_bounds__(64) k_un0(DevMesh m, DevState st) {
+    const int nMixed = *st.nMixed;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nMixed; i += gridDim.x * blockDim.x) {
+        double u = 0.0;
+        if (st.status[i] == 0) u = dot(interpolate_u(m, st, ld3(st.ifC, i), st.mixed[i]), ld3(st.ifN, i));
+        st.un0[i] = u;
+    }
+}
+
+// timeIntegratedFlux with one thread per (cut cell, cell face): each downwind
+// face's volume is independent of the others
+template <bool TRIQUAD>
+__global__ void __launch_bounds__(64) k_interface_flux_faces(DevMesh m, DevState st, double dt, int slots) {
+    const int nMixed = *st.nMixed;
+    const long total = (long)nMixed * slots;
+    for (long w = (long)blockIdx.x * blockDim.x + threadIdx.x; w < total; w += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(w / slots), k = (int)(w % slots);
+        if (st.status[i] != 0) continue;
+        const int c = st.mixed[i];
+        const int fb = m.cOff[c];
+        if (k >= m.cOff[c + 1] - fb) continue;
+        const int f = m.cFaces[fb + k];
+        const double ph = st.phi[f];
+        bool down;
+        if (f < m.nIF) down = (m.own[f] == c) ? (ph >= 0.0) : (ph < 0.0);
+        else down = !m.fEmpty[f] && (ph >= 0.0);
+        if (!down) continue;
+        const V3 n = ld3(st.ifN, i);
+        const double d = st.ifD[i], Un0 = st.un0[i];
+        const int s0 = m.fOff[f], fsz = m.fOff[f + 1] - s0;
+        bool ok = true;
+        double v;
+        if (fsz == 4) {
+            V3 p[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) p[q] = ld3(m.pts, m.fPts[s0 + q]);
+            v = tiff_regs<4>(m, f, p, n, d, Un0, dt, ph, m.magSf[f], ok);
+        } else if (TRIQUAD || fsz == 3) {
+            V3 p[3];
+#pragma unroll
+            for (int q = 0; q < 3; ++q) p[q] = ld3(m.pts, m.fPts[s0 + q]);
+            v = tiff_regs<3>(m, f, p, n, d, Un0, dt, ph, m.magSf[f], ok);
+        } else {
+            v = time_integrated_face_flux(m, f, n, d, Un0, dt, ph, m.magSf[f], ok);
+        }
+        st.dVf[f] = v;
+        if (!ok) atomicOr(st.errFlag, ERR_TOO_MANY_CROSSINGS);
+    }
+}
+
 // advection::timeIntegratedFlux (advection.C:85-221): each downwind face of a
 // cut cell gets its time-integrated volume; a face is downwind of exactly one
 // cell, so the writes never collide.
@@ -974,64 +1227,82 @@ __device__ inline int bound_first_faces(const DevMesh& m, const DevState& st, in
 template <int MF>
 __device__ inline void bound_cell(const DevMesh& m, const DevState& st, int c, double dt, double rDeltaT,
                                   const double* Sp, const double* Su) {
+    // every per-face array is indexed by the unrolled face slot only, so the
+    // whole cell stays in registers
     const double aTol = 100.0 * SVP_SMALL;
     const int b = m.cOff[c], e = m.cOff[c + 1];
     const int nf = e - b;
+    const double Vi = m.V[c];
+    const double ac = st.alpha[c];
+    const double aOld = st.alphaOld[c];
     int fid[MF];
     bool isOwn[MF], down[MF], writable[MF];
     double phf[MF], dvf[MF], cor[MF];
-    for (int k = 0; k < nf; ++k) {
-        const int f = m.cFaces[b + k];
-        fid[k] = f;
-        isOwn[k] = (m.own[f] == c);
-        phf[k] = face_value(m, st.phi, f);
-        dvf[k] = face_value(m, st.dVf, f);
-        down[k] = isOwn[k] ? (phf[k] >= 0) : (phf[k] < 0);
-        writable[k] = (f < m.nIF) || !m.fEmpty[f];
-        double cv = 0.0;
-        if (!down[k] && f < m.nIF) {
-            const int j = isOwn[k] ? m.nbr[f] : m.own[f];
-            if (j < c) cv = st.corr[f];
+#pragma unroll
+    for (int k = 0; k < MF; ++k) {
+        fid[k] = 0;
+        isOwn[k] = false;
+        down[k] = false;
+        writable[k] = false;
+        phf[k] = 0.0;
+        dvf[k] = 0.0;
+        cor[k] = 0.0;
+        if (k < nf) {
+            const int f = m.cFaces[b + k];
+            fid[k] = f;
+            isOwn[k] = (m.own[f] == c);
+            phf[k] = face_value(m, st.phi, f);
+            dvf[k] = face_value(m, st.dVf, f);
+            down[k] = isOwn[k] ? (phf[k] >= 0) : (phf[k] < 0);
+            writable[k] = (f < m.nIF) || !m.fEmpty[f];
+            if (!down[k] && f < m.nIF) {
+                const int j = isOwn[k] ? m.nbr[f] : m.own[f];
+                if (j < c) cor[k] = st.corr[f];
+            }
         }
-        cor[k] = cv;
     }
     double nfd = 0.0;
-    for (int k = 0; k < nf; ++k) nfd = isOwn[k] ? nfd + dvf[k] : nfd - dvf[k];
-    const double Vi = m.V[c];
-    const double ac = st.alpha[c];
+#pragma unroll
+    for (int k = 0; k < MF; ++k)
+        if (k < nf) nfd = isOwn[k] ? nfd + dvf[k] : nfd - dvf[k];
     double alphaOvershoot = pos0(ac - 1.0) * (ac - 1.0) + neg0(ac) * ac;
     double fluidToPassOn = alphaOvershoot * Vi;
     int nFaces = 1;
     for (int iter = 0; iter < 10; iter++) {
         if (fabs(alphaOvershoot) < aTol || nFaces == 0) break;
-        int sel[MF];
+        bool sel[MF];
         double smax[MF];
-        int ns = 0;
         double dVftot = 0.0;
         nFaces = 0;
-        for (int k = 0; k < nf; ++k) {
-            if (!down[k]) continue;
-            const double dVff = dvf[k] + cor[k];
-            const double maxExtra = fabs(pos0(fluidToPassOn) * phf[k] * dt - dVff);
-            if (maxExtra / Vi > aTol) {
-                sel[ns] = k;
-                smax[ns] = maxExtra;
-                ns++;
-                dVftot += fabs(phf[k] * dt);
+#pragma unroll
+        for (int k = 0; k < MF; ++k) {
+            sel[k] = false;
+            smax[k] = 0.0;
+            if (k < nf && down[k]) {
+                const double dVff = dvf[k] + cor[k];
+                const double maxExtra = fabs(pos0(fluidToPassOn) * phf[k] * dt - dVff);
+                if (maxExtra / Vi > aTol) {
+                    sel[k] = true;
+                    smax[k] = maxExtra;
+                    dVftot += fabs(phf[k] * dt);
+                }
             }
         }
-        for (int q = 0; q < ns; ++q) {
-            const int k = sel[q];
+#pragma unroll
+        for (int k = 0; k < MF; ++k) {
+            if (!sel[k]) continue;
             double fl = fabs(fluidToPassOn) * fabs(phf[k] * dt) / dVftot;
-            nFaces += (int)pos0(smax[q] - fl);
-            fl = fmin(fl, smax[q]);
+            nFaces += (int)pos0(smax[k] - fl);
+            fl = fmin(fl, smax[k]);
             double dVff = cor[k];
             dVff += sgn(phf[k]) * sgn(fluidToPassOn) * fl;
             if (writable[k]) cor[k] = dVff;
         }
         double nfc = 0.0;
-        for (int k = 0; k < nf; ++k) nfc = isOwn[k] ? nfc + cor[k] : nfc - cor[k];
-        double num = st.alphaOld[c] * rDeltaT;
+#pragma unroll
+        for (int k = 0; k < MF; ++k)
+            if (k < nf) nfc = isOwn[k] ? nfc + cor[k] : nfc - cor[k];
+        double num = aOld * rDeltaT;
         if (Su) num = num + Su[c];
         num = num - nfd / Vi * rDeltaT - nfc / Vi * rDeltaT;
         const double den = Sp ? (rDeltaT - Sp[c]) : rDeltaT;
@@ -1039,8 +1310,9 @@ __device__ inline void bound_cell(const DevMesh& m, const DevState& st, int c, d
         alphaOvershoot = pos0(alpha1New - 1.0) * (alpha1New - 1.0) + neg0(alpha1New) * alpha1New;
         fluidToPassOn = alphaOvershoot * Vi;
     }
-    for (int k = 0; k < nf; ++k)
-        if (down[k] && writable[k]) st.corr[fid[k]] = cor[k];
+#pragma unroll
+    for (int k = 0; k < MF; ++k)
+        if (k < nf && down[k] && writable[k]) st.corr[fid[k]] = cor[k];
 }
 
 // limitFlux (advectionTemplates.C:118-215), one pass = the kernels below.
@@ -1123,7 +1395,7 @@ __global__ void __launch_bounds__(256) k_bnd_write(DevState st, int nWords) {
     }
 }
 
-__global__ void k_bnd_prep(DevMesh m, DevState st, double dt) {
+__global__ void __launch_bounds__(256) k_bnd_prep(DevMesh m, DevState st, double dt) {
     const int nCand = st.bcount[0];
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nCand; i += gridDim.x * blockDim.x) {
         const int c = st.cand[i];
@@ -1155,7 +1427,7 @@ __global__ void __launch_bounds__(1024) k_bnd_scan2(DevState st) {
 }
 
 template <int MF>
-__global__ void k_bnd_scatter(DevMesh m, DevState st, double dt) {
+__global__ void __launch_bounds__(256) k_bnd_scatter(DevMesh m, DevState st, double dt) {
     const int nCand = st.bcount[0];
     int faces[MF];
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nCand; i += gridDim.x * blockDim.x) {
@@ -1184,8 +1456,9 @@ template <int MF>
 __device__ inline void bnd_complete(const DevMesh& m, const DevState& st, int i, double dt, double rDeltaT,
                                     const double* Sp, const double* Su, int* qnext, int* nNext) {
     const int c = st.cand[i];
+    // successors read these corrections in a later round: the next kernel of
+    // the grid rounds, or after the workgroup barrier of the tail
     bound_cell<MF>(m, st, c, dt, rDeltaT, Sp, Su);
-    __threadfence();
     for (int kk = m.cOff[c]; kk < m.cOff[c + 1]; ++kk) {
         const int f = m.cFaces[kk];
         if (f >= m.nIF || !face_downwind_of(m, st.phi, f, c)) continue;
@@ -1198,7 +1471,7 @@ __device__ inline void bnd_complete(const DevMesh& m, const DevState& st, int i,
 }
 
 template <int MF>
-__global__ void k_bnd_round(DevMesh m, DevState st, int r, double dt, const double* Sp, const double* Su) {
+__global__ void __launch_bounds__(256) k_bnd_round(DevMesh m, DevState st, int r, double dt, const double* Sp, const double* Su) {
     const int n = st.bcount[3 + r];
     const int* q = st.queue + (long)(r & 1) * m.nC;
     int* qnext = st.queue + (long)((r + 1) & 1) * m.nC;
@@ -1240,7 +1513,7 @@ __global__ void __launch_bounds__(1024) k_bnd_tail(DevMesh m, DevState st, doubl
     if (t == 0) st.bcount[BND_CTRL - 1] = sRounds;
 }
 
-__global__ void k_bnd_touch(DevMesh m, DevState st) {
+__global__ void __launch_bounds__(256) k_bnd_touch(DevMesh m, DevState st) {
     const int nCorr = st.bcount[1];
     for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < nCorr; q += gridDim.x * blockDim.x) {
         const int f = st.corrList[q];
@@ -1255,7 +1528,7 @@ __global__ void k_bnd_touch(DevMesh m, DevState st) {
 // alpha of every touched cell in correctedFaces order (advectionTemplates.C:164-192)
 // and the dVf of every corrected face
 template <int MF>
-__global__ void k_bnd_apply(DevMesh m, DevState st) {
+__global__ void __launch_bounds__(256) k_bnd_apply(DevMesh m, DevState st) {
     const double aTol = 100.0 * SVP_SMALL;
     const int nT = st.bcount[2], nCorr = st.bcount[1];
     for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < nT + nCorr; idx += gridDim.x * blockDim.x) {
@@ -1299,7 +1572,7 @@ __global__ void k_bnd_apply(DevMesh m, DevState st) {
     }
 }
 
-__global__ void k_bnd_reset(DevMesh m, DevState st, int pass) {
+__global__ void __launch_bounds__(256) k_bnd_reset(DevMesh m, DevState st, int pass) {
     const int nCand = st.bcount[0];
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nCand; i += gridDim.x * blockDim.x) {
         if (!st.active[i]) continue;
@@ -1376,7 +1649,9 @@ void launch_advect_stage(int stage, const DevMesh& m, const DevState& st, const 
             k_upwind<<<cdiv(m.nF, 256), 256, 0, s>>>(m, st, dt);
             break;
         case STAGE_IFLUX:
-            k_interface_flux<<<1024, 64, 0, s>>>(m, st, dt);
+            if (!st.un0Fused) k_un0<<<512, 64, 0, s>>>(m, st);
+            if (m.triQuadFaces) k_interface_flux_faces<true><<<2048, 64, 0, s>>>(m, st, dt, st.maxCellFaces);
+            else k_interface_flux_faces<false><<<2048, 64, 0, s>>>(m, st, dt, st.maxCellFaces);
             break;
         case STAGE_UPDATE:
             k_update<<<nb, DETECT_BLOCK, 0, s>>>(m, st, dt, Sp, Su);

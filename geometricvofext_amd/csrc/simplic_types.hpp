@@ -82,6 +82,7 @@ struct DevState {
     int* touchedFlag;   // nC
     int* boundStats;    // [4*maxPasses]: candidates, active, rounds, corrected faces
     int* bcount;        // [16] per-pass counters of the bounding kernels
+    int un0Fused;       // plane placement by 8-lane groups, which also computes Un0
     int maxCellFaces;
     double* minmax;     // [4]: min, max before bounding, min, max after
     int* errFlag;       // [1] device error bits
