@@ -18,7 +18,8 @@ from .mesh import PolyMesh, hex_box, tet_box  # noqa: F401
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libsimplic_amd.so")
-HEADERS = [os.path.join(os.path.dirname(_HERE), "include", h) for h in ("simplic_amd.h", "simplic_harness.h")]
+HEADERS = [os.path.join(os.path.dirname(_HERE), "include", h)
+           for h in ("simplic_amd.h", "simplic_harness.h", "simplic_learned.h")]
 
 SVP_HOST = 0
 SVP_DEVICE = 1
@@ -41,6 +42,7 @@ class SvpDict(C.Structure):
         ("clip", C.c_int32),
         ("split_warped_face", C.c_int32),
         ("write_plic_fields", C.c_int32),
+        ("learned_residual_tol", C.c_double),
     ]
 
 
@@ -101,6 +103,20 @@ def lib():
     L.svp_harness_init_alpha.argtypes = [vp, vp, C.c_int, C.c_int]
     L.svp_profile_set.argtypes = [vp, C.c_int]
     L.svp_profile_get.argtypes = [vp, vp, vp, C.c_int]
+    i64, u64 = C.c_int64, C.c_uint64
+    L.svp_nm_create.argtypes = [C.c_int, u64, C.POINTER(vp)]
+    L.svp_nm_destroy.argtypes = [vp]
+    L.svp_nm_set_params.argtypes = [vp, vp, i32]
+    L.svp_nm_get_params.argtypes = [vp, vp, i32]
+    L.svp_nm_generate.argtypes = [vp, i64, u64, i32]
+    L.svp_nm_dataset_get.argtypes = [vp, i64, i64, vp, vp, vp, vp]
+    L.svp_nm_train.argtypes = [vp, i32, i32, C.c_float, C.c_float, u64, vp, C.POINTER(dbl)]
+    L.svp_nm_forward.argtypes = [vp, vp, i64, vp, C.c_int]
+    L.svp_nm_gradient.argtypes = [vp, vp, vp, i64, vp, C.POINTER(dbl), C.c_int]
+    L.svp_nm_cube_fractions.argtypes = [vp, i64, vp, C.c_int]
+    L.svp_set_normal_model.argtypes = [vp, vp]
+    L.svp_learned_stats.argtypes = [vp, vp]
+    L.svp_learned_residuals.argtypes = [vp, vp, C.c_int]
     _lib = L
     return L
 
@@ -131,7 +147,8 @@ def make_dict(**kw) -> SvpDict:
     d = dict_defaults()
     names = {"orientationMethod": "orientation_method", "mixedCellTol": "mixed_cell_tol",
              "nAlphaBounds": "n_alpha_bounds", "snapTol": "snap_tol", "clip": "clip",
-             "splitWarpedFace": "split_warped_face", "writePlicFields": "write_plic_fields"}
+             "splitWarpedFace": "split_warped_face", "writePlicFields": "write_plic_fields",
+             "learnedResidualTol": "learned_residual_tol"}
     for k, v in kw.items():
         k = names.get(k, k)
         if k == "orientation_method":
@@ -324,3 +341,21 @@ class SolveVofEqu:
         cnt = np.zeros(len(self.STAGES), np.int64)
         _check(lib().svp_profile_get(self.h, sec.ctypes.data, cnt.ctypes.data, int(reset)))
         return {n: (s, int(c)) for n, s, c in zip(self.STAGES, sec, cnt)}
+
+    # learned orientation method (include/simplic_learned.h)
+    def set_normal_model(self, model):
+        _check(lib().svp_set_normal_model(self.h, model.h))
+
+    def learned_stats(self):
+        """(interface cells, learned kept, residual fallbacks, least-squares only)."""
+        out = np.zeros(4, np.int64)
+        _check(lib().svp_learned_stats(self.h, out.ctypes.data))
+        return tuple(int(v) for v in out)
+
+    def learned_residuals(self):
+        out = np.zeros(self.n_mixed())
+        _check(lib().svp_learned_residuals(self.h, out.ctypes.data, SVP_HOST))
+        return out
+
+
+from .learned import NormalModel, cube_fractions  # noqa: E402,F401

@@ -15,8 +15,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libsimplic_amd.so")
-SOURCES = ["simplic_kernels.hip", "harness_kernels.hip", "simplic_api.cpp"]
-HEADERS = ["simplic_device.hpp", "simplic_types.hpp"]
+SOURCES = ["simplic_kernels.hip", "harness_kernels.hip", "learned_kernels.hip", "simplic_api.cpp", "learned_api.cpp"]
+HEADERS = ["simplic_device.hpp", "simplic_types.hpp", "learned_device.hpp", "learned_kernels.hpp",
+           "learned_model.hpp", "api_common.hpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
@@ -39,7 +40,7 @@ def _objects_and_commands(build_dir):
     return objs, cmds
 
 
-def build(verbose: bool = False, jobs: int = 3) -> str:
+def build(verbose: bool = False, jobs: int = 5) -> str:
     build_dir = os.path.join(HERE, "_build")
     os.makedirs(build_dir, exist_ok=True)
     objs, cmds = _objects_and_commands(build_dir)

@@ -15,10 +15,14 @@
 #include <string>
 #include <vector>
 
+#include "api_common.hpp"
 #include "simplic_amd.h"
 #include "simplic_harness.h"
 #include "simplic_device.hpp"
+#include "simplic_learned.h"
 #include "simplic_types.hpp"
+#include "learned_kernels.hpp"
+#include "learned_model.hpp"
 
 namespace svp {
 void launch_scale_flow(int nC, int nF, int nB, const double* U0, const double* phi0, const double* U0b,
@@ -30,62 +34,7 @@ void launch_init_alpha(const DevMesh& m, const void* shapes, int nShapes, int ma
                        hipStream_t s);
 }  // namespace svp
 
-namespace {
-
-thread_local std::string g_err;
-
-struct SvpError : std::runtime_error {
-    int code;
-    SvpError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
-};
-
-#define HIPCHECK(x)                                                                            \
-    do {                                                                                       \
-        hipError_t e_ = (x);                                                                   \
-        if (e_ != hipSuccess)                                                                  \
-            throw SvpError(SVP_E_HIP, std::string(#x) + ": " + hipGetErrorString(e_));         \
-    } while (0)
-
-template <class F>
-int guarded(F&& f) {
-    try {
-        f();
-        return SVP_OK;
-    } catch (const SvpError& e) {
-        g_err = e.what();
-        return e.code;
-    } catch (const std::exception& e) {
-        g_err = e.what();
-        return SVP_E_ARG;
-    }
-}
-
-// Owning device buffer
-struct DBuf {
-    void* p = nullptr;
-    size_t bytes = 0;
-    DBuf() = default;
-    DBuf(const DBuf&) = delete;
-    DBuf& operator=(const DBuf&) = delete;
-    ~DBuf() {
-        if (p) (void)hipFree(p);
-    }
-    template <class T>
-    T* alloc(size_t n) {
-        bytes = std::max<size_t>(n, 1) * sizeof(T);
-        HIPCHECK(hipMalloc(&p, bytes));
-        HIPCHECK(hipMemset(p, 0, bytes));
-        return (T*)p;
-    }
-    template <class T>
-    T* upload(const std::vector<T>& v) {
-        T* d = alloc<T>(v.size());
-        if (!v.empty()) HIPCHECK(hipMemcpy(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
-        return d;
-    }
-};
-
-}  // namespace
+using namespace svp::api;
 
 namespace gve {
 namespace SimPLIC {
@@ -104,6 +53,8 @@ public:
         params_.snapTol = dict.snap_tol;
         params_.clip = dict.clip;
         params_.splitWarpedFace = dict.split_warped_face;
+        learned_.active = std::string(dict.orientation_method) == "learned";
+        learned_.tol = dict.learned_residual_tol;
         buildMesh(mesh);
         allocState();
         launch_geometry(m_, stream_);
@@ -126,8 +77,12 @@ public:
         hipEvent_t a = event(), b = event();
         HIPCHECK(hipEventRecord(a, stream_));
         stage(STAGE_DETECT, [&] { launch_detect(m_, st_, params_, stream_); });
-        stage(STAGE_NORMAL, [&] { launch_ls_normal(m_, st_, stream_); });
-        stage(STAGE_FSD, [&] { launch_find_signed_distance(m_, st_, params_, stream_); });
+        if (learned_.active) {
+            reconstructLearned();
+        } else {
+            stage(STAGE_NORMAL, [&] { launch_ls_normal(m_, st_, stream_); });
+            stage(STAGE_FSD, [&] { launch_find_signed_distance(m_, st_, params_, stream_); });
+        }
         HIPCHECK(hipEventRecord(b, stream_));
         HIPCHECK(hipGetLastError());
         pendingRecon_.push_back({a, b});
@@ -150,6 +105,76 @@ public:
         lastDt_ = dt;
         alphaPhiValid_ = false;
         reconstructed_ = false;
+    }
+
+    // orientationMethod learned: network normals on cells with a uniform
+    // 3x3x3 stencil, least-squares normals elsewhere; plane placement; the
+    // residual test; least-squares normal and plane placement again on the
+    // cells that fail it.
+    void reconstructLearned() {
+        if (!learned_.ready)
+            throw SvpError(SVP_E_ARG, "orientationMethod learned needs a model (svp_set_normal_model)");
+        int* nLs = learned_.counts;
+        int* nFb = learned_.counts + 1;
+        stage(STAGE_NORMAL, [&] {
+            nm::launch_zero_ints(learned_.counts, 2, stream_);
+            nm::launch_infer(learned_.lat, st_.alpha, st_.mixed, st_.nMixed, learned_.wb, learned_.pq, st_.ifN,
+                             learned_.learned, learned_.lsList, nLs, m_.nC, stream_);
+            launch_ls_normal(m_, st_, stream_, learned_.lsList, nLs);
+        });
+        stage(STAGE_FSD, [&] {
+            launch_find_signed_distance(m_, st_, params_, stream_);
+            nm::launch_residual(learned_.lat, st_.alpha, m_.C, m_.V, st_.mixed, st_.nMixed, st_.ifN, st_.ifD,
+                                st_.status, learned_.learned, learned_.tol, learned_.resid, learned_.fbList, nFb,
+                                stream_);
+            launch_ls_normal(m_, st_, stream_, learned_.fbList, nFb);
+            launch_find_signed_distance(m_, st_, params_, stream_, learned_.fbList, nFb);
+        });
+    }
+
+    void setNormalModel(const svp_normal_model& nmod) {
+        if (!learned_.active)
+            throw SvpError(SVP_E_ARG, "svp_set_normal_model: the handle's orientationMethod is not learned");
+        if (nmod.device != device_) throw SvpError(SVP_E_ARG, "svp_set_normal_model: model on another device");
+        HIPCHECK(hipStreamSynchronize(nmod.stream));
+        if (!learned_.lat) {
+            learned_.lat = dalloc<int>(27L * m_.nC);
+            learned_.pq = dalloc<float>(nm::N_PADDED);
+            learned_.wb = dalloc<uint16_t>(nm::N_BF16);
+            learned_.learned = dalloc<unsigned char>(m_.nC);
+            learned_.lsList = dalloc<int>(m_.nC);
+            learned_.fbList = dalloc<int>(m_.nC);
+            learned_.counts = dalloc<int>(2);
+            learned_.resid = dalloc<double>(m_.nC);
+            DBuf cube;
+            cube.alloc<unsigned char>(m_.nC);
+            nm::launch_lattice(m_, (unsigned char*)cube.p, learned_.lat, stream_);
+            HIPCHECK(hipGetLastError());
+            HIPCHECK(hipStreamSynchronize(stream_));
+        }
+        HIPCHECK(hipMemcpyAsync(learned_.pq, nmod.pq.p, sizeof(float) * nm::N_PADDED, hipMemcpyDeviceToDevice,
+                                stream_));
+        HIPCHECK(hipMemcpyAsync(learned_.wb, nmod.wb.p, sizeof(uint16_t) * nm::N_BF16, hipMemcpyDeviceToDevice,
+                                stream_));
+        HIPCHECK(hipStreamSynchronize(stream_));
+        learned_.ready = true;
+        reconstructed_ = false;
+    }
+
+    void learnedStats(int64_t* out4) {
+        if (!learned_.ready) throw SvpError(SVP_E_ARG, "svp_learned_stats: no learned reconstruction");
+        int c[2] = {0, 0};
+        const int n = nMixed();
+        HIPCHECK(hipMemcpy(c, learned_.counts, sizeof(c), hipMemcpyDeviceToHost));
+        out4[0] = n;
+        out4[1] = n - c[0] - c[1];
+        out4[2] = c[1];
+        out4[3] = c[0];
+    }
+
+    void learnedResiduals(double* out, int where) {
+        if (!learned_.ready) throw SvpError(SVP_E_ARG, "svp_learned_residuals: no learned reconstruction");
+        copyOut(out, learned_.resid, nMixed(), where);
     }
 
     // Per-stage HIP-event timing on this handle's stream (bit i of mask =
@@ -256,6 +281,19 @@ public:
     unsigned long long* courant_ = nullptr;
     int* nonfinite_ = nullptr;
     std::vector<double> hostC_, hostCf_, hostSf_;
+    // learned orientation method
+    struct {
+        bool active = false, ready = false;
+        double tol = 0.0;
+        int* lat = nullptr;          // 27 per cell, -1 without a uniform stencil
+        float* pq = nullptr;         // network parameters (padded FP32)
+        uint16_t* wb = nullptr;      // BF16 weights
+        unsigned char* learned = nullptr;
+        int* lsList = nullptr;       // positions needing the least-squares normal first
+        int* fbList = nullptr;       // positions that failed the residual test
+        int* counts = nullptr;       // [2]: lsList, fbList lengths
+        double* resid = nullptr;
+    } learned_;
 
     void copyIn(double* dst, const double* src, long n, int where) {
         if (!src) throw SvpError(SVP_E_ARG, "null input pointer");
@@ -550,6 +588,8 @@ private:
         st_.boundStats = dalloc<int>(4 * MAX_BOUND_PASSES);
         st_.bcount = dalloc<int>(16);
         st_.maxCellFaces = maxCellFaces_;
+        st_.bndRec = maxCellFaces_ <= NS_CELL_FACES ? (void*)dalloc<BndRec<NS_CELL_FACES>>(nC)
+                                                    : (void*)dalloc<BndRec<MAX_CELL_FACES>>(nC);
         st_.un0Fused = (!dict_.split_warped_face && maxCellFaces_ <= 8 && m_.maxCellPoints <= NS_CELL_POINTS &&
                         m_.triQuadFaces) ? 1 : 0;
         st_.minmax = dalloc<double>(4);
@@ -587,6 +627,7 @@ void svp_dict_defaults(svp_dict* d) {
     d->clip = 1;
     d->split_warped_face = 0;
     d->write_plic_fields = 0;
+    d->learned_residual_tol = 0.02;
 }
 
 // OpenFOAM dictionary entries "key value;" with comments; sub-dictionaries
@@ -643,15 +684,17 @@ int svp_dict_parse(const char* text, svp_dict* out) {
             else if (key == "clip") d.clip = toBool(val);
             else if (key == "splitWarpedFace") d.split_warped_face = toBool(val);
             else if (key == "writePlicFields") d.write_plic_fields = toBool(val);
+            else if (key == "learnedResidualTol") d.learned_residual_tol = std::stod(val);
         }
         if (!method.empty()) {
-            if (method == "LS" || method == "isoAlphaGrad") {
-                std::snprintf(d.orientation_method, sizeof d.orientation_method, "%s", method.c_str());
+            if (method == "LS" || method == "isoAlphaGrad" || method == "learned" || method == "MLP") {
+                std::snprintf(d.orientation_method, sizeof d.orientation_method, "%s",
+                              method == "MLP" ? "learned" : method.c_str());
             } else if (method == "alphaGrad" || method == "isoRDF" || method == "RDF") {
                 throw SvpError(SVP_E_UNSUPPORTED, "orientationMethod " + method + " is not provided (LS/isoAlphaGrad are)");
             } else {
                 throw SvpError(SVP_E_DICT, "Invalid orientationMethod " + method +
-                                               ". Valid methods are: alphaGrad isoAlphaGrad isoRDF LS RDF");
+                                               ". Valid methods are: alphaGrad isoAlphaGrad isoRDF LS RDF learned");
             }
         }
         *out = d;
@@ -665,12 +708,12 @@ int svp_create(const svp_mesh* mesh, const svp_dict* dict, int device, svp_handl
         if (dict) d = *dict;
         else svp_dict_defaults(&d);
         const std::string m(d.orientation_method);
-        if (m != "LS" && m != "isoAlphaGrad") {
+        if (m != "LS" && m != "isoAlphaGrad" && m != "learned") {
             if (m == "alphaGrad" || m == "isoRDF" || m == "RDF")
                 throw SvpError(SVP_E_UNSUPPORTED, "orientationMethod " + m + " is not provided");
             throw SvpError(SVP_E_DICT, "Invalid orientationMethod " + m);
         }
-        if (d.n_alpha_bounds < 0 || !(d.mixed_cell_tol >= 0.0) || d.snap_tol < 0.0)
+        if (d.n_alpha_bounds < 0 || !(d.mixed_cell_tol >= 0.0) || d.snap_tol < 0.0 || !(d.learned_residual_tol >= 0.0))
             throw SvpError(SVP_E_DICT, "invalid tolerance or bound count");
         auto h = std::make_unique<svp_handle>();
         h->s = std::make_unique<solveVofEqu>(*mesh, d, device);
@@ -849,6 +892,24 @@ int svp_synchronize(svp_handle* h) {
         HIPCHECK(hipStreamSynchronize(s.stream_));
         s.checkDeviceErrors();
     });
+}
+
+int svp_set_normal_model(svp_handle* h, const svp_normal_model* m) {
+    return guarded([&] {
+        if (!m) throw SvpError(SVP_E_ARG, "svp_set_normal_model: null model");
+        H(h).setNormalModel(*m);
+    });
+}
+
+int svp_learned_stats(svp_handle* h, int64_t* out4) {
+    return guarded([&] {
+        if (!out4) throw SvpError(SVP_E_ARG, "svp_learned_stats: null output");
+        H(h).learnedStats(out4);
+    });
+}
+
+int svp_learned_residuals(svp_handle* h, double* out, int where) {
+    return guarded([&] { H(h).learnedResiduals(out, where); });
 }
 
 const char* svp_last_error(void) { return g_err.c_str(); }

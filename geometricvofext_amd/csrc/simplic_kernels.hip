@@ -246,9 +246,11 @@ __device__ inline V3 ls_gradient(const DevMesh& m, const double* alpha, int c) {
     return v3(x[1], x[2], x[3]);
 }
 
-__global__ void __launch_bounds__(64) k_ls_normal(DevMesh m, DevState st) {
-    const int nMixed = *st.nMixed;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nMixed; i += gridDim.x * blockDim.x) {
+// wl: optional list of mixed-cell positions (*wn entries) instead of all of them
+__global__ void __launch_bounds__(64) k_ls_normal(DevMesh m, DevState st, const int* wl, const int* wn) {
+    const int nWork = wl ? *wn : *st.nMixed;
+    for (int wi = blockIdx.x * blockDim.x + threadIdx.x; wi < nWork; wi += gridDim.x * blockDim.x) {
+        const int i = wl ? wl[wi] : wi;
         const int c = st.mixed[i];
         V3 n = -ls_gradient(m, st.alpha, c);
         n = n / (mag(n) + SVP_SMALL);
@@ -256,8 +258,8 @@ __global__ void __launch_bounds__(64) k_ls_normal(DevMesh m, DevState st) {
     }
 }
 
-void launch_ls_normal(const DevMesh& m, const DevState& st, hipStream_t s) {
-    k_ls_normal<<<1024, 64, 0, s>>>(m, st);
+void launch_ls_normal(const DevMesh& m, const DevState& st, hipStream_t s, const int* wl, const int* wn) {
+    k_ls_normal<<<1024, 64, 0, s>>>(m, st, wl, wn);
 }
 
 // ---------------------------------------------------------------------------
@@ -438,9 +440,11 @@ __device__ inline void find_signed_distance(const Poly& P, V3 Cc, double Vc, dou
 }
 
 template <class Poly>
-__global__ void __launch_bounds__(64) k_find_signed_distance(DevMesh m, DevState st, int split) {
-    const int nMixed = *st.nMixed;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nMixed; i += gridDim.x * blockDim.x) {
+__global__ void __launch_bounds__(64) k_find_signed_distance(DevMesh m, DevState st, int split, const int* wl,
+                                                             const int* wn) {
+    const int nWork = wl ? *wn : *st.nMixed;
+    for (int wi = blockIdx.x * blockDim.x + threadIdx.x; wi < nWork; wi += gridDim.x * blockDim.x) {
+        const int i = wl ? wl[wi] : wi;
         const int c = st.mixed[i];
         Poly P;
         FaceCut fc[Poly::maxFaces];
@@ -478,13 +482,14 @@ __device__ inline void fsd_face_cut(int fsz, const V3 (&q4)[4], const V3 (&q3)[3
     else ok &= calcSubFaceN<3>(q3, n, dist, fc);
 }
 
-__global__ void __launch_bounds__(64) k_fsd_group(DevMesh m, DevState st) {
+__global__ void __launch_bounds__(64) k_fsd_group(DevMesh m, DevState st, const int* wl, const int* wn) {
     __shared__ FaceCut rec[64 / FSD_G][FSD_G];
     __shared__ double vraw[64 / FSD_G][NS_CELL_POINTS];
     __shared__ double vsrt[64 / FSD_G][NS_CELL_POINTS];
     const int g = threadIdx.x / FSD_G, l = threadIdx.x % FSD_G;
-    const int nMixed = *st.nMixed;
-    for (int i = blockIdx.x * (64 / FSD_G) + g; i < nMixed; i += gridDim.x * (64 / FSD_G)) {
+    const int nWork = wl ? *wn : *st.nMixed;
+    for (int wi = blockIdx.x * (64 / FSD_G) + g; wi < nWork; wi += gridDim.x * (64 / FSD_G)) {
+        const int i = wl ? wl[wi] : wi;
         const int c = st.mixed[i];
         const int fb = m.cOff[c], nF = m.cOff[c + 1] - fb;
         const V3 normalI = ld3(st.ifN, i);
@@ -715,10 +720,11 @@ __global__ void __launch_bounds__(64) k_fsd_group(DevMesh m, DevState st) {
     }
 }
 
-void launch_find_signed_distance(const DevMesh& m, const DevState& st, const Params& p, hipStream_t s) {
-    if (p.splitWarpedFace) k_find_signed_distance<CellPolySP><<<1024, 64, 0, s>>>(m, st, 1);
-    else if (st.un0Fused) k_fsd_group<<<2048, 64, 0, s>>>(m, st);
-    else k_find_signed_distance<CellPolyNS><<<1024, 64, 0, s>>>(m, st, 0);
+void launch_find_signed_distance(const DevMesh& m, const DevState& st, const Params& p, hipStream_t s,
+                                 const int* wl, const int* wn) {
+    if (p.splitWarpedFace) k_find_signed_distance<CellPolySP><<<1024, 64, 0, s>>>(m, st, 1, wl, wn);
+    else if (st.un0Fused) k_fsd_group<<<2048, 64, 0, s>>>(m, st, wl, wn);
+    else k_find_signed_distance<CellPolyNS><<<1024, 64, 0, s>>>(m, st, 0, wl, wn);
 }
 
 // ---------------------------------------------------------------------------
@@ -1439,32 +1445,141 @@ __global__ void __launch_bounds__(256) k_bnd_scatter(DevMesh m, DevState st, dou
             st.corrPos[faces[k]] = pos + k;
             st.corrList[pos + k] = faces[k];
         }
+        // the record the dependent rounds work from
+        BndRec<MF>& R = ((BndRec<MF>*)st.bndRec)[i];
+        const int b = m.cOff[c], nf = m.cOff[c + 1] - b;
         int np = 0;
-        for (int k = m.cOff[c]; k < m.cOff[c + 1]; ++k) {
-            const int f = m.cFaces[k];
-            if (f >= m.nIF || face_downwind_of(m, st.phi, f, c)) continue;
-            const int j = (m.own[f] == c) ? m.nbr[f] : m.own[f];
-            if (j < c && st.actIdx[j] >= 0) np++;
+        unsigned int flg[(MF + 7) / 8];
+#pragma unroll
+        for (int w = 0; w < (MF + 7) / 8; ++w) flg[w] = 0u;
+#pragma unroll
+        for (int k = 0; k < MF; ++k) {
+            int f = 0, sc = -1;
+            double ph = 0.0, dv = 0.0;
+            if (k < nf) {
+                f = m.cFaces[b + k];
+                const bool own = (m.own[f] == c);
+                ph = face_value(m, st.phi, f);
+                dv = face_value(m, st.dVf, f);
+                const bool down = own ? (ph >= 0) : (ph < 0);
+                const bool writable = (f < m.nIF) || !m.fEmpty[f];
+                bool lowerWriter = false;
+                if (f < m.nIF) {
+                    const int o = own ? m.nbr[f] : m.own[f];
+                    if (!down && o < c) {
+                        lowerWriter = true;
+                        if (st.actIdx[o] >= 0) np++;
+                    }
+                    if (down && o > c) sc = st.actIdx[o];
+                }
+                flg[k >> 3] |= ((own ? 1u : 0u) | (down ? 2u : 0u) | (writable ? 4u : 0u) | (lowerWriter ? 8u : 0u))
+                               << (4 * (k & 7));
+            }
+            R.fid[k] = f;
+            R.succ[k] = sc;
+            R.phf[k] = ph;
+            R.dvf[k] = dv;
         }
+#pragma unroll
+        for (int w = 0; w < (MF + 7) / 8; ++w) R.flg[w] = flg[w];
+        R.nf = nf;
+        R.Vi = m.V[c];
+        R.ac = st.alpha[c];
+        R.aOld = st.alphaOld[c];
         st.pend[i] = np;
         if (np == 0) st.queue[atomicAdd(&st.bcount[3], 1)] = i;
     }
+}
+
+// Whole boundFlux of one cell (advectionTemplates.C:243-347) from its record.
+// Corrections of the downwind faces live in registers; those of upwind faces
+// come from lower-numbered cells that have completed (a higher-numbered writer
+// has not run yet in the reference's ascending walk, so it contributes zero).
+template <int MF>
+__device__ inline void bound_cell_rec(const BndRec<MF>& R, const DevState& st, int c, double dt, double rDeltaT,
+                                      const double* Sp, const double* Su) {
+    const double aTol = 100.0 * SVP_SMALL;
+    unsigned int flg[(MF + 7) / 8];
+#pragma unroll
+    for (int w = 0; w < (MF + 7) / 8; ++w) flg[w] = R.flg[w];
+    const int nf = R.nf;
+    const double Vi = R.Vi, ac = R.ac, aOld = R.aOld;
+    auto bits = [&](int k) { return flg[k >> 3] >> (4 * (k & 7)); };
+    auto isOwn = [&](int k) { return (bits(k) & 1u) != 0u; };
+    auto down = [&](int k) { return (bits(k) & 2u) != 0u; };
+    auto writable = [&](int k) { return (bits(k) & 4u) != 0u; };
+    double phf[MF], dvf[MF], cor[MF];
+#pragma unroll
+    for (int k = 0; k < MF; ++k) {
+        phf[k] = R.phf[k];
+        dvf[k] = R.dvf[k];
+        cor[k] = (bits(k) & 8u) ? st.corr[R.fid[k]] : 0.0;
+    }
+    double nfd = 0.0;
+#pragma unroll
+    for (int k = 0; k < MF; ++k)
+        if (k < nf) nfd = isOwn(k) ? nfd + dvf[k] : nfd - dvf[k];
+    double alphaOvershoot = pos0(ac - 1.0) * (ac - 1.0) + neg0(ac) * ac;
+    double fluidToPassOn = alphaOvershoot * Vi;
+    int nFaces = 1;
+    for (int iter = 0; iter < 10; iter++) {
+        if (fabs(alphaOvershoot) < aTol || nFaces == 0) break;
+        bool sel[MF];
+        double smax[MF];
+        double dVftot = 0.0;
+        nFaces = 0;
+#pragma unroll
+        for (int k = 0; k < MF; ++k) {
+            sel[k] = false;
+            smax[k] = 0.0;
+            if (k < nf && down(k)) {
+                const double dVff = dvf[k] + cor[k];
+                const double maxExtra = fabs(pos0(fluidToPassOn) * phf[k] * dt - dVff);
+                if (maxExtra / Vi > aTol) {
+                    sel[k] = true;
+                    smax[k] = maxExtra;
+                    dVftot += fabs(phf[k] * dt);
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < MF; ++k) {
+            if (!sel[k]) continue;
+            double fl = fabs(fluidToPassOn) * fabs(phf[k] * dt) / dVftot;
+            nFaces += (int)pos0(smax[k] - fl);
+            fl = fmin(fl, smax[k]);
+            double dVff = cor[k];
+            dVff += sgn(phf[k]) * sgn(fluidToPassOn) * fl;
+            if (writable(k)) cor[k] = dVff;
+        }
+        double nfc = 0.0;
+#pragma unroll
+        for (int k = 0; k < MF; ++k)
+            if (k < nf) nfc = isOwn(k) ? nfc + cor[k] : nfc - cor[k];
+        double num = aOld * rDeltaT;
+        if (Su) num = num + Su[c];
+        num = num - nfd / Vi * rDeltaT - nfc / Vi * rDeltaT;
+        const double den = Sp ? (rDeltaT - Sp[c]) : rDeltaT;
+        const double alpha1New = num / den;
+        alphaOvershoot = pos0(alpha1New - 1.0) * (alpha1New - 1.0) + neg0(alpha1New) * alpha1New;
+        fluidToPassOn = alphaOvershoot * Vi;
+    }
+#pragma unroll
+    for (int k = 0; k < MF; ++k)
+        if (k < nf && down(k) && writable(k)) st.corr[R.fid[k]] = cor[k];
 }
 
 // completes cell i of frontier r and releases its successors into frontier r+1
 template <int MF>
 __device__ inline void bnd_complete(const DevMesh& m, const DevState& st, int i, double dt, double rDeltaT,
                                     const double* Sp, const double* Su, int* qnext, int* nNext) {
-    const int c = st.cand[i];
+    const BndRec<MF>& R = ((const BndRec<MF>*)st.bndRec)[i];
     // successors read these corrections in a later round: the next kernel of
     // the grid rounds, or after the workgroup barrier of the tail
-    bound_cell<MF>(m, st, c, dt, rDeltaT, Sp, Su);
-    for (int kk = m.cOff[c]; kk < m.cOff[c + 1]; ++kk) {
-        const int f = m.cFaces[kk];
-        if (f >= m.nIF || !face_downwind_of(m, st.phi, f, c)) continue;
-        const int d = (m.own[f] == c) ? m.nbr[f] : m.own[f];
-        if (d < c) continue;
-        const int a = st.actIdx[d];
+    bound_cell_rec<MF>(R, st, st.cand[i], dt, rDeltaT, Sp, Su);
+#pragma unroll
+    for (int k = 0; k < MF; ++k) {
+        const int a = R.succ[k];
         if (a < 0) continue;
         if (atomicSub(&st.pend[a], 1) == 1) qnext[atomicAdd(nNext, 1)] = a;
     }

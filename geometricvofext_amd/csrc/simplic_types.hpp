@@ -83,6 +83,7 @@ struct DevState {
     int* boundStats;    // [4*maxPasses]: candidates, active, rounds, corrected faces
     int* bcount;        // [16] per-pass counters of the bounding kernels
     int un0Fused;       // plane placement by 8-lane groups, which also computes Un0
+    void* bndRec;       // nC records BndRec<MF> of the active bounding candidates
     int maxCellFaces;
     double* minmax;     // [4]: min, max before bounding, min, max after
     int* errFlag;       // [1] device error bits
@@ -107,11 +108,27 @@ constexpr int MAX_BOUND_PASSES = 64;
 
 constexpr int DETECT_BLOCK = 256;
 
+// Face data of an active bounding candidate, gathered once per pass so that
+// the dependent rounds read one contiguous record.  flags: 4 bits per face
+// (owner, downwind, writable, upwind writer with a lower label).
+template <int MF>
+struct BndRec {
+    double phf[MF], dvf[MF];
+    int fid[MF];
+    int succ[MF];       // candidate index of the active downwind neighbour with a higher label, else -1
+    unsigned int flg[(MF + 7) / 8];
+    int nf;
+    double Vi, ac, aOld;
+};
+
 // launchers (simplic_kernels.hip)
 void launch_geometry(const DevMesh& m, hipStream_t s);
 void launch_detect(const DevMesh& m, const DevState& st, const Params& p, hipStream_t s);
-void launch_ls_normal(const DevMesh& m, const DevState& st, hipStream_t s);
-void launch_find_signed_distance(const DevMesh& m, const DevState& st, const Params& p, hipStream_t s);
+// wl/wn: optional device list of mixed-cell positions and its length
+void launch_ls_normal(const DevMesh& m, const DevState& st, hipStream_t s, const int* wl = nullptr,
+                      const int* wn = nullptr);
+void launch_find_signed_distance(const DevMesh& m, const DevState& st, const Params& p, hipStream_t s,
+                                 const int* wl = nullptr, const int* wn = nullptr);
 // pipeline stages of one step, in launch order (also the profiler's slots)
 enum Stage : int {
     STAGE_DETECT = 0,

@@ -57,13 +57,14 @@ typedef struct {
 /* The fvSolution solvers{"alpha.*"{...}} keys the reference reads
  * (reconstruction.C:502-516, advection.C:455-457). */
 typedef struct {
-    char orientation_method[32];   /* "isoAlphaGrad" (default) or "LS" */
+    char orientation_method[32];   /* "isoAlphaGrad" (default), "LS", or "learned" (simplic_learned.h) */
     double mixed_cell_tol;         /* 1e-8 */
     int32_t n_alpha_bounds;        /* 10 */
     double snap_tol;               /* 0 */
     int32_t clip;                  /* 1 */
     int32_t split_warped_face;     /* 0 */
     int32_t write_plic_fields;     /* 0 */
+    double learned_residual_tol;   /* learnedResidualTol, "learned" method only: 0.02 */
 } svp_dict;
 
 /* Defaults of every key (reconstruction.C:502-516, advection.C:455-457). */
@@ -73,7 +74,8 @@ void svp_dict_defaults(svp_dict* d);
  * C/C++ comments allowed).  Unknown keys are ignored, as OpenFOAM does.
  * `reconstructionScheme` is accepted as an alias of `orientationMethod`.
  * Returns SVP_E_DICT for an unknown orientation method
- * (reconstruction.C:610-625) and SVP_E_UNSUPPORTED for alphaGrad / isoRDF / RDF. */
+ * (reconstruction.C:610-625) and SVP_E_UNSUPPORTED for alphaGrad / isoRDF / RDF.
+ * "learned" (alias "MLP") selects the learned normal of simplic_learned.h. */
 int svp_dict_parse(const char* text, svp_dict* out);
 
 /* solveVofEqu constructor (solveVofEqu.C:55-91): uploads the mesh, computes
