@@ -47,6 +47,13 @@ WORKLOADS = {
 
 ALPHA_DICT = dict(orientationMethod="LS", nAlphaBounds=3, snapTol=0.0, clip=False, mixedCellTol=1e-8)
 
+# learned-normal comparison: 128^3 uniform hex, 3 seeded ellipsoids, same
+# deformation; the network is trained on the GPU at the start of the run
+LEARNED_WORKLOAD = dict(kind="hex", N=128, perturb=0.0, seed=0, shapes_seed=2024, flow="deformation",
+                        end_time=3.0, desc="128^3 uniform hex, 3 seeded ellipsoids (semi-axes 0.08-0.15), "
+                                           "deformation T=3, Co 0.5")
+LEARNED_TRAINING = dict(samples=4 << 20, depth=5, epochs=10, batch=4096, lr0=3e-3, lr1=3e-5, seed=11)
+
 
 def algorithmic_bytes(stage, n_cells, n_faces, n_ic, cell_kind):
     """Algorithmic HBM bytes of one launch of a stage (FP64 values, int32 labels).
@@ -90,6 +97,65 @@ def build_case(wl):
     U0, phi0 = solver.deformation_field()
     solver.upload_base_flow(U0, phi0)
     return mesh, solver, alpha0, U0, phi0
+
+
+def learned_comparison(say, residual_tol=None):
+    """Plain least-squares normal against the learned normal on the
+    ellipsoid workload: L1 shape error at t=T and interface-cell updates/s of
+    the same timed loop, plus the network's training cost."""
+    import geometricvofext_amd as g
+    from geometricvofext_amd import cases, harness
+    tr = LEARNED_TRAINING
+    model = g.NormalModel(seed=tr["seed"])
+    t0 = time.time()
+    model.generate(tr["samples"], seed=tr["seed"] + 12, depth=tr["depth"])
+    gen_s = time.time() - t0
+    losses, train_s = model.train(epochs=tr["epochs"], batch=tr["batch"], lr0=tr["lr0"], lr1=tr["lr1"],
+                                  seed=tr["seed"] + 1)
+    say(f"learned normal: {tr['samples']} samples in {gen_s:.1f}s, {tr['epochs']} epochs in {train_s:.1f}s")
+    wl = LEARNED_WORKLOAD
+    mesh = g.hex_box(wl["N"])
+    shapes = cases.random_ellipsoids(wl["shapes_seed"])
+    out = {"workload": wl["desc"], "training": {"samples": tr["samples"], "epochs": tr["epochs"],
+                                                "generate_s": gen_s, "train_device_s": train_s,
+                                                "final_loss": float(losses[-1])}}
+    seq = None
+    for method in ("LS", "learned"):
+        d = dict(ALPHA_DICT, orientationMethod=method)
+        if residual_tol is not None:
+            d["learnedResidualTol"] = residual_tol
+        s = g.SolveVofEqu(mesh, d)
+        if method == "learned":
+            s.set_normal_model(model)
+        s.init_alpha(shapes, max_depth=5)
+        a0 = s.alpha()
+        U0, phi0 = s.deformation_field()
+        s.upload_base_flow(U0, phi0)
+        if seq is None:
+            seq = time_sequence(s, a0, wl["end_time"])
+        s.set_alpha(a0)
+        device_sync(s)
+        r0, ad0, n0 = s.timers()
+        run_steps(s, seq)
+        device_sync(s)
+        r1, ad1, n1 = s.timers()
+        hot = (r1 - r0) + (ad1 - ad0)
+        V = s.geometry()["V"]
+        a1 = s.alpha()
+        row = {"updates_per_s": (n1 - n0) / hot, "ms_per_step": 1e3 * hot / len(seq),
+               "E_s": harness.shape_error(a1, a0, V),
+               "E_v": harness.volume_error(a1, V, float(np.sum(a0 * V))), "steps": len(seq)}
+        if method == "learned":
+            st = s.learned_stats()
+            row["last_step_cells"] = {"interface": st[0], "learned": st[1], "residual_fallback": st[2],
+                                      "least_squares_only": st[3]}
+        out["plain" if method == "LS" else "learned"] = row
+        say(f"{method}: E_s {row['E_s']:.5f}, {row['updates_per_s'] / 1e6:.2f}M updates/s")
+        s.close()
+    model.close()
+    out["speedup_learned_over_plain"] = out["learned"]["updates_per_s"] / out["plain"]["updates_per_s"]
+    out["E_s_ratio_learned_over_plain"] = out["learned"]["E_s"] / out["plain"]["E_s"]
+    return out
 
 
 def time_sequence(solver, alpha0, end_time, max_steps=None):
@@ -159,6 +225,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=8, help="steps of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--stage", default="auto", help="stage timed for the roofline (auto = slowest)")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-normal comparison")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -260,6 +327,9 @@ def main():
     if K == n_total:
         res["accuracy"] = {"E_s_vs_t0": harness.shape_error(alpha_end, alpha0, V),
                            "E_v": harness.volume_error(alpha_end, V, float(np.sum(alpha0 * V)))}
+
+    if not args.no_learned:
+        res["learned_vs_plain"] = learned_comparison(say)
 
     if not args.no_cpu_baseline:
         ncpu = os.cpu_count() or 1

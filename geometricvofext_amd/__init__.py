@@ -114,6 +114,10 @@ def lib():
     L.svp_nm_forward.argtypes = [vp, vp, i64, vp, C.c_int]
     L.svp_nm_gradient.argtypes = [vp, vp, vp, i64, vp, C.POINTER(dbl), C.c_int]
     L.svp_nm_cube_fractions.argtypes = [vp, i64, vp, C.c_int]
+    L.svp_batch_define.argtypes = [vp, i32, vp]
+    L.svp_batch_set_dt.argtypes = [vp, vp, C.c_int]
+    L.svp_harness_scale_flow_batch.argtypes = [vp, vp]
+    L.svp_harness_courant_batch.argtypes = [vp, vp, vp]
     L.svp_set_normal_model.argtypes = [vp, vp]
     L.svp_learned_stats.argtypes = [vp, vp]
     L.svp_learned_residuals.argtypes = [vp, vp, C.c_int]
@@ -341,6 +345,26 @@ class SolveVofEqu:
         cnt = np.zeros(len(self.STAGES), np.int64)
         _check(lib().svp_profile_get(self.h, sec.ctypes.data, cnt.ctypes.data, int(reset)))
         return {n: (s, int(c)) for n, s, c in zip(self.STAGES, sec, cnt)}
+
+    # batch of independent cases (svp_batch_define)
+    def batch_define(self, cell_offsets):
+        off = np.ascontiguousarray(cell_offsets, dtype=np.int32)
+        _check(lib().svp_batch_define(self.h, int(off.size - 1), off.ctypes.data))
+        self.n_cases = int(off.size - 1)
+
+    def batch_set_dt(self, dt):
+        dt = np.ascontiguousarray(dt, dtype=np.float64)
+        _check(lib().svp_batch_set_dt(self.h, dt.ctypes.data, SVP_HOST))
+
+    def scale_flow_batch(self, factors):
+        f = np.ascontiguousarray(factors, dtype=np.float64)
+        _check(lib().svp_harness_scale_flow_batch(self.h, f.ctypes.data))
+
+    def courant_batch(self):
+        co = np.zeros(self.n_cases)
+        aco = np.zeros(self.n_cases)
+        _check(lib().svp_harness_courant_batch(self.h, co.ctypes.data, aco.ctypes.data))
+        return co, aco
 
     # learned orientation method (include/simplic_learned.h)
     def set_normal_model(self, model):

@@ -16,6 +16,24 @@ __global__ void k_scale_flow(int nC, int nF, int nB, const double* U0, const dou
     if (i < 3 * nB) Ub[i] = U0b ? U0b[i] * factor : 0.0;
 }
 
+// batch: each case scaled by its own factor (faces by their owner's case)
+__global__ void k_scale_flow_batch(DevMesh m, const unsigned short* caseOf, const double* factors,
+                                   const double* U0, const double* phi0, const double* U0b, double* U,
+                                   double* phi, double* Ub) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nB = m.nF - m.nIF;
+    if (i < m.nF) phi[i] = phi0[i] * factors[caseOf[m.own[i]]];
+    if (i < 3 * m.nC) U[i] = U0[i] * factors[caseOf[i / 3]];
+    if (i < 3 * nB) Ub[i] = U0b ? U0b[i] * factors[caseOf[m.own[m.nIF + i / 3]]] : 0.0;
+}
+
+void launch_scale_flow_batch(const DevMesh& m, const unsigned short* caseOf, const double* factors,
+                             const double* U0, const double* phi0, const double* U0b, double* U, double* phi,
+                             double* Ub, hipStream_t s) {
+    const long n = std::max<long>(m.nF, 3L * m.nC);
+    k_scale_flow_batch<<<cdiv(n, 256), 256, 0, s>>>(m, caseOf, factors, U0, phi0, U0b, U, phi, Ub);
+}
+
 void launch_scale_flow(int nC, int nF, int nB, const double* U0, const double* phi0, const double* U0b,
                        double factor, double* U, double* phi, double* Ub, hipStream_t s) {
     const long n = std::max<long>(std::max<long>(nF, 3L * nC), 3L * nB);
@@ -24,44 +42,92 @@ void launch_scale_flow(int nC, int nF, int nB, const double* U0, const double* p
 
 // fvc::surfaceSum(mag(phi)) in ascending face order, then max of sum/V and of
 // (pos0(alpha-0.01)*pos0(0.99-alpha)*sum)/V.  All values are >= 0, so the max
-// is an integer max over the IEEE bit patterns.
-__global__ void k_courant(DevMesh m, const double* phi, const double* alpha, unsigned long long* out) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    double r = 0.0, ra = 0.0;
-    if (c < m.nC) {
-        double s = 0.0;
-        for (int k = m.cOff[c]; k < m.cOff[c + 1]; ++k) {
-            const int e = m.cFacesAsc[k];
-            const int f = e >= 0 ? e : ~e;
-            if (f >= m.nIF && m.fEmpty[f]) continue;
-            s += fabs(phi[f]);
+// is an integer max over the IEEE bit patterns.  Grid-stride blocks reduce in
+// LDS and issue one atomic pair per block (per case in a batch, where a block
+// holding cells of several cases falls back to one atomic pair per wave).
+__device__ inline void courant_cell(const DevMesh& m, const double* phi, const double* alpha, int c, double& r,
+                                    double& ra) {
+    double s = 0.0;
+    for (int k = m.cOff[c]; k < m.cOff[c + 1]; ++k) {
+        const int e = m.cFacesAsc[k];
+        const int f = e >= 0 ? e : ~e;
+        if (f >= m.nIF && m.fEmpty[f]) continue;
+        s += fabs(phi[f]);
+    }
+    const double a = alpha[c];
+    r = s / m.V[c];
+    ra = (pos0(a - 0.01) * pos0(0.99 - a) * s) / m.V[c];
+}
+
+__device__ inline unsigned long long umax64(unsigned long long a, unsigned long long b) { return a > b ? a : b; }
+
+__global__ void __launch_bounds__(256) k_courant(DevMesh m, const double* phi, const double* alpha,
+                                                 const unsigned short* caseOf, unsigned long long* out) {
+    __shared__ unsigned long long lr[4], lra[4];
+    if (caseOf) {
+        // block-uniform loop; a wave whose cells share one case reduces first
+        for (int c0 = blockIdx.x * 256; c0 < m.nC; c0 += gridDim.x * 256) {
+            const int c = c0 + threadIdx.x;
+            const bool valid = c < m.nC;
+            double r = 0.0, ra = 0.0;
+            if (valid) courant_cell(m, phi, alpha, c, r, ra);
+            const int k = valid ? caseOf[c] : -1;
+            unsigned long long br = __double_as_longlong(r), bra = __double_as_longlong(ra);
+            const int k0 = __shfl(k, 0);
+            if (__all(!valid || k == k0)) {
+                for (int o = 32; o > 0; o >>= 1) {
+                    br = umax64(br, __shfl_xor(br, o));
+                    bra = umax64(bra, __shfl_xor(bra, o));
+                }
+                if ((threadIdx.x & 63) == 0 && k0 >= 0) {
+                    atomicMax(out + 2 * k0, br);
+                    atomicMax(out + 2 * k0 + 1, bra);
+                }
+            } else if (valid) {
+                atomicMax(out + 2 * k, br);
+                atomicMax(out + 2 * k + 1, bra);
+            }
         }
-        const double a = alpha[c];
-        r = s / m.V[c];
-        ra = (pos0(a - 0.01) * pos0(0.99 - a) * s) / m.V[c];
+        return;
     }
-    // wave max then one atomic per wave
-    unsigned long long br = __double_as_longlong(r), bra = __double_as_longlong(ra);
+    unsigned long long br = 0ull, bra = 0ull;
+    for (int c = blockIdx.x * 256 + threadIdx.x; c < m.nC; c += gridDim.x * 256) {
+        double r, ra;
+        courant_cell(m, phi, alpha, c, r, ra);
+        br = umax64(br, (unsigned long long)__double_as_longlong(r));
+        bra = umax64(bra, (unsigned long long)__double_as_longlong(ra));
+    }
     for (int o = 32; o > 0; o >>= 1) {
-        const unsigned long long x = __shfl_xor(br, o), y = __shfl_xor(bra, o);
-        br = x > br ? x : br;
-        bra = y > bra ? y : bra;
+        br = umax64(br, __shfl_xor(br, o));
+        bra = umax64(bra, __shfl_xor(bra, o));
     }
+    const int w = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
-        atomicMax(out, br);
-        atomicMax(out + 1, bra);
+        lr[w] = br;
+        lra[w] = bra;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        atomicMax(out, umax64(umax64(lr[0], lr[1]), umax64(lr[2], lr[3])));
+        atomicMax(out + 1, umax64(umax64(lra[0], lra[1]), umax64(lra[2], lra[3])));
     }
 }
 
 __global__ void k_zero_u64(unsigned long long* p, int n) {
-    const int i = threadIdx.x;
-    if (i < n) p[i] = 0ull;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0ull;
 }
 
 void launch_courant(const DevMesh& m, const double* phi, const double* alpha, unsigned long long* out,
                     hipStream_t s) {
     k_zero_u64<<<1, 64, 0, s>>>(out, 2);
-    k_courant<<<cdiv(m.nC, 256), 256, 0, s>>>(m, phi, alpha, out);
+    k_courant<<<std::min(2048, cdiv(m.nC, 256)), 256, 0, s>>>(m, phi, alpha, nullptr, out);
+}
+
+// per-case Courant numbers of a batch: out[2k], out[2k+1]
+void launch_courant_batch(const DevMesh& m, const double* phi, const double* alpha, const unsigned short* caseOf,
+                          int nCases, unsigned long long* out, hipStream_t s) {
+    k_zero_u64<<<cdiv(2 * nCases, 256), 256, 0, s>>>(out, 2 * nCases);
+    k_courant<<<std::min(4096, cdiv(m.nC, 256)), 256, 0, s>>>(m, phi, alpha, caseOf, out);
 }
 
 // ---------------------------------------------------------------------------

@@ -150,7 +150,8 @@ __global__ void __launch_bounds__(DETECT_BLOCK) k_detect_flags(DevMesh m, DevSta
     bool flag = false;
     if (c < m.nC) {
         const double a = st.alpha[c];
-        flag = (tol < a) && (a < 1.0 - tol);
+        // cells of an idle case of a batch are not reconstructed
+        flag = (tol < a) && (a < 1.0 - tol) && (!st.caseOf || st.dtCase[st.caseOf[c]] > 0.0);
         st.mixedFlag[c] = flag ? 1 : 0;
     }
     int total;
@@ -734,9 +735,10 @@ void launch_find_signed_distance(const DevMesh& m, const DevState& st, const Par
 // dVf = upwind(phi).flux(alpha)*dt (advectionTemplates.C:371): face value
 // lambda*(aP - aN) + aN with lambda = pos0(phi); zeroGradient boundary value
 // is the owner value; empty patches carry nothing.
-__global__ void __launch_bounds__(256) k_upwind(DevMesh m, DevState st, double dt) {
+__global__ void __launch_bounds__(256) k_upwind(DevMesh m, DevState st, double dtStep) {
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= m.nF) return;
+    const double dt = cell_dt(st, m.own[f], dtStep);
     const double ph = st.phi[f];
     double v;
     if (f < m.nIF) {
@@ -1046,7 +1048,7 @@ __global__ void __launch_bounds__(64) k_un0(DevMesh m, DevState st) {
 // timeIntegratedFlux with one thread per (cut cell, cell face): each downwind
 // face's volume is independent of the others
 template <bool TRIQUAD>
-__global__ void __launch_bounds__(64) k_interface_flux_faces(DevMesh m, DevState st, double dt, int slots) {
+__global__ void __launch_bounds__(64) k_interface_flux_faces(DevMesh m, DevState st, double dtStep, int slots) {
     const int nMixed = *st.nMixed;
     const long total = (long)nMixed * slots;
     for (long w = (long)blockIdx.x * blockDim.x + threadIdx.x; w < total; w += (long)gridDim.x * blockDim.x) {
@@ -1055,6 +1057,7 @@ __global__ void __launch_bounds__(64) k_interface_flux_faces(DevMesh m, DevState
         const int c = st.mixed[i];
         const int fb = m.cOff[c];
         if (k >= m.cOff[c + 1] - fb) continue;
+        const double dt = cell_dt(st, c, dtStep);
         const int f = m.cFaces[fb + k];
         const double ph = st.phi[f];
         bool down;
@@ -1084,34 +1087,6 @@ __global__ void __launch_bounds__(64) k_interface_flux_faces(DevMesh m, DevState
     }
 }
 
-// advection::timeIntegratedFlux (advection.C:85-221): each downwind face of a
-// cut cell gets its time-integrated volume; a face is downwind of exactly one
-// cell, so the writes never collide.
-__global__ void __launch_bounds__(64) k_interface_flux(DevMesh m, DevState st, double dt) {
-    const int nMixed = *st.nMixed;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nMixed; i += gridDim.x * blockDim.x) {
-        if (st.status[i] != 0) {
-            st.un0[i] = 0.0;
-            continue;
-        }
-        const int c = st.mixed[i];
-        const V3 n = ld3(st.ifN, i);
-        const double d = st.ifD[i];
-        const double Un0 = dot(interpolate_u(m, st, ld3(st.ifC, i), c), n);
-        st.un0[i] = Un0;
-        bool ok = true;
-        for (int k = m.cOff[c]; k < m.cOff[c + 1]; ++k) {
-            const int f = m.cFaces[k];
-            const double ph = st.phi[f];
-            bool down;
-            if (f < m.nIF) down = (m.own[f] == c) ? (ph >= 0.0) : (ph < 0.0);
-            else down = !m.fEmpty[f] && (ph >= 0.0);
-            if (down) st.dVf[f] = time_integrated_face_flux(m, f, n, d, Un0, dt, ph, m.magSf[f], ok);
-        }
-        if (!ok) atomicOr(st.errFlag, ERR_TOO_MANY_CROSSINGS);
-    }
-}
-
 __device__ inline bool loop_oob(double a, double aTol) { return (a - 1.0 > aTol) || (a < -aTol); }
 __device__ inline bool cell_oob(double a, double aTol) { return (a < -aTol) || (a > 1.0 + aTol); }
 
@@ -1133,13 +1108,16 @@ __device__ inline bool need_bounding(const DevMesh& m, const unsigned char* mixe
 // [-aTol, 1+aTol] for the bounding loop and writes the bounding candidates
 // (outside the bounds and next to the interface) as a bitmap, one 64-bit word
 // per wave.
-__global__ void __launch_bounds__(DETECT_BLOCK) k_update(DevMesh m, DevState st, double dt,
+__global__ void __launch_bounds__(DETECT_BLOCK) k_update(DevMesh m, DevState st, double dtStep,
                                                          const double* Sp, const double* Su) {
     const double aTol = 100.0 * SVP_SMALL;
-    const double rDeltaT = 1.0 / dt;
     const int c = blockIdx.x * DETECT_BLOCK + threadIdx.x;
     bool cand = false, oob = false;
-    if (c < m.nC) {
+    const double dt = c < m.nC ? cell_dt(st, c, dtStep) : 1.0;
+    if (c < m.nC && dt == 0.0) {
+        st.alpha[c] = st.alphaOld[c];  // idle case of a batch: alpha unchanged
+    } else if (c < m.nC) {
+        const double rDeltaT = 1.0 / dt;
         double sI = 0.0;
         for (int k = m.cOff[c]; k < m.cOff[c + 1]; ++k) {
             const int e = m.cFacesAsc[k];
@@ -1182,20 +1160,32 @@ __device__ inline bool face_downwind_of(const DevMesh& m, const double* phi, int
 }
 
 // workgroup exclusive scan of one int per thread (blockDim.x == 1024)
-__device__ inline int wg_exclusive_scan(int v, int* lds, int& total) {
-    const int t = threadIdx.x;
-    lds[t] = v;
-    __syncthreads();
-    for (int o = 1; o < 1024; o <<= 1) {
-        const int x = (t >= o) ? lds[t - o] : 0;
-        __syncthreads();
-        lds[t] += x;
-        __syncthreads();
+// exclusive scan over a workgroup of 1024 threads: wave scans by shuffles,
+// then a scan of the 16 wave totals; lds needs 16 ints
+__device__ inline int wg_scan1024(int v, int* lds, int& total) {
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    int x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o);
+        if (lane >= o) x += y;
     }
-    total = lds[1023];
-    const int r = lds[t] - v;
+    if (lane == 63) lds[w] = x;
     __syncthreads();
-    return r;
+    if (w == 0) {
+        int s2 = lane < 16 ? lds[lane] : 0;
+#pragma unroll
+        for (int o = 1; o < 16; o <<= 1) {
+            const int y = __shfl_up(s2, o);
+            if (lane >= o) s2 += y;
+        }
+        if (lane < 16) lds[lane] = s2;
+    }
+    __syncthreads();
+    const int pre = w > 0 ? lds[w - 1] : 0;
+    total = lds[15];
+    __syncthreads();
+    return pre + x - v;
 }
 
 // First redistribution loop of boundFlux (advectionTemplates.C:260-330 with
@@ -1324,112 +1314,77 @@ __device__ inline void bound_cell(const DevMesh& m, const DevState& st, int c, d
 // limitFlux (advectionTemplates.C:118-215), one pass = the kernels below.
 // The pass runs only while cells lie outside [-aTol, 1+aTol] (bound count of
 // the update kernel, kept current by the apply kernel):
-//   count/scan/write  ascending candidates from the bitmap
+//   collect           ascending candidates from the bitmap
 //   prep              first redistribution loop of every candidate (it
 //                     decides which cells write and opens correctedFaces)
-//   scan2/scatter     correctedFaces positions, predecessor counts, frontier
+//   scan2/scatter     correctedFaces positions and the cells they touch,
+//                     per-candidate records, predecessor counts, frontier
 //   round r           full boundFlux of the ready cells; successors whose
 //                     last predecessor completed join the next frontier
 //   tail              the remaining rounds inside one workgroup
-//   touch/apply       corrections applied per cell in correctedFaces order,
+//   apply             corrections applied per cell in correctedFaces order,
 //                     dVf of the corrected faces, bitmap and bound count
 //   reset             dVfCorrectionValues = 0 for the next pass
 // bcount: [0] candidates, [1] corrected faces, [2] touched cells, [3+r]
-// frontier size of round r.
-constexpr int BND_GRID_ROUNDS = 4;
-constexpr int BND_CTRL = 3 + BND_GRID_ROUNDS + 2;
+// frontier size of round r (r <= st.gridRounds), [BND_CTRL-1] rounds.
+constexpr int BND_CTRL = 3 + MAX_GRID_ROUNDS + 2;
 
-__global__ void __launch_bounds__(256) k_bnd_count(DevState st, int nWords) {
-    __shared__ int lds[4];
-    if (*st.oobCount == 0) return;
-    const int w = blockIdx.x * 256 + threadIdx.x;
-    const int c = (w < nWords) ? __popcll(st.candBits[w]) : 0;
-    int s = c;
-    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
-    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) st.blockCount[blockIdx.x] = lds[0] + lds[1] + lds[2] + lds[3];
-}
-
-// exclusive scan of counts[0..n) in one workgroup; total -> *out.  With a
-// zero bound count the pass is off: total 0.
-__global__ void __launch_bounds__(1024) k_bnd_scan(int* counts, int n, int* out, const int* gate, int* zero,
-                                                   int nZero) {
-    __shared__ int lds[1024];
-    __shared__ int carry;
+// ascending candidates of the pass from the bitmap, in one workgroup: every
+// thread owns a contiguous run of bitmap words.  With a zero bound count the
+// pass is off (no candidates).  Also clears the per-pass counters.
+__global__ void __launch_bounds__(1024) k_bnd_collect(DevState st, int nWords) {
+    __shared__ int lds[16];
     const int t = threadIdx.x;
-    for (int i = t; i < nZero; i += 1024) zero[i] = 0;
-    if (gate && *gate == 0) {
-        if (t == 0) *out = 0;
+    if (t >= 1 && t < BND_CTRL) st.bcount[t] = 0;
+    if (*st.oobCount == 0) {
+        if (t == 0) st.bcount[0] = 0;
         return;
     }
-    if (t == 0) carry = 0;
-    __syncthreads();
-    for (int base = 0; base < n; base += 1024) {
-        const int i = base + t;
-        const int v = (i < n) ? counts[i] : 0;
-        int tot;
-        const int ex = wg_exclusive_scan(v, lds, tot);
-        if (i < n) counts[i] = carry + ex;
-        __syncthreads();
-        if (t == 0) carry += tot;
-        __syncthreads();
+    const int per = (nWords + 1023) / 1024;
+    const int w0 = min(nWords, t * per), w1 = min(nWords, w0 + per);
+    int cnt = 0;
+    for (int w = w0; w < w1; ++w) cnt += __popcll(st.candBits[w]);
+    int total;
+    int off = wg_scan1024(cnt, lds, total);
+    for (int w = w0; w < w1; ++w) {
+        unsigned long long bits = st.candBits[w];
+        while (bits) {
+            const int bit = __ffsll((long long)bits) - 1;
+            bits &= bits - 1;
+            st.cand[off++] = w * 64 + bit;
+        }
     }
-    if (t == 0) *out = carry;
-}
-
-__global__ void __launch_bounds__(256) k_bnd_write(DevState st, int nWords) {
-    __shared__ int lds[256];
-    if (*st.bcount == 0) return;
-    const int w = blockIdx.x * 256 + threadIdx.x;
-    unsigned long long bits = (w < nWords) ? st.candBits[w] : 0ull;
-    // block exclusive scan of popcounts
-    const int c = __popcll(bits);
-    lds[threadIdx.x] = c;
-    __syncthreads();
-    for (int o = 1; o < 256; o <<= 1) {
-        const int x = (threadIdx.x >= o) ? lds[threadIdx.x - o] : 0;
-        __syncthreads();
-        lds[threadIdx.x] += x;
-        __syncthreads();
-    }
-    int off = st.blockCount[blockIdx.x] + lds[threadIdx.x] - c;
-    while (bits) {
-        const int bit = __ffsll((long long)bits) - 1;
-        bits &= bits - 1;
-        st.cand[off++] = w * 64 + bit;
-    }
+    if (t == 0) st.bcount[0] = total;
 }
 
 __global__ void __launch_bounds__(256) k_bnd_prep(DevMesh m, DevState st, double dt) {
     const int nCand = st.bcount[0];
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nCand; i += gridDim.x * blockDim.x) {
         const int c = st.cand[i];
-        const int n = bound_first_faces(m, st, c, dt, nullptr);
+        const int n = bound_first_faces(m, st, c, cell_dt(st, c, dt), nullptr);
         st.nFirst[i] = n;
         st.active[i] = n > 0 ? 1 : 0;
         if (n > 0) st.actIdx[c] = i;
     }
 }
 
+// correctedFaces offsets: exclusive scan of the first-loop face counts
 __global__ void __launch_bounds__(1024) k_bnd_scan2(DevState st) {
-    __shared__ int lds[1024];
-    __shared__ int carry;
+    __shared__ int lds[16];
     const int n = st.bcount[0];
     const int t = threadIdx.x;
-    if (t == 0) carry = 0;
-    __syncthreads();
-    for (int base = 0; base < n; base += 1024) {
-        const int i = base + t;
-        const int v = (i < n) ? st.nFirst[i] : 0;
-        int tot;
-        const int ex = wg_exclusive_scan(v, lds, tot);
-        if (i < n) st.nFirst[i] = carry + ex;
-        __syncthreads();
-        if (t == 0) carry += tot;
-        __syncthreads();
+    const int per = (n + 1023) / 1024;
+    const int i0 = min(n, t * per), i1 = min(n, i0 + per);
+    int sum = 0;
+    for (int i = i0; i < i1; ++i) sum += st.nFirst[i];
+    int total;
+    int off = wg_scan1024(sum, lds, total);
+    for (int i = i0; i < i1; ++i) {
+        const int v = st.nFirst[i];
+        st.nFirst[i] = off;
+        off += v;
     }
-    if (t == 0) st.bcount[1] = carry;
+    if (t == 0) st.bcount[1] = total;
 }
 
 template <int MF>
@@ -1439,11 +1394,17 @@ __global__ void __launch_bounds__(256) k_bnd_scatter(DevMesh m, DevState st, dou
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nCand; i += gridDim.x * blockDim.x) {
         if (!st.active[i]) continue;
         const int c = st.cand[i];
-        const int n = bound_first_faces(m, st, c, dt, faces);
+        const int n = bound_first_faces(m, st, c, cell_dt(st, c, dt), faces);
         const int pos = st.nFirst[i];
         for (int k = 0; k < n; ++k) {
-            st.corrPos[faces[k]] = pos + k;
-            st.corrList[pos + k] = faces[k];
+            const int f = faces[k];
+            st.corrPos[f] = pos + k;
+            st.corrList[pos + k] = f;
+            // cells whose alpha takes this face's correction
+            const int cs[2] = {m.own[f], f < m.nIF ? m.nbr[f] : -1};
+            for (int q = 0; q < 2; ++q)
+                if (cs[q] >= 0 && atomicCAS(&st.touchedFlag[cs[q]], 0, 1) == 0)
+                    st.touched[atomicAdd(&st.bcount[2], 1)] = cs[q];
         }
         // the record the dependent rounds work from
         BndRec<MF>& R = ((BndRec<MF>*)st.bndRec)[i];
@@ -1569,19 +1530,27 @@ __device__ inline void bound_cell_rec(const BndRec<MF>& R, const DevState& st, i
         if (k < nf && down(k) && writable(k)) st.corr[R.fid[k]] = cor[k];
 }
 
-// completes cell i of frontier r and releases its successors into frontier r+1
-template <int MF>
+// completes cell i of frontier r and releases its successors into frontier
+// r+1; release(a) drops one pending predecessor of candidate a and returns
+// true when it was the last
+template <int MF, class Release>
 __device__ inline void bnd_complete(const DevMesh& m, const DevState& st, int i, double dt, double rDeltaT,
-                                    const double* Sp, const double* Su, int* qnext, int* nNext) {
+                                    const double* Sp, const double* Su, int* qnext, int* nNext, Release release) {
     const BndRec<MF>& R = ((const BndRec<MF>*)st.bndRec)[i];
+    const int c = st.cand[i];
     // successors read these corrections in a later round: the next kernel of
     // the grid rounds, or after the workgroup barrier of the tail
-    bound_cell_rec<MF>(R, st, st.cand[i], dt, rDeltaT, Sp, Su);
+    if (st.caseOf) {
+        const double dtc = st.dtCase[st.caseOf[c]];
+        bound_cell_rec<MF>(R, st, c, dtc, 1.0 / dtc, Sp, Su);
+    } else {
+        bound_cell_rec<MF>(R, st, c, dt, rDeltaT, Sp, Su);
+    }
 #pragma unroll
     for (int k = 0; k < MF; ++k) {
         const int a = R.succ[k];
         if (a < 0) continue;
-        if (atomicSub(&st.pend[a], 1) == 1) qnext[atomicAdd(nNext, 1)] = a;
+        if (release(a)) qnext[atomicAdd(nNext, 1)] = a;
     }
 }
 
@@ -1591,31 +1560,59 @@ __global__ void __launch_bounds__(256) k_bnd_round(DevMesh m, DevState st, int r
     const int* q = st.queue + (long)(r & 1) * m.nC;
     int* qnext = st.queue + (long)((r + 1) & 1) * m.nC;
     const double rDeltaT = 1.0 / dt;
+    auto release = [&](int a) { return atomicSub(&st.pend[a], 1) == 1; };
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x)
-        bnd_complete<MF>(m, st, q[k], dt, rDeltaT, Sp, Su, qnext, &st.bcount[3 + r + 1]);
+        bnd_complete<MF>(m, st, q[k], dt, rDeltaT, Sp, Su, qnext, &st.bcount[3 + r + 1], release);
 }
 
-// remaining rounds in one workgroup (frontiers are small by now); every round
-// completes at least one cell because edges point to higher cell numbers
+// Remaining rounds in one workgroup (frontiers are small by now); every round
+// completes at least one cell because edges point to higher cell numbers.
+// With at most TAIL_PEND_CAP candidates the predecessor counts move to LDS
+// (one byte each, four per word), so releasing a successor is an LDS atomic.
+constexpr int TAIL_PEND_WORDS = 8192;
+constexpr int TAIL_PEND_CAP = 4 * TAIL_PEND_WORDS;
+
 template <int MF>
 __global__ void __launch_bounds__(1024) k_bnd_tail(DevMesh m, DevState st, double dt, const double* Sp,
                                                    const double* Su) {
     __shared__ int sQ, sNext, sRounds;
+    __shared__ unsigned int sPend[TAIL_PEND_WORDS];
     const int t = threadIdx.x;
     const double rDeltaT = 1.0 / dt;
+    const int R = st.gridRounds;
+    const int nCand = st.bcount[0];
+    const bool ldsPend = nCand <= TAIL_PEND_CAP;
     if (t == 0) {
-        sQ = st.bcount[3 + BND_GRID_ROUNDS];
+        sQ = st.bcount[3 + R];
         sNext = 0;
-        sRounds = BND_GRID_ROUNDS;
+        sRounds = R;
     }
+    if (ldsPend)
+        for (int w = t; w < (nCand + 3) / 4; w += 1024) {
+            unsigned int word = 0u;
+            for (int b = 0; b < 4; ++b)
+                if (4 * w + b < nCand) word |= ((unsigned int)st.pend[4 * w + b] & 0xFFu) << (8 * b);
+            sPend[w] = word;
+        }
     __syncthreads();
-    int cur = BND_GRID_ROUNDS & 1;
+    auto releaseLds = [&](int a) {
+        const int sh = 8 * (a & 3);
+        const unsigned int old = atomicSub(&sPend[a >> 2], 1u << sh);
+        return ((old >> sh) & 0xFFu) == 1u;
+    };
+    auto releaseGlobal = [&](int a) { return atomicSub(&st.pend[a], 1) == 1; };
+    int cur = R & 1;
     while (true) {
         const int qn = sQ;
         if (qn == 0) break;
         const int* q = st.queue + (long)cur * m.nC;
         int* qnext = st.queue + (long)(cur ^ 1) * m.nC;
-        for (int k = t; k < qn; k += 1024) bnd_complete<MF>(m, st, q[k], dt, rDeltaT, Sp, Su, qnext, &sNext);
+        if (ldsPend)
+            for (int k = t; k < qn; k += 1024)
+                bnd_complete<MF>(m, st, q[k], dt, rDeltaT, Sp, Su, qnext, &sNext, releaseLds);
+        else
+            for (int k = t; k < qn; k += 1024)
+                bnd_complete<MF>(m, st, q[k], dt, rDeltaT, Sp, Su, qnext, &sNext, releaseGlobal);
         __syncthreads();
         if (t == 0) {
             sQ = sNext;
@@ -1626,18 +1623,6 @@ __global__ void __launch_bounds__(1024) k_bnd_tail(DevMesh m, DevState st, doubl
         __syncthreads();
     }
     if (t == 0) st.bcount[BND_CTRL - 1] = sRounds;
-}
-
-__global__ void __launch_bounds__(256) k_bnd_touch(DevMesh m, DevState st) {
-    const int nCorr = st.bcount[1];
-    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < nCorr; q += gridDim.x * blockDim.x) {
-        const int f = st.corrList[q];
-        const int cs[2] = {m.own[f], f < m.nIF ? m.nbr[f] : -1};
-        for (int s = 0; s < 2; ++s) {
-            if (cs[s] < 0) continue;
-            if (atomicCAS(&st.touchedFlag[cs[s]], 0, 1) == 0) st.touched[atomicAdd(&st.bcount[2], 1)] = cs[s];
-        }
-    }
 }
 
 // alpha of every touched cell in correctedFaces order (advectionTemplates.C:164-192)
@@ -1653,36 +1638,60 @@ __global__ void __launch_bounds__(256) k_bnd_apply(DevMesh m, DevState st) {
             continue;
         }
         const int c = st.touched[idx];
-        int kpos[MF];
+        const int b = m.cOff[c], nf = m.cOff[c + 1] - b;
+        // all face labels, then all correctedFaces positions, as independent loads
+        int fid[MF], kpos[MF];
+#pragma unroll
+        for (int k = 0; k < MF; ++k) fid[k] = k < nf ? m.cFaces[b + k] : 0;
+#pragma unroll
+        for (int k = 0; k < MF; ++k) kpos[k] = k < nf ? st.corrPos[fid[k]] : -1;
         double kval[MF];
         bool kown[MF];
-        int nk = 0;
-        for (int kk = m.cOff[c]; kk < m.cOff[c + 1]; ++kk) {
-            const int f = m.cFaces[kk];
-            const int ps = st.corrPos[f];
-            if (ps < 0) continue;
-            int j = nk - 1;
-            while (j >= 0 && kpos[j] > ps) {
-                kpos[j + 1] = kpos[j];
-                kval[j + 1] = kval[j];
-                kown[j + 1] = kown[j];
-                --j;
+#pragma unroll
+        for (int k = 0; k < MF; ++k) {
+            kval[k] = 0.0;
+            kown[k] = false;
+            if (kpos[k] >= 0) {
+                kval[k] = face_value(m, st.corr, fid[k]);
+                kown[k] = (m.own[fid[k]] == c);
             }
-            kpos[j + 1] = ps;
-            kval[j + 1] = face_value(m, st.corr, f);
-            kown[j + 1] = (m.own[f] == c);
-            nk++;
         }
         const double before = st.alpha[c];
+        const double Vc = m.V[c];
         double a = before;
-        for (int q = 0; q < nk; ++q) a = kown[q] ? a - kval[q] / m.V[c] : a + kval[q] / m.V[c];
+        // corrections in ascending correctedFaces position (selection over
+        // the unrolled slots: positions are distinct)
+        int last = -1;
+        for (int q = 0; q < MF; ++q) {
+            int best = -1, bpos = 0x7fffffff;
+#pragma unroll
+            for (int k = 0; k < MF; ++k)
+                if (kpos[k] > last && kpos[k] < bpos) {
+                    bpos = kpos[k];
+                    best = k;
+                }
+            if (best < 0) break;
+            double v = 0.0;
+            bool own = false;
+#pragma unroll
+            for (int k = 0; k < MF; ++k)
+                if (k == best) {
+                    v = kval[k];
+                    own = kown[k];
+                }
+            a = own ? a - v / Vc : a + v / Vc;
+            last = bpos;
+        }
         st.alpha[c] = a;
         const int dOob = (int)loop_oob(a, aTol) - (int)loop_oob(before, aTol);
         if (dOob) atomicAdd(st.oobCount, dOob);
         const bool candNew = cell_oob(a, aTol) && need_bounding(m, st.mixedFlag, c);
         const unsigned long long bit = 1ull << (c & 63);
-        if (candNew) atomicOr(&st.candBits[c >> 6], bit);
-        else atomicAnd(&st.candBits[c >> 6], ~bit);
+        const bool candOld = (st.candBits[c >> 6] & bit) != 0ull;
+        if (candNew != candOld) {
+            if (candNew) atomicOr(&st.candBits[c >> 6], bit);
+            else atomicAnd(&st.candBits[c >> 6], ~bit);
+        }
         st.touchedFlag[c] = 0;
     }
 }
@@ -1703,7 +1712,7 @@ __global__ void __launch_bounds__(256) k_bnd_reset(DevMesh m, DevState st, int p
     }
     if (blockIdx.x == 0 && threadIdx.x == 0 && pass < MAX_BOUND_PASSES && nCand > 0) {
         int nAct = 0;
-        for (int r = 0; r <= BND_GRID_ROUNDS; ++r) nAct += st.bcount[3 + r];
+        for (int r = 0; r <= st.gridRounds; ++r) nAct += st.bcount[3 + r];
         st.boundStats[4 * pass + 0] = nCand;
         st.boundStats[4 * pass + 1] = nAct;
         st.boundStats[4 * pass + 2] = st.bcount[BND_CTRL - 1];
@@ -1716,17 +1725,15 @@ template <int MF>
 void launch_bound_pass(const DevMesh& m, const DevState& st, int pass, double dt, const double* Sp,
                        const double* Su, hipStream_t s) {
     const int nWords = (m.nC + 63) / 64;
-    const int nbw = cdiv(nWords, 256);
-    const int grid = std::min(1024, cdiv(m.nC, 256));
-    k_bnd_count<<<nbw, 256, 0, s>>>(st, nWords);
-    k_bnd_scan<<<1, 1024, 0, s>>>(st.blockCount, nbw, st.bcount, st.oobCount, st.bcount + 1, BND_CTRL - 1);
-    k_bnd_write<<<nbw, 256, 0, s>>>(st, nWords);
+    // candidate-sized work: a modest grid-stride grid launches faster than
+    // one thread per cell of the mesh
+    const int grid = std::max(1, std::min(st.bndGrid, cdiv(m.nC, 256)));
+    k_bnd_collect<<<1, 1024, 0, s>>>(st, nWords);
     k_bnd_prep<<<grid, 256, 0, s>>>(m, st, dt);
     k_bnd_scan2<<<1, 1024, 0, s>>>(st);
     k_bnd_scatter<MF><<<grid, 256, 0, s>>>(m, st, dt);
-    for (int r = 0; r < BND_GRID_ROUNDS; ++r) k_bnd_round<MF><<<grid, 256, 0, s>>>(m, st, r, dt, Sp, Su);
+    for (int r = 0; r < st.gridRounds; ++r) k_bnd_round<MF><<<grid, 256, 0, s>>>(m, st, r, dt, Sp, Su);
     k_bnd_tail<MF><<<1, 1024, 0, s>>>(m, st, dt, Sp, Su);
-    k_bnd_touch<<<grid, 256, 0, s>>>(m, st);
     k_bnd_apply<MF><<<grid, 256, 0, s>>>(m, st);
     k_bnd_reset<<<grid, 256, 0, s>>>(m, st, pass);
 }
@@ -1741,9 +1748,10 @@ void launch_bound(const DevMesh& m, const DevState& st, const Params& p, double 
 }
 
 // applyBruteForceBounding (advection.C:291-308)
-__global__ void k_snap_clip(int nC, double* alpha, double snapTol, int clip) {
+__global__ void k_snap_clip(int nC, double* alpha, double snapTol, int clip, DevState st) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nC) return;
+    if (st.caseOf && st.dtCase[st.caseOf[c]] == 0.0) return;  // idle case
     double a = alpha[c];
     if (snapTol > 0.0) a = a * pos0(a - snapTol) * neg0(a - (1.0 - snapTol)) + pos0(a - (1.0 - snapTol));
     if (clip) a = fmin(1.0, fmax(0.0, a));
@@ -1776,20 +1784,22 @@ void launch_advect_stage(int stage, const DevMesh& m, const DevState& st, const 
             break;
         case STAGE_SNAPCLIP:
             if (p.snapTol > 0.0 || p.clip)
-                k_snap_clip<<<cdiv(m.nC, 256), 256, 0, s>>>(m.nC, st.alpha, p.snapTol, p.clip);
+                k_snap_clip<<<cdiv(m.nC, 256), 256, 0, s>>>(m.nC, st.alpha, p.snapTol, p.clip, st);
             break;
         default:
             break;
     }
 }
 
-__global__ void k_alpha_phi(int nF, const double* dVf, double* alphaPhi, double dt) {
+__global__ void k_alpha_phi(DevMesh m, DevState st, double dtStep) {
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
-    if (f < nF) alphaPhi[f] = dVf[f] / dt;
+    if (f >= m.nF) return;
+    const double dt = cell_dt(st, m.own[f], dtStep);
+    st.alphaPhi[f] = dt > 0.0 ? st.dVf[f] / dt : 0.0;
 }
 
 void launch_alpha_phi(const DevMesh& m, const DevState& st, double dt, hipStream_t s) {
-    k_alpha_phi<<<cdiv(m.nF, 256), 256, 0, s>>>(m.nF, st.dVf, st.alphaPhi, dt);
+    k_alpha_phi<<<cdiv(m.nF, 256), 256, 0, s>>>(m, st, dt);
 }
 
 }  // namespace svp

@@ -83,12 +83,24 @@ struct DevState {
     int* boundStats;    // [4*maxPasses]: candidates, active, rounds, corrected faces
     int* bcount;        // [16] per-pass counters of the bounding kernels
     int un0Fused;       // plane placement by 8-lane groups, which also computes Un0
+    int gridRounds;     // bounding rounds run as grid-wide kernels before the one-workgroup tail
+    int bndGrid;        // workgroups of the grid-stride bounding kernels
     void* bndRec;       // nC records BndRec<MF> of the active bounding candidates
     int maxCellFaces;
+    // batch of independent cases in one mesh: case of every cell and the
+    // per-case time step (0 = the case is idle this step); null when the
+    // handle holds one case and the step's dt is passed by value
+    const unsigned short* caseOf;
+    const double* dtCase;
     double* minmax;     // [4]: min, max before bounding, min, max after
     int* errFlag;       // [1] device error bits
     unsigned long long* nIcTotal;  // [1] running sum of mixed-cell counts
 };
+
+// time step of cell c: its case's in a batch, else the step's
+__device__ inline double cell_dt(const DevState& st, int c, double dt) {
+    return st.caseOf ? st.dtCase[st.caseOf[c]] : dt;
+}
 
 struct Params {
     double mixedCellTol;
@@ -105,6 +117,7 @@ enum : int {
     ERR_BOUND_SCHEDULE = 4,
 };
 constexpr int MAX_BOUND_PASSES = 64;
+constexpr int MAX_GRID_ROUNDS = 8;
 
 constexpr int DETECT_BLOCK = 256;
 

@@ -294,3 +294,55 @@ def tet_box(N: int, perturb: float = 0.0, seed: int = 0, length: float = 1.0) ->
         patch_type=np.full(len(names), PATCH_WALL, dtype=np.int32),
         name=f"tet{N}" + (f"_p{perturb}_s{seed}" if perturb > 0 else ""),
     )
+
+
+def union_mesh(meshes, name: str = "batch") -> tuple[PolyMesh, np.ndarray]:
+    """Disjoint union of independent meshes as one polyMesh, for running many
+    cases in the same launches.  Internal faces come case by case (each case's
+    upper-triangular order, shifted), then every case's boundary patches.
+    Within every cell the relative order of its faces, and every stencil and
+    point-source order, stay those of the case on its own, so each case's
+    results are bit-identical to a separate run.  Returns the mesh and the
+    cell offsets of the cases (n_cases + 1)."""
+    pt_off = np.cumsum([0] + [m.n_points for m in meshes])
+    c_off = np.cumsum([0] + [m.n_cells for m in meshes])
+    nif_tot = sum(m.n_internal_faces for m in meshes)
+    pts = np.concatenate([m.points for m in meshes])
+    fl_int, fl_bnd, own_int, own_bnd, nei = [], [], [], [], []
+    names, starts, sizes, types = [], [], [], []
+    start = nif_tot
+    for k, m in enumerate(meshes):
+        nif = m.n_internal_faces
+        lens = np.diff(m.face_offsets)
+        fp = m.face_points.astype(np.int64) + pt_off[k]
+        split = m.face_offsets[nif]
+        fl_int.append((lens[:nif], fp[:split]))
+        own_int.append(m.owner[:nif].astype(np.int64) + c_off[k])
+        nei.append(m.neighbour.astype(np.int64) + c_off[k])
+        fl_bnd.append((lens[nif:], fp[split:]))
+        own_bnd.append(m.owner[nif:].astype(np.int64) + c_off[k])
+        for p, pname in enumerate(m.patch_names):
+            names.append(f"case{k}_{pname}")
+            starts.append(start + int(m.patch_start[p]) - nif)
+            sizes.append(int(m.patch_size[p]))
+            types.append(int(m.patch_type[p]))
+        start += m.n_faces - nif
+    lens = np.concatenate([l for l, _ in fl_int] + [l for l, _ in fl_bnd])
+    fpts = np.concatenate([f for _, f in fl_int] + [f for _, f in fl_bnd]).astype(np.int32)
+    owner = np.concatenate(own_int + own_bnd).astype(np.int32)
+    offs = np.zeros(lens.size + 1, dtype=np.int64)
+    np.cumsum(lens, out=offs[1:])
+    u = PolyMesh(
+        points=pts,
+        face_offsets=offs.astype(np.int32),
+        face_points=fpts,
+        owner=owner,
+        neighbour=np.concatenate(nei).astype(np.int32),
+        n_cells=int(c_off[-1]),
+        patch_names=names,
+        patch_start=np.array(starts, dtype=np.int32),
+        patch_size=np.array(sizes, dtype=np.int32),
+        patch_type=np.array(types, dtype=np.int32),
+        name=name,
+    )
+    return u, c_off.astype(np.int32)

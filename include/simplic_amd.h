@@ -64,7 +64,7 @@ typedef struct {
     int32_t clip;                  /* 1 */
     int32_t split_warped_face;     /* 0 */
     int32_t write_plic_fields;     /* 0 */
-    double learned_residual_tol;   /* learnedResidualTol, "learned" method only: 0.02 */
+    double learned_residual_tol;   /* learnedResidualTol, "learned" method only: 0.25 (>= 1: no fallback test) */
 } svp_dict;
 
 /* Defaults of every key (reconstruction.C:502-516, advection.C:455-457). */
@@ -160,6 +160,20 @@ int svp_profile_get(svp_handle* h, double* seconds, int64_t* counts, int reset);
 int svp_synchronize(svp_handle* h);
 
 /* Message of the last failing call on this thread. */
+/* Batch of independent cases in one mesh (no reference counterpart: the
+ * reference runs one case per process).  Cells [cell_offsets[k],
+ * cell_offsets[k+1]) form case k; no internal face may join two cases
+ * (SVP_E_MESH).  Build the mesh as the disjoint union of the cases with each
+ * case's internal faces in its own order, then its boundary faces: every case
+ * then gives bit-identical results to a separate handle.  Afterwards the dt
+ * argument of svp_advect is ignored: each case advances by its entry of
+ * svp_batch_set_dt, and a case with dt 0 is idle (not reconstructed, alpha
+ * unchanged).  Cases share nAlphaBounds; a bounding pass only touches cases
+ * with cells out of bounds, as each case on its own would. */
+int svp_batch_define(svp_handle* h, int32_t n_cases, const int32_t* cell_offsets);
+/* Per-case time steps (n_cases values, >= 0) for the next reconstruct/advect. */
+int svp_batch_set_dt(svp_handle* h, const double* dt, int where);
+
 const char* svp_last_error(void);
 
 #ifdef __cplusplus

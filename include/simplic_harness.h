@@ -43,6 +43,11 @@ int svp_harness_init_alpha(svp_handle* h, const double* shapes, int n_shapes, in
  * Synchronises with the handle's stream. */
 int svp_harness_courant(svp_handle* h, double* max_sum_phi_by_v, double* max_alpha_sum_phi_by_v);
 
+/* Batch handles (svp_batch_define): each case's flow scaled by its own
+ * factor (n_cases host values), and the two maxima per case (n_cases each). */
+int svp_harness_scale_flow_batch(svp_handle* h, const double* factors);
+int svp_harness_courant_batch(svp_handle* h, double* max_sum_phi_by_v, double* max_alpha_sum_phi_by_v);
+
 #ifdef __cplusplus
 }
 #endif

@@ -90,7 +90,7 @@ int svp_learned_stats(svp_handle* h, int64_t* out4);
 
 /* Residuals (mean |fraction - alpha| over the 26 neighbours) of the last
  * reconstruct, per interface-cell position; -1 where no learned normal was
- * tried. */
+ * tried.  SVP_E_ARG when learnedResidualTol >= 1 (the test is not run). */
 int svp_learned_residuals(svp_handle* h, double* out, int where);
 
 #ifdef __cplusplus

@@ -213,7 +213,7 @@ def test_learned_reconstruction_against_oracle(g, trained, oracle_lib):
     from geometricvofext_amd import harness  # noqa: F401
     m, _, _ = trained
     mesh = g.hex_box(20)
-    tol = 0.03
+    tol = 0.1
     d = dict(orientationMethod="learned", nAlphaBounds=3, snapTol=0.0, clip=False, learnedResidualTol=tol)
     s = g.SolveVofEqu(mesh, d)
     s.set_normal_model(m)

@@ -156,3 +156,19 @@ def test_parity_split_warped_faces(oracle_lib):
     _say("parity", mesh.name, "splitWarpedFace")
     exact, _, _ = _run_parity(mesh, oracle_lib, a0, 4, splitWarpedFace=True)
     assert exact
+
+
+def test_parity_tets_rotation_spheres(oracle_lib):
+    """Tetrahedral case at reduced size: 6-tet split with 10% vertex
+    perturbation, 8 seeded spheres from the exact-cut initialiser, solid-body
+    rotation about a seeded axis through the cube centre."""
+    import geometricvofext_amd as g
+    from geometricvofext_amd import cases
+    mesh = g.tet_box(12, perturb=0.1, seed=2)
+    init = g.SolveVofEqu(mesh, dict(orientationMethod="LS"))
+    init.init_alpha(cases.random_spheres(7), max_depth=4)
+    a0 = init.alpha()
+    U0, phi0 = init.rotation_field(cases.rotation_omega(3, 3.0), (0.5, 0.5, 0.5))
+    _say("parity", mesh.name, "rotation, 8 spheres")
+    exact, _, _ = _run_parity(mesh, oracle_lib, a0, 6, flow=(U0, phi0), nAlphaBounds=3, clip=False)
+    assert exact
