@@ -114,6 +114,8 @@ def lib():
     L.svp_nm_forward.argtypes = [vp, vp, i64, vp, C.c_int]
     L.svp_nm_gradient.argtypes = [vp, vp, vp, i64, vp, C.POINTER(dbl), C.c_int]
     L.svp_nm_cube_fractions.argtypes = [vp, i64, vp, C.c_int]
+    L.svp_bound_stats.argtypes = [vp, vp, C.c_int]
+    L.svp_bound_totals.argtypes = [vp, vp]
     L.svp_batch_define.argtypes = [vp, i32, vp]
     L.svp_batch_set_dt.argtypes = [vp, vp, C.c_int]
     L.svp_harness_scale_flow_batch.argtypes = [vp, vp]
@@ -345,6 +347,21 @@ class SolveVofEqu:
         cnt = np.zeros(len(self.STAGES), np.int64)
         _check(lib().svp_profile_get(self.h, sec.ctypes.data, cnt.ctypes.data, int(reset)))
         return {n: (s, int(c)) for n, s, c in zip(self.STAGES, sec, cnt)}
+
+    def bound_stats(self):
+        """Per bounding pass of the last advect (and since the last call):
+        rows of (candidates, active, rounds, corrected faces); resets."""
+        out = np.zeros(4 * 64, np.int32)
+        _check(lib().svp_bound_stats(self.h, out.ctypes.data, out.size))
+        rows = out.reshape(-1, 4)
+        return rows[rows[:, 0] > 0]
+
+    def bound_totals(self):
+        """Running sums over the bounding passes with candidates: candidates,
+        active, rounds, corrected faces, passes."""
+        out = np.zeros(5, np.uint64)
+        _check(lib().svp_bound_totals(self.h, out.ctypes.data))
+        return [int(v) for v in out]
 
     # batch of independent cases (svp_batch_define)
     def batch_define(self, cell_offsets):

@@ -537,7 +537,6 @@ __global__ void __launch_bounds__(64) k_infer(const int* lat, const double* alph
                                               const int* nMixedP, const uint16_t* wb, const float* pq, double* ifN,
                                               unsigned char* learned, int* lsList, int* nLs) {
     __shared__ double raw[16][NIN];
-    __shared__ __attribute__((aligned(16))) uint16_t in[16][KIN];
     const int l = threadIdx.x, r = l % 16, q = l / 16;
     const int nMixed = *nMixedP;
     for (int i0 = blockIdx.x * 16; i0 < nMixed; i0 += gridDim.x * 16) {
@@ -547,15 +546,18 @@ __global__ void __launch_bounds__(64) k_infer(const int* lat, const double* alph
         const bool valid = live && lat[(long)c * NIN] >= 0;
         for (int L = q; L < NIN; L += 4) raw[r][L] = valid ? alpha[lat[(long)c * NIN + L]] : 0.0;
         __builtin_amdgcn_wave_barrier();
-        Canon cn;
-        if (q == 0) {
-            cn = canon_of(&raw[r][0]);
-            for (int Lc = 0; Lc < KIN; ++Lc)
-                in[r][Lc] = (valid && Lc < NIN) ? input_of(raw[r][canon_source(cn, Lc)]) : (uint16_t)0;
+        // every lane of the cell derives the same frame and builds its own
+        // eight inputs (K slots 8q..8q+7 of the first layer)
+        const Canon cn = canon_of(&raw[r][0]);
+        uint16_t xin[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int Lc = 8 * q + j;
+            xin[j] = (valid && Lc < NIN) ? input_of(raw[r][canon_source(cn, Lc)]) : (uint16_t)0;
         }
         __builtin_amdgcn_wave_barrier();
         Fwd F;
-        forward16(ld_bf16x8(&in[r][8 * q]), wb, pq, r, q, F);
+        forward16(__builtin_bit_cast(bf16x8, xin), wb, pq, r, q, F);
         if (q == 0 && live) {
             bool ok = valid;
             if (ok) {
@@ -573,7 +575,7 @@ __global__ void __launch_bounds__(64) k_infer(const int* lat, const double* alph
             learned[i] = ok ? 1 : 0;
             if (!ok) lsList[atomicAdd(nLs, 1)] = i;
         }
-        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_wave_barrier();  // raw is rewritten by the next cells
     }
 }
 
@@ -714,7 +716,7 @@ void launch_sync_bf16(const float* pq, uint16_t* wb, hipStream_t s) {
 
 void launch_infer(const int* lat, const double* alpha, const int* mixed, const int* nMixed, const uint16_t* wb,
                   const float* pq, double* ifN, unsigned char* learned, int* lsList, int* nLs, int nC, hipStream_t s) {
-    k_infer<<<std::max(1, std::min(4096, cdiv(nC, 16))), 64, 0, s>>>(lat, alpha, mixed, nMixed, wb, pq, ifN, learned,
+    k_infer<<<std::max(1, std::min(2048, cdiv(nC, 16))), 64, 0, s>>>(lat, alpha, mixed, nMixed, wb, pq, ifN, learned,
                                                                       lsList, nLs);
 }
 

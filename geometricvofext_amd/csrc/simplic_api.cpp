@@ -622,7 +622,7 @@ private:
         st_.ifC = dalloc<double>(3L * nC);
         st_.ifS = dalloc<double>(3L * nC);
         st_.un0 = dalloc<double>(nC);
-        st_.oobCount = dalloc<int>(1);
+        st_.oobCount = dalloc<int>(OOB_SLOTS * OOB_STRIDE);
         st_.candCount = dalloc<int>(1);
         st_.candBits = dalloc<unsigned long long>((nC + 63) / 64);
         st_.cand = dalloc<int>(nC);
@@ -638,6 +638,7 @@ private:
         st_.touched = dalloc<int>(nC);
         st_.touchedFlag = dalloc<int>(nC);
         st_.boundStats = dalloc<int>(4 * MAX_BOUND_PASSES);
+        st_.boundTotals = dalloc<unsigned long long>(5);
         st_.bcount = dalloc<int>(16);
         st_.maxCellFaces = maxCellFaces_;
         // bounding rounds run grid-wide before the one-workgroup tail (tuning
@@ -925,6 +926,14 @@ int svp_device_view_get(svp_handle* h, svp_device_view* out) {
         out->C = s.m_.C;
         out->V = s.m_.V;
         out->stream = (void*)s.stream_;
+    });
+}
+
+int svp_bound_totals(svp_handle* h, uint64_t* out5) {
+    return guarded([&] {
+        solveVofEqu& s = H(h);
+        if (!out5) throw SvpError(SVP_E_ARG, "svp_bound_totals: null output");
+        s.copyOutT((unsigned long long*)out5, s.st_.boundTotals, 5, SVP_HOST);
     });
 }
 

@@ -1144,7 +1144,7 @@ __global__ void __launch_bounds__(DETECT_BLOCK) k_update(DevMesh m, DevState st,
     if (lane == 0) {
         const int word = c >> 6;
         if (word < (m.nC + 63) / 64) st.candBits[word] = bits;
-        if (oobBits) atomicAdd(st.oobCount, __popcll(oobBits));
+        if (oobBits) atomicAdd(&st.oobCount[OOB_STRIDE * (blockIdx.x & (OOB_SLOTS - 1))], __popcll(oobBits));
     }
 }
 
@@ -1314,7 +1314,7 @@ __device__ inline void bound_cell(const DevMesh& m, const DevState& st, int c, d
 // limitFlux (advectionTemplates.C:118-215), one pass = the kernels below.
 // The pass runs only while cells lie outside [-aTol, 1+aTol] (bound count of
 // the update kernel, kept current by the apply kernel):
-//   collect           ascending candidates from the bitmap
+//   count/scan/write  ascending candidates from the bitmap
 //   prep              first redistribution loop of every candidate (it
 //                     decides which cells write and opens correctedFaces)
 //   scan2/scatter     correctedFaces positions and the cells they touch,
@@ -1329,32 +1329,88 @@ __device__ inline void bound_cell(const DevMesh& m, const DevState& st, int c, d
 // frontier size of round r (r <= st.gridRounds), [BND_CTRL-1] rounds.
 constexpr int BND_CTRL = 3 + MAX_GRID_ROUNDS + 2;
 
-// ascending candidates of the pass from the bitmap, in one workgroup: every
-// thread owns a contiguous run of bitmap words.  With a zero bound count the
-// pass is off (no candidates).  Also clears the per-pass counters.
-__global__ void __launch_bounds__(1024) k_bnd_collect(DevState st, int nWords) {
+// Ascending candidates of the pass from the bitmap: per-block popcounts,
+// one-workgroup scan of the block counts, then every block writes its
+// candidates (coalesced word reads throughout).  With a zero bound count the
+// pass is off (no candidates).  The scan also clears the per-pass counters.
+constexpr int BND_WORDS_PER_BLOCK = 1024;
+
+// any cell out of bounds: sum of the spread counters (each slot is one cache
+// line, so the update kernel's blocks do not queue on a single address)
+__device__ inline bool oob_any(const DevState& st) {
+    int n = 0;
+#pragma unroll 8
+    for (int k = 0; k < OOB_SLOTS; ++k) n += st.oobCount[OOB_STRIDE * k];
+    return n != 0;
+}
+
+__global__ void __launch_bounds__(256) k_bnd_count(DevState st, int nWords) {
+    __shared__ int lds[4];
+    if (!oob_any(st)) return;
+    int c = 0;
+    for (int w = blockIdx.x * BND_WORDS_PER_BLOCK + threadIdx.x;
+         w < min(nWords, (blockIdx.x + 1) * BND_WORDS_PER_BLOCK); w += 256)
+        c += __popcll(st.candBits[w]);
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) st.blockCount[blockIdx.x] = lds[0] + lds[1] + lds[2] + lds[3];
+}
+
+__global__ void __launch_bounds__(1024) k_bnd_scan(DevState st, int nBlocks) {
     __shared__ int lds[16];
     const int t = threadIdx.x;
     if (t >= 1 && t < BND_CTRL) st.bcount[t] = 0;
-    if (*st.oobCount == 0) {
+    if (!oob_any(st)) {
         if (t == 0) st.bcount[0] = 0;
         return;
     }
-    const int per = (nWords + 1023) / 1024;
-    const int w0 = min(nWords, t * per), w1 = min(nWords, w0 + per);
-    int cnt = 0;
-    for (int w = w0; w < w1; ++w) cnt += __popcll(st.candBits[w]);
+    // nBlocks <= 1024 * per
+    const int per = (nBlocks + 1023) / 1024;
+    const int i0 = min(nBlocks, t * per), i1 = min(nBlocks, i0 + per);
+    int sum = 0;
+    for (int i = i0; i < i1; ++i) sum += st.blockCount[i];
     int total;
-    int off = wg_scan1024(cnt, lds, total);
-    for (int w = w0; w < w1; ++w) {
-        unsigned long long bits = st.candBits[w];
+    int off = wg_scan1024(sum, lds, total);
+    for (int i = i0; i < i1; ++i) {
+        const int v = st.blockCount[i];
+        st.blockCount[i] = off;
+        off += v;
+    }
+    if (t == 0) st.bcount[0] = total;
+}
+
+__global__ void __launch_bounds__(256) k_bnd_write(DevState st, int nWords) {
+    __shared__ int lds[16];
+    if (st.bcount[0] == 0) return;
+    int base = st.blockCount[blockIdx.x];
+    // the block's words in chunks of 256: a block scan of the popcounts per chunk
+    for (int w0 = blockIdx.x * BND_WORDS_PER_BLOCK; w0 < min(nWords, (blockIdx.x + 1) * BND_WORDS_PER_BLOCK);
+         w0 += 256) {
+        const int w = w0 + threadIdx.x;
+        unsigned long long bits = (w < nWords) ? st.candBits[w] : 0ull;
+        const int c = __popcll(bits);
+        // exclusive scan over the 256 threads: wave scans, then the 4 wave totals
+        const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+        int x = c;
+        for (int o = 1; o < 64; o <<= 1) {
+            const int y = __shfl_up(x, o);
+            if (lane >= o) x += y;
+        }
+        if (lane == 63) lds[wv] = x;
+        __syncthreads();
+        int pre = 0;
+        for (int k = 0; k < wv; ++k) pre += lds[k];
+        const int tot = lds[0] + lds[1] + lds[2] + lds[3];
+        int off = base + pre + x - c;
         while (bits) {
             const int bit = __ffsll((long long)bits) - 1;
             bits &= bits - 1;
             st.cand[off++] = w * 64 + bit;
         }
+        base += tot;
+        __syncthreads();
     }
-    if (t == 0) st.bcount[0] = total;
 }
 
 __global__ void __launch_bounds__(256) k_bnd_prep(DevMesh m, DevState st, double dt) {
@@ -1368,23 +1424,29 @@ __global__ void __launch_bounds__(256) k_bnd_prep(DevMesh m, DevState st, double
     }
 }
 
-// correctedFaces offsets: exclusive scan of the first-loop face counts
+// correctedFaces offsets: exclusive scan of the first-loop face counts, in
+// tiles of 4096 (four consecutive counts per thread, coalesced)
 __global__ void __launch_bounds__(1024) k_bnd_scan2(DevState st) {
     __shared__ int lds[16];
     const int n = st.bcount[0];
     const int t = threadIdx.x;
-    const int per = (n + 1023) / 1024;
-    const int i0 = min(n, t * per), i1 = min(n, i0 + per);
-    int sum = 0;
-    for (int i = i0; i < i1; ++i) sum += st.nFirst[i];
-    int total;
-    int off = wg_scan1024(sum, lds, total);
-    for (int i = i0; i < i1; ++i) {
-        const int v = st.nFirst[i];
-        st.nFirst[i] = off;
-        off += v;
+    int carry = 0;
+    for (int base = 0; base < n; base += 4096) {
+        int v[4];
+        const int i = base + 4 * t;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) v[j] = (i + j < n) ? st.nFirst[i + j] : 0;
+        const int sum = v[0] + v[1] + v[2] + v[3];
+        int total;
+        int off = carry + wg_scan1024(sum, lds, total);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (i + j < n) st.nFirst[i + j] = off;
+            off += v[j];
+        }
+        carry += total;
     }
-    if (t == 0) st.bcount[1] = total;
+    if (t == 0) st.bcount[1] = carry;
 }
 
 template <int MF>
@@ -1532,10 +1594,10 @@ __device__ inline void bound_cell_rec(const BndRec<MF>& R, const DevState& st, i
 
 // completes cell i of frontier r and releases its successors into frontier
 // r+1; release(a) drops one pending predecessor of candidate a and returns
-// true when it was the last
-template <int MF, class Release>
+// true when it was the last, push(a) appends a to the next frontier
+template <int MF, class Release, class Push>
 __device__ inline void bnd_complete(const DevMesh& m, const DevState& st, int i, double dt, double rDeltaT,
-                                    const double* Sp, const double* Su, int* qnext, int* nNext, Release release) {
+                                    const double* Sp, const double* Su, Release release, Push push) {
     const BndRec<MF>& R = ((const BndRec<MF>*)st.bndRec)[i];
     const int c = st.cand[i];
     // successors read these corrections in a later round: the next kernel of
@@ -1550,7 +1612,7 @@ __device__ inline void bnd_complete(const DevMesh& m, const DevState& st, int i,
     for (int k = 0; k < MF; ++k) {
         const int a = R.succ[k];
         if (a < 0) continue;
-        if (release(a)) qnext[atomicAdd(nNext, 1)] = a;
+        if (release(a)) push(a);
     }
 }
 
@@ -1559,41 +1621,53 @@ __global__ void __launch_bounds__(256) k_bnd_round(DevMesh m, DevState st, int r
     const int n = st.bcount[3 + r];
     const int* q = st.queue + (long)(r & 1) * m.nC;
     int* qnext = st.queue + (long)((r + 1) & 1) * m.nC;
+    int* nNext = &st.bcount[3 + r + 1];
     const double rDeltaT = 1.0 / dt;
     auto release = [&](int a) { return atomicSub(&st.pend[a], 1) == 1; };
+    auto push = [&](int a) { qnext[atomicAdd(nNext, 1)] = a; };
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x)
-        bnd_complete<MF>(m, st, q[k], dt, rDeltaT, Sp, Su, qnext, &st.bcount[3 + r + 1], release);
+        bnd_complete<MF>(m, st, q[k], dt, rDeltaT, Sp, Su, release, push);
 }
 
-// Remaining rounds in one workgroup (frontiers are small by now); every round
+// Remaining rounds in one workgroup of TAIL_THREADS (frontiers are small by now); every round
 // completes at least one cell because edges point to higher cell numbers.
-// With at most TAIL_PEND_CAP candidates the predecessor counts move to LDS
-// (one byte each, four per word), so releasing a successor is an LDS atomic.
+// The frontiers live in LDS up to TAIL_Q entries (further entries in the
+// global queue at the same positions), and with at most TAIL_PEND_CAP
+// candidates the predecessor counts move to LDS as well (one byte each, four
+// per word), so a round touches global memory only for the cells' records
+// and corrections.
 constexpr int TAIL_PEND_WORDS = 8192;
 constexpr int TAIL_PEND_CAP = 4 * TAIL_PEND_WORDS;
+constexpr int TAIL_Q = 4096;
+
+constexpr int TAIL_THREADS = 512;
 
 template <int MF>
-__global__ void __launch_bounds__(1024) k_bnd_tail(DevMesh m, DevState st, double dt, const double* Sp,
-                                                   const double* Su) {
+__global__ void __launch_bounds__(TAIL_THREADS) k_bnd_tail(DevMesh m, DevState st, double dt, const double* Sp,
+                                                           const double* Su) {
     __shared__ int sQ, sNext, sRounds;
     __shared__ unsigned int sPend[TAIL_PEND_WORDS];
+    __shared__ int sQueue[2][TAIL_Q];
     const int t = threadIdx.x;
     const double rDeltaT = 1.0 / dt;
     const int R = st.gridRounds;
     const int nCand = st.bcount[0];
     const bool ldsPend = nCand <= TAIL_PEND_CAP;
+    const int q0 = st.bcount[3 + R];
     if (t == 0) {
-        sQ = st.bcount[3 + R];
+        sQ = q0;
         sNext = 0;
         sRounds = R;
     }
     if (ldsPend)
-        for (int w = t; w < (nCand + 3) / 4; w += 1024) {
+        for (int w = t; w < (nCand + 3) / 4; w += TAIL_THREADS) {
             unsigned int word = 0u;
             for (int b = 0; b < 4; ++b)
                 if (4 * w + b < nCand) word |= ((unsigned int)st.pend[4 * w + b] & 0xFFu) << (8 * b);
             sPend[w] = word;
         }
+    int gcur = R & 1;  // global queue buffer holding the overflow of the current frontier
+    for (int k = t; k < min(q0, TAIL_Q); k += TAIL_THREADS) sQueue[0][k] = st.queue[(long)gcur * m.nC + k];
     __syncthreads();
     auto releaseLds = [&](int a) {
         const int sh = 8 * (a & 3);
@@ -1601,18 +1675,23 @@ __global__ void __launch_bounds__(1024) k_bnd_tail(DevMesh m, DevState st, doubl
         return ((old >> sh) & 0xFFu) == 1u;
     };
     auto releaseGlobal = [&](int a) { return atomicSub(&st.pend[a], 1) == 1; };
-    int cur = R & 1;
+    int cur = 0;
     while (true) {
         const int qn = sQ;
         if (qn == 0) break;
-        const int* q = st.queue + (long)cur * m.nC;
-        int* qnext = st.queue + (long)(cur ^ 1) * m.nC;
-        if (ldsPend)
-            for (int k = t; k < qn; k += 1024)
-                bnd_complete<MF>(m, st, q[k], dt, rDeltaT, Sp, Su, qnext, &sNext, releaseLds);
-        else
-            for (int k = t; k < qn; k += 1024)
-                bnd_complete<MF>(m, st, q[k], dt, rDeltaT, Sp, Su, qnext, &sNext, releaseGlobal);
+        const int* gq = st.queue + (long)gcur * m.nC;
+        int* gqNext = st.queue + (long)(gcur ^ 1) * m.nC;
+        int* lqNext = sQueue[cur ^ 1];
+        auto push = [&](int a) {
+            const int idx = atomicAdd(&sNext, 1);
+            if (idx < TAIL_Q) lqNext[idx] = a;
+            else gqNext[idx] = a;
+        };
+        for (int k = t; k < qn; k += TAIL_THREADS) {
+            const int i = k < TAIL_Q ? sQueue[cur][k] : gq[k];
+            if (ldsPend) bnd_complete<MF>(m, st, i, dt, rDeltaT, Sp, Su, releaseLds, push);
+            else bnd_complete<MF>(m, st, i, dt, rDeltaT, Sp, Su, releaseGlobal, push);
+        }
         __syncthreads();
         if (t == 0) {
             sQ = sNext;
@@ -1620,6 +1699,7 @@ __global__ void __launch_bounds__(1024) k_bnd_tail(DevMesh m, DevState st, doubl
             sRounds++;
         }
         cur ^= 1;
+        gcur ^= 1;
         __syncthreads();
     }
     if (t == 0) st.bcount[BND_CTRL - 1] = sRounds;
@@ -1684,7 +1764,7 @@ __global__ void __launch_bounds__(256) k_bnd_apply(DevMesh m, DevState st) {
         }
         st.alpha[c] = a;
         const int dOob = (int)loop_oob(a, aTol) - (int)loop_oob(before, aTol);
-        if (dOob) atomicAdd(st.oobCount, dOob);
+        if (dOob) atomicAdd(&st.oobCount[OOB_STRIDE * ((c >> 8) & (OOB_SLOTS - 1))], dOob);
         const bool candNew = cell_oob(a, aTol) && need_bounding(m, st.mixedFlag, c);
         const unsigned long long bit = 1ull << (c & 63);
         const bool candOld = (st.candBits[c >> 6] & bit) != 0ull;
@@ -1717,6 +1797,11 @@ __global__ void __launch_bounds__(256) k_bnd_reset(DevMesh m, DevState st, int p
         st.boundStats[4 * pass + 1] = nAct;
         st.boundStats[4 * pass + 2] = st.bcount[BND_CTRL - 1];
         st.boundStats[4 * pass + 3] = st.bcount[1];
+        st.boundTotals[0] += (unsigned long long)nCand;
+        st.boundTotals[1] += (unsigned long long)nAct;
+        st.boundTotals[2] += (unsigned long long)st.bcount[BND_CTRL - 1];
+        st.boundTotals[3] += (unsigned long long)st.bcount[1];
+        st.boundTotals[4] += 1ull;
         *st.candCount = nCand;
     }
 }
@@ -1728,12 +1813,15 @@ void launch_bound_pass(const DevMesh& m, const DevState& st, int pass, double dt
     // candidate-sized work: a modest grid-stride grid launches faster than
     // one thread per cell of the mesh
     const int grid = std::max(1, std::min(st.bndGrid, cdiv(m.nC, 256)));
-    k_bnd_collect<<<1, 1024, 0, s>>>(st, nWords);
+    const int nbw = cdiv(nWords, BND_WORDS_PER_BLOCK);
+    k_bnd_count<<<nbw, 256, 0, s>>>(st, nWords);
+    k_bnd_scan<<<1, 1024, 0, s>>>(st, nbw);
+    k_bnd_write<<<nbw, 256, 0, s>>>(st, nWords);
     k_bnd_prep<<<grid, 256, 0, s>>>(m, st, dt);
     k_bnd_scan2<<<1, 1024, 0, s>>>(st);
     k_bnd_scatter<MF><<<grid, 256, 0, s>>>(m, st, dt);
     for (int r = 0; r < st.gridRounds; ++r) k_bnd_round<MF><<<grid, 256, 0, s>>>(m, st, r, dt, Sp, Su);
-    k_bnd_tail<MF><<<1, 1024, 0, s>>>(m, st, dt, Sp, Su);
+    k_bnd_tail<MF><<<1, TAIL_THREADS, 0, s>>>(m, st, dt, Sp, Su);
     k_bnd_apply<MF><<<grid, 256, 0, s>>>(m, st);
     k_bnd_reset<<<grid, 256, 0, s>>>(m, st, pass);
 }
@@ -1759,8 +1847,7 @@ __global__ void k_snap_clip(int nC, double* alpha, double snapTol, int clip, Dev
 }
 
 __global__ void k_zero_int(int* p, int n) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
 }
 
 void launch_advect_stage(int stage, const DevMesh& m, const DevState& st, const Params& p, double dt,
@@ -1768,7 +1855,7 @@ void launch_advect_stage(int stage, const DevMesh& m, const DevState& st, const 
     const int nb = detect_blocks(m.nC);
     switch (stage) {
         case STAGE_UPWIND:
-            k_zero_int<<<1, 64, 0, s>>>(st.oobCount, 1);
+            k_zero_int<<<1, 256, 0, s>>>(st.oobCount, OOB_SLOTS * OOB_STRIDE);
             k_upwind<<<cdiv(m.nF, 256), 256, 0, s>>>(m, st, dt);
             break;
         case STAGE_IFLUX:

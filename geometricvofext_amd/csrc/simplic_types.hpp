@@ -67,7 +67,7 @@ struct DevState {
     double* ifS;        // 3*cap
     double* un0;
     // bounding bookkeeping (limitFlux/boundFlux)
-    int* oobCount;      // [1] cells with alpha-1 > aTol or alpha < -aTol
+    int* oobCount;      // cells with alpha-1 > aTol or alpha < -aTol, spread over OOB_SLOTS counters
     int* candCount;     // [1] candidates of the last pass (diagnostics)
     unsigned long long* candBits;  // nC bits: alpha outside [-aTol,1+aTol] and needBounding
     int* cand;          // nC, ascending candidates of the current pass
@@ -81,6 +81,7 @@ struct DevState {
     int* touched;       // nC, cells whose alpha gets corrections
     int* touchedFlag;   // nC
     int* boundStats;    // [4*maxPasses]: candidates, active, rounds, corrected faces
+    unsigned long long* boundTotals;  // [5] running sums: candidates, active, rounds, corrected faces, passes
     int* bcount;        // [16] per-pass counters of the bounding kernels
     int un0Fused;       // plane placement by 8-lane groups, which also computes Un0
     int gridRounds;     // bounding rounds run as grid-wide kernels before the one-workgroup tail
@@ -118,6 +119,8 @@ enum : int {
 };
 constexpr int MAX_BOUND_PASSES = 64;
 constexpr int MAX_GRID_ROUNDS = 8;
+constexpr int OOB_SLOTS = 64;   // counters of the out-of-bounds count, one per 64-byte line
+constexpr int OOB_STRIDE = 16;
 
 constexpr int DETECT_BLOCK = 256;
 

@@ -153,6 +153,10 @@ int svp_device_view_get(svp_handle* h, svp_device_view* out);
  * candidates, cells that wrote corrections, dependency rounds, corrected
  * faces.  Zero entries after the last pass that ran. */
 int svp_bound_stats(svp_handle* h, int32_t* out, int cap);
+/* Running sums over every bounding pass that had candidates since the
+ * handle was created: candidates, active candidates, dependency rounds,
+ * corrected faces, passes. */
+int svp_bound_totals(svp_handle* h, uint64_t* out5);
 int svp_profile_set(svp_handle* h, int mask);
 int svp_profile_get(svp_handle* h, double* seconds, int64_t* counts, int reset);
 
