@@ -292,6 +292,38 @@ __device__ inline void forward16(const bf16x8 xop, const uint16_t* wb, const flo
     for (int j = 0; j < 4; ++j) F.o[j] = acc[j] + pq[Q_B3 + 4 * q + j];
 }
 
+// forward16 with the weights of the first two layers in wb (layout of the
+// BF16 copy up to W2T), W3 in w3 and the three bias vectors apart
+__device__ inline void forward16_split(const bf16x8 xop, const uint16_t* wb, const uint16_t* w3, const float* b1,
+                                       const float* b2, const float* b3, int r, int q, Fwd& F) {
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+        f4 acc = {0.f, 0.f, 0.f, 0.f};
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ld_bf16x8(wb + B_W1 + (16 * t + r) * KIN + 8 * q), xop, acc, 0, 0, 0);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) F.h1[t][j] = f2bf(gelu(acc[j] + b1[16 * t + 4 * q + j]));
+    }
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+        f4 acc = {0.f, 0.f, 0.f, 0.f};
+        const uint16_t* row = wb + B_W2 + (16 * t + r) * NH;
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks)
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ld_perm(row, ks, q), pack_pair(F.h1[2 * ks], F.h1[2 * ks + 1]),
+                                                          acc, 0, 0, 0);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) F.h2[t][j] = f2bf(gelu(acc[j] + b2[16 * t + 4 * q + j]));
+    }
+    f4 acc = {0.f, 0.f, 0.f, 0.f};
+    const uint16_t* row = w3 + r * NH;
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks)
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ld_perm(row, ks, q), pack_pair(F.h2[2 * ks], F.h2[2 * ks + 1]),
+                                                      acc, 0, 0, 0);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) F.o[j] = acc[j] + b3[4 * q + j];
+}
+
 // sample at position k of the epoch order (affine permutation mod n)
 __device__ inline long sample_at(long k, long n, long pa, long pb) {
     return (long)(((unsigned __int128)pa * (unsigned __int128)k + (unsigned __int128)pb) % (unsigned __int128)n);
@@ -533,85 +565,127 @@ __global__ void __launch_bounds__(256) k_sync_bf16(const float* pq, uint16_t* wb
 // Cells without a full uniform 3x3x3 lattice stencil (lat[27c] < 0) and
 // degenerate outputs are listed for the least-squares normal.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(64) k_infer(const int* lat, const double* alpha, const int* mixed,
-                                              const int* nMixedP, const uint16_t* wb, const float* pq, double* ifN,
-                                              unsigned char* learned, int* lsList, int* nLs) {
-    __shared__ double raw[16][NIN];
-    const int l = threadIdx.x, r = l % 16, q = l / 16;
+constexpr int INFER_WAVES = 4;
+constexpr int RESID_ROUNDS = 16;  // multisection rounds of the residual plane (bracket / 5 each)
+
+// Inference, one wave per 16 interface cells (lane r + 16 q: cell r, part q).
+// The four lanes of a cell gather its 27 stencil values, derive the same
+// canonical frame and each build eight network inputs; the wave runs the
+// network; group 0 holds the outputs, which every lane of the cell maps back
+// to the mesh frame.  With tol < 1 the residual test follows: the plane of
+// the network normal that leaves the cell's own alpha in its cube is found by
+// multisection of the plane constant D over the vertex range (the liquid
+// fraction falls monotonically as D grows; each round the four lanes
+// evaluate four interior points and the bracket shrinks 5x), and the
+// residual is the mean |fraction - alpha| over the 26 neighbour cubes
+// lo + e h of the lattice (lane q takes the neighbours L = q, q+4, ...).
+// Cells without a full uniform 3x3x3 stencil (lat[27c] < 0), degenerate
+// outputs and cells above tol are listed for the least-squares normal.
+__global__ void __launch_bounds__(64 * INFER_WAVES, 2) k_infer(const int* lat, const double* alpha, const int* mixed,
+                                                             const int* nMixedP, const double* C, const double* V,
+                                                             const uint16_t* wbG, const float* pqG, double tol,
+                                                             double* ifN, unsigned char* learned, double* resid,
+                                                             int* lsList, int* nLs, int* nFb) {
+    // weights and biases of the network, staged once per workgroup
+    __shared__ __attribute__((aligned(16))) uint16_t wb[B_W2T];           // W1b, W2b (W2T is not needed)
+    __shared__ __attribute__((aligned(16))) uint16_t w3[NOUTP * NH];
+    __shared__ float bias[NH + NH + NOUTP];
+    __shared__ double raw[INFER_WAVES][16][NIN];
+    {
+        const uint4* src = (const uint4*)wbG;
+        uint4* dst = (uint4*)wb;
+        for (int k = threadIdx.x; k < B_W2T * 2 / 16; k += blockDim.x) dst[k] = src[k];
+        const uint4* src3 = (const uint4*)(wbG + B_W3);
+        uint4* dst3 = (uint4*)w3;
+        for (int k = threadIdx.x; k < NOUTP * NH * 2 / 16; k += blockDim.x) dst3[k] = src3[k];
+        for (int k = threadIdx.x; k < NH; k += blockDim.x) {
+            bias[k] = pqG[Q_B1 + k];
+            bias[NH + k] = pqG[Q_B2 + k];
+        }
+        for (int k = threadIdx.x; k < NOUTP; k += blockDim.x) bias[2 * NH + k] = pqG[Q_B3 + k];
+    }
+    __syncthreads();
+    const int wv = threadIdx.x / 64, l = threadIdx.x % 64, r = l % 16, q = l / 16;
     const int nMixed = *nMixedP;
-    for (int i0 = blockIdx.x * 16; i0 < nMixed; i0 += gridDim.x * 16) {
+    double (*rw)[NIN] = raw[wv];
+    for (int i0 = (blockIdx.x * INFER_WAVES + wv) * 16; i0 < nMixed; i0 += gridDim.x * INFER_WAVES * 16) {
         const int i = i0 + r;
         const bool live = i < nMixed;
         const int c = live ? mixed[i] : 0;
         const bool valid = live && lat[(long)c * NIN] >= 0;
-        for (int L = q; L < NIN; L += 4) raw[r][L] = valid ? alpha[lat[(long)c * NIN + L]] : 0.0;
+        for (int L = q; L < NIN; L += 4) rw[r][L] = valid ? alpha[lat[(long)c * NIN + L]] : 0.0;
         __builtin_amdgcn_wave_barrier();
         // every lane of the cell derives the same frame and builds its own
         // eight inputs (K slots 8q..8q+7 of the first layer)
-        const Canon cn = canon_of(&raw[r][0]);
+        const Canon cn = canon_of(&rw[r][0]);
         uint16_t xin[8];
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
             const int Lc = 8 * q + j;
-            xin[j] = (valid && Lc < NIN) ? input_of(raw[r][canon_source(cn, Lc)]) : (uint16_t)0;
+            xin[j] = (valid && Lc < NIN) ? input_of(rw[r][canon_source(cn, Lc)]) : (uint16_t)0;
         }
         __builtin_amdgcn_wave_barrier();
         Fwd F;
-        forward16(__builtin_bit_cast(bf16x8, xin), wb, pq, r, q, F);
-        if (q == 0 && live) {
-            bool ok = valid;
-            if (ok) {
-                const double vc[3] = {(double)F.o[0], (double)F.o[1], (double)F.o[2]};
-                double v[3];
-                from_canon(cn, vc, v);
-                const double mg = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
-                ok = mg > 1e-6;
-                if (ok) {
-                    ifN[3L * i] = v[0] / mg;
-                    ifN[3L * i + 1] = v[1] / mg;
-                    ifN[3L * i + 2] = v[2] / mg;
-                }
+        forward16_split(__builtin_bit_cast(bf16x8, xin), wb, w3, bias, bias + NH, bias + 2 * NH, r, q, F);
+        // outputs of cell r sit in lane r (group 0): every lane of the cell takes them
+        const double vc[3] = {(double)__shfl(F.o[0], r), (double)__shfl(F.o[1], r), (double)__shfl(F.o[2], r)};
+        double v[3];
+        from_canon(cn, vc, v);
+        const double mg = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        bool ok = valid && mg > 1e-6;
+        const double nx = ok ? v[0] / mg : 0.0, ny = ok ? v[1] / mg : 0.0, nz = ok ? v[2] / mg : 0.0;
+        double res = -1.0;
+        if (tol < 1.0) {
+            // cell cube (lattice cell: an axis-aligned cube of edge h at C)
+            const double h = live ? cbrt(V[c]) : 1.0;
+            const double lox = live ? C[3L * c] - 0.5 * h : 0.0, loy = live ? C[3L * c + 1] - 0.5 * h : 0.0,
+                         loz = live ? C[3L * c + 2] - 0.5 * h : 0.0;
+            const double a = rw[r][13];
+            const double base = -(nx * lox + ny * loy + nz * loz);
+            const double sx = nx * h, sy = ny * h, sz = nz * h;
+            const double tmin = fmin(sx, 0.0) + fmin(sy, 0.0) + fmin(sz, 0.0);
+            const double tmax = fmax(sx, 0.0) + fmax(sy, 0.0) + fmax(sz, 0.0);
+            double dlo = base - tmax, dhi = base - tmin;  // fraction 1 at dlo, 0 at dhi
+            for (int round = 0; round < RESID_ROUNDS; ++round) {
+                const double w = (dhi - dlo) / 5.0;
+                const bool above = plane_cube_fraction(nx, ny, nz, dlo + w * (double)(q + 1), lox, loy, loz, h) > a;
+                // the cell's lanes are r, r+16, r+32, r+48: count those left of the crossing
+                const unsigned long long bal = __ballot(above);
+                const int k = (int)((bal >> r) & 1ull) + (int)((bal >> (r + 16)) & 1ull) +
+                              (int)((bal >> (r + 32)) & 1ull) + (int)((bal >> (r + 48)) & 1ull);
+                const double newLo = k > 0 ? dlo + w * (double)k : dlo;
+                const double newHi = k < 4 ? dlo + w * (double)(k + 1) : dhi;
+                dlo = newLo;
+                dhi = newHi;
             }
-            learned[i] = ok ? 1 : 0;
-            if (!ok) lsList[atomicAdd(nLs, 1)] = i;
+            const double d = 0.5 * (dlo + dhi);
+            double part = 0.0;
+            for (int L = q; L < NIN; L += 4) {
+                if (L == 13) continue;
+                const double ex = (double)(L / 9 - 1), ey = (double)((L / 3) % 3 - 1), ez = (double)(L % 3 - 1);
+                const double f = plane_cube_fraction(nx, ny, nz, d, lox + ex * h, loy + ey * h, loz + ez * h, h);
+                part += fabs(f - rw[r][L]);
+            }
+            part += __shfl_xor(part, 16);
+            part += __shfl_xor(part, 32);
+            res = part * (1.0 / 26.0);
+        }
+        if (q == 0 && live) {
+            if (ok) {
+                ifN[3L * i] = nx;
+                ifN[3L * i + 1] = ny;
+                ifN[3L * i + 2] = nz;
+            }
+            if (resid) resid[i] = ok ? res : -1.0;
+            const bool fb = ok && tol < 1.0 && res > tol;
+            learned[i] = (ok && !fb) ? 1 : 0;
+            if (!ok || fb) lsList[atomicAdd(nLs, 1)] = i;
+            if (fb) atomicAdd(nFb, 1);
         }
         __builtin_amdgcn_wave_barrier();  // raw is rewritten by the next cells
     }
 }
 
-// Mean absolute difference, over the 26 lattice neighbours, between the
-// liquid fraction the cell's plane n.x + d = 0 leaves in the neighbour's
-// cube and the neighbour's alpha.  Cells above tol join the fallback list.
-__global__ void __launch_bounds__(256) k_residual(const int* lat, const double* alpha, const double* C,
-                                                  const double* V, const int* mixed, const int* nMixedP,
-                                                  const double* ifN, const double* ifD, const int* status,
-                                                  unsigned char* learned, double tol, double* resid, int* fbList,
-                                                  int* nFb) {
-    const int nMixed = *nMixedP;
-    for (int i = blockIdx.x * 256 + threadIdx.x; i < nMixed; i += gridDim.x * 256) {
-        if (!learned[i]) {
-            resid[i] = -1.0;
-            continue;
-        }
-        const int c = mixed[i];
-        const double nx = ifN[3L * i], ny = ifN[3L * i + 1], nz = ifN[3L * i + 2], d = ifD[i];
-        const double h = cbrt(V[c]);
-        double res = 0.0;
-        for (int L = 0; L < NIN; ++L) {
-            if (L == 13) continue;
-            const int j = lat[(long)c * NIN + L];
-            const double f = plane_cube_fraction(nx, ny, nz, d, C[3L * j] - 0.5 * h, C[3L * j + 1] - 0.5 * h,
-                                                 C[3L * j + 2] - 0.5 * h, h);
-            res += fabs(f - alpha[j]);
-        }
-        res *= (1.0 / 26.0);
-        resid[i] = res;
-        if (res > tol || status[i] != 0) {
-            learned[i] = 0;
-            fbList[atomicAdd(nFb, 1)] = i;
-        }
-    }
-}
 
 // cube fractions for the closed-form check: in[k*8] = n, d, lo, h
 __global__ void __launch_bounds__(256) k_cube_fractions(const double* in, long k, double* out) {
@@ -714,18 +788,13 @@ void launch_sync_bf16(const float* pq, uint16_t* wb, hipStream_t s) {
     k_sync_bf16<<<cdiv(N_PADDED, 256), 256, 0, s>>>(pq, wb);
 }
 
-void launch_infer(const int* lat, const double* alpha, const int* mixed, const int* nMixed, const uint16_t* wb,
-                  const float* pq, double* ifN, unsigned char* learned, int* lsList, int* nLs, int nC, hipStream_t s) {
-    k_infer<<<std::max(1, std::min(2048, cdiv(nC, 16))), 64, 0, s>>>(lat, alpha, mixed, nMixed, wb, pq, ifN, learned,
-                                                                      lsList, nLs);
+void launch_infer(const int* lat, const double* alpha, const int* mixed, const int* nMixed, const double* C,
+                  const double* V, const uint16_t* wb, const float* pq, double tol, double* ifN, unsigned char* learned,
+                  double* resid, int* lsList, int* nLs, int* nFb, int nC, hipStream_t s) {
+    k_infer<<<std::max(1, std::min(512, cdiv(nC, 16 * INFER_WAVES))), 64 * INFER_WAVES, 0, s>>>(
+        lat, alpha, mixed, nMixed, C, V, wb, pq, tol, ifN, learned, resid, lsList, nLs, nFb);
 }
 
-void launch_residual(const int* lat, const double* alpha, const double* C, const double* V, const int* mixed,
-                     const int* nMixed, const double* ifN, const double* ifD, const int* status,
-                     unsigned char* learned, double tol, double* resid, int* fbList, int* nFb, hipStream_t s) {
-    k_residual<<<1024, 256, 0, s>>>(lat, alpha, C, V, mixed, nMixed, ifN, ifD, status, learned, tol, resid, fbList,
-                                    nFb);
-}
 
 }  // namespace nm
 }  // namespace svp

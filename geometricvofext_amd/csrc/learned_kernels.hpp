@@ -27,11 +27,9 @@ void launch_reduce(const float* partial, int nWG, float* grad, hipStream_t s);
 void launch_adam(const float* partial, int nWG, float* pq, float* m1, float* m2, uint16_t* wb, float lr, float b1,
                  float b2, float eps, float c1, float c2, hipStream_t s);
 void launch_sync_bf16(const float* pq, uint16_t* wb, hipStream_t s);
-void launch_infer(const int* lat, const double* alpha, const int* mixed, const int* nMixed, const uint16_t* wb,
-                  const float* pq, double* ifN, unsigned char* learned, int* lsList, int* nLs, int nC, hipStream_t s);
-void launch_residual(const int* lat, const double* alpha, const double* C, const double* V, const int* mixed,
-                     const int* nMixed, const double* ifN, const double* ifD, const int* status,
-                     unsigned char* learned, double tol, double* resid, int* fbList, int* nFb, hipStream_t s);
+void launch_infer(const int* lat, const double* alpha, const int* mixed, const int* nMixed, const double* C,
+                  const double* V, const uint16_t* wb, const float* pq, double tol, double* ifN, unsigned char* learned,
+                  double* resid, int* lsList, int* nLs, int* nFb, int nC, hipStream_t s);
 
 void launch_cube_fractions(const double* in, long k, double* out, hipStream_t s);
 void launch_lattice(const DevMesh& m, unsigned char* cubeScratch, int* lat, hipStream_t s);

@@ -152,9 +152,9 @@ public:
     }
 
     // orientationMethod learned: network normals on cells with a uniform
-    // 3x3x3 stencil, least-squares normals elsewhere; plane placement; the
-    // residual test; least-squares normal and plane placement again on the
-    // cells that fail it.
+    // 3x3x3 stencil; the residual test of each network normal's cube plane;
+    // least-squares normals on the other cells and on those that fail it; one
+    // plane placement for all.
     void reconstructLearned() {
         if (!learned_.ready)
             throw SvpError(SVP_E_ARG, "orientationMethod learned needs a model (svp_set_normal_model)");
@@ -162,20 +162,13 @@ public:
         int* nFb = learned_.counts + 1;
         stage(STAGE_NORMAL, [&] {
             nm::launch_zero_ints(learned_.counts, 2, stream_);
-            nm::launch_infer(learned_.lat, st_.alpha, st_.mixed, st_.nMixed, learned_.wb, learned_.pq, st_.ifN,
-                             learned_.learned, learned_.lsList, nLs, m_.nC, stream_);
+            // the residual test runs inside the inference kernel (tol < 1)
+            nm::launch_infer(learned_.lat, st_.alpha, st_.mixed, st_.nMixed, m_.C, m_.V, learned_.wb, learned_.pq,
+                             learned_.tol, st_.ifN, learned_.learned, learned_.resid, learned_.lsList, nLs, nFb,
+                             m_.nC, stream_);
             launch_ls_normal(m_, st_, stream_, learned_.lsList, nLs);
         });
-        stage(STAGE_FSD, [&] {
-            launch_find_signed_distance(m_, st_, params_, stream_);
-            // a mean absolute fraction difference never exceeds 1: no test needed
-            if (learned_.tol >= 1.0) return;
-            nm::launch_residual(learned_.lat, st_.alpha, m_.C, m_.V, st_.mixed, st_.nMixed, st_.ifN, st_.ifD,
-                                st_.status, learned_.learned, learned_.tol, learned_.resid, learned_.fbList, nFb,
-                                stream_);
-            launch_ls_normal(m_, st_, stream_, learned_.fbList, nFb);
-            launch_find_signed_distance(m_, st_, params_, stream_, learned_.fbList, nFb);
-        });
+        stage(STAGE_FSD, [&] { launch_find_signed_distance(m_, st_, params_, stream_); });
     }
 
     void setNormalModel(const svp_normal_model& nmod) {
@@ -189,7 +182,6 @@ public:
             learned_.wb = dalloc<uint16_t>(nm::N_BF16);
             learned_.learned = dalloc<unsigned char>(m_.nC);
             learned_.lsList = dalloc<int>(m_.nC);
-            learned_.fbList = dalloc<int>(m_.nC);
             learned_.counts = dalloc<int>(2);
             learned_.resid = dalloc<double>(m_.nC);
             DBuf cube;
@@ -213,9 +205,9 @@ public:
         const int n = nMixed();
         HIPCHECK(hipMemcpy(c, learned_.counts, sizeof(c), hipMemcpyDeviceToHost));
         out4[0] = n;
-        out4[1] = n - c[0] - c[1];
+        out4[1] = n - c[0];
         out4[2] = c[1];
-        out4[3] = c[0];
+        out4[3] = c[0] - c[1];
     }
 
     void learnedResiduals(double* out, int where) {
@@ -341,9 +333,8 @@ public:
         float* pq = nullptr;         // network parameters (padded FP32)
         uint16_t* wb = nullptr;      // BF16 weights
         unsigned char* learned = nullptr;
-        int* lsList = nullptr;       // positions needing the least-squares normal first
-        int* fbList = nullptr;       // positions that failed the residual test
-        int* counts = nullptr;       // [2]: lsList, fbList lengths
+        int* lsList = nullptr;       // positions that take the least-squares normal
+        int* counts = nullptr;       // [2]: lsList length, of which residual fallbacks
         double* resid = nullptr;
     } learned_;
 

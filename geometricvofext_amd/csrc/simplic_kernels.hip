@@ -483,7 +483,16 @@ __device__ inline void fsd_face_cut(int fsz, const V3 (&q4)[4], const V3 (&q3)[3
     else ok &= calcSubFaceN<3>(q3, n, dist, fc);
 }
 
-__global__ void __launch_bounds__(64) k_fsd_group(DevMesh m, DevState st, const int* wl, const int* wn) {
+// waves per SIMD the register allocation of the interface-cell kernels
+// targets (build-time tuning; results do not depend on it)
+#ifndef SVP_FSD_WAVES
+#define SVP_FSD_WAVES 2
+#endif
+#ifndef SVP_FLUX_WAVES
+#define SVP_FLUX_WAVES 1
+#endif
+
+__global__ void __launch_bounds__(64, SVP_FSD_WAVES) k_fsd_group(DevMesh m, DevState st, const int* wl, const int* wn) {
     __shared__ FaceCut rec[64 / FSD_G][FSD_G];
     __shared__ double vraw[64 / FSD_G][NS_CELL_POINTS];
     __shared__ double vsrt[64 / FSD_G][NS_CELL_POINTS];
@@ -1048,7 +1057,7 @@ __global__ void __launch_bounds__(64) k_un0(DevMesh m, DevState st) {
 // timeIntegratedFlux with one thread per (cut cell, cell face): each downwind
 // face's volume is independent of the others
 template <bool TRIQUAD>
-__global__ void __launch_bounds__(64) k_interface_flux_faces(DevMesh m, DevState st, double dtStep, int slots) {
+__global__ void __launch_bounds__(64, SVP_FLUX_WAVES) k_interface_flux_faces(DevMesh m, DevState st, double dtStep, int slots) {
     const int nMixed = *st.nMixed;
     const long total = (long)nMixed * slots;
     for (long w = (long)blockIdx.x * blockDim.x + threadIdx.x; w < total; w += (long)gridDim.x * blockDim.x) {

@@ -5,11 +5,13 @@
  * "learned" next to the selector map of reconstruction.C:60-68: a small
  * network (27 -> 128 -> 128 -> 3, GELU; BF16 weights and activations, FP32
  * accumulation on MFMA) maps the 3x3x3 volume-fraction stencil of an
- * interface cell of a uniform hex mesh to its normal.  The unchanged FP64
- * findSignedDistance (cutCell.C:611-799) places the plane; where the plane's
- * volume residual over the 26 neighbours exceeds learnedResidualTol, and on
- * cells without a full uniform stencil, the least-squares normal
- * (reconstruction.C:85-141) is used instead.
+ * interface cell of a uniform hex mesh to its normal.  Where the volume
+ * residual of that normal exceeds learnedResidualTol, and on cells without a
+ * full uniform stencil, the least-squares normal (reconstruction.C:85-141) is
+ * used instead.  The residual is the mean |fraction - alpha| over the 26
+ * neighbour cubes of the plane with the network normal that holds the cell's
+ * own alpha in its cube.  The unchanged FP64 findSignedDistance
+ * (cutCell.C:611-799) then places every plane.
  *
  * The network is trained on the GPU by this library: training pairs are cut
  * from random planes, spheres and ellipsoids (curvature radius 1.5-50 cells)
@@ -88,9 +90,9 @@ int svp_set_normal_model(svp_handle* h, const svp_normal_model* m);
  * [3] without a full uniform stencil or with a degenerate network output. */
 int svp_learned_stats(svp_handle* h, int64_t* out4);
 
-/* Residuals (mean |fraction - alpha| over the 26 neighbours) of the last
- * reconstruct, per interface-cell position; -1 where no learned normal was
- * tried.  SVP_E_ARG when learnedResidualTol >= 1 (the test is not run). */
+/* Residuals of the last reconstruct, per interface-cell position; -1 where no
+ * network normal was tried.  SVP_E_ARG when learnedResidualTol >= 1 (the test
+ * is not run). */
 int svp_learned_residuals(svp_handle* h, double* out, int where);
 
 #ifdef __cplusplus

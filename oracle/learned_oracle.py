@@ -16,8 +16,10 @@ reference, so these functions restate this library's own design:
   BF16 rounding points of the MFMA kernels (``rounded=True``) or in plain
   FP64 (``rounded=False``).  The FP64 backward is pinned by central finite
   differences.
-* ``residual``: mean |fraction - alpha| of the plane over the 26 lattice
-  neighbours.
+* ``cube_plane_offset`` / ``residual``: the plane of a normal that holds a
+  cube's own alpha, and its mean |fraction - alpha| over the 26 lattice
+  neighbours.  The offset is pinned against the oracle's findSignedDistance
+  on unit cubes.
 """
 from __future__ import annotations
 
@@ -225,12 +227,33 @@ def mlp_loss_grad(params, x, y, rounded=True):
 # ---------------------------------------------------------------------------
 # residual of a plane over the lattice neighbours
 # ---------------------------------------------------------------------------
-def residual(n, d, centres27, alpha27, h):
-    """Mean |fraction - alpha| over the 26 neighbours of cubes of edge h at centres27."""
+def cube_plane_offset(n, a, lo, h, rounds=16):
+    """Plane constant D of the cube [lo, lo+h]^3 with liquid fraction a on the
+    side n.x + D < 0, by the inference kernel's multisection: per round 4
+    interior points of the bracket, which shrinks to the interval of the
+    crossing."""
+    base = -(n[0] * lo[0] + n[1] * lo[1] + n[2] * lo[2])
+    sx, sy, sz = n[0] * h, n[1] * h, n[2] * h
+    tmin = min(sx, 0.0) + min(sy, 0.0) + min(sz, 0.0)
+    tmax = max(sx, 0.0) + max(sy, 0.0) + max(sz, 0.0)
+    dlo, dhi = base - tmax, base - tmin
+    for _ in range(rounds):
+        w = (dhi - dlo) / 5.0
+        k = sum(1 for q in range(4) if plane_cube_fraction(n, dlo + w * (q + 1), lo, h) > a)
+        new_lo = dlo + w * k if k > 0 else dlo
+        new_hi = dlo + w * (k + 1) if k < 4 else dhi
+        dlo, dhi = new_lo, new_hi
+    return 0.5 * (dlo + dhi)
+
+
+def residual(n, d, lo, h, alpha27):
+    """Mean |fraction - alpha| over the 26 lattice neighbour cubes lo + e h of
+    the cell cube [lo, lo+h]^3 for the plane n.x + d = 0."""
     acc = 0.0
     for L in range(NIN):
         if L == 13:
             continue
-        lo = [centres27[L][k] - 0.5 * h for k in range(3)]
-        acc += abs(plane_cube_fraction(n, d, lo, h) - alpha27[L])
+        e = (L // 9 - 1, (L // 3) % 3 - 1, L % 3 - 1)
+        clo = [lo[k] + e[k] * h for k in range(3)]
+        acc += abs(plane_cube_fraction(n, d, clo, h) - alpha27[L])
     return acc / 26.0

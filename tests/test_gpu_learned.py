@@ -254,13 +254,18 @@ def test_learned_reconstruction_against_oracle(g, trained, oracle_lib):
                 c27[L] = C[j]
             perm, sgn = LO.canon_of(a27)
             o = m.forward(LO.canonical_input(a27)[None]).astype(np.float64)[0]
-            v = np.array(LO.from_canon(perm, sgn, o))
-            v /= np.linalg.norm(v)
+            v = LO.from_canon(perm, sgn, o)
+            mg = math.sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2])
+            v = np.array([v[0] / mg, v[1] / mg, v[2] / mg])
+            # the residual of the network normal's cube plane
+            lo = C[c] - 0.5 * h
+            dv = LO.cube_plane_offset(v, alpha[c], lo, h)
+            assert abs(LO.residual(v, dv, lo, h, a27) - res[k]) < 1e-12
             if np.allclose(n, v, atol=1e-12, rtol=0):
                 n_kept += 1
                 assert res[k] <= tol
-                assert abs(LO.residual(n, itf["d"][k], c27, a27, h) - res[k]) < 1e-12
                 continue
+            assert res[k] > tol
         # least-squares normal (no full stencil, or fallback)
         assert np.array_equal(n, pitf["n"][k])
     assert n_kept == kept

@@ -150,14 +150,28 @@ def test_rounded_network_close_to_fp64():
 
 def test_residual_of_exact_plane_is_zero():
     rng = np.random.default_rng(4)
-    cen = [(L // 9 - 1, (L // 3) % 3 - 1, L % 3 - 1) for L in range(27)]
     for _ in range(20):
         n = rng.normal(size=3)
         n /= np.linalg.norm(n)
         d = rng.uniform(-0.4, 0.4)
         a = _plane_stencil(n, d)
-        assert LO.residual(n, d, cen, a, 1.0) < 1e-14
+        assert LO.residual(n, d, (-0.5, -0.5, -0.5), 1.0, a) < 1e-14
         # a tilted plane leaves a positive residual
         n2 = n + 0.3 * rng.normal(size=3)
         n2 /= np.linalg.norm(n2)
-        assert LO.residual(n2, d, cen, a, 1.0) > 1e-4
+        assert LO.residual(n2, d, (-0.5, -0.5, -0.5), 1.0, a) > 1e-4
+
+
+def test_cube_plane_offset_matches_find_signed_distance(unit_cube):
+    """The multisection plane of a cube at a given alpha (16 rounds, bracket
+    width 5^-16 of the vertex range) is the plane the oracle's
+    findSignedDistance places in the same cube (cutCell.C:611-799)."""
+    rng = np.random.default_rng(8)
+    for _ in range(100):
+        n = rng.normal(size=3)
+        n /= np.linalg.norm(n)
+        a = rng.uniform(0.01, 0.99)
+        d = LO.cube_plane_offset(n, a, (0.0, 0.0, 0.0), 1.0)
+        assert abs(LO.plane_cube_fraction(n, d, (0.0, 0.0, 0.0), 1.0) - a) < 1e-9
+        st, out = unit_cube.find_signed_distance(0, a, n)
+        assert st == 0 and abs(out[0] - d) < 1e-10, (out[0], d)
