@@ -248,6 +248,23 @@ __device__ inline T pick(const T (&a)[N], int j) {
     return r;
 }
 
+// Rotate register arrays left by r (0 <= r < N): a[i] <- a[(i + r) % N], by
+// conditional moves with static indices only (binary stages of the shift),
+// so the arrays stay in registers.
+template <int N, class T>
+__device__ inline void rotate_left(T (&a)[N], int r) {
+#pragma unroll
+    for (int s = 1; s < N; s <<= 1) {
+        if (r & s) {
+            T t[N];
+#pragma unroll
+            for (int i = 0; i < N; ++i) t[i] = a[(i + s) % N];
+#pragma unroll
+            for (int i = 0; i < N; ++i) a[i] = t[i];
+        }
+    }
+}
+
 // calcSubFace for a face of exactly N points held in registers; the same
 // operations in the same order as calcSubFace above.
 template <int N>
@@ -278,6 +295,16 @@ __device__ inline bool calcSubFaceN(const V3 (&fPts)[N], V3 normal, double dista
         return true;
     }
     bool ok = true;
+    // the walk starts at the first submerged point: rotate it to slot 0
+    V3 rp[N];
+    double rd[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        rp[i] = fPts[i];
+        rd[i] = dist[i];
+    }
+    rotate_left(rp, first);
+    rotate_left(rd, first);
     int nPts = 0;
     V3 p0 = VZERO, p1 = VZERO, p2 = VZERO, fCentre = VZERO;
     auto emit1 = [&](V3 q) {
@@ -287,11 +314,9 @@ __device__ inline bool calcSubFaceN(const V3 (&fPts)[N], V3 normal, double dista
     };
 #pragma unroll
     for (int k = 0; k < N; ++k) {
-        int cur = first + k;
-        if (cur >= N) cur -= N;
-        const int nxt = (cur + 1 == N) ? 0 : cur + 1;
-        const double dc = pick(dist, cur), dn = pick(dist, nxt);
-        const V3 pc = pick(fPts, cur), pn = pick(fPts, nxt);
+        const int nxt = (k + 1 == N) ? 0 : k + 1;
+        const double dc = rd[k], dn = rd[nxt];
+        const V3 pc = rp[k], pn = rp[nxt];
         if (dc < 0) emit1(pc);
         if ((dc * dn) < 0) {
             const double w = dc / (dc - dn);
@@ -338,11 +363,9 @@ __device__ inline bool calcSubFaceN(const V3 (&fPts)[N], V3 normal, double dista
     };
 #pragma unroll
     for (int k = 0; k < N; ++k) {
-        int cur = first + k;
-        if (cur >= N) cur -= N;
-        const int nxt = (cur + 1 == N) ? 0 : cur + 1;
-        const double dc = pick(dist, cur), dn = pick(dist, nxt);
-        const V3 pc = pick(fPts, cur), pn = pick(fPts, nxt);
+        const int nxt = (k + 1 == N) ? 0 : k + 1;
+        const double dc = rd[k], dn = rd[nxt];
+        const V3 pc = rp[k], pn = rp[nxt];
         if (dc < 0) emit2(pc);
         if ((dc * dn) < 0) {
             const double w = dc / (dc - dn);

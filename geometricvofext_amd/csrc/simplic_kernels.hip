@@ -1013,11 +1013,21 @@ __device__ inline double tiff_regs(const DevMesh& m, int f, const V3 (&p)[N], V3
         const V3 fc = ld3(m.Cf, f);
         const double t0 = (dot(fc, normal) + distance) / Un0;
         const double tc = fabs(t0) < SVP_TSMALL ? 0.0 : t0;
+        // fan triangle pi uses points pi and pi+1: the arrays rotate one slot
+        // per triangle so that those are always slots 0 and 1
+        V3 rp[N];
+        double rt[N];
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            rp[i] = p[i];
+            rt[i] = pTimes[i];
+        }
 #pragma unroll 1
         for (int pi = 0; pi < N; ++pi) {
-            const int pn = (pi + 1 == N) ? 0 : pi + 1;
-            const V3 tri[3] = {fc, pick(p, pi), pick(p, pn)};
-            const double tt[3] = {tc, pick(pTimes, pi), pick(pTimes, pn)};
+            const V3 tri[3] = {fc, rp[0], rp[1]};
+            const double tt[3] = {tc, rt[0], rt[1]};
+            rotate_left(rp, 1);
+            rotate_left(rt, 1);
             const double magSfTri = mag(triAreaNormal(tri[0], tri[1], tri[2]));
             const double phiTri = phi * magSfTri / magSf;
             dVf += phiTri / magSfTri * tia_regs<3>(tri, tt, normal, distance, Un0, dt, magSfTri, ok);
