@@ -43,6 +43,11 @@ WORKLOADS = {
     "hex64": dict(kind="hex", N=64, perturb=0.0, seed=0, shapes=[((0.35, 0.35, 0.35), 0.15)],
                   flow="deformation", end_time=3.0,
                   desc="64^3 uniform hex, sphere r=0.15 at 0.35, deformation T=3, Co 0.5"),
+    # tetrahedral workload: 96^3 cubes x 6 Kuhn tets, vertices +-10% h, 8 seeded
+    # spheres, one revolution of solid-body rotation about a seeded axis
+    "tet96": dict(kind="tet", N=96, perturb=0.1, seed=2, shapes="spheres:7", flow="rotation:3",
+                  end_time=1.0, desc="96^3 x 6 tets (5.3M cells), vertices +-10% h (seed 2), 8 seeded "
+                                     "spheres r 0.05-0.12, one revolution of solid-body rotation, Co 0.5"),
 }
 
 ALPHA_DICT = dict(orientationMethod="LS", nAlphaBounds=3, snapTol=0.0, clip=False, mixedCellTol=1e-8)
@@ -105,12 +110,25 @@ def build_case(wl):
         mesh = g.hex_box(wl["N"], perturb=wl["perturb"], seed=wl["seed"])
     else:
         mesh = g.tet_box(wl["N"], perturb=wl["perturb"], seed=wl["seed"])
+    from geometricvofext_amd import cases
     solver = g.SolveVofEqu(mesh, ALPHA_DICT)
-    solver.init_alpha(wl["shapes"], max_depth=5)
+    shapes = wl["shapes"]
+    if isinstance(shapes, str):
+        shapes = cases.random_spheres(int(shapes.split(":")[1]))
+    solver.init_alpha(shapes, max_depth=5 if wl["kind"] == "hex" else 4)
     alpha0 = solver.alpha()
-    U0, phi0 = solver.deformation_field()
+    if wl["flow"].startswith("rotation"):
+        omega = cases.rotation_omega(int(wl["flow"].split(":")[1]), wl["end_time"])
+        U0, phi0 = solver.rotation_field(omega, (0.5, 0.5, 0.5))
+    else:
+        U0, phi0 = solver.deformation_field()
     solver.upload_base_flow(U0, phi0)
     return mesh, solver, alpha0, U0, phi0
+
+
+def flow_period(wl):
+    """Period of the harness time factor: 2T for the reversing deformation, none for rotation."""
+    return 0.0 if wl["flow"].startswith("rotation") else 2.0 * wl["end_time"]
 
 
 def learned_comparison(say):
@@ -178,7 +196,7 @@ def learned_comparison(say):
     return out
 
 
-def time_sequence(solver, alpha0, end_time, max_steps=None):
+def time_sequence(solver, alpha0, end_time, max_steps=None, period=6.0):
     """(t, dt, uFactor) of every step to end_time.  With maxCo == maxAlphaCo
     the alpha-restricted Courant number never exceeds the full one, so the
     sequence does not depend on alpha and is computed from the flow alone."""
@@ -192,7 +210,7 @@ def time_sequence(solver, alpha0, end_time, max_steps=None):
         co, aco = solver.courant()
         rt.set_delta_t(0.5 * co * rt.delta_t, 0.5 * aco * rt.delta_t)
         rt.increment()
-        uf = harness.u_factor(rt.value, rt.delta_t, 6.0)
+        uf = harness.u_factor(rt.value, rt.delta_t, period)
         seq.append((rt.value, rt.delta_t, uf))
         solver.scale_flow(uf)
         if max_steps and len(seq) >= max_steps:
@@ -258,7 +276,7 @@ def main():
     wl = WORKLOADS[args.workload]
     t_setup = time.time()
     mesh, solver, alpha0, U0, phi0 = build_case(wl)
-    seq = time_sequence(solver, alpha0, wl["end_time"])
+    seq = time_sequence(solver, alpha0, wl["end_time"], period=flow_period(wl))
     n_total = len(seq)
     K = args.steps if args.steps > 0 else n_total
     K = min(K, n_total)

@@ -245,7 +245,8 @@ public:
             char buf[256];
             std::snprintf(buf, sizeof buf,
                           "device error bits 0x%x (1: cell exceeds cut limits, 2: more than %d plane "
-                          "crossings on a face, 4: bounding candidate overflow, 8: bounding work overflow)",
+                          "crossings on a face, 4: bounding candidate overflow, 8: bounding work overflow, "
+                          "16: bounding dependency wait without progress)",
                           err, MAX_IF_POINTS);
             throw SvpError(SVP_E_DEVICE, buf);
         }
@@ -643,6 +644,14 @@ private:
         if (const char* e = std::getenv("SVP_BND_GRID")) {
             const int g = std::atoi(e);
             if (g >= 1 && g <= 4096) st_.bndGrid = g;
+        }
+        // dependent part of a bounding pass: one dataflow launch of this many
+        // workgroups (knob SVP_BND_FLOW, 0 = grid rounds and the one-workgroup
+        // tail; results do not depend on it)
+        st_.bndFlow = 64;
+        if (const char* e = std::getenv("SVP_BND_FLOW")) {
+            const int g = std::atoi(e);
+            if (g >= 0 && g <= 4096) st_.bndFlow = g;
         }
         st_.bndRec = maxCellFaces_ <= NS_CELL_FACES ? (void*)dalloc<BndRec<NS_CELL_FACES>>(nC)
                                                     : (void*)dalloc<BndRec<MAX_CELL_FACES>>(nC);

@@ -1347,6 +1347,14 @@ __device__ inline void bound_cell(const DevMesh& m, const DevState& st, int c, d
 // bcount: [0] candidates, [1] corrected faces, [2] touched cells, [3+r]
 // frontier size of round r (r <= st.gridRounds), [BND_CTRL-1] rounds.
 constexpr int BND_CTRL = 3 + MAX_GRID_ROUNDS + 2;
+// dataflow pass (one launch in place of the rounds and the tail): chunk
+// ticket and completion count, cleared with the per-pass counters
+constexpr int BND_TICKET = BND_CTRL;
+constexpr int BND_PROGRESS = BND_CTRL + 1;
+static_assert(BND_PROGRESS < 16, "bcount holds 16 counters");
+// correction of a face whose writer has not completed yet in the dataflow
+// pass: a NaN payload that no arithmetic produces (stored and compared as bits)
+constexpr unsigned long long CORR_PENDING = 0x7FF4D00DF00DD00Dull;
 
 // Ascending candidates of the pass from the bitmap: per-block popcounts,
 // one-workgroup scan of the block counts, then every block writes its
@@ -1379,7 +1387,7 @@ __global__ void __launch_bounds__(256) k_bnd_count(DevState st, int nWords) {
 __global__ void __launch_bounds__(1024) k_bnd_scan(DevState st, int nBlocks) {
     __shared__ int lds[16];
     const int t = threadIdx.x;
-    if (t >= 1 && t < BND_CTRL) st.bcount[t] = 0;
+    if (t >= 1 && t <= BND_PROGRESS) st.bcount[t] = 0;
     if (!oob_any(st)) {
         if (t == 0) st.bcount[0] = 0;
         return;
@@ -1528,8 +1536,16 @@ __global__ void __launch_bounds__(256) k_bnd_scatter(DevMesh m, DevState st, dou
         R.Vi = m.V[c];
         R.ac = st.alpha[c];
         R.aOld = st.alphaOld[c];
-        st.pend[i] = np;
-        if (np == 0) st.queue[atomicAdd(&st.bcount[3], 1)] = i;
+        if (st.bndFlow) {
+            // every face this cell writes reads as pending until it completes
+#pragma unroll
+            for (int k = 0; k < MF; ++k)
+                if (k < nf && (flg[k >> 3] >> (4 * (k & 7)) & 6u) == 6u)
+                    st.corr[R.fid[k]] = __longlong_as_double((long long)CORR_PENDING);
+        } else {
+            st.pend[i] = np;
+            if (np == 0) st.queue[atomicAdd(&st.bcount[3], 1)] = i;
+        }
     }
 }
 
@@ -1537,9 +1553,12 @@ __global__ void __launch_bounds__(256) k_bnd_scatter(DevMesh m, DevState st, dou
 // Corrections of the downwind faces live in registers; those of upwind faces
 // come from lower-numbered cells that have completed (a higher-numbered writer
 // has not run yet in the reference's ascending walk, so it contributes zero).
-template <int MF>
-__device__ inline void bound_cell_rec(const BndRec<MF>& R, const DevState& st, int c, double dt, double rDeltaT,
-                                      const double* Sp, const double* Su) {
+// cin[k]: correction of face slot k written by its lower-numbered upwind
+// neighbour (0 where the flags mark no such writer); out(f, v) stores the
+// cell's final correction of downwind face f.
+template <int MF, class Out>
+__device__ inline void bound_cell_rec(const BndRec<MF>& R, const double (&cin)[MF], int c, double dt,
+                                      double rDeltaT, const double* Sp, const double* Su, Out out) {
     const double aTol = 100.0 * SVP_SMALL;
     unsigned int flg[(MF + 7) / 8];
 #pragma unroll
@@ -1555,7 +1574,7 @@ __device__ inline void bound_cell_rec(const BndRec<MF>& R, const DevState& st, i
     for (int k = 0; k < MF; ++k) {
         phf[k] = R.phf[k];
         dvf[k] = R.dvf[k];
-        cor[k] = (bits(k) & 8u) ? st.corr[R.fid[k]] : 0.0;
+        cor[k] = cin[k];
     }
     double nfd = 0.0;
 #pragma unroll
@@ -1608,7 +1627,20 @@ __device__ inline void bound_cell_rec(const BndRec<MF>& R, const DevState& st, i
     }
 #pragma unroll
     for (int k = 0; k < MF; ++k)
-        if (k < nf && down(k) && writable(k)) st.corr[R.fid[k]] = cor[k];
+        if (k < nf && down(k) && writable(k)) out(R.fid[k], cor[k]);
+}
+
+// boundFlux of candidate i from its record: its own case's time step in a batch
+template <int MF, class Out>
+__device__ inline void bound_candidate(const DevState& st, const BndRec<MF>& R, const double (&cin)[MF], int i,
+                                       double dt, double rDeltaT, const double* Sp, const double* Su, Out out) {
+    const int c = st.cand[i];
+    if (st.caseOf) {
+        const double dtc = st.dtCase[st.caseOf[c]];
+        bound_cell_rec<MF>(R, cin, c, dtc, 1.0 / dtc, Sp, Su, out);
+    } else {
+        bound_cell_rec<MF>(R, cin, c, dt, rDeltaT, Sp, Su, out);
+    }
 }
 
 // completes cell i of frontier r and releases its successors into frontier
@@ -1618,15 +1650,13 @@ template <int MF, class Release, class Push>
 __device__ inline void bnd_complete(const DevMesh& m, const DevState& st, int i, double dt, double rDeltaT,
                                     const double* Sp, const double* Su, Release release, Push push) {
     const BndRec<MF>& R = ((const BndRec<MF>*)st.bndRec)[i];
-    const int c = st.cand[i];
-    // successors read these corrections in a later round: the next kernel of
-    // the grid rounds, or after the workgroup barrier of the tail
-    if (st.caseOf) {
-        const double dtc = st.dtCase[st.caseOf[c]];
-        bound_cell_rec<MF>(R, st, c, dtc, 1.0 / dtc, Sp, Su);
-    } else {
-        bound_cell_rec<MF>(R, st, c, dt, rDeltaT, Sp, Su);
-    }
+    // predecessors wrote these corrections in an earlier round: the previous
+    // kernel of the grid rounds, or before the workgroup barrier of the tail
+    double cin[MF];
+#pragma unroll
+    for (int k = 0; k < MF; ++k)
+        cin[k] = ((R.flg[k >> 3] >> (4 * (k & 7))) & 8u) ? st.corr[R.fid[k]] : 0.0;
+    bound_candidate<MF>(st, R, cin, i, dt, rDeltaT, Sp, Su, [&](int f, double v) { st.corr[f] = v; });
 #pragma unroll
     for (int k = 0; k < MF; ++k) {
         const int a = R.succ[k];
@@ -1722,6 +1752,89 @@ __global__ void __launch_bounds__(TAIL_THREADS) k_bnd_tail(DevMesh m, DevState s
         __syncthreads();
     }
     if (t == 0) st.bcount[BND_CTRL - 1] = sRounds;
+}
+
+// The dependent part of a pass in one launch.  Waves take chunks of 64
+// consecutive candidates from a ticket, so every lower-numbered candidate is
+// held by a wave that is already running; a candidate runs as soon as none of
+// the corrections it reads (faces written by lower-numbered active upwind
+// neighbours) is CORR_PENDING.  Those corrections are the only data passed
+// between candidates: each is written once, by one cell, with an agent-scope
+// atomic store and read with agent-scope atomic loads, so the value is its
+// own ready flag.  The lowest unfinished candidate is always ready, so the
+// waits end; a wave that sees no completion anywhere in the grid for about
+// a second flags ERR_BOUND_STALL and leaves.
+constexpr int FLOW_STALL_CHECKS = 1 << 14;  // idle checks, 64 sleeps each, without progress
+
+__device__ inline unsigned long long corr_load(const double* p) {
+    return __hip_atomic_load((const unsigned long long*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <int MF>
+__global__ void __launch_bounds__(256) k_bnd_flow(DevMesh m, DevState st, double dt, const double* Sp,
+                                                  const double* Su) {
+    const int nCand = st.bcount[0];
+    const int lane = threadIdx.x & 63;
+    const double rDeltaT = 1.0 / dt;
+    int* ticket = &st.bcount[BND_TICKET];
+    int* progress = &st.bcount[BND_PROGRESS];
+    auto out = [&](int f, double v) {
+        __hip_atomic_store((unsigned long long*)&st.corr[f], (unsigned long long)__double_as_longlong(v),
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    };
+    while (true) {
+        int base = nCand;
+        // a wave that left on a stall ends the pass for everyone
+        if (lane == 0 && !(__hip_atomic_load(st.errFlag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & ERR_BOUND_STALL))
+            base = atomicAdd(ticket, 64);
+        base = __shfl(base, 0);
+        if (base >= nCand) break;
+        const int i = base + lane;
+        bool pending = i < nCand && st.active[i] != 0;
+        const unsigned long long act = __ballot(pending);
+        if (lane == 0 && act) atomicAdd(&st.bcount[3], __popcll(act));
+        int idle = 0, seen = -1, stalls = 0;
+        while (__any(pending)) {
+            bool fin = false;
+            if (pending) {
+                const BndRec<MF>& R = ((const BndRec<MF>*)st.bndRec)[i];
+                double cin[MF];
+                bool ready = true;
+#pragma unroll
+                for (int k = 0; k < MF; ++k) {
+                    cin[k] = 0.0;
+                    if ((R.flg[k >> 3] >> (4 * (k & 7))) & 8u) {
+                        const unsigned long long u = corr_load(&st.corr[R.fid[k]]);
+                        ready = ready && u != CORR_PENDING;
+                        cin[k] = __longlong_as_double((long long)u);
+                    }
+                }
+                if (ready) {
+                    bound_candidate<MF>(st, R, cin, i, dt, rDeltaT, Sp, Su, out);
+                    pending = false;
+                    fin = true;
+                }
+            }
+            const unsigned long long done = __ballot(fin);
+            if (done) {
+                if (lane == 0) atomicAdd(progress, __popcll(done));
+                idle = 0;
+                continue;
+            }
+            __builtin_amdgcn_s_sleep(1);
+            if (++idle < 64) continue;
+            idle = 0;
+            int now = 0;
+            if (lane == 0) now = __hip_atomic_load(progress, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            now = __shfl(now, 0);
+            stalls = (now == seen) ? stalls + 1 : 0;
+            seen = now;
+            if (stalls >= FLOW_STALL_CHECKS) {
+                if (lane == 0) atomicOr(st.errFlag, ERR_BOUND_STALL);
+                return;
+            }
+        }
+    }
 }
 
 // alpha of every touched cell in correctedFaces order (advectionTemplates.C:164-192)
@@ -1839,8 +1952,12 @@ void launch_bound_pass(const DevMesh& m, const DevState& st, int pass, double dt
     k_bnd_prep<<<grid, 256, 0, s>>>(m, st, dt);
     k_bnd_scan2<<<1, 1024, 0, s>>>(st);
     k_bnd_scatter<MF><<<grid, 256, 0, s>>>(m, st, dt);
-    for (int r = 0; r < st.gridRounds; ++r) k_bnd_round<MF><<<grid, 256, 0, s>>>(m, st, r, dt, Sp, Su);
-    k_bnd_tail<MF><<<1, TAIL_THREADS, 0, s>>>(m, st, dt, Sp, Su);
+    if (st.bndFlow) {
+        k_bnd_flow<MF><<<std::max(1, std::min(st.bndFlow, cdiv(m.nC, 256))), 256, 0, s>>>(m, st, dt, Sp, Su);
+    } else {
+        for (int r = 0; r < st.gridRounds; ++r) k_bnd_round<MF><<<grid, 256, 0, s>>>(m, st, r, dt, Sp, Su);
+        k_bnd_tail<MF><<<1, TAIL_THREADS, 0, s>>>(m, st, dt, Sp, Su);
+    }
     k_bnd_apply<MF><<<grid, 256, 0, s>>>(m, st);
     k_bnd_reset<<<grid, 256, 0, s>>>(m, st, pass);
 }

@@ -86,6 +86,7 @@ struct DevState {
     int un0Fused;       // plane placement by 8-lane groups, which also computes Un0
     int gridRounds;     // bounding rounds run as grid-wide kernels before the one-workgroup tail
     int bndGrid;        // workgroups of the grid-stride bounding kernels
+    int bndFlow;        // workgroups of the one-launch dataflow pass, 0 = rounds and tail
     void* bndRec;       // nC records BndRec<MF> of the active bounding candidates
     int maxCellFaces;
     // batch of independent cases in one mesh: case of every cell and the
@@ -116,6 +117,7 @@ enum : int {
     ERR_CELL_TOO_LARGE = 1,
     ERR_TOO_MANY_CROSSINGS = 2,
     ERR_BOUND_SCHEDULE = 4,
+    ERR_BOUND_STALL = 16,
 };
 constexpr int MAX_BOUND_PASSES = 64;
 constexpr int MAX_GRID_ROUNDS = 8;
