@@ -648,7 +648,7 @@ private:
         // dependent part of a bounding pass: one dataflow launch of this many
         // workgroups (knob SVP_BND_FLOW, 0 = grid rounds and the one-workgroup
         // tail; results do not depend on it)
-        st_.bndFlow = 64;
+        st_.bndFlow = 128;
         if (const char* e = std::getenv("SVP_BND_FLOW")) {
             const int g = std::atoi(e);
             if (g >= 0 && g <= 4096) st_.bndFlow = g;

@@ -1207,83 +1207,169 @@ __device__ inline int wg_scan1024(int v, int* lds, int& total) {
     return pre + x - v;
 }
 
-// First redistribution loop of boundFlux (advectionTemplates.C:260-330 with
-// iter = 0): it only reads the cell's own data, because the corrections of a
-// face are written solely by the cell that face is downwind of.  Returns the
-// number of faces the loop passes fluid through (they open correctedFaces in
-// this order) or 0 when the cell writes no corrections in the whole pass.
-__device__ inline int bound_first_faces(const DevMesh& m, const DevState& st, int c, double dt,
-                                        int* facesOut) {
+// limitFlux (advectionTemplates.C:118-215), one pass = the kernels below.
+// The pass runs only while cells lie outside [-aTol, 1+aTol] (bound count of
+// the update kernel, kept current by the apply kernel):
+//   count/scan/write  ascending candidates from the bitmap
+//   prep              every candidate's first redistribution loop, which
+//                     decides whether it writes corrections and which faces
+//                     open correctedFaces; the record of each writing cell
+//   scan2             correctedFaces offsets
+//   flow              all remaining boundFlux work in dependency order in
+//                     one launch, and the correctedFaces positions and
+//                     touched cells (the default), or
+//   scatter/round/tail  the same as Kahn rounds: grid-wide kernels, then the
+//                     remaining rounds inside one workgroup
+//   apply             corrections applied per cell in correctedFaces order,
+//                     dVf of the corrected faces, bitmap and bound count
+//   reset             dVfCorrectionValues = 0 for the next pass
+// bcount: [0] candidates, [1] corrected faces, [2] touched cells, [3+r]
+// frontier size of round r (r <= st.gridRounds; [3] = active cells of the
+// dataflow pass), [BND_CTRL-1] rounds.
+constexpr int BND_CTRL = 3 + MAX_GRID_ROUNDS + 2;
+// dataflow pass (one launch in place of the rounds and the tail): chunk
+// ticket and completion count, cleared with the per-pass counters
+constexpr int BND_TICKET = BND_CTRL;
+constexpr int BND_PROGRESS = BND_CTRL + 1;
+static_assert(BND_PROGRESS < 16, "bcount holds 16 counters");
+// correction of a face whose writer has not completed yet in the dataflow
+// pass: a NaN payload that no arithmetic produces (stored and compared as bits)
+constexpr unsigned long long CORR_PENDING = 0x7FF4D00DF00DD00Dull;
+
+// Record of candidate c for the dependent part of the pass, with iteration 0
+// of boundFlux's redistribution loop (advectionTemplates.C:260-330) already
+// done: that iteration reads only the cell's own downwind faces, whose
+// corrections start at zero because a face is written solely by the cell it
+// is downwind of.  Its selected faces are the first loop that opens
+// correctedFaces.  Returns the number of those faces; 0 means the cell writes
+// no corrections this pass and gets no record.  Every face the cell will
+// write is marked CORR_PENDING.
+template <int MF>
+__device__ inline int bound_record(const DevMesh& m, const DevState& st, int c, double dt, const double* Sp,
+                                   const double* Su, BndRec<MF>& R) {
     const double aTol = 100.0 * SVP_SMALL;
     const double ac = st.alpha[c];
     const double Vi = m.V[c];
     const double alphaOvershoot = pos0(ac - 1.0) * (ac - 1.0) + neg0(ac) * ac;
     if (fabs(alphaOvershoot) < aTol) return 0;
     const double fluidToPassOn = alphaOvershoot * Vi;
-    int n = 0;
-    for (int k = m.cOff[c]; k < m.cOff[c + 1]; ++k) {
-        const int f = m.cFaces[k];
-        if (!face_downwind_of(m, st.phi, f, c)) continue;
-        const double ph = face_value(m, st.phi, f);
-        const double dVff = face_value(m, st.dVf, f) + 0.0;
-        const double maxExtra = fabs(pos0(fluidToPassOn) * ph * dt - dVff);
-        if (maxExtra / Vi > aTol) {
-            if (facesOut) facesOut[n] = f;
-            n++;
-        }
-    }
-    return n;
-}
-
-// Whole boundFlux of one cell (advectionTemplates.C:243-347).  Corrections of
-// the cell's downwind faces live in registers; those of its upwind faces come
-// from lower-numbered cells that have completed (the reference processes cells
-// in ascending order, so a higher-numbered writer contributes zero).
-template <int MF>
-__device__ inline void bound_cell(const DevMesh& m, const DevState& st, int c, double dt, double rDeltaT,
-                                  const double* Sp, const double* Su) {
-    // every per-face array is indexed by the unrolled face slot only, so the
-    // whole cell stays in registers
-    const double aTol = 100.0 * SVP_SMALL;
-    const int b = m.cOff[c], e = m.cOff[c + 1];
-    const int nf = e - b;
-    const double Vi = m.V[c];
-    const double ac = st.alpha[c];
-    const double aOld = st.alphaOld[c];
+    const int b = m.cOff[c], nf = m.cOff[c + 1] - b;
     int fid[MF];
-    bool isOwn[MF], down[MF], writable[MF];
-    double phf[MF], dvf[MF], cor[MF];
+    double phf[MF], dvf[MF], smax[MF];
+    unsigned int flg[(MF + 7) / 8];
+    unsigned int first = 0u;
+    int n = 0;
+    double dVftot = 0.0;
+#pragma unroll
+    for (int w = 0; w < (MF + 7) / 8; ++w) flg[w] = 0u;
 #pragma unroll
     for (int k = 0; k < MF; ++k) {
         fid[k] = 0;
-        isOwn[k] = false;
-        down[k] = false;
-        writable[k] = false;
         phf[k] = 0.0;
         dvf[k] = 0.0;
-        cor[k] = 0.0;
+        smax[k] = 0.0;
         if (k < nf) {
             const int f = m.cFaces[b + k];
             fid[k] = f;
-            isOwn[k] = (m.own[f] == c);
-            phf[k] = face_value(m, st.phi, f);
-            dvf[k] = face_value(m, st.dVf, f);
-            down[k] = isOwn[k] ? (phf[k] >= 0) : (phf[k] < 0);
-            writable[k] = (f < m.nIF) || !m.fEmpty[f];
-            if (!down[k] && f < m.nIF) {
-                const int j = isOwn[k] ? m.nbr[f] : m.own[f];
-                if (j < c) cor[k] = st.corr[f];
+            const bool own = (m.own[f] == c);
+            const double ph = face_value(m, st.phi, f);
+            const double dv = face_value(m, st.dVf, f);
+            phf[k] = ph;
+            dvf[k] = dv;
+            const bool down = own ? (ph >= 0) : (ph < 0);
+            const bool writable = (f < m.nIF) || !m.fEmpty[f];
+            const bool lowerWriter = f < m.nIF && !down && (own ? m.nbr[f] : m.own[f]) < c;
+            flg[k >> 3] |= ((own ? 1u : 0u) | (down ? 2u : 0u) | (writable ? 4u : 0u) | (lowerWriter ? 8u : 0u))
+                           << (4 * (k & 7));
+            if (down) {
+                const double dVff = dv + 0.0;
+                const double maxExtra = fabs(pos0(fluidToPassOn) * ph * dt - dVff);
+                if (maxExtra / Vi > aTol) {
+                    first |= 1u << k;
+                    n++;
+                    smax[k] = maxExtra;
+                    dVftot += fabs(ph * dt);
+                }
             }
         }
+    }
+    if (n == 0) return 0;
+    auto bits = [&](int k) { return flg[k >> 3] >> (4 * (k & 7)); };
+    int nFaces = 0;
+#pragma unroll
+    for (int k = 0; k < MF; ++k) {
+        R.cor0[k] = 0.0;
+        if (!((first >> k) & 1u)) continue;
+        double fl = fabs(fluidToPassOn) * fabs(phf[k] * dt) / dVftot;
+        nFaces += (int)pos0(smax[k] - fl);
+        fl = fmin(fl, smax[k]);
+        double dVff = 0.0;
+        dVff += sgn(phf[k]) * sgn(fluidToPassOn) * fl;
+        if (bits(k) & 4u) R.cor0[k] = dVff;
     }
     double nfd = 0.0;
 #pragma unroll
     for (int k = 0; k < MF; ++k)
-        if (k < nf) nfd = isOwn[k] ? nfd + dvf[k] : nfd - dvf[k];
-    double alphaOvershoot = pos0(ac - 1.0) * (ac - 1.0) + neg0(ac) * ac;
-    double fluidToPassOn = alphaOvershoot * Vi;
-    int nFaces = 1;
-    for (int iter = 0; iter < 10; iter++) {
+        if (k < nf) nfd = (bits(k) & 1u) ? nfd + dvf[k] : nfd - dvf[k];
+    const double rDeltaT = 1.0 / dt;
+    double num = st.alphaOld[c] * rDeltaT;
+    if (Su) num = num + Su[c];
+    R.num0 = num - nfd / Vi * rDeltaT;
+    R.den = Sp ? (rDeltaT - Sp[c]) : rDeltaT;
+#pragma unroll
+    for (int k = 0; k < MF; ++k) {
+        R.fid[k] = fid[k];
+        R.phf[k] = phf[k];
+        R.dvf[k] = dvf[k];
+        if (k < nf && (bits(k) & 6u) == 6u) st.corr[fid[k]] = __longlong_as_double((long long)CORR_PENDING);
+    }
+#pragma unroll
+    for (int w = 0; w < (MF + 7) / 8; ++w) R.flg[w] = flg[w];
+    R.nf = nf;
+    R.first = first;
+    R.nFaces0 = nFaces;
+    R.Vi = Vi;
+    return n;
+}
+
+// boundFlux of one cell (advectionTemplates.C:243-347) from iteration 0 of
+// its record on: cin[k] is the correction of face slot k written by its
+// lower-numbered upwind neighbour (0 where the flags mark no such writer; a
+// higher-numbered writer has not run yet in the reference's ascending walk,
+// so it contributes zero); out(k, f, v) stores the final correction of
+// downwind face f in slot k.
+template <int MF, class Out>
+__device__ inline void bound_finish(const BndRec<MF>& R, const double (&cin)[MF], double dt, double rDeltaT,
+                                    Out out) {
+    const double aTol = 100.0 * SVP_SMALL;
+    unsigned int flg[(MF + 7) / 8];
+#pragma unroll
+    for (int w = 0; w < (MF + 7) / 8; ++w) flg[w] = R.flg[w];
+    const int nf = R.nf;
+    const double Vi = R.Vi;
+    auto bits = [&](int k) { return flg[k >> 3] >> (4 * (k & 7)); };
+    auto isOwn = [&](int k) { return (bits(k) & 1u) != 0u; };
+    auto down = [&](int k) { return (bits(k) & 2u) != 0u; };
+    auto writable = [&](int k) { return (bits(k) & 4u) != 0u; };
+    double phf[MF], cor[MF];
+#pragma unroll
+    for (int k = 0; k < MF; ++k) {
+        phf[k] = R.phf[k];
+        cor[k] = down(k) ? R.cor0[k] : cin[k];
+    }
+    double alphaOvershoot = 0.0, fluidToPassOn = 0.0;
+    auto settle = [&]() {
+        double nfc = 0.0;
+#pragma unroll
+        for (int k = 0; k < MF; ++k)
+            if (k < nf) nfc = isOwn(k) ? nfc + cor[k] : nfc - cor[k];
+        const double alpha1New = (R.num0 - nfc / Vi * rDeltaT) / R.den;
+        alphaOvershoot = pos0(alpha1New - 1.0) * (alpha1New - 1.0) + neg0(alpha1New) * alpha1New;
+        fluidToPassOn = alphaOvershoot * Vi;
+    };
+    settle();
+    int nFaces = R.nFaces0;
+    for (int iter = 1; iter < 10; iter++) {
         if (fabs(alphaOvershoot) < aTol || nFaces == 0) break;
         bool sel[MF];
         double smax[MF];
@@ -1293,8 +1379,8 @@ __device__ inline void bound_cell(const DevMesh& m, const DevState& st, int c, d
         for (int k = 0; k < MF; ++k) {
             sel[k] = false;
             smax[k] = 0.0;
-            if (k < nf && down[k]) {
-                const double dVff = dvf[k] + cor[k];
+            if (k < nf && down(k)) {
+                const double dVff = R.dvf[k] + cor[k];
                 const double maxExtra = fabs(pos0(fluidToPassOn) * phf[k] * dt - dVff);
                 if (maxExtra / Vi > aTol) {
                     sel[k] = true;
@@ -1311,50 +1397,14 @@ __device__ inline void bound_cell(const DevMesh& m, const DevState& st, int c, d
             fl = fmin(fl, smax[k]);
             double dVff = cor[k];
             dVff += sgn(phf[k]) * sgn(fluidToPassOn) * fl;
-            if (writable[k]) cor[k] = dVff;
+            if (writable(k)) cor[k] = dVff;
         }
-        double nfc = 0.0;
-#pragma unroll
-        for (int k = 0; k < MF; ++k)
-            if (k < nf) nfc = isOwn[k] ? nfc + cor[k] : nfc - cor[k];
-        double num = aOld * rDeltaT;
-        if (Su) num = num + Su[c];
-        num = num - nfd / Vi * rDeltaT - nfc / Vi * rDeltaT;
-        const double den = Sp ? (rDeltaT - Sp[c]) : rDeltaT;
-        const double alpha1New = num / den;
-        alphaOvershoot = pos0(alpha1New - 1.0) * (alpha1New - 1.0) + neg0(alpha1New) * alpha1New;
-        fluidToPassOn = alphaOvershoot * Vi;
+        settle();
     }
 #pragma unroll
     for (int k = 0; k < MF; ++k)
-        if (k < nf && down[k] && writable[k]) st.corr[fid[k]] = cor[k];
+        if (k < nf && down(k) && writable(k)) out(k, R.fid[k], cor[k]);
 }
-
-// limitFlux (advectionTemplates.C:118-215), one pass = the kernels below.
-// The pass runs only while cells lie outside [-aTol, 1+aTol] (bound count of
-// the update kernel, kept current by the apply kernel):
-//   count/scan/write  ascending candidates from the bitmap
-//   prep              first redistribution loop of every candidate (it
-//                     decides which cells write and opens correctedFaces)
-//   scan2/scatter     correctedFaces positions and the cells they touch,
-//                     per-candidate records, predecessor counts, frontier
-//   round r           full boundFlux of the ready cells; successors whose
-//                     last predecessor completed join the next frontier
-//   tail              the remaining rounds inside one workgroup
-//   apply             corrections applied per cell in correctedFaces order,
-//                     dVf of the corrected faces, bitmap and bound count
-//   reset             dVfCorrectionValues = 0 for the next pass
-// bcount: [0] candidates, [1] corrected faces, [2] touched cells, [3+r]
-// frontier size of round r (r <= st.gridRounds), [BND_CTRL-1] rounds.
-constexpr int BND_CTRL = 3 + MAX_GRID_ROUNDS + 2;
-// dataflow pass (one launch in place of the rounds and the tail): chunk
-// ticket and completion count, cleared with the per-pass counters
-constexpr int BND_TICKET = BND_CTRL;
-constexpr int BND_PROGRESS = BND_CTRL + 1;
-static_assert(BND_PROGRESS < 16, "bcount holds 16 counters");
-// correction of a face whose writer has not completed yet in the dataflow
-// pass: a NaN payload that no arithmetic produces (stored and compared as bits)
-constexpr unsigned long long CORR_PENDING = 0x7FF4D00DF00DD00Dull;
 
 // Ascending candidates of the pass from the bitmap: per-block popcounts,
 // one-workgroup scan of the block counts, then every block writes its
@@ -1440,11 +1490,13 @@ __global__ void __launch_bounds__(256) k_bnd_write(DevState st, int nWords) {
     }
 }
 
-__global__ void __launch_bounds__(256) k_bnd_prep(DevMesh m, DevState st, double dt) {
+template <int MF>
+__global__ void __launch_bounds__(256) k_bnd_prep(DevMesh m, DevState st, double dt, const double* Sp,
+                                                  const double* Su) {
     const int nCand = st.bcount[0];
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nCand; i += gridDim.x * blockDim.x) {
         const int c = st.cand[i];
-        const int n = bound_first_faces(m, st, c, cell_dt(st, c, dt), nullptr);
+        const int n = bound_record<MF>(m, st, c, cell_dt(st, c, dt), Sp, Su, ((BndRec<MF>*)st.bndRec)[i]);
         st.nFirst[i] = n;
         st.active[i] = n > 0 ? 1 : 0;
         if (n > 0) st.actIdx[c] = i;
@@ -1476,170 +1528,63 @@ __global__ void __launch_bounds__(1024) k_bnd_scan2(DevState st) {
     if (t == 0) st.bcount[1] = carry;
 }
 
+// correctedFaces positions and the cells they touch (both schedules), for a
+// candidate whose record is written
 template <int MF>
-__global__ void __launch_bounds__(256) k_bnd_scatter(DevMesh m, DevState st, double dt) {
-    const int nCand = st.bcount[0];
-    int faces[MF];
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nCand; i += gridDim.x * blockDim.x) {
-        if (!st.active[i]) continue;
-        const int c = st.cand[i];
-        const int n = bound_first_faces(m, st, c, cell_dt(st, c, dt), faces);
-        const int pos = st.nFirst[i];
-        for (int k = 0; k < n; ++k) {
-            const int f = faces[k];
-            st.corrPos[f] = pos + k;
-            st.corrList[pos + k] = f;
-            // cells whose alpha takes this face's correction
-            const int cs[2] = {m.own[f], f < m.nIF ? m.nbr[f] : -1};
-            for (int q = 0; q < 2; ++q)
-                if (cs[q] >= 0 && atomicCAS(&st.touchedFlag[cs[q]], 0, 1) == 0)
-                    st.touched[atomicAdd(&st.bcount[2], 1)] = cs[q];
-        }
-        // the record the dependent rounds work from
-        BndRec<MF>& R = ((BndRec<MF>*)st.bndRec)[i];
-        const int b = m.cOff[c], nf = m.cOff[c + 1] - b;
-        int np = 0;
-        unsigned int flg[(MF + 7) / 8];
+__device__ inline void bound_open_faces(const DevMesh& m, const DevState& st, const BndRec<MF>& R, int i) {
+    const unsigned int first = R.first;
+    int pos = st.nFirst[i];
 #pragma unroll
-        for (int w = 0; w < (MF + 7) / 8; ++w) flg[w] = 0u;
-#pragma unroll
-        for (int k = 0; k < MF; ++k) {
-            int f = 0, sc = -1;
-            double ph = 0.0, dv = 0.0;
-            if (k < nf) {
-                f = m.cFaces[b + k];
-                const bool own = (m.own[f] == c);
-                ph = face_value(m, st.phi, f);
-                dv = face_value(m, st.dVf, f);
-                const bool down = own ? (ph >= 0) : (ph < 0);
-                const bool writable = (f < m.nIF) || !m.fEmpty[f];
-                bool lowerWriter = false;
-                if (f < m.nIF) {
-                    const int o = own ? m.nbr[f] : m.own[f];
-                    if (!down && o < c) {
-                        lowerWriter = true;
-                        if (st.actIdx[o] >= 0) np++;
-                    }
-                    if (down && o > c) sc = st.actIdx[o];
-                }
-                flg[k >> 3] |= ((own ? 1u : 0u) | (down ? 2u : 0u) | (writable ? 4u : 0u) | (lowerWriter ? 8u : 0u))
-                               << (4 * (k & 7));
-            }
-            R.fid[k] = f;
-            R.succ[k] = sc;
-            R.phf[k] = ph;
-            R.dvf[k] = dv;
-        }
-#pragma unroll
-        for (int w = 0; w < (MF + 7) / 8; ++w) R.flg[w] = flg[w];
-        R.nf = nf;
-        R.Vi = m.V[c];
-        R.ac = st.alpha[c];
-        R.aOld = st.alphaOld[c];
-        if (st.bndFlow) {
-            // every face this cell writes reads as pending until it completes
-#pragma unroll
-            for (int k = 0; k < MF; ++k)
-                if (k < nf && (flg[k >> 3] >> (4 * (k & 7)) & 6u) == 6u)
-                    st.corr[R.fid[k]] = __longlong_as_double((long long)CORR_PENDING);
-        } else {
-            st.pend[i] = np;
-            if (np == 0) st.queue[atomicAdd(&st.bcount[3], 1)] = i;
-        }
+    for (int k = 0; k < MF; ++k) {
+        if (!((first >> k) & 1u)) continue;
+        const int f = R.fid[k];
+        st.corrPos[f] = pos;
+        st.corrList[pos] = f;
+        pos++;
+        const int cs[2] = {m.own[f], f < m.nIF ? m.nbr[f] : -1};
+        for (int q = 0; q < 2; ++q)
+            if (cs[q] >= 0 && atomicCAS(&st.touchedFlag[cs[q]], 0, 1) == 0)
+                st.touched[atomicAdd(&st.bcount[2], 1)] = cs[q];
     }
 }
 
-// Whole boundFlux of one cell (advectionTemplates.C:243-347) from its record.
-// Corrections of the downwind faces live in registers; those of upwind faces
-// come from lower-numbered cells that have completed (a higher-numbered writer
-// has not run yet in the reference's ascending walk, so it contributes zero).
-// cin[k]: correction of face slot k written by its lower-numbered upwind
-// neighbour (0 where the flags mark no such writer); out(f, v) stores the
-// cell's final correction of downwind face f.
-template <int MF, class Out>
-__device__ inline void bound_cell_rec(const BndRec<MF>& R, const double (&cin)[MF], int c, double dt,
-                                      double rDeltaT, const double* Sp, const double* Su, Out out) {
-    const double aTol = 100.0 * SVP_SMALL;
-    unsigned int flg[(MF + 7) / 8];
-#pragma unroll
-    for (int w = 0; w < (MF + 7) / 8; ++w) flg[w] = R.flg[w];
-    const int nf = R.nf;
-    const double Vi = R.Vi, ac = R.ac, aOld = R.aOld;
-    auto bits = [&](int k) { return flg[k >> 3] >> (4 * (k & 7)); };
-    auto isOwn = [&](int k) { return (bits(k) & 1u) != 0u; };
-    auto down = [&](int k) { return (bits(k) & 2u) != 0u; };
-    auto writable = [&](int k) { return (bits(k) & 4u) != 0u; };
-    double phf[MF], dvf[MF], cor[MF];
-#pragma unroll
-    for (int k = 0; k < MF; ++k) {
-        phf[k] = R.phf[k];
-        dvf[k] = R.dvf[k];
-        cor[k] = cin[k];
-    }
-    double nfd = 0.0;
-#pragma unroll
-    for (int k = 0; k < MF; ++k)
-        if (k < nf) nfd = isOwn(k) ? nfd + dvf[k] : nfd - dvf[k];
-    double alphaOvershoot = pos0(ac - 1.0) * (ac - 1.0) + neg0(ac) * ac;
-    double fluidToPassOn = alphaOvershoot * Vi;
-    int nFaces = 1;
-    for (int iter = 0; iter < 10; iter++) {
-        if (fabs(alphaOvershoot) < aTol || nFaces == 0) break;
-        bool sel[MF];
-        double smax[MF];
-        double dVftot = 0.0;
-        nFaces = 0;
+// Kahn schedule: correctedFaces, successors of every active candidate,
+// predecessor counts and frontier 0
+template <int MF>
+__global__ void __launch_bounds__(256) k_bnd_scatter(DevMesh m, DevState st) {
+    const int nCand = st.bcount[0];
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nCand; i += gridDim.x * blockDim.x) {
+        if (!st.active[i]) continue;
+        BndRec<MF>& R = ((BndRec<MF>*)st.bndRec)[i];
+        bound_open_faces<MF>(m, st, R, i);
+        const int c = st.cand[i];
+        int np = 0;
 #pragma unroll
         for (int k = 0; k < MF; ++k) {
-            sel[k] = false;
-            smax[k] = 0.0;
-            if (k < nf && down(k)) {
-                const double dVff = dvf[k] + cor[k];
-                const double maxExtra = fabs(pos0(fluidToPassOn) * phf[k] * dt - dVff);
-                if (maxExtra / Vi > aTol) {
-                    sel[k] = true;
-                    smax[k] = maxExtra;
-                    dVftot += fabs(phf[k] * dt);
-                }
+            const unsigned int bits = R.flg[k >> 3] >> (4 * (k & 7));
+            int sc = -1;
+            if (k < R.nf && R.fid[k] < m.nIF) {
+                const int f = R.fid[k];
+                const int o = (bits & 1u) ? m.nbr[f] : m.own[f];
+                if ((bits & 8u) && st.actIdx[o] >= 0) np++;
+                if ((bits & 2u) && o > c) sc = st.actIdx[o];
             }
+            R.succ[k] = sc;
         }
-#pragma unroll
-        for (int k = 0; k < MF; ++k) {
-            if (!sel[k]) continue;
-            double fl = fabs(fluidToPassOn) * fabs(phf[k] * dt) / dVftot;
-            nFaces += (int)pos0(smax[k] - fl);
-            fl = fmin(fl, smax[k]);
-            double dVff = cor[k];
-            dVff += sgn(phf[k]) * sgn(fluidToPassOn) * fl;
-            if (writable(k)) cor[k] = dVff;
-        }
-        double nfc = 0.0;
-#pragma unroll
-        for (int k = 0; k < MF; ++k)
-            if (k < nf) nfc = isOwn(k) ? nfc + cor[k] : nfc - cor[k];
-        double num = aOld * rDeltaT;
-        if (Su) num = num + Su[c];
-        num = num - nfd / Vi * rDeltaT - nfc / Vi * rDeltaT;
-        const double den = Sp ? (rDeltaT - Sp[c]) : rDeltaT;
-        const double alpha1New = num / den;
-        alphaOvershoot = pos0(alpha1New - 1.0) * (alpha1New - 1.0) + neg0(alpha1New) * alpha1New;
-        fluidToPassOn = alphaOvershoot * Vi;
+        st.pend[i] = np;
+        if (np == 0) st.queue[atomicAdd(&st.bcount[3], 1)] = i;
     }
-#pragma unroll
-    for (int k = 0; k < MF; ++k)
-        if (k < nf && down(k) && writable(k)) out(R.fid[k], cor[k]);
 }
 
 // boundFlux of candidate i from its record: its own case's time step in a batch
 template <int MF, class Out>
 __device__ inline void bound_candidate(const DevState& st, const BndRec<MF>& R, const double (&cin)[MF], int i,
-                                       double dt, double rDeltaT, const double* Sp, const double* Su, Out out) {
-    const int c = st.cand[i];
+                                       double dt, double rDeltaT, Out out) {
     if (st.caseOf) {
-        const double dtc = st.dtCase[st.caseOf[c]];
-        bound_cell_rec<MF>(R, cin, c, dtc, 1.0 / dtc, Sp, Su, out);
+        const double dtc = st.dtCase[st.caseOf[st.cand[i]]];
+        bound_finish<MF>(R, cin, dtc, 1.0 / dtc, out);
     } else {
-        bound_cell_rec<MF>(R, cin, c, dt, rDeltaT, Sp, Su, out);
+        bound_finish<MF>(R, cin, dt, rDeltaT, out);
     }
 }
 
@@ -1647,8 +1592,8 @@ __device__ inline void bound_candidate(const DevState& st, const BndRec<MF>& R, 
 // r+1; release(a) drops one pending predecessor of candidate a and returns
 // true when it was the last, push(a) appends a to the next frontier
 template <int MF, class Release, class Push>
-__device__ inline void bnd_complete(const DevMesh& m, const DevState& st, int i, double dt, double rDeltaT,
-                                    const double* Sp, const double* Su, Release release, Push push) {
+__device__ inline void bnd_complete(const DevState& st, int i, double dt, double rDeltaT, Release release,
+                                    Push push) {
     const BndRec<MF>& R = ((const BndRec<MF>*)st.bndRec)[i];
     // predecessors wrote these corrections in an earlier round: the previous
     // kernel of the grid rounds, or before the workgroup barrier of the tail
@@ -1656,7 +1601,7 @@ __device__ inline void bnd_complete(const DevMesh& m, const DevState& st, int i,
 #pragma unroll
     for (int k = 0; k < MF; ++k)
         cin[k] = ((R.flg[k >> 3] >> (4 * (k & 7))) & 8u) ? st.corr[R.fid[k]] : 0.0;
-    bound_candidate<MF>(st, R, cin, i, dt, rDeltaT, Sp, Su, [&](int f, double v) { st.corr[f] = v; });
+    bound_candidate<MF>(st, R, cin, i, dt, rDeltaT, [&](int, int f, double v) { st.corr[f] = v; });
 #pragma unroll
     for (int k = 0; k < MF; ++k) {
         const int a = R.succ[k];
@@ -1666,7 +1611,7 @@ __device__ inline void bnd_complete(const DevMesh& m, const DevState& st, int i,
 }
 
 template <int MF>
-__global__ void __launch_bounds__(256) k_bnd_round(DevMesh m, DevState st, int r, double dt, const double* Sp, const double* Su) {
+__global__ void __launch_bounds__(256) k_bnd_round(DevMesh m, DevState st, int r, double dt) {
     const int n = st.bcount[3 + r];
     const int* q = st.queue + (long)(r & 1) * m.nC;
     int* qnext = st.queue + (long)((r + 1) & 1) * m.nC;
@@ -1675,7 +1620,7 @@ __global__ void __launch_bounds__(256) k_bnd_round(DevMesh m, DevState st, int r
     auto release = [&](int a) { return atomicSub(&st.pend[a], 1) == 1; };
     auto push = [&](int a) { qnext[atomicAdd(nNext, 1)] = a; };
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x)
-        bnd_complete<MF>(m, st, q[k], dt, rDeltaT, Sp, Su, release, push);
+        bnd_complete<MF>(st, q[k], dt, rDeltaT, release, push);
 }
 
 // Remaining rounds in one workgroup of TAIL_THREADS (frontiers are small by now); every round
@@ -1692,8 +1637,7 @@ constexpr int TAIL_Q = 4096;
 constexpr int TAIL_THREADS = 512;
 
 template <int MF>
-__global__ void __launch_bounds__(TAIL_THREADS) k_bnd_tail(DevMesh m, DevState st, double dt, const double* Sp,
-                                                           const double* Su) {
+__global__ void __launch_bounds__(TAIL_THREADS) k_bnd_tail(DevMesh m, DevState st, double dt) {
     __shared__ int sQ, sNext, sRounds;
     __shared__ unsigned int sPend[TAIL_PEND_WORDS];
     __shared__ int sQueue[2][TAIL_Q];
@@ -1738,8 +1682,8 @@ __global__ void __launch_bounds__(TAIL_THREADS) k_bnd_tail(DevMesh m, DevState s
         };
         for (int k = t; k < qn; k += TAIL_THREADS) {
             const int i = k < TAIL_Q ? sQueue[cur][k] : gq[k];
-            if (ldsPend) bnd_complete<MF>(m, st, i, dt, rDeltaT, Sp, Su, releaseLds, push);
-            else bnd_complete<MF>(m, st, i, dt, rDeltaT, Sp, Su, releaseGlobal, push);
+            if (ldsPend) bnd_complete<MF>(st, i, dt, rDeltaT, releaseLds, push);
+            else bnd_complete<MF>(st, i, dt, rDeltaT, releaseGlobal, push);
         }
         __syncthreads();
         if (t == 0) {
@@ -1754,16 +1698,18 @@ __global__ void __launch_bounds__(TAIL_THREADS) k_bnd_tail(DevMesh m, DevState s
     if (t == 0) st.bcount[BND_CTRL - 1] = sRounds;
 }
 
-// The dependent part of a pass in one launch.  Waves take chunks of 64
+// The dependent part of a pass in one launch.  Workgroups take chunks of 256
 // consecutive candidates from a ticket, so every lower-numbered candidate is
-// held by a wave that is already running; a candidate runs as soon as none of
-// the corrections it reads (faces written by lower-numbered active upwind
-// neighbours) is CORR_PENDING.  Those corrections are the only data passed
-// between candidates: each is written once, by one cell, with an agent-scope
-// atomic store and read with agent-scope atomic loads, so the value is its
-// own ready flag.  The lowest unfinished candidate is always ready, so the
-// waits end; a wave that sees no completion anywhere in the grid for about
-// a second flags ERR_BOUND_STALL and leaves.
+// held by a workgroup that is already running; a candidate runs as soon as
+// none of the corrections it reads (faces written by lower-numbered active
+// upwind neighbours) is CORR_PENDING.  Those corrections are the only data
+// passed between candidates, and each is written once, by one cell: inside
+// the chunk through LDS, across chunks with agent-scope atomic stores and
+// loads in global memory, so the value is its own ready flag.  The lowest
+// unfinished candidate is always ready, so the waits end; a wave that sees no
+// completion anywhere in the grid for about a second flags ERR_BOUND_STALL and
+// the pass stops.
+constexpr int FLOW_CHUNK = 256;
 constexpr int FLOW_STALL_CHECKS = 1 << 14;  // idle checks, 64 sleeps each, without progress
 
 __device__ inline unsigned long long corr_load(const double* p) {
@@ -1771,46 +1717,90 @@ __device__ inline unsigned long long corr_load(const double* p) {
 }
 
 template <int MF>
-__global__ void __launch_bounds__(256) k_bnd_flow(DevMesh m, DevState st, double dt, const double* Sp,
-                                                  const double* Su) {
+__global__ void __launch_bounds__(FLOW_CHUNK) k_bnd_flow(DevMesh m, DevState st, double dt) {
+    // per chunk slot and face slot: the correction once written, and the face
+    __shared__ unsigned long long sOut[FLOW_CHUNK * MF];
+    __shared__ int sFid[FLOW_CHUNK * MF];
+    __shared__ int sBase;
+    // records of up to 8 faces stay in registers from the chunk's start, so
+    // a cell that turns ready reads nothing but its upwind corrections
+    constexpr bool kHold = MF <= 8;
     const int nCand = st.bcount[0];
-    const int lane = threadIdx.x & 63;
+    const int t = threadIdx.x, lane = t & 63;
     const double rDeltaT = 1.0 / dt;
-    int* ticket = &st.bcount[BND_TICKET];
     int* progress = &st.bcount[BND_PROGRESS];
-    auto out = [&](int f, double v) {
-        __hip_atomic_store((unsigned long long*)&st.corr[f], (unsigned long long)__double_as_longlong(v),
-                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    };
+    const BndRec<MF>* rec = (const BndRec<MF>*)st.bndRec;
     while (true) {
-        int base = nCand;
-        // a wave that left on a stall ends the pass for everyone
-        if (lane == 0 && !(__hip_atomic_load(st.errFlag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & ERR_BOUND_STALL))
-            base = atomicAdd(ticket, 64);
-        base = __shfl(base, 0);
+        __syncthreads();
+        if (t == 0) {
+            // after a stall no more chunks are taken
+            const bool stalled = __hip_atomic_load(st.errFlag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) &
+                                 ERR_BOUND_STALL;
+            sBase = stalled ? nCand : atomicAdd(&st.bcount[BND_TICKET], FLOW_CHUNK);
+        }
+        __syncthreads();
+        const int base = sBase;
         if (base >= nCand) break;
-        const int i = base + lane;
-        bool pending = i < nCand && st.active[i] != 0;
-        const unsigned long long act = __ballot(pending);
+        const int i = base + t;
+        const bool active = i < nCand && st.active[i] != 0;
+        bool pending = active;
+        const unsigned long long act = __ballot(active);
         if (lane == 0 && act) atomicAdd(&st.bcount[3], __popcll(act));
+        BndRec<MF> held;
+        if (kHold && active) held = rec[i];
+        const BndRec<MF>& R = kHold ? held : rec[active ? i : 0];
+        int fid[MF];
+        unsigned int flg[(MF + 7) / 8];
+#pragma unroll
+        for (int w = 0; w < (MF + 7) / 8; ++w) flg[w] = active ? R.flg[w] : 0u;
+        auto bits = [&](int k) { return flg[k >> 3] >> (4 * (k & 7)); };
+#pragma unroll
+        for (int k = 0; k < MF; ++k) {
+            fid[k] = active ? R.fid[k] : 0;
+            // faces this candidate writes, for the readers of the chunk
+            sFid[t * MF + k] = ((bits(k) & 6u) == 6u) ? fid[k] : -1;
+            sOut[t * MF + k] = CORR_PENDING;
+        }
+        __syncthreads();
+        // where each read correction comes from: a slot of the chunk in LDS,
+        // a face in global memory, or nowhere (inactive writer: zero)
+        int wl[MF], wf[MF];
+#pragma unroll
+        for (int k = 0; k < MF; ++k) {
+            wl[k] = -1;
+            wf[k] = -1;
+            if (bits(k) & 8u) {
+                const int f = fid[k];
+                const int o = (bits(k) & 1u) ? m.nbr[f] : m.own[f];
+                const int p = st.actIdx[o];
+                if (p >= base) {
+                    for (int q = 0; q < MF; ++q)
+                        if (sFid[(p - base) * MF + q] == f) wl[k] = (p - base) * MF + q;
+                }
+                if (p >= 0 && wl[k] < 0) wf[k] = f;
+            }
+        }
+        auto out = [&](int k, int f, double v) {
+            const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+            __hip_atomic_store(&sOut[t * MF + k], u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            __hip_atomic_store((unsigned long long*)&st.corr[f], u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        };
         int idle = 0, seen = -1, stalls = 0;
         while (__any(pending)) {
             bool fin = false;
             if (pending) {
-                const BndRec<MF>& R = ((const BndRec<MF>*)st.bndRec)[i];
                 double cin[MF];
                 bool ready = true;
 #pragma unroll
                 for (int k = 0; k < MF; ++k) {
-                    cin[k] = 0.0;
-                    if ((R.flg[k >> 3] >> (4 * (k & 7))) & 8u) {
-                        const unsigned long long u = corr_load(&st.corr[R.fid[k]]);
-                        ready = ready && u != CORR_PENDING;
-                        cin[k] = __longlong_as_double((long long)u);
-                    }
+                    unsigned long long u = 0ull;
+                    if (wl[k] >= 0) u = __hip_atomic_load(&sOut[wl[k]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    else if (wf[k] >= 0) u = corr_load(&st.corr[wf[k]]);
+                    ready = ready && u != CORR_PENDING;
+                    cin[k] = __longlong_as_double((long long)u);
                 }
                 if (ready) {
-                    bound_candidate<MF>(st, R, cin, i, dt, rDeltaT, Sp, Su, out);
+                    bound_candidate<MF>(st, R, cin, i, dt, rDeltaT, out);
                     pending = false;
                     fin = true;
                 }
@@ -1831,9 +1821,10 @@ __global__ void __launch_bounds__(256) k_bnd_flow(DevMesh m, DevState st, double
             seen = now;
             if (stalls >= FLOW_STALL_CHECKS) {
                 if (lane == 0) atomicOr(st.errFlag, ERR_BOUND_STALL);
-                return;
+                break;
             }
         }
+        if (active) bound_open_faces<MF>(m, st, R, i);
     }
 }
 
@@ -1949,14 +1940,14 @@ void launch_bound_pass(const DevMesh& m, const DevState& st, int pass, double dt
     k_bnd_count<<<nbw, 256, 0, s>>>(st, nWords);
     k_bnd_scan<<<1, 1024, 0, s>>>(st, nbw);
     k_bnd_write<<<nbw, 256, 0, s>>>(st, nWords);
-    k_bnd_prep<<<grid, 256, 0, s>>>(m, st, dt);
+    k_bnd_prep<MF><<<grid, 256, 0, s>>>(m, st, dt, Sp, Su);
     k_bnd_scan2<<<1, 1024, 0, s>>>(st);
-    k_bnd_scatter<MF><<<grid, 256, 0, s>>>(m, st, dt);
     if (st.bndFlow) {
-        k_bnd_flow<MF><<<std::max(1, std::min(st.bndFlow, cdiv(m.nC, 256))), 256, 0, s>>>(m, st, dt, Sp, Su);
+        k_bnd_flow<MF><<<std::max(1, std::min(st.bndFlow, cdiv(m.nC, FLOW_CHUNK))), FLOW_CHUNK, 0, s>>>(m, st, dt);
     } else {
-        for (int r = 0; r < st.gridRounds; ++r) k_bnd_round<MF><<<grid, 256, 0, s>>>(m, st, r, dt, Sp, Su);
-        k_bnd_tail<MF><<<1, TAIL_THREADS, 0, s>>>(m, st, dt, Sp, Su);
+        k_bnd_scatter<MF><<<grid, 256, 0, s>>>(m, st);
+        for (int r = 0; r < st.gridRounds; ++r) k_bnd_round<MF><<<grid, 256, 0, s>>>(m, st, r, dt);
+        k_bnd_tail<MF><<<1, TAIL_THREADS, 0, s>>>(m, st, dt);
     }
     k_bnd_apply<MF><<<grid, 256, 0, s>>>(m, st);
     k_bnd_reset<<<grid, 256, 0, s>>>(m, st, pass);

@@ -126,17 +126,23 @@ constexpr int OOB_STRIDE = 16;
 
 constexpr int DETECT_BLOCK = 256;
 
-// Face data of an active bounding candidate, gathered once per pass so that
-// the dependent rounds read one contiguous record.  flags: 4 bits per face
-// (owner, downwind, writable, upwind writer with a lower label).
+// Face data of an active bounding candidate, gathered once per pass with the
+// first iteration of its redistribution loop done, so that the dependent
+// part reads one contiguous record.  flags: 4 bits per face (owner,
+// downwind, writable, upwind writer with a lower label).
 template <int MF>
 struct BndRec {
     double phf[MF], dvf[MF];
+    double cor0[MF];    // downwind corrections after iteration 0
     int fid[MF];
-    int succ[MF];       // candidate index of the active downwind neighbour with a higher label, else -1
+    int succ[MF];       // Kahn schedule: candidate index of the active downwind neighbour with a higher label, else -1
     unsigned int flg[(MF + 7) / 8];
     int nf;
-    double Vi, ac, aOld;
+    unsigned int first;  // face slots of the first redistribution loop, in correctedFaces order
+    int nFaces0;         // faces still able to take fluid after iteration 0
+    double Vi;
+    double num0;         // alpha_old/dt (+Su) - sum(dVf)/V/dt, the part of the new alpha's numerator without corrections
+    double den;          // 1/dt (-Sp)
 };
 
 // launchers (simplic_kernels.hip)
