@@ -618,15 +618,16 @@ private:
         st_.candCount = dalloc<int>(1);
         st_.candBits = dalloc<unsigned long long>((nC + 63) / 64);
         st_.cand = dalloc<int>(nC);
-        st_.nFirst = dalloc<int>(nC);
         st_.active = dalloc<unsigned char>(nC);
         st_.pend = dalloc<int>(nC);
         st_.queue = dalloc<int>(2L * nC);
         st_.actIdx = dalloc<int>(nC);
         HIPCHECK(hipMemset(st_.actIdx, 0xFF, sizeof(int) * nC));
+        // correctedFaces order keys are cell * (max faces) + rank in 31 bits
+        if ((long long)nC * (maxCellFaces_ <= NS_CELL_FACES ? NS_CELL_FACES : MAX_CELL_FACES) >= 0x7FFFFFFFll)
+            throw SvpError(SVP_E_UNSUPPORTED, "mesh too large for 31-bit bounding order keys");
         st_.corrPos = dalloc<int>(nF);
         HIPCHECK(hipMemset(st_.corrPos, 0xFF, sizeof(int) * nF));
-        st_.corrList = dalloc<int>(nF);
         st_.touched = dalloc<int>(nC);
         st_.touchedFlag = dalloc<int>(nC);
         st_.boundStats = dalloc<int>(4 * MAX_BOUND_PASSES);

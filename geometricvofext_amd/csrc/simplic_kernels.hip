@@ -1494,53 +1494,33 @@ template <int MF>
 __global__ void __launch_bounds__(256) k_bnd_prep(DevMesh m, DevState st, double dt, const double* Sp,
                                                   const double* Su) {
     const int nCand = st.bcount[0];
+    int nOpen = 0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nCand; i += gridDim.x * blockDim.x) {
         const int c = st.cand[i];
         const int n = bound_record<MF>(m, st, c, cell_dt(st, c, dt), Sp, Su, ((BndRec<MF>*)st.bndRec)[i]);
-        st.nFirst[i] = n;
         st.active[i] = n > 0 ? 1 : 0;
         if (n > 0) st.actIdx[c] = i;
+        nOpen += n;
     }
+    // correctedFaces length of the pass (statistics)
+    for (int o = 32; o > 0; o >>= 1) nOpen += __shfl_xor(nOpen, o);
+    if ((threadIdx.x & 63) == 0 && nOpen) atomicAdd(&st.bcount[1], nOpen);
 }
 
-// correctedFaces offsets: exclusive scan of the first-loop face counts, in
-// tiles of 4096 (four consecutive counts per thread, coalesced)
-__global__ void __launch_bounds__(1024) k_bnd_scan2(DevState st) {
-    __shared__ int lds[16];
-    const int n = st.bcount[0];
-    const int t = threadIdx.x;
-    int carry = 0;
-    for (int base = 0; base < n; base += 4096) {
-        int v[4];
-        const int i = base + 4 * t;
-#pragma unroll
-        for (int j = 0; j < 4; ++j) v[j] = (i + j < n) ? st.nFirst[i + j] : 0;
-        const int sum = v[0] + v[1] + v[2] + v[3];
-        int total;
-        int off = carry + wg_scan1024(sum, lds, total);
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            if (i + j < n) st.nFirst[i + j] = off;
-            off += v[j];
-        }
-        carry += total;
-    }
-    if (t == 0) st.bcount[1] = carry;
-}
-
-// correctedFaces positions and the cells they touch (both schedules), for a
-// candidate whose record is written
+// correctedFaces keys and the cells they touch (both schedules), for candidate
+// c whose record is written.  correctedFaces lists the first-loop faces of
+// the candidates in ascending cell order, so the key c * MF + (rank among
+// the cell's first-loop faces) orders any two of its faces as their positions
+// in that list would; only that order is used.
 template <int MF>
-__device__ inline void bound_open_faces(const DevMesh& m, const DevState& st, const BndRec<MF>& R, int i) {
+__device__ inline void bound_open_faces(const DevMesh& m, const DevState& st, const BndRec<MF>& R, int c) {
     const unsigned int first = R.first;
-    int pos = st.nFirst[i];
+    int key = c * MF;
 #pragma unroll
     for (int k = 0; k < MF; ++k) {
         if (!((first >> k) & 1u)) continue;
         const int f = R.fid[k];
-        st.corrPos[f] = pos;
-        st.corrList[pos] = f;
-        pos++;
+        st.corrPos[f] = key++;
         const int cs[2] = {m.own[f], f < m.nIF ? m.nbr[f] : -1};
         for (int q = 0; q < 2; ++q)
             if (cs[q] >= 0 && atomicCAS(&st.touchedFlag[cs[q]], 0, 1) == 0)
@@ -1556,8 +1536,8 @@ __global__ void __launch_bounds__(256) k_bnd_scatter(DevMesh m, DevState st) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nCand; i += gridDim.x * blockDim.x) {
         if (!st.active[i]) continue;
         BndRec<MF>& R = ((BndRec<MF>*)st.bndRec)[i];
-        bound_open_faces<MF>(m, st, R, i);
         const int c = st.cand[i];
+        bound_open_faces<MF>(m, st, R, c);
         int np = 0;
 #pragma unroll
         for (int k = 0; k < MF; ++k) {
@@ -1824,20 +1804,29 @@ __global__ void __launch_bounds__(FLOW_CHUNK) k_bnd_flow(DevMesh m, DevState st,
                 break;
             }
         }
-        if (active) bound_open_faces<MF>(m, st, R, i);
+        if (active) bound_open_faces<MF>(m, st, R, st.cand[i]);
     }
 }
 
 // alpha of every touched cell in correctedFaces order (advectionTemplates.C:164-192)
-// and the dVf of every corrected face
+// and the dVf of every corrected face (the first-loop faces of the active
+// candidates)
 template <int MF>
 __global__ void __launch_bounds__(256) k_bnd_apply(DevMesh m, DevState st) {
     const double aTol = 100.0 * SVP_SMALL;
-    const int nT = st.bcount[2], nCorr = st.bcount[1];
-    for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < nT + nCorr; idx += gridDim.x * blockDim.x) {
+    const int nT = st.bcount[2], nCand = st.bcount[0];
+    for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < nT + nCand; idx += gridDim.x * blockDim.x) {
         if (idx >= nT) {
-            const int f = st.corrList[idx - nT];
-            if (f < m.nIF || !m.fEmpty[f]) st.dVf[f] = face_value(m, st.dVf, f) + face_value(m, st.corr, f);
+            const int i = idx - nT;
+            if (!st.active[i]) continue;
+            const BndRec<MF>& R = ((const BndRec<MF>*)st.bndRec)[i];
+            const unsigned int first = R.first;
+#pragma unroll
+            for (int k = 0; k < MF; ++k) {
+                if (!((first >> k) & 1u)) continue;
+                const int f = R.fid[k];
+                if (f < m.nIF || !m.fEmpty[f]) st.dVf[f] = face_value(m, st.dVf, f) + face_value(m, st.corr, f);
+            }
             continue;
         }
         const int c = st.touched[idx];
@@ -1941,7 +1930,6 @@ void launch_bound_pass(const DevMesh& m, const DevState& st, int pass, double dt
     k_bnd_scan<<<1, 1024, 0, s>>>(st, nbw);
     k_bnd_write<<<nbw, 256, 0, s>>>(st, nWords);
     k_bnd_prep<MF><<<grid, 256, 0, s>>>(m, st, dt, Sp, Su);
-    k_bnd_scan2<<<1, 1024, 0, s>>>(st);
     if (st.bndFlow) {
         k_bnd_flow<MF><<<std::max(1, std::min(st.bndFlow, cdiv(m.nC, FLOW_CHUNK))), FLOW_CHUNK, 0, s>>>(m, st, dt);
     } else {

@@ -71,13 +71,11 @@ struct DevState {
     int* candCount;     // [1] candidates of the last pass (diagnostics)
     unsigned long long* candBits;  // nC bits: alpha outside [-aTol,1+aTol] and needBounding
     int* cand;          // nC, ascending candidates of the current pass
-    int* nFirst;        // nC, per candidate: faces of its first redistribution loop
     unsigned char* active;  // nC, per candidate: writes corrections this pass
     int* pend;          // nC, per candidate: active upwind predecessors not yet done
     int* queue;         // 2*nC, Kahn frontiers
     int* actIdx;        // nC cells -> candidate index of active cells, else -1
-    int* corrPos;       // nF faces -> position in correctedFaces, else -1
-    int* corrList;      // nF, correctedFaces of the current pass
+    int* corrPos;       // nF faces -> order key in correctedFaces (cell * max faces + rank), else -1
     int* touched;       // nC, cells whose alpha gets corrections
     int* touchedFlag;   // nC
     int* boundStats;    // [4*maxPasses]: candidates, active, rounds, corrected faces
