@@ -1254,7 +1254,7 @@ __device__ inline int bound_record(const DevMesh& m, const DevState& st, int c, 
     if (fabs(alphaOvershoot) < aTol) return 0;
     const double fluidToPassOn = alphaOvershoot * Vi;
     const int b = m.cOff[c], nf = m.cOff[c + 1] - b;
-    int fid[MF];
+    int fid[MF], oth[MF];
     double phf[MF], dvf[MF], smax[MF];
     unsigned int flg[(MF + 7) / 8];
     unsigned int first = 0u;
@@ -1265,6 +1265,7 @@ __device__ inline int bound_record(const DevMesh& m, const DevState& st, int c, 
 #pragma unroll
     for (int k = 0; k < MF; ++k) {
         fid[k] = 0;
+        oth[k] = -1;
         phf[k] = 0.0;
         dvf[k] = 0.0;
         smax[k] = 0.0;
@@ -1272,13 +1273,14 @@ __device__ inline int bound_record(const DevMesh& m, const DevState& st, int c, 
             const int f = m.cFaces[b + k];
             fid[k] = f;
             const bool own = (m.own[f] == c);
+            if (f < m.nIF) oth[k] = own ? m.nbr[f] : m.own[f];
             const double ph = face_value(m, st.phi, f);
             const double dv = face_value(m, st.dVf, f);
             phf[k] = ph;
             dvf[k] = dv;
             const bool down = own ? (ph >= 0) : (ph < 0);
             const bool writable = (f < m.nIF) || !m.fEmpty[f];
-            const bool lowerWriter = f < m.nIF && !down && (own ? m.nbr[f] : m.own[f]) < c;
+            const bool lowerWriter = oth[k] >= 0 && !down && oth[k] < c;
             flg[k >> 3] |= ((own ? 1u : 0u) | (down ? 2u : 0u) | (writable ? 4u : 0u) | (lowerWriter ? 8u : 0u))
                            << (4 * (k & 7));
             if (down) {
@@ -1319,6 +1321,7 @@ __device__ inline int bound_record(const DevMesh& m, const DevState& st, int c, 
 #pragma unroll
     for (int k = 0; k < MF; ++k) {
         R.fid[k] = fid[k];
+        R.oth[k] = oth[k];
         R.phf[k] = phf[k];
         R.dvf[k] = dvf[k];
         if (k < nf && (bits(k) & 6u) == 6u) st.corr[fid[k]] = __longlong_as_double((long long)CORR_PENDING);
@@ -1511,21 +1514,34 @@ __global__ void __launch_bounds__(256) k_bnd_prep(DevMesh m, DevState st, double
 // c whose record is written.  correctedFaces lists the first-loop faces of
 // the candidates in ascending cell order, so the key c * MF + (rank among
 // the cell's first-loop faces) orders any two of its faces as their positions
-// in that list would; only that order is used.
+// in that list would; only that order is used.  The touched-cell claims are
+// issued together, then one counter add reserves the list entries.
 template <int MF>
-__device__ inline void bound_open_faces(const DevMesh& m, const DevState& st, const BndRec<MF>& R, int c) {
+__device__ inline void bound_open_faces(const DevState& st, const BndRec<MF>& R, int c) {
     const unsigned int first = R.first;
     int key = c * MF;
+    // claims first, results read afterwards, so that the claims overlap
+    int old[MF];
 #pragma unroll
     for (int k = 0; k < MF; ++k) {
+        old[k] = 1;
         if (!((first >> k) & 1u)) continue;
-        const int f = R.fid[k];
-        st.corrPos[f] = key++;
-        const int cs[2] = {m.own[f], f < m.nIF ? m.nbr[f] : -1};
-        for (int q = 0; q < 2; ++q)
-            if (cs[q] >= 0 && atomicCAS(&st.touchedFlag[cs[q]], 0, 1) == 0)
-                st.touched[atomicAdd(&st.bcount[2], 1)] = cs[q];
+        st.corrPos[R.fid[k]] = key++;
+        if (R.oth[k] >= 0) old[k] = atomicCAS(&st.touchedFlag[R.oth[k]], 0, 1);
     }
+    const int oldSelf = first ? atomicCAS(&st.touchedFlag[c], 0, 1) : 1;
+    unsigned long long won = 0ull;
+#pragma unroll
+    for (int k = 0; k < MF; ++k)
+        if (old[k] == 0) won |= 1ull << k;
+    const bool self = oldSelf == 0;
+    const int nWon = __popcll(won) + (self ? 1 : 0);
+    if (nWon == 0) return;
+    int slot = atomicAdd(&st.bcount[2], nWon);
+    if (self) st.touched[slot++] = c;
+#pragma unroll
+    for (int k = 0; k < MF; ++k)
+        if ((won >> k) & 1ull) st.touched[slot++] = R.oth[k];
 }
 
 // Kahn schedule: correctedFaces, successors of every active candidate,
@@ -1537,7 +1553,7 @@ __global__ void __launch_bounds__(256) k_bnd_scatter(DevMesh m, DevState st) {
         if (!st.active[i]) continue;
         BndRec<MF>& R = ((BndRec<MF>*)st.bndRec)[i];
         const int c = st.cand[i];
-        bound_open_faces<MF>(m, st, R, c);
+        bound_open_faces<MF>(st, R, c);
         int np = 0;
 #pragma unroll
         for (int k = 0; k < MF; ++k) {
@@ -1678,36 +1694,69 @@ __global__ void __launch_bounds__(TAIL_THREADS) k_bnd_tail(DevMesh m, DevState s
     if (t == 0) st.bcount[BND_CTRL - 1] = sRounds;
 }
 
-// The dependent part of a pass in one launch.  Workgroups take chunks of 256
-// consecutive candidates from a ticket, so every lower-numbered candidate is
-// held by a workgroup that is already running; a candidate runs as soon as
-// none of the corrections it reads (faces written by lower-numbered active
-// upwind neighbours) is CORR_PENDING.  Those corrections are the only data
-// passed between candidates, and each is written once, by one cell: inside
-// the chunk through LDS, across chunks with agent-scope atomic stores and
-// loads in global memory, so the value is its own ready flag.  The lowest
-// unfinished candidate is always ready, so the waits end; a wave that sees no
-// completion anywhere in the grid for about a second flags ERR_BOUND_STALL and
-// the pass stops.
-constexpr int FLOW_CHUNK = 256;
-constexpr int FLOW_STALL_CHECKS = 1 << 14;  // idle checks, 64 sleeps each, without progress
+// The dependent part of a pass in one launch.  Workgroups take chunks of
+// consecutive candidates from a ticket, so every lower-numbered
+// candidate is held by a workgroup that is already running; a candidate runs
+// as soon as none of the corrections it reads (faces written by
+// lower-numbered active upwind neighbours) is CORR_PENDING.  Those
+// corrections are the only data passed between candidates, and each is
+// written once, by one cell: to LDS for the readers of its own chunk and to
+// global memory with an agent-scope atomic store for the others, so the
+// value is its own ready flag.  One extra wave per workgroup polls global
+// memory (agent-scope atomic loads) for the corrections its chunk reads from
+// earlier chunks and copies them into LDS, so the computing waves only ever
+// wait on LDS.  The lowest unfinished candidate is always ready, so the
+// waits end; a wave that sees no completion anywhere in the grid for about a
+// second flags ERR_BOUND_STALL and the pass stops.
+// chunk of candidates per workgroup: one per thread of the computing waves;
+// 7 + 1 waves at two waves per SIMD for cells of up to 8 faces (held
+// records fill the registers), smaller chunks for larger cells (LDS)
+template <int MF>
+constexpr int flow_chunk() { return MF <= 8 ? 448 : 128; }
+template <int MF>
+constexpr int flow_threads() { return flow_chunk<MF>() + 64; }  // computing waves + the polling wave
+constexpr int FLOW_POLL_BATCH = 8;          // loads a polling lane keeps in flight
+constexpr int FLOW_STALL_CHECKS = 1 << 14;   // idle checks, 64 sleeps each, without progress
 
 __device__ inline unsigned long long corr_load(const double* p) {
     return __hip_atomic_load((const unsigned long long*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// stall check of a waiting wave: true once the grid-wide completion count
+// has not moved over FLOW_STALL_CHECKS checks
+struct FlowStall {
+    int idle = 0, seen = -1, stalls = 0;
+    __device__ bool wait(const int* progress, int lane) {
+        __builtin_amdgcn_s_sleep(1);
+        if (++idle < 64) return false;
+        idle = 0;
+        int now = 0;
+        if (lane == 0) now = __hip_atomic_load(progress, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        now = __shfl(now, 0);
+        stalls = (now == seen) ? stalls + 1 : 0;
+        seen = now;
+        return stalls >= FLOW_STALL_CHECKS;
+    }
+};
+
 template <int MF>
-__global__ void __launch_bounds__(FLOW_CHUNK) k_bnd_flow(DevMesh m, DevState st, double dt) {
-    // per chunk slot and face slot: the correction once written, and the face
-    __shared__ unsigned long long sOut[FLOW_CHUNK * MF];
-    __shared__ int sFid[FLOW_CHUNK * MF];
-    __shared__ int sBase;
+__global__ void __launch_bounds__(flow_threads<MF>()) k_bnd_flow(DevMesh m, DevState st, double dt) {
+    constexpr int CH = flow_chunk<MF>();
+    // LDS slots [0, CH*MF): corrections written by the chunk's cells (slot
+    // t*MF+k: cell t, face slot k); [CH*MF, 2*CH*MF): corrections read from
+    // earlier chunks, copied in by the polling wave
+    constexpr int NS = CH * MF;
+    __shared__ unsigned long long sVal[2 * NS];
+    __shared__ int sFid[NS];                 // faces the chunk's cells write
+    __shared__ int sPoll[NS][2];             // polled corrections: face, LDS slot
+    __shared__ int sBase, sNPoll;
     // records of up to 8 faces stay in registers from the chunk's start, so
     // a cell that turns ready reads nothing but its upwind corrections
     constexpr bool kHold = MF <= 8;
     const int nCand = st.bcount[0];
     const int t = threadIdx.x, lane = t & 63;
-    const double rDeltaT = 1.0 / dt;
+    const bool poller = t >= CH;
+    static_assert(NS <= 64 * 64, "a polling lane tracks at most 64 entries");
     int* progress = &st.bcount[BND_PROGRESS];
     const BndRec<MF>* rec = (const BndRec<MF>*)st.bndRec;
     while (true) {
@@ -1716,71 +1765,132 @@ __global__ void __launch_bounds__(FLOW_CHUNK) k_bnd_flow(DevMesh m, DevState st,
             // after a stall no more chunks are taken
             const bool stalled = __hip_atomic_load(st.errFlag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) &
                                  ERR_BOUND_STALL;
-            sBase = stalled ? nCand : atomicAdd(&st.bcount[BND_TICKET], FLOW_CHUNK);
+            sBase = stalled ? nCand : atomicAdd(&st.bcount[BND_TICKET], CH);
+            sNPoll = 0;
         }
         __syncthreads();
         const int base = sBase;
         if (base >= nCand) break;
         const int i = base + t;
-        const bool active = i < nCand && st.active[i] != 0;
+        const bool active = !poller && i < nCand && st.active[i] != 0;
         bool pending = active;
         const unsigned long long act = __ballot(active);
         if (lane == 0 && act) atomicAdd(&st.bcount[3], __popcll(act));
         BndRec<MF> held;
         if (kHold && active) held = rec[i];
         const BndRec<MF>& R = kHold ? held : rec[active ? i : 0];
-        int fid[MF];
         unsigned int flg[(MF + 7) / 8];
 #pragma unroll
         for (int w = 0; w < (MF + 7) / 8; ++w) flg[w] = active ? R.flg[w] : 0u;
         auto bits = [&](int k) { return flg[k >> 3] >> (4 * (k & 7)); };
+        // candidate index of each upwind writer (-1: it writes nothing this
+        // pass); the loads are unconditional so that they are all in flight
+        // together
+        int wr[MF];
 #pragma unroll
-        for (int k = 0; k < MF; ++k) {
-            fid[k] = active ? R.fid[k] : 0;
-            // faces this candidate writes, for the readers of the chunk
-            sFid[t * MF + k] = ((bits(k) & 6u) == 6u) ? fid[k] : -1;
-            sOut[t * MF + k] = CORR_PENDING;
+        for (int k = 0; k < MF; ++k) wr[k] = st.actIdx[(bits(k) & 8u) ? R.oth[k] : 0];
+#pragma unroll
+        for (int k = 0; k < MF; ++k)
+            if (!(bits(k) & 8u)) wr[k] = -1;
+        if (!poller) {
+#pragma unroll
+            for (int k = 0; k < MF; ++k) {
+                sFid[t * MF + k] = ((bits(k) & 6u) == 6u) ? R.fid[k] : -1;
+                sVal[t * MF + k] = CORR_PENDING;
+                sVal[NS + t * MF + k] = CORR_PENDING;
+            }
         }
         __syncthreads();
-        // where each read correction comes from: a slot of the chunk in LDS,
-        // a face in global memory, or nowhere (inactive writer: zero)
-        int wl[MF], wf[MF];
+        // where each read correction comes from: the writer's slot when it is
+        // a cell of this chunk, else a polled slot.  Nothing in the waiting
+        // loop reads global memory, so the loop never waits on the global
+        // stores of earlier completions.
+        int src[MF];
+        unsigned int need = 0u;
+        double cin[MF];
 #pragma unroll
         for (int k = 0; k < MF; ++k) {
-            wl[k] = -1;
-            wf[k] = -1;
-            if (bits(k) & 8u) {
-                const int f = fid[k];
-                const int o = (bits(k) & 1u) ? m.nbr[f] : m.own[f];
-                const int p = st.actIdx[o];
-                if (p >= base) {
-                    for (int q = 0; q < MF; ++q)
-                        if (sFid[(p - base) * MF + q] == f) wl[k] = (p - base) * MF + q;
+            src[k] = -1;
+            cin[k] = 0.0;
+            const int p = wr[k];
+            if (p < 0) continue;
+            const int f = R.fid[k];
+            need |= 1u << k;
+            if (p >= base) {
+                const int j = p - base;
+                for (int q = 0; q < MF; ++q)
+                    if (sFid[j * MF + q] == f) src[k] = j * MF + q;
+            } else {
+                const int g = atomicAdd(&sNPoll, 1);
+                sPoll[g][0] = f;
+                sPoll[g][1] = NS + t * MF + k;
+                src[k] = NS + t * MF + k;
+            }
+        }
+        // time step of the candidate (its case's in a batch)
+        double dtc = dt;
+        if (st.caseOf && active) dtc = st.dtCase[st.caseOf[st.cand[i]]];
+        const double rdc = 1.0 / dtc;
+        __syncthreads();
+        FlowStall stall;
+        if (poller) {
+            // copy corrections of earlier chunks into LDS as they complete:
+            // entry lane + 64 b of the list is this lane's bit b, and the
+            // loads of FLOW_POLL_BATCH entries are issued together
+            const int nPoll = sNPoll;
+            unsigned long long left = 0ull;
+            for (int b = 0; lane + 64 * b < nPoll; ++b) left |= 1ull << b;
+            while (__any(left != 0ull)) {
+                bool got = false;
+                for (int b0 = 0; (left >> b0) != 0ull; b0 += FLOW_POLL_BATCH) {
+                    unsigned long long u[FLOW_POLL_BATCH];
+#pragma unroll
+                    for (int b = 0; b < FLOW_POLL_BATCH; ++b) {
+                        const int e = lane + 64 * (b0 + b);
+                        u[b] = ((left >> (b0 + b)) & 1ull) ? corr_load(&st.corr[sPoll[e][0]]) : CORR_PENDING;
+                    }
+#pragma unroll
+                    for (int b = 0; b < FLOW_POLL_BATCH; ++b) {
+                        if (u[b] != CORR_PENDING) {
+                            const int e = lane + 64 * (b0 + b);
+                            __hip_atomic_store(&sVal[sPoll[e][1]], u[b], __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_WORKGROUP);
+                            left &= ~(1ull << (b0 + b));
+                            got = true;
+                        }
+                    }
                 }
-                if (p >= 0 && wl[k] < 0) wf[k] = f;
+                if (__any(got)) {
+                    stall.idle = 0;
+                    continue;
+                }
+                if (stall.wait(progress, lane)) {
+                    if (lane == 0) atomicOr(st.errFlag, ERR_BOUND_STALL);
+                    break;
+                }
             }
         }
         auto out = [&](int k, int f, double v) {
             const unsigned long long u = (unsigned long long)__double_as_longlong(v);
-            __hip_atomic_store(&sOut[t * MF + k], u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            __hip_atomic_store(&sVal[t * MF + k], u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             __hip_atomic_store((unsigned long long*)&st.corr[f], u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         };
-        int idle = 0, seen = -1, stalls = 0;
-        while (__any(pending)) {
+        while (!poller && __any(pending)) {
             bool fin = false;
             if (pending) {
-                double cin[MF];
-                bool ready = true;
+                // corrections not seen yet; each one is read until it arrives
 #pragma unroll
                 for (int k = 0; k < MF; ++k) {
-                    unsigned long long u = 0ull;
-                    if (wl[k] >= 0) u = __hip_atomic_load(&sOut[wl[k]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                    else if (wf[k] >= 0) u = corr_load(&st.corr[wf[k]]);
-                    ready = ready && u != CORR_PENDING;
-                    cin[k] = __longlong_as_double((long long)u);
+                    if (!((need >> k) & 1u)) continue;
+                    const unsigned long long u =
+                        __hip_atomic_load(&sVal[src[k]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    if (u != CORR_PENDING) {
+                        cin[k] = __longlong_as_double((long long)u);
+                        need &= ~(1u << k);
+                    }
                 }
-                if (ready) {
-                    bound_candidate<MF>(st, R, cin, i, dt, rDeltaT, out);
+                if (need == 0u) {
+                    bound_finish<MF>(R, cin, dtc, rdc, out);
                     pending = false;
                     fin = true;
                 }
@@ -1788,23 +1898,15 @@ __global__ void __launch_bounds__(FLOW_CHUNK) k_bnd_flow(DevMesh m, DevState st,
             const unsigned long long done = __ballot(fin);
             if (done) {
                 if (lane == 0) atomicAdd(progress, __popcll(done));
-                idle = 0;
+                stall.idle = 0;
                 continue;
             }
-            __builtin_amdgcn_s_sleep(1);
-            if (++idle < 64) continue;
-            idle = 0;
-            int now = 0;
-            if (lane == 0) now = __hip_atomic_load(progress, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            now = __shfl(now, 0);
-            stalls = (now == seen) ? stalls + 1 : 0;
-            seen = now;
-            if (stalls >= FLOW_STALL_CHECKS) {
+            if (stall.wait(progress, lane)) {
                 if (lane == 0) atomicOr(st.errFlag, ERR_BOUND_STALL);
                 break;
             }
         }
-        if (active) bound_open_faces<MF>(m, st, R, st.cand[i]);
+        if (active) bound_open_faces<MF>(st, R, st.cand[i]);
     }
 }
 
@@ -1931,7 +2033,8 @@ void launch_bound_pass(const DevMesh& m, const DevState& st, int pass, double dt
     k_bnd_write<<<nbw, 256, 0, s>>>(st, nWords);
     k_bnd_prep<MF><<<grid, 256, 0, s>>>(m, st, dt, Sp, Su);
     if (st.bndFlow) {
-        k_bnd_flow<MF><<<std::max(1, std::min(st.bndFlow, cdiv(m.nC, FLOW_CHUNK))), FLOW_CHUNK, 0, s>>>(m, st, dt);
+        k_bnd_flow<MF><<<std::max(1, std::min(st.bndFlow, cdiv(m.nC, flow_chunk<MF>()))), flow_threads<MF>(), 0, s>>>(
+            m, st, dt);
     } else {
         k_bnd_scatter<MF><<<grid, 256, 0, s>>>(m, st);
         for (int r = 0; r < st.gridRounds; ++r) k_bnd_round<MF><<<grid, 256, 0, s>>>(m, st, r, dt);

@@ -133,6 +133,7 @@ struct BndRec {
     double phf[MF], dvf[MF];
     double cor0[MF];    // downwind corrections after iteration 0
     int fid[MF];
+    int oth[MF];        // cell on the other side of the face, -1 on boundary faces
     int succ[MF];       // Kahn schedule: candidate index of the active downwind neighbour with a higher label, else -1
     unsigned int flg[(MF + 7) / 8];
     int nf;
