@@ -104,6 +104,75 @@ def step_bytes(n_cells, n_faces, n_ic, cell_kind):
     return 24 * n_cells + 24 * n_faces + (512 if cell_kind == "hex" else 272) * n_ic
 
 
+# kernels of each pipeline stage, for the HBM traffic read from profiler counters
+STAGE_KERNELS = {
+    "detect": ("k_detect_flags", "k_scan_counts", "k_compact_flags", "k_zero_int"),
+    "normal": ("k_ls_normal",),
+    "findSignedDistance": ("k_fsd_group", "k_find_signed_distance"),
+    "upwind": ("k_upwind",),
+    "interfaceFlux": ("k_interface_flux", "k_un0"),
+    "update": ("k_update",),
+    "bounding": ("k_bnd_",),
+    "snapClip": ("k_snap_clip",),
+}
+COPY_PROBE_BYTES = 256 << 20
+
+
+def copy_probe():
+    """One device-to-device copy of COPY_PROBE_BYTES: the reference kernel
+    whose FETCH_SIZE / WRITE_SIZE readings calibrate the counters."""
+    import torch
+    a = torch.ones(COPY_PROBE_BYTES // 8, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    print(f"copied {COPY_PROBE_BYTES} bytes", flush=True)
+
+
+def _counter_sums(run_dir):
+    """kernel name -> [dispatches, counter sum (KiB)] of a rocprofv3 --pmc run."""
+    import csv
+    import glob
+    sums = {}
+    for path in glob.glob(os.path.join(run_dir, "**", "*counter_collection.csv"), recursive=True):
+        with open(path) as fh:
+            for row in csv.DictReader(fh):
+                name = row["Kernel_Name"].split("(")[0].replace("void ", "")
+                e = sums.setdefault(name, [0, 0.0])
+                e[0] += 1
+                e[1] += float(row["Counter_Value"])
+    return sums
+
+
+def pmc_summary(copy_fetch_dir, copy_write_dir, run_fetch_dir, run_write_dir):
+    """HBM bytes per launch of every stage from FETCH_SIZE and WRITE_SIZE runs,
+    each scaled by what the same counter reads for the copy probe (a copy of
+    COPY_PROBE_BYTES reads and writes exactly that many bytes)."""
+    def probe_kib(d):
+        sums = _counter_sums(d)
+        name, (n, v) = max(sums.items(), key=lambda kv: kv[1][1] / max(kv[1][0], 1))
+        return v / n
+    f_fac = COPY_PROBE_BYTES / (1024.0 * probe_kib(copy_fetch_dir))
+    w_fac = COPY_PROBE_BYTES / (1024.0 * probe_kib(copy_write_dir))
+    fetch, write = _counter_sums(run_fetch_dir), _counter_sums(run_write_dir)
+    steps = max(v[0] for k, v in fetch.items() if k.endswith("k_update"))
+    out = {"fetch_factor": f_fac, "write_factor": w_fac, "steps": steps, "stages": {}, "kernels": {}}
+    for k in sorted(set(fetch) | set(write)):
+        fb = 1024.0 * f_fac * fetch.get(k, [0, 0.0])[1] / steps
+        wb = 1024.0 * w_fac * write.get(k, [0, 0.0])[1] / steps
+        out["kernels"][k] = {"fetch_bytes_per_step": fb, "write_bytes_per_step": wb,
+                             "dispatches_per_step": fetch.get(k, [0, 0.0])[0] / steps}
+        for stage, pats in STAGE_KERNELS.items():
+            if any(p in k for p in pats):
+                e = out["stages"].setdefault(stage, {"fetch_bytes": 0.0, "write_bytes": 0.0})
+                e["fetch_bytes"] += fb
+                e["write_bytes"] += wb
+    for e in out["stages"].values():
+        e["bytes"] = e["fetch_bytes"] + e["write_bytes"]
+    return out
+
+
 def build_case(wl):
     import geometricvofext_amd as g
     if wl["kind"] == "hex":
@@ -124,6 +193,18 @@ def build_case(wl):
         U0, phi0 = solver.deformation_field()
     solver.upload_base_flow(U0, phi0)
     return mesh, solver, alpha0, U0, phi0
+
+
+def traffic_of(workload, stage):
+    """HBM bytes per launch of `stage` measured with profiler counters on this
+    workload (profiles/hbm_traffic_<workload>.json, written by --pmc-summary),
+    or None when no such measurement is committed."""
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_{workload}.json")
+    try:
+        with open(path) as fh:
+            return json.load(fh)["stages"][stage]["bytes"]
+    except (OSError, KeyError, ValueError):
+        return None
 
 
 def flow_period(wl):
@@ -264,7 +345,17 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--stage", default="auto", help="stage timed for the roofline (auto = slowest)")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-normal comparison")
+    ap.add_argument("--copy-probe", action="store_true", help="run only the counter-calibration copy")
+    ap.add_argument("--pmc-summary", nargs=4, metavar=("COPY_FETCH", "COPY_WRITE", "RUN_FETCH", "RUN_WRITE"),
+                    help="summarise rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of the copy probe and "
+                         "of this benchmark into HBM bytes per stage launch (JSON)")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe()
+        return
+    if args.pmc_summary:
+        print(json.dumps(pmc_summary(*args.pmc_summary), indent=1))
+        return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU; the alpha transport shards by spatial domain "
@@ -359,7 +450,7 @@ def main():
         "config": {"workload": wl["desc"], "cells": mesh.n_cells, "faces": mesh.n_faces,
                    "alpha_dict": ALPHA_DICT, "path": "plain LS"},
         "roofline": {"bound": "hbm", "stage": stage, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_of(args.workload, stage),
                      "algorithmic_bytes_per_launch": st_bytes, "avg_launch_us": 1e6 * st_avg},
         "step_roofline": {"achieved": step_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                           "frac": step_gbs / HBM_PEAK_GBS},
