@@ -121,6 +121,52 @@ __host__ __device__ inline double plane_cube_fraction(double nx, double ny, doub
     return cube_fraction(m0, m1, m2, rhs);
 }
 
+// plane_cube_fraction for one normal and cube edge over many plane constants
+// and cube corners: the reflected, normalised and sorted coefficients are
+// computed once, and every evaluation runs the same operations on the same
+// values as plane_cube_fraction, so the results are identical.
+struct CubePlane {
+    double nx, ny, nz, h;
+    double neg[3];      // reflected coefficients -m_i (0 where m_i >= 0), subtracted in axis order
+    double s;           // m0 + m1 + m2 after reflection
+    double sm[3];       // normalised coefficients, ascending
+    __host__ __device__ CubePlane(double nx_, double ny_, double nz_, double h_) : nx(nx_), ny(ny_), nz(nz_), h(h_) {
+        double m0 = nx * h, m1 = ny * h, m2 = nz * h;
+        neg[0] = m0 < 0.0 ? m0 : 0.0;
+        neg[1] = m1 < 0.0 ? m1 : 0.0;
+        neg[2] = m2 < 0.0 ? m2 : 0.0;
+        if (m0 < 0.0) m0 = -m0;
+        if (m1 < 0.0) m1 = -m1;
+        if (m2 < 0.0) m2 = -m2;
+        s = m0 + m1 + m2;
+        if (s > 0.0) {
+            m0 /= s;
+            m1 /= s;
+            m2 /= s;
+        }
+        double t;
+        if (m0 > m1) { t = m0; m0 = m1; m1 = t; }
+        if (m1 > m2) { t = m1; m1 = m2; m2 = t; }
+        if (m0 > m1) { t = m0; m0 = m1; m1 = t; }
+        sm[0] = m0;
+        sm[1] = m1;
+        sm[2] = m2;
+    }
+    // liquid fraction of [lo, lo + h]^3 on the side n . x + D < 0
+    __host__ __device__ double fraction(double D, double lox, double loy, double loz) const {
+        double rhs = -D - (nx * lox + ny * loy + nz * loz);
+        if (neg[0] < 0.0) rhs -= neg[0];
+        if (neg[1] < 0.0) rhs -= neg[1];
+        if (neg[2] < 0.0) rhs -= neg[2];
+        if (!(s > 0.0)) return rhs > 0.0 ? 1.0 : 0.0;
+        const double a = rhs / s;
+        if (a <= 0.0) return 0.0;
+        if (a >= 1.0) return 1.0;
+        if (a <= 0.5) return cube_fraction_sorted(sm[0], sm[1], sm[2], a);
+        return 1.0 - cube_fraction_sorted(sm[0], sm[1], sm[2], 1.0 - a);
+    }
+};
+
 // ---------------------------------------------------------------------------
 // Stencil symmetry.  Lattice index of offset (dx, dy, dz) in {-1,0,1}^3:
 // L = 9 (dx+1) + 3 (dy+1) + (dz+1).  The canonical frame reflects every axis

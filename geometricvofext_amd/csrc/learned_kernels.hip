@@ -612,8 +612,21 @@ __global__ void __launch_bounds__(64 * INFER_WAVES, 2) k_infer(const int* lat, c
         const int i = i0 + r;
         const bool live = i < nMixed;
         const int c = live ? mixed[i] : 0;
-        const bool valid = live && lat[(long)c * NIN] >= 0;
-        for (int L = q; L < NIN; L += 4) rw[r][L] = valid ? alpha[lat[(long)c * NIN + L]] : 0.0;
+        // this lane's stencil entries L = q + 4j: all labels, then all
+        // alphas, in flight together; the cell's centre and volume with them
+        constexpr int NJ = (NIN + 3) / 4;
+        int sl[NJ];
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) sl[j] = (q + 4 * j < NIN) ? lat[(long)c * NIN + q + 4 * j] : -1;
+        const double cV = V[c];
+        const double cCx = C[3L * c], cCy = C[3L * c + 1], cCz = C[3L * c + 2];
+        const bool valid = live && __shfl(sl[0], r) >= 0;  // lattice entry 0 is held by lane r (q = 0)
+        double av[NJ];
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) av[j] = (valid && sl[j] >= 0) ? alpha[sl[j]] : 0.0;
+#pragma unroll
+        for (int j = 0; j < NJ; ++j)
+            if (q + 4 * j < NIN) rw[r][q + 4 * j] = av[j];
         __builtin_amdgcn_wave_barrier();
         // every lane of the cell derives the same frame and builds its own
         // eight inputs (K slots 8q..8q+7 of the first layer)
@@ -637,18 +650,19 @@ __global__ void __launch_bounds__(64 * INFER_WAVES, 2) k_infer(const int* lat, c
         double res = -1.0;
         if (tol < 1.0) {
             // cell cube (lattice cell: an axis-aligned cube of edge h at C)
-            const double h = live ? cbrt(V[c]) : 1.0;
-            const double lox = live ? C[3L * c] - 0.5 * h : 0.0, loy = live ? C[3L * c + 1] - 0.5 * h : 0.0,
-                         loz = live ? C[3L * c + 2] - 0.5 * h : 0.0;
+            const double h = live ? cbrt(cV) : 1.0;
+            const double lox = live ? cCx - 0.5 * h : 0.0, loy = live ? cCy - 0.5 * h : 0.0,
+                         loz = live ? cCz - 0.5 * h : 0.0;
             const double a = rw[r][13];
             const double base = -(nx * lox + ny * loy + nz * loz);
             const double sx = nx * h, sy = ny * h, sz = nz * h;
             const double tmin = fmin(sx, 0.0) + fmin(sy, 0.0) + fmin(sz, 0.0);
             const double tmax = fmax(sx, 0.0) + fmax(sy, 0.0) + fmax(sz, 0.0);
+            const CubePlane cp(nx, ny, nz, h);
             double dlo = base - tmax, dhi = base - tmin;  // fraction 1 at dlo, 0 at dhi
             for (int round = 0; round < RESID_ROUNDS; ++round) {
                 const double w = (dhi - dlo) / 5.0;
-                const bool above = plane_cube_fraction(nx, ny, nz, dlo + w * (double)(q + 1), lox, loy, loz, h) > a;
+                const bool above = cp.fraction(dlo + w * (double)(q + 1), lox, loy, loz) > a;
                 // the cell's lanes are r, r+16, r+32, r+48: count those left of the crossing
                 const unsigned long long bal = __ballot(above);
                 const int k = (int)((bal >> r) & 1ull) + (int)((bal >> (r + 16)) & 1ull) +
@@ -660,10 +674,12 @@ __global__ void __launch_bounds__(64 * INFER_WAVES, 2) k_infer(const int* lat, c
             }
             const double d = 0.5 * (dlo + dhi);
             double part = 0.0;
-            for (int L = q; L < NIN; L += 4) {
-                if (L == 13) continue;
+#pragma unroll
+            for (int j = 0; j < NJ; ++j) {
+                const int L = q + 4 * j;
+                if (L >= NIN || L == 13) continue;
                 const double ex = (double)(L / 9 - 1), ey = (double)((L / 3) % 3 - 1), ez = (double)(L % 3 - 1);
-                const double f = plane_cube_fraction(nx, ny, nz, d, lox + ex * h, loy + ey * h, loz + ez * h, h);
+                const double f = cp.fraction(d, lox + ex * h, loy + ey * h, loz + ez * h);
                 part += fabs(f - rw[r][L]);
             }
             part += __shfl_xor(part, 16);

@@ -186,16 +186,42 @@ __device__ inline V3 ls_gradient(const DevMesh& m, const double* alpha, int c) {
     double A[4][4] = {};
     double B[4] = {};
     const V3 cc = ld3(m.C, c);
-    for (int k = m.stOff[c]; k < m.stOff[c + 1]; ++k) {
-        const int g = m.st[k];
-        const V3 d = ld3(m.C, g) - cc;
-        const double t[4] = {1.0, d.x, d.y, d.z};
-        const double v = alpha[g];
+    // stencil cells in batches of 8: the labels, then their centres and
+    // alphas, are loaded together; the sums still run in stencil order
+    constexpr int SB = 8;
+    const int b = m.stOff[c], e = m.stOff[c + 1];
+    for (int k0 = b; k0 < e; k0 += SB) {
+        // every load is unconditional (past the end: the last entry again)
+        // and entries past the end leave the sums unchanged by selection,
+        // so nothing splits the loads into dependent groups
+        int g[SB];
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
+        for (int j = 0; j < SB; ++j) g[j] = m.st[min(k0 + j, e - 1)];
+        V3 pc[SB];
+        double av[SB];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) A[r][q] += t[r] * t[q];
-            B[r] += t[r] * v;
+        for (int j = 0; j < SB; ++j) {
+            pc[j] = ld3(m.C, g[j]);
+            av[j] = alpha[g[j]];
+        }
+        // keep the scheduler from sinking those loads into the sums
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int j = 0; j < SB; ++j) {
+            const bool in = k0 + j < e;
+            const V3 d = pc[j] - cc;
+            const double t[4] = {1.0, d.x, d.y, d.z};
+            const double v = av[j];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const double a = A[r][q] + t[r] * t[q];
+                    A[r][q] = in ? a : A[r][q];
+                }
+                const double bb = B[r] + t[r] * v;
+                B[r] = in ? bb : B[r];
+            }
         }
     }
     // LUDecompose(scalarSymmetricSquareMatrix)
@@ -248,7 +274,7 @@ __device__ inline V3 ls_gradient(const DevMesh& m, const double* alpha, int c) {
 }
 
 // wl: optional list of mixed-cell positions (*wn entries) instead of all of them
-__global__ void __launch_bounds__(64) k_ls_normal(DevMesh m, DevState st, const int* wl, const int* wn) {
+__global__ void __launch_bounds__(64, 1) k_ls_normal(DevMesh m, DevState st, const int* wl, const int* wn) {
     const int nWork = wl ? *wn : *st.nMixed;
     for (int wi = blockIdx.x * blockDim.x + threadIdx.x; wi < nWork; wi += gridDim.x * blockDim.x) {
         const int i = wl ? wl[wi] : wi;
