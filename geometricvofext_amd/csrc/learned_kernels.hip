@@ -6,6 +6,7 @@
 // placement that consumes these normals is findSignedDistance
 // (cutCell.C:611-799), unchanged.
 #include <hip/hip_runtime.h>
+#include <cstdlib>
 #include <stdint.h>
 
 #include <algorithm>
@@ -592,12 +593,19 @@ __global__ void __launch_bounds__(64 * INFER_WAVES, 2) k_infer(const int* lat, c
     __shared__ float bias[NH + NH + NOUTP];
     __shared__ double raw[INFER_WAVES][16][NIN];
     {
+        // all of a thread's weight loads in flight together, then the LDS stores
+        static_assert(B_W2T * 2 / 16 % (64 * INFER_WAVES) == 0 && NOUTP * NH * 2 / 16 == 64 * INFER_WAVES,
+                      "staging covers the weights exactly");
+        constexpr int PER = B_W2T * 2 / 16 / (64 * INFER_WAVES);
         const uint4* src = (const uint4*)wbG;
         uint4* dst = (uint4*)wb;
-        for (int k = threadIdx.x; k < B_W2T * 2 / 16; k += blockDim.x) dst[k] = src[k];
-        const uint4* src3 = (const uint4*)(wbG + B_W3);
-        uint4* dst3 = (uint4*)w3;
-        for (int k = threadIdx.x; k < NOUTP * NH * 2 / 16; k += blockDim.x) dst3[k] = src3[k];
+        uint4 tmp[PER + 1];
+#pragma unroll
+        for (int j = 0; j < PER; ++j) tmp[j] = src[threadIdx.x + 64 * INFER_WAVES * j];
+        tmp[PER] = ((const uint4*)(wbG + B_W3))[threadIdx.x];
+#pragma unroll
+        for (int j = 0; j < PER; ++j) dst[threadIdx.x + 64 * INFER_WAVES * j] = tmp[j];
+        ((uint4*)w3)[threadIdx.x] = tmp[PER];
         for (int k = threadIdx.x; k < NH; k += blockDim.x) {
             bias[k] = pqG[Q_B1 + k];
             bias[NH + k] = pqG[Q_B2 + k];
@@ -807,7 +815,13 @@ void launch_sync_bf16(const float* pq, uint16_t* wb, hipStream_t s) {
 void launch_infer(const int* lat, const double* alpha, const int* mixed, const int* nMixed, const double* C,
                   const double* V, const uint16_t* wb, const float* pq, double tol, double* ifN, unsigned char* learned,
                   double* resid, int* lsList, int* nLs, int* nFb, int nC, hipStream_t s) {
-    k_infer<<<std::max(1, std::min(512, cdiv(nC, 16 * INFER_WAVES))), 64 * INFER_WAVES, 0, s>>>(
+    // workgroups (each stages the 45 KB of weights once): knob SVP_INFER_GRID
+    static const int maxGrid = [] {
+        const char* e = std::getenv("SVP_INFER_GRID");
+        const int g = e ? std::atoi(e) : 0;
+        return (g >= 1 && g <= 4096) ? g : 512;
+    }();
+    k_infer<<<std::max(1, std::min(maxGrid, cdiv(nC, 16 * INFER_WAVES))), 64 * INFER_WAVES, 0, s>>>(
         lat, alpha, mixed, nMixed, C, V, wb, pq, tol, ifN, learned, resid, lsList, nLs, nFb);
 }
 
