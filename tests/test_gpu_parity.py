@@ -172,3 +172,38 @@ def test_parity_tets_rotation_spheres(oracle_lib):
     _say("parity", mesh.name, "rotation, 8 spheres")
     exact, _, _ = _run_parity(mesh, oracle_lib, a0, 6, flow=(U0, phi0), nAlphaBounds=3, clip=False)
     assert exact
+
+
+def _bound_stats(gpu):
+    """(candidates, active, rounds, corrected faces) per bounding pass of the last step."""
+    return [tuple(int(v) for v in row) for row in gpu.bound_stats()]
+
+
+def test_parity_perturbed_hex_many_chunks(oracle_lib):
+    """Bounding passes with thousands of candidates, so the one-launch
+    dataflow pass spreads them over several workgroup chunks and corrections
+    cross chunks through global memory and the polling wave."""
+    import geometricvofext_amd as g
+    mesh = g.hex_box(48, perturb=0.2, seed=13)
+    om = oracle_lib.OracleMesh(mesh)
+    a0 = ramp_alpha(om.C, [((0.4, 0.45, 0.5), 0.3)], 1.5 / 48)
+    _say("parity", mesh.name, "many bounding chunks")
+    exact, gpu, _ = _run_parity(mesh, oracle_lib, a0, 4, nAlphaBounds=3, clip=False)
+    passes = _bound_stats(gpu)
+    _say("bounding passes (candidates, active, rounds, corrected faces):", passes)
+    assert passes and max(p[0] for p in passes) > 3 * 448
+    assert exact
+
+
+def test_parity_kahn_schedule(oracle_lib, monkeypatch):
+    """The alternative bounding schedule (grid-wide Kahn rounds and the
+    one-workgroup tail, SVP_BND_FLOW=0) on the heavy-bounding case."""
+    import geometricvofext_amd as g
+    monkeypatch.setenv("SVP_BND_FLOW", "0")
+    mesh = g.hex_box(24, perturb=0.2, seed=11)
+    om = oracle_lib.OracleMesh(mesh)
+    a0 = ramp_alpha(om.C, [((0.35, 0.35, 0.35), 0.15)], 1.5 / 24)
+    _say("parity", mesh.name, "Kahn schedule")
+    exact, gpu, _ = _run_parity(mesh, oracle_lib, a0, 8, nAlphaBounds=3, clip=False)
+    assert any(p[2] > 0 for p in _bound_stats(gpu)), "the Kahn schedule reports its rounds"
+    assert exact
