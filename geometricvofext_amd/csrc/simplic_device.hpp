@@ -37,6 +37,12 @@ struct V3 {
     double x, y, z;
 };
 __host__ __device__ inline V3 v3(double a, double b, double c) { return V3{a, b, c}; }
+// component-wise choice: a conditional V3 assignment written this way stays in
+// registers (a struct assignment under a condition can become a store through
+// a selected address, which moves the variables to scratch memory)
+__host__ __device__ inline V3 vsel(bool c, V3 a, V3 b) {
+    return V3{c ? a.x : b.x, c ? a.y : b.y, c ? a.z : b.z};
+}
 __host__ __device__ inline V3 operator+(V3 a, V3 b) { return v3(a.x + b.x, a.y + b.y, a.z + b.z); }
 __host__ __device__ inline V3 operator-(V3 a, V3 b) { return v3(a.x - b.x, a.y - b.y, a.z - b.z); }
 __host__ __device__ inline V3 operator-(V3 a) { return v3(-a.x, -a.y, -a.z); }
@@ -267,8 +273,11 @@ __device__ inline void rotate_left(T (&a)[N], int r) {
 
 // calcSubFace for a face of exactly N points held in registers; the same
 // operations in the same order as calcSubFace above.
-template <int N>
-__device__ inline bool calcSubFaceN(const V3 (&fPts)[N], V3 normal, double distance, FaceCut& out) {
+// IF: record the plane crossings in out.ifPts (the cell cut needs them; the
+// face-flux integrals only need the area).  out may live in LDS: it is then
+// written in place, which keeps a per-lane copy of the record out of scratch.
+template <int N, bool IF = true, class FC>
+__device__ inline bool calcSubFaceN(const V3 (&fPts)[N], V3 normal, double distance, FC& out) {
     double dist[N];
     int nSub = 0, first = -1;
 #pragma unroll
@@ -308,10 +317,15 @@ __device__ inline bool calcSubFaceN(const V3 (&fPts)[N], V3 normal, double dista
     int nPts = 0;
     V3 p0 = VZERO, p1 = VZERO, p2 = VZERO, fCentre = VZERO;
     auto emit1 = [&](V3 q) {
-        if (nPts == 0) fCentre = q; else fCentre = fCentre + q;
-        if (nPts == 0) p0 = q; else if (nPts == 1) p1 = q; else if (nPts == 2) p2 = q;
+        fCentre = vsel(nPts == 0, q, fCentre + q);
+        p0 = vsel(nPts == 0, q, p0);
+        p1 = vsel(nPts == 1, q, p1);
+        p2 = vsel(nPts == 2, q, p2);
         nPts++;
     };
+    static_assert(MAX_IF_POINTS == 4, "four crossing registers");
+    V3 ip0 = VZERO, ip1 = VZERO, ip2 = VZERO, ip3 = VZERO;
+    int nIf = 0;
 #pragma unroll
     for (int k = 0; k < N; ++k) {
         const int nxt = (k + 1 == N) ? 0 : k + 1;
@@ -322,14 +336,23 @@ __device__ inline bool calcSubFaceN(const V3 (&fPts)[N], V3 normal, double dista
             const double w = dc / (dc - dn);
             const V3 q = pc + w * (pn - pc);
             emit1(q);
-#pragma unroll
-            for (int s = 0; s < MAX_IF_POINTS; ++s)
-                if (out.nIf == s) out.ifPts[s] = q;
-            if (out.nIf >= MAX_IF_POINTS) ok = false;
-            out.nIf++;
+            if constexpr (IF) {
+                ip0 = vsel(nIf == 0, q, ip0);
+                ip1 = vsel(nIf == 1, q, ip1);
+                ip2 = vsel(nIf == 2, q, ip2);
+                ip3 = vsel(nIf == 3, q, ip3);
+            }
+            if (nIf >= MAX_IF_POINTS) ok = false;
+            nIf++;
         }
     }
-    if (out.nIf > MAX_IF_POINTS) out.nIf = MAX_IF_POINTS;
+    if constexpr (IF) {
+        out.ifPts[0] = ip0;
+        out.ifPts[1] = ip1;
+        out.ifPts[2] = ip2;
+        out.ifPts[3] = ip3;
+    }
+    out.nIf = nIf > MAX_IF_POINTS ? MAX_IF_POINTS : nIf;
     if (nPts < 3) {
         out.status = -1;
         out.centre = faceCentre(fPts, N);
@@ -348,9 +371,9 @@ __device__ inline bool calcSubFaceN(const V3 (&fPts)[N], V3 normal, double dista
     V3 prev = VZERO, firstP = VZERO;
     int kk = 0;
     auto emit2 = [&](V3 q) {
-        if (kk == 0) {
-            firstP = q;
-        } else {
+        const bool on = kk > 0;
+        firstP = vsel(on, firstP, q);
+        if (on) {
             const V3 c = prev + q + fCentre;
             const V3 nn = cross(q - prev, fCentre - prev);
             const double ar = mag(nn);

@@ -585,68 +585,82 @@ __global__ void __launch_bounds__(64, SVP_FSD_WAVES) k_fsd_group(DevMesh m, DevS
         __builtin_amdgcn_wave_barrier();
         bool ok = true;
         auto cut = [&](double dist, CellCut& cc) {
-            if (l < nF) {
-                FaceCut fc;
-                fsd_face_cut(fsz, q4, q3, normalI, dist, fc, ok);
-                rec[g][l] = fc;
-            }
+            if (l < nF) fsd_face_cut(fsz, q4, q3, normalI, dist, rec[g][l], ok);
             __builtin_amdgcn_wave_barrier();
             combineSubCell(&rec[g][0], nF, Cc, Vc, cc);
             __builtin_amdgcn_wave_barrier();
         };
-        // cutCell::findSignedDistance (cutCell.C:611-799)
+        // cutCell::findSignedDistance (cutCell.C:611-799) as one loop around a
+        // single cut, so the cut's code exists once (the kernel's instruction
+        // footprint decides its speed): phase 0 bisects over the vertex
+        // levels, 1 and 2 cut at a third and two thirds of the last bracket,
+        // 3 is the final cut at the plane found
         double lowDistance = vsrt[g][0], upDistance = vsrt[g][np - 1];
         int lowLabel = 0, upLabel = np - 1;
         double lowAlpha = 0.0, upAlpha = 1.0;
         CellCut cc;
-        bool done = false;
-        double dOut = 0.0;
-        while ((upLabel - lowLabel) > 1) {
-            const int midLabel = (int)round(0.5 * (double)(upLabel + lowLabel));
-            const double midDistance = vsrt[g][midLabel];
-            cut(midDistance, cc);
-            const double midAlpha = cc.VOF;
-            if (fabs(midAlpha - alphaI) < SVP_TSMALL) {
-                dOut = midDistance;
-                done = true;
+        double dOut = 0.0, deltaDistance = 0.0, alphaOneThird = 0.0;
+        int phase = 0, midLabel = 0;
+        while (true) {
+            double dist;
+            if (phase == 0) {
+                if ((upLabel - lowLabel) > 1) {
+                    midLabel = (int)round(0.5 * (double)(upLabel + lowLabel));
+                    dist = vsrt[g][midLabel];
+                } else if (fabs(lowDistance - upDistance) < SVP_TSMALL) {
+                    dOut = 0.5 * (lowDistance + upDistance);
+                    dist = dOut;
+                    phase = 3;
+                } else {
+                    deltaDistance = (upDistance - lowDistance) / 3.0;
+                    dist = lowDistance + deltaDistance;
+                    phase = 1;
+                }
+            } else if (phase == 2) {
+                dist = lowDistance + 2.0 * deltaDistance;
+            } else {
+                dist = dOut;
+            }
+            cut(dist, cc);
+            if (phase == 0) {
+                const double midDistance = dist;
+                const double midAlpha = cc.VOF;
+                if (fabs(midAlpha - alphaI) < SVP_TSMALL) {
+                    dOut = midDistance;
+                    break;
+                }
+                if (midAlpha > alphaI) {
+                    upLabel = midLabel;
+                    upDistance = midDistance;
+                    upAlpha = midAlpha;
+                } else {
+                    lowLabel = midLabel;
+                    lowDistance = midDistance;
+                    lowAlpha = midAlpha;
+                }
+            } else if (phase == 1) {
+                alphaOneThird = cc.VOF - lowAlpha;
+                phase = 2;
+            } else if (phase == 2) {
+                const double alphaPrismatoid = upAlpha - lowAlpha;
+                const double alphaTwoThirds = cc.VOF - lowAlpha;
+                const double a = 13.5 * alphaOneThird - 13.5 * alphaTwoThirds + 4.5 * alphaPrismatoid;
+                const double b = -22.5 * alphaOneThird + 18.0 * alphaTwoThirds - 4.5 * alphaPrismatoid;
+                const double cq = 9.0 * alphaOneThird - 4.5 * alphaTwoThirds + 1.0 * alphaPrismatoid;
+                const double d = lowAlpha - alphaI;
+                double lambda = 0.5;
+                for (int iter = 0; iter < 100; iter++) {
+                    const double func = a * (lambda * lambda * lambda) + b * (lambda * lambda) + cq * lambda + d;
+                    const double funcPrime = 3.0 * a * (lambda * lambda) + 2.0 * b * lambda + cq;
+                    const double lambdaNew = lambda - (func / funcPrime);
+                    if (fabs(lambdaNew - lambda) < SVP_TSMALL) break;
+                    lambda = lambdaNew;
+                }
+                dOut = lowDistance - lambda * (lowDistance - upDistance);
+                phase = 3;
+            } else {
                 break;
             }
-            if (midAlpha > alphaI) {
-                upLabel = midLabel;
-                upDistance = midDistance;
-                upAlpha = midAlpha;
-            } else {
-                lowLabel = midLabel;
-                lowDistance = midDistance;
-                lowAlpha = midAlpha;
-            }
-        }
-        if (!done && fabs(lowDistance - upDistance) < SVP_TSMALL) {
-            dOut = 0.5 * (lowDistance + upDistance);
-            cut(dOut, cc);
-            done = true;
-        }
-        if (!done) {
-            const double alphaPrismatoid = upAlpha - lowAlpha;
-            const double deltaDistance = (upDistance - lowDistance) / 3.0;
-            cut(lowDistance + deltaDistance, cc);
-            const double alphaOneThird = cc.VOF - lowAlpha;
-            cut(lowDistance + 2.0 * deltaDistance, cc);
-            const double alphaTwoThirds = cc.VOF - lowAlpha;
-            const double a = 13.5 * alphaOneThird - 13.5 * alphaTwoThirds + 4.5 * alphaPrismatoid;
-            const double b = -22.5 * alphaOneThird + 18.0 * alphaTwoThirds - 4.5 * alphaPrismatoid;
-            const double cq = 9.0 * alphaOneThird - 4.5 * alphaTwoThirds + 1.0 * alphaPrismatoid;
-            const double d = lowAlpha - alphaI;
-            double lambda = 0.5;
-            for (int iter = 0; iter < 100; iter++) {
-                const double func = a * (lambda * lambda * lambda) + b * (lambda * lambda) + cq * lambda + d;
-                const double funcPrime = 3.0 * a * (lambda * lambda) + 2.0 * b * lambda + cq;
-                const double lambdaNew = lambda - (func / funcPrime);
-                if (fabs(lambdaNew - lambda) < SVP_TSMALL) break;
-                lambda = lambdaNew;
-            }
-            dOut = lowDistance - lambda * (lowDistance - upDistance);
-            cut(dOut, cc);
         }
         if (l == 0) {
             unsigned long long fcls = 0ull;
@@ -994,16 +1008,16 @@ __device__ inline double tia_regs(const V3 (&P)[N], const double (&ptIn)[N], V3 
     } else {
         prevTime = 0.0;
         tauOld = 0.0;
-        ok &= calcSubFaceN<N>(P, normal, distance, fc);
+        ok &= calcSubFaceN<N, false>(P, normal, distance, fc);
         subAreaOld = mag(fc.area);
     }
     if (!(lastTime > dt)) tIntArea += magSf * (dt - lastTime) * pos0(Un0);
     const double smallTime = fmax(SVP_TSMALL / fabs(Un0), SVP_TSMALL);
     auto simpson = [&](double tauNew) {
         const double deltaTau = 0.5 * (tauNew - tauOld);
-        ok &= calcSubFaceN<N>(P, normal, distance - tauNew * Un0, fc);
+        ok &= calcSubFaceN<N, false>(P, normal, distance - tauNew * Un0, fc);
         const double subAreaNew = mag(fc.area);
-        ok &= calcSubFaceN<N>(P, normal, distance - (tauOld + deltaTau) * Un0, fc);
+        ok &= calcSubFaceN<N, false>(P, normal, distance - (tauOld + deltaTau) * Un0, fc);
         const double subAreaMid = mag(fc.area);
         tIntArea += (deltaTau / 3.0) * (subAreaOld + 4.0 * subAreaMid + subAreaNew);
         subAreaOld = subAreaNew;
@@ -1062,7 +1076,7 @@ __device__ inline double tiff_regs(const DevMesh& m, int f, const V3 (&p)[N], V3
     }
     FaceCut fcut;
     if (flat) {
-        ok &= calcSubFaceN<N>(p, normal, distance, fcut);
+        ok &= calcSubFaceN<N, false>(p, normal, distance, fcut);
         const double alphaf = mag(fcut.area) / magSf;
         return (phi * dt * alphaf);
     }
@@ -1073,7 +1087,7 @@ __device__ inline double tiff_regs(const DevMesh& m, int f, const V3 (&p)[N], V3
         const V3 tri[3] = {fc, p[pi], p[(pi + 1) % N]};
         const double magSfTri = mag(triAreaNormal(tri[0], tri[1], tri[2]));
         const double phiTri = phi * magSfTri / magSf;
-        ok &= calcSubFaceN<3>(tri, normal, distance, fcut);
+        ok &= calcSubFaceN<3, false>(tri, normal, distance, fcut);
         const double alphafTri = mag(fcut.area) / magSfTri;
         dVf += (phiTri * dt * alphafTri);
     }
