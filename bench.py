@@ -57,6 +57,9 @@ ALPHA_DICT = dict(orientationMethod="LS", nAlphaBounds=3, snapTol=0.0, clip=Fals
 LEARNED_WORKLOAD = dict(kind="hex", N=128, perturb=0.0, seed=0, shapes_seed=2024, flow="deformation",
                         end_time=3.0, desc="128^3 uniform hex, 3 seeded ellipsoids (semi-axes 0.08-0.15), "
                                            "deformation T=3, Co 0.5")
+# finer plain meshes of the learned-normal case, to find the plain path's
+# cost at the learned path's error
+EQUAL_ERROR_MESHES = (144, 160)
 LEARNED_TRAINING = dict(samples=2 << 20, depth=4, epochs=20, batch=4096, lr0=3e-3, lr1=3e-5, seed=11)
 
 
@@ -274,7 +277,49 @@ def learned_comparison(say):
     for name in ("learned", "learned_no_fallback"):
         out[name]["speedup_over_plain"] = out[name]["updates_per_s"] / out["plain"]["updates_per_s"]
         out[name]["E_s_ratio_to_plain"] = out[name]["E_s"] / out["plain"]["E_s"]
+    out["equal_error"] = equal_error_comparison(out, shapes, wl, say)
     return out
+
+
+def equal_error_comparison(out, shapes, wl, say):
+    """The plain path on finer meshes of the same case: the time it needs to
+    reach the learned path's L1 shape error at t=T (hot-path seconds of the
+    whole run, linear in E_s between the bracketing resolutions), against
+    the learned path's time on the base mesh."""
+    import geometricvofext_amd as g
+    from geometricvofext_amd import harness
+    pts = [(wl["N"], out["plain"]["E_s"], out["plain"]["ms_per_step"] * out["plain"]["steps"] / 1e3,
+            out["plain"]["updates_per_s"])]
+    for N in EQUAL_ERROR_MESHES:
+        s = g.SolveVofEqu(g.hex_box(N), dict(ALPHA_DICT))
+        s.init_alpha(shapes, max_depth=5)
+        a0 = s.alpha()
+        U0, phi0 = s.deformation_field()
+        s.upload_base_flow(U0, phi0)
+        seq = time_sequence(s, a0, wl["end_time"])
+        s.set_alpha(a0)
+        device_sync(s)
+        r0, ad0, n0 = s.timers()
+        run_steps(s, seq)
+        device_sync(s)
+        r1, ad1, n1 = s.timers()
+        hot = (r1 - r0) + (ad1 - ad0)
+        e = harness.shape_error(s.alpha(), a0, s.geometry()["V"])
+        pts.append((N, e, hot, (n1 - n0) / hot))
+        say(f"plain {N}^3: E_s {e:.5f} in {hot:.3f}s")
+        s.close()
+    res = {"plain_runs": [{"N": n, "E_s": e, "hot_s": t, "updates_per_s": u} for n, e, t, u in pts]}
+    for name in ("learned", "learned_no_fallback"):
+        target = out[name]["E_s"]
+        t_learn = out[name]["ms_per_step"] * out[name]["steps"] / 1e3
+        t_plain = None
+        for (n0_, e0, t0, _), (n1_, e1, t1, _) in zip(pts, pts[1:]):
+            if e0 >= target >= e1:
+                t_plain = t0 + (t1 - t0) * (e0 - target) / (e0 - e1)
+                break
+        res[name] = {"E_s": target, "hot_s": t_learn, "plain_hot_s_at_equal_error": t_plain,
+                     "time_to_solution_speedup": (t_plain / t_learn) if t_plain else None}
+    return res
 
 
 def time_sequence(solver, alpha0, end_time, max_steps=None, period=6.0):
