@@ -633,6 +633,7 @@ private:
         st_.boundStats = dalloc<int>(4 * MAX_BOUND_PASSES);
         st_.boundTotals = dalloc<unsigned long long>(5);
         st_.bcount = dalloc<int>(16);
+        HIPCHECK(hipMemset(st_.bcount, 0, 16 * sizeof(int)));
         st_.maxCellFaces = maxCellFaces_;
         // bounding rounds run grid-wide before the one-workgroup tail (tuning
         // knob SVP_BND_GRID_ROUNDS, 0..MAX_GRID_ROUNDS; results do not depend on it)

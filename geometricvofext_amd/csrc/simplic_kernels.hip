@@ -112,29 +112,60 @@ __device__ inline int block_prefix(bool flag, int* lds, int& total) {
     return off + inWave;
 }
 
-// single-workgroup exclusive scan of counts[0..nb) in place; total -> *out
-__global__ void k_scan_counts(int* counts, int nb, int* out, unsigned long long* accum) {
-    __shared__ int lds[1024];
-    __shared__ int carry;
-    if (threadIdx.x == 0) carry = 0;
-    __syncthreads();
-    for (int base = 0; base < nb; base += 1024) {
-        const int i = base + threadIdx.x;
-        const int v = (i < nb) ? counts[i] : 0;
-        lds[threadIdx.x] = v;
-        __syncthreads();
-        for (int o = 1; o < 1024; o <<= 1) {
-            const int t = (threadIdx.x >= o) ? lds[threadIdx.x - o] : 0;
-            __syncthreads();
-            lds[threadIdx.x] += t;
-            __syncthreads();
-        }
-        if (i < nb) counts[i] = carry + lds[threadIdx.x] - v;
-        __syncthreads();
-        if (threadIdx.x == 1023) carry += lds[1023];
-        __syncthreads();
+// workgroup exclusive scan of one int per thread (blockDim.x == 1024)
+// exclusive scan over a workgroup of 1024 threads: wave scans by shuffles,
+// then a scan of the 16 wave totals; lds needs 16 ints
+__device__ inline int wg_scan1024(int v, int* lds, int& total) {
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    int x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o);
+        if (lane >= o) x += y;
     }
-    if (threadIdx.x == 0) {
+    if (lane == 63) lds[w] = x;
+    __syncthreads();
+    if (w == 0) {
+        int s2 = lane < 16 ? lds[lane] : 0;
+#pragma unroll
+        for (int o = 1; o < 16; o <<= 1) {
+            const int y = __shfl_up(s2, o);
+            if (lane >= o) s2 += y;
+        }
+        if (lane < 16) lds[lane] = s2;
+    }
+    __syncthreads();
+    const int pre = w > 0 ? lds[w - 1] : 0;
+    total = lds[15];
+    __syncthreads();
+    return pre + x - v;
+}
+
+// single-workgroup exclusive scan of counts[0..nb) in place; total -> *out.
+// Tiles of 8192: eight consecutive counts per thread, one workgroup scan.
+__global__ void __launch_bounds__(1024) k_scan_counts(int* counts, int nb, int* out, unsigned long long* accum) {
+    __shared__ int lds[16];
+    constexpr int PT = 8;
+    const int t = threadIdx.x;
+    int carry = 0;
+    for (int base = 0; base < nb; base += 1024 * PT) {
+        int v[PT];
+        const int i = base + PT * t;
+#pragma unroll
+        for (int j = 0; j < PT; ++j) v[j] = (i + j < nb) ? counts[i + j] : 0;
+        int sum = 0;
+#pragma unroll
+        for (int j = 0; j < PT; ++j) sum += v[j];
+        int total;
+        int off = carry + wg_scan1024(sum, lds, total);
+#pragma unroll
+        for (int j = 0; j < PT; ++j) {
+            if (i + j < nb) counts[i + j] = off;
+            off += v[j];
+        }
+        carry += total;
+    }
+    if (t == 0) {
         *out = carry;
         if (accum) *accum += (unsigned long long)carry;
     }
@@ -1218,34 +1249,6 @@ __device__ inline bool face_downwind_of(const DevMesh& m, const double* phi, int
     return (m.own[facei] == c) ? (ph >= 0) : (ph < 0);
 }
 
-// workgroup exclusive scan of one int per thread (blockDim.x == 1024)
-// exclusive scan over a workgroup of 1024 threads: wave scans by shuffles,
-// then a scan of the 16 wave totals; lds needs 16 ints
-__device__ inline int wg_scan1024(int v, int* lds, int& total) {
-    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    int x = v;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const int y = __shfl_up(x, o);
-        if (lane >= o) x += y;
-    }
-    if (lane == 63) lds[w] = x;
-    __syncthreads();
-    if (w == 0) {
-        int s2 = lane < 16 ? lds[lane] : 0;
-#pragma unroll
-        for (int o = 1; o < 16; o <<= 1) {
-            const int y = __shfl_up(s2, o);
-            if (lane >= o) s2 += y;
-        }
-        if (lane < 16) lds[lane] = s2;
-    }
-    __syncthreads();
-    const int pre = w > 0 ? lds[w - 1] : 0;
-    total = lds[15];
-    __syncthreads();
-    return pre + x - v;
-}
 
 // limitFlux (advectionTemplates.C:118-215), one pass = the kernels below.
 // The pass runs only while cells lie outside [-aTol, 1+aTol] (bound count of
