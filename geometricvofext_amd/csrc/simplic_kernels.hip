@@ -1305,24 +1305,41 @@ __device__ inline int bound_record(const DevMesh& m, const DevState& st, int c, 
     double dVftot = 0.0;
 #pragma unroll
     for (int w = 0; w < (MF + 7) / 8; ++w) flg[w] = 0u;
+    // all face loads first, unconditional (slots past nf repeat the last
+    // face), so they are in flight together instead of one face at a time
+    int fo[MF], fn[MF];
+    double fph[MF], fdv[MF];
+    unsigned char fem[MF];
+#pragma unroll
+    for (int k = 0; k < MF; ++k) fid[k] = m.cFaces[b + min(k, nf - 1)];
 #pragma unroll
     for (int k = 0; k < MF; ++k) {
-        fid[k] = 0;
+        const int f = fid[k];
+        fo[k] = m.own[f];
+        fn[k] = m.nbr[f];
+        fph[k] = st.phi[f];
+        fdv[k] = st.dVf[f];
+        fem[k] = m.fEmpty[f];
+    }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int k = 0; k < MF; ++k) {
         oth[k] = -1;
         phf[k] = 0.0;
         dvf[k] = 0.0;
         smax[k] = 0.0;
         if (k < nf) {
-            const int f = m.cFaces[b + k];
-            fid[k] = f;
-            const bool own = (m.own[f] == c);
-            if (f < m.nIF) oth[k] = own ? m.nbr[f] : m.own[f];
-            const double ph = face_value(m, st.phi, f);
-            const double dv = face_value(m, st.dVf, f);
+            const int f = fid[k];
+            const bool own = (fo[k] == c);
+            if (f < m.nIF) oth[k] = own ? fn[k] : fo[k];
+            // face_value: empty-patch faces carry nothing
+            const bool live = (f < m.nIF) || !fem[k];
+            const double ph = live ? fph[k] : 0.0;
+            const double dv = live ? fdv[k] : 0.0;
             phf[k] = ph;
             dvf[k] = dv;
             const bool down = own ? (ph >= 0) : (ph < 0);
-            const bool writable = (f < m.nIF) || !m.fEmpty[f];
+            const bool writable = live;
             const bool lowerWriter = oth[k] >= 0 && !down && oth[k] < c;
             flg[k >> 3] |= ((own ? 1u : 0u) | (down ? 2u : 0u) | (writable ? 4u : 0u) | (lowerWriter ? 8u : 0u))
                            << (4 * (k & 7));
@@ -1976,25 +1993,31 @@ __global__ void __launch_bounds__(256) k_bnd_apply(DevMesh m, DevState st) {
         }
         const int c = st.touched[idx];
         const int b = m.cOff[c], nf = m.cOff[c + 1] - b;
-        // all face labels, then all correctedFaces positions, as independent loads
-        int fid[MF], kpos[MF];
-#pragma unroll
-        for (int k = 0; k < MF; ++k) fid[k] = k < nf ? m.cFaces[b + k] : 0;
-#pragma unroll
-        for (int k = 0; k < MF; ++k) kpos[k] = k < nf ? st.corrPos[fid[k]] : -1;
+        // every load unconditional (slots past nf repeat the last face), in
+        // three batches: labels; keys, corrections, owners, empty flags; alpha
+        int fid[MF], kpos[MF], kon[MF];
         double kval[MF];
-        bool kown[MF];
+        unsigned char kem[MF];
+#pragma unroll
+        for (int k = 0; k < MF; ++k) fid[k] = m.cFaces[b + min(k, nf - 1)];
 #pragma unroll
         for (int k = 0; k < MF; ++k) {
-            kval[k] = 0.0;
-            kown[k] = false;
-            if (kpos[k] >= 0) {
-                kval[k] = face_value(m, st.corr, fid[k]);
-                kown[k] = (m.own[fid[k]] == c);
-            }
+            kpos[k] = st.corrPos[fid[k]];
+            kval[k] = st.corr[fid[k]];
+            kon[k] = m.own[fid[k]];
+            kem[k] = m.fEmpty[fid[k]];
         }
         const double before = st.alpha[c];
         const double Vc = m.V[c];
+        __builtin_amdgcn_sched_barrier(0);
+        bool kown[MF];
+#pragma unroll
+        for (int k = 0; k < MF; ++k) {
+            if (k >= nf) kpos[k] = -1;
+            // face_value: empty-patch faces carry nothing
+            kval[k] = (kpos[k] >= 0 && (fid[k] < m.nIF || !kem[k])) ? kval[k] : 0.0;
+            kown[k] = kpos[k] >= 0 && kon[k] == c;
+        }
         double a = before;
         // corrections in ascending correctedFaces position (selection over
         // the unrolled slots: positions are distinct)
