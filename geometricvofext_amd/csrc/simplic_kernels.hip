@@ -1068,7 +1068,7 @@ __device__ inline double tia_regs(const V3 (&P)[N], const double (&ptIn)[N], V3 
 
 // timeIntegratedFaceFlux (cutFace.C:262-389) for a face of N points in registers
 template <int N>
-__device__ inline double tiff_regs(const DevMesh& m, int f, const V3 (&p)[N], V3 normal, double distance,
+__device__ __attribute__((always_inline)) inline double tiff_regs(const DevMesh& m, int f, const V3 (&p)[N], V3 normal, double distance,
                                    double Un0, double dt, double phi, double magSf, bool& ok) {
     if (fabs(phi) <= SVP_TSMALL) return 0.0;
     const bool flat = m.flat[f] > (1.0 - SVP_TSMALL);
