@@ -1257,18 +1257,18 @@ __device__ inline bool face_downwind_of(const DevMesh& m, const double* phi, int
 //   prep              every candidate's first redistribution loop, which
 //                     decides whether it writes corrections and which faces
 //                     open correctedFaces; the record of each writing cell
-//   scan2             correctedFaces offsets
 //   flow              all remaining boundFlux work in dependency order in
-//                     one launch, and the correctedFaces positions and
+//                     one launch, and the correctedFaces order keys and
 //                     touched cells (the default), or
 //   scatter/round/tail  the same as Kahn rounds: grid-wide kernels, then the
 //                     remaining rounds inside one workgroup
 //   apply             corrections applied per cell in correctedFaces order,
 //                     dVf of the corrected faces, bitmap and bound count
 //   reset             dVfCorrectionValues = 0 for the next pass
-// bcount: [0] candidates, [1] corrected faces, [2] touched cells, [3+r]
-// frontier size of round r (r <= st.gridRounds; [3] = active cells of the
-// dataflow pass), [BND_CTRL-1] rounds.
+// bcount: [0] candidates, [1] corrected faces (prep), [2] touched cells,
+// [3+r] frontier size of round r (r <= st.gridRounds; [3] = active cells of
+// the dataflow pass), [BND_CTRL-1] rounds, then the dataflow ticket and
+// completion count.
 constexpr int BND_CTRL = 3 + MAX_GRID_ROUNDS + 2;
 // dataflow pass (one launch in place of the rounds and the tail): chunk
 // ticket and completion count, cleared with the per-pass counters
