@@ -60,6 +60,7 @@ public:
         params_.clip = dict.clip;
         params_.splitWarpedFace = dict.split_warped_face;
         learned_.active = std::string(dict.orientation_method) == "learned";
+        gaussNormal_ = std::string(dict.orientation_method) == "alphaGrad";
         learned_.tol = dict.learned_residual_tol;
         buildMesh(mesh);
         allocState();
@@ -86,7 +87,10 @@ public:
         if (learned_.active) {
             reconstructLearned();
         } else {
-            stage(STAGE_NORMAL, [&] { launch_ls_normal(m_, st_, stream_); });
+            stage(STAGE_NORMAL, [&] {
+                if (gaussNormal_) launch_grad_normal(m_, st_, stream_);
+                else launch_ls_normal(m_, st_, stream_);
+            });
             stage(STAGE_FSD, [&] { launch_find_signed_distance(m_, st_, params_, stream_); });
         }
         HIPCHECK(hipEventRecord(b, stream_));
@@ -388,6 +392,7 @@ private:
     std::vector<hipEvent_t> freeEvents_;
     double reconTime_ = 0.0, advectTime_ = 0.0;
     bool reconstructed_ = false;
+    bool gaussNormal_ = false;  // orientationMethod alphaGrad
     bool alphaPhiValid_ = false;
     int maxCellFaces_ = 0;
     int profMask_ = 0;
@@ -590,6 +595,7 @@ private:
         m_.C = dalloc<double>(3L * nC);
         m_.V = dalloc<double>(nC);
         m_.psW = dalloc<double>(psSrc.size());
+        m_.linW = dalloc<double>(std::max(nIF, 1));
     }
 
     void allocState() {
@@ -754,11 +760,13 @@ int svp_dict_parse(const char* text, svp_dict* out) {
             else if (key == "learnedResidualTol") d.learned_residual_tol = std::stod(val);
         }
         if (!method.empty()) {
-            if (method == "LS" || method == "isoAlphaGrad" || method == "learned" || method == "MLP") {
+            if (method == "LS" || method == "isoAlphaGrad" || method == "alphaGrad" || method == "learned" ||
+                method == "MLP") {
                 std::snprintf(d.orientation_method, sizeof d.orientation_method, "%s",
                               method == "MLP" ? "learned" : method.c_str());
-            } else if (method == "alphaGrad" || method == "isoRDF" || method == "RDF") {
-                throw SvpError(SVP_E_UNSUPPORTED, "orientationMethod " + method + " is not provided (LS/isoAlphaGrad are)");
+            } else if (method == "isoRDF" || method == "RDF") {
+                throw SvpError(SVP_E_UNSUPPORTED,
+                               "orientationMethod " + method + " is not provided (LS/isoAlphaGrad/alphaGrad are)");
             } else {
                 throw SvpError(SVP_E_DICT, "Invalid orientationMethod " + method +
                                                ". Valid methods are: alphaGrad isoAlphaGrad isoRDF LS RDF learned");
@@ -775,8 +783,8 @@ int svp_create(const svp_mesh* mesh, const svp_dict* dict, int device, svp_handl
         if (dict) d = *dict;
         else svp_dict_defaults(&d);
         const std::string m(d.orientation_method);
-        if (m != "LS" && m != "isoAlphaGrad" && m != "learned") {
-            if (m == "alphaGrad" || m == "isoRDF" || m == "RDF")
+        if (m != "LS" && m != "isoAlphaGrad" && m != "alphaGrad" && m != "learned") {
+            if (m == "isoRDF" || m == "RDF")
                 throw SvpError(SVP_E_UNSUPPORTED, "orientationMethod " + m + " is not provided");
             throw SvpError(SVP_E_DICT, "Invalid orientationMethod " + m);
         }

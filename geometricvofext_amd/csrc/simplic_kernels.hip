@@ -84,10 +84,22 @@ __global__ void k_point_weights(DevMesh m) {
     for (int k = m.psOff[p]; k < m.psOff[p + 1]; ++k) m.psW[k] = m.psW[k] / sumW;
 }
 
+// surfaceInterpolation::makeWeights (OpenFOAM v2312): owner weight of the
+// linear interpolation, |Sf.(C_N - Cf)| / (|Sf.(Cf - C_P)| + |Sf.(C_N - Cf)|)
+__global__ void k_lin_weights(DevMesh m) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= m.nIF) return;
+    const V3 Sf = ld3(m.Sf, f), Cf = ld3(m.Cf, f);
+    const double sfdOwn = fabs(dot(Sf, Cf - ld3(m.C, m.own[f])));
+    const double sfdNei = fabs(dot(Sf, ld3(m.C, m.nbr[f]) - Cf));
+    m.linW[f] = sfdNei / (sfdOwn + sfdNei);
+}
+
 void launch_geometry(const DevMesh& m, hipStream_t s) {
     k_face_geom<<<cdiv(m.nF, 256), 256, 0, s>>>(m);
     k_cell_geom<<<cdiv(m.nC, 256), 256, 0, s>>>(m);
     k_point_weights<<<cdiv(m.nP, 256), 256, 0, s>>>(m);
+    if (m.nIF > 0) k_lin_weights<<<cdiv(m.nIF, 256), 256, 0, s>>>(m);
 }
 
 // ---------------------------------------------------------------------------
@@ -318,6 +330,66 @@ __global__ void __launch_bounds__(64, 1) k_ls_normal(DevMesh m, DevState st, con
 
 void launch_ls_normal(const DevMesh& m, const DevState& st, hipStream_t s, const int* wl, const int* wn) {
     k_ls_normal<<<1024, 64, 0, s>>>(m, st, wl, wn);
+}
+
+// calcInterfaceNFromRegAlphaGrad (reconstruction.C:74-82): n = -grad(alpha) /
+// (|grad| + SMALL) with fvc::grad = Gauss linear, i.e. OpenFOAM's
+// gaussGrad::gradf: the faces of the cell in ascending order add Sf*ssf
+// (owner) or subtract it (neighbour), boundary faces of non-empty patches
+// add Sf*alpha (zeroGradient), then the sum is divided by V.  Only mixed
+// cells need the normal.  Faces in batches of 8 with all loads in flight.
+__global__ void __launch_bounds__(64, 1) k_grad_normal(DevMesh m, DevState st) {
+    const int nMixed = *st.nMixed;
+    constexpr int FB = 8;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nMixed; i += gridDim.x * blockDim.x) {
+        const int c = st.mixed[i];
+        const double ac = st.alpha[c];
+        const double Vc = m.V[c];
+        V3 g = VZERO;
+        const int b = m.cOff[c], e = m.cOff[c + 1];
+        for (int k0 = b; k0 < e; k0 += FB) {
+            int ev[FB];
+#pragma unroll
+            for (int j = 0; j < FB; ++j) ev[j] = m.cFacesAsc[min(k0 + j, e - 1)];
+            V3 sf[FB];
+            double w[FB], ao[FB];
+            unsigned char em[FB];
+#pragma unroll
+            for (int j = 0; j < FB; ++j) {
+                const int f = ev[j] >= 0 ? ev[j] : ~ev[j];
+                const bool in = f < m.nIF;
+                sf[j] = ld3(m.Sf, f);
+                w[j] = m.linW[in ? f : 0];
+                em[j] = m.fEmpty[f];
+                // the other cell's alpha (the cell's own on boundary faces)
+                const int o = ev[j] >= 0 ? (in ? m.nbr[f] : c) : m.own[f];
+                ao[j] = st.alpha[o];
+            }
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int j = 0; j < FB; ++j) {
+                if (k0 + j >= e) break;
+                const int ef = ev[j];
+                if (ef >= 0) {
+                    if (ef < m.nIF) {
+                        const double ssf = w[j] * (ac - ao[j]) + ao[j];
+                        g = g + ssf * sf[j];
+                    } else if (!em[j]) {
+                        g = g + ac * sf[j];
+                    }
+                } else {
+                    const double ssf = w[j] * (ao[j] - ac) + ac;
+                    g = g - ssf * sf[j];
+                }
+            }
+        }
+        const V3 n = -(g / Vc);
+        st3(st.ifN, i, n / (mag(n) + SVP_SMALL));
+    }
+}
+
+void launch_grad_normal(const DevMesh& m, const DevState& st, hipStream_t s) {
+    k_grad_normal<<<1024, 64, 0, s>>>(m, st);
 }
 
 // ---------------------------------------------------------------------------

@@ -41,6 +41,7 @@ struct DevMesh {
     double* C;       // 3*nC
     double* V;       // nC
     double* psW;     // point weights, psOff CSR
+    double* linW;    // nIF, linear interpolation weight of the owner value
 };
 
 // Per-step state.  Interface data are compact arrays indexed by the position
@@ -150,6 +151,8 @@ void launch_detect(const DevMesh& m, const DevState& st, const Params& p, hipStr
 // wl/wn: optional device list of mixed-cell positions and its length
 void launch_ls_normal(const DevMesh& m, const DevState& st, hipStream_t s, const int* wl = nullptr,
                       const int* wn = nullptr);
+// alphaGrad normal: minus the Gauss-linear gradient of alpha, normalised
+void launch_grad_normal(const DevMesh& m, const DevState& st, hipStream_t s);
 void launch_find_signed_distance(const DevMesh& m, const DevState& st, const Params& p, hipStream_t s,
                                  const int* wl = nullptr, const int* wn = nullptr);
 // pipeline stages of one step, in launch order (also the profiler's slots)

@@ -74,7 +74,8 @@ void svp_dict_defaults(svp_dict* d);
  * C/C++ comments allowed).  Unknown keys are ignored, as OpenFOAM does.
  * `reconstructionScheme` is accepted as an alias of `orientationMethod`.
  * Returns SVP_E_DICT for an unknown orientation method
- * (reconstruction.C:610-625) and SVP_E_UNSUPPORTED for alphaGrad / isoRDF / RDF.
+ * (reconstruction.C:610-625) and SVP_E_UNSUPPORTED for isoRDF / RDF.
+ * alphaGrad is the Gauss-linear gradient normal (reconstruction.C:74-82).
  * "learned" (alias "MLP") selects the learned normal of simplic_learned.h. */
 int svp_dict_parse(const char* text, svp_dict* out);
 

@@ -59,6 +59,7 @@ def lib():
     L.or_solver_new.restype = vp
     L.or_solver_new.argtypes = [vp, C.c_double, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int]
     L.or_solver_free.argtypes = [vp]
+    L.or_solver_set_orientation.argtypes = [vp, C.c_int]
     L.or_set_alpha.argtypes = [vp, _dp]
     L.or_set_flow.argtypes = [vp, _dp, _dp, C.c_void_p]
     L.or_reconstruct.argtypes = [vp]
@@ -169,11 +170,14 @@ def ls_grad(pos, val):
 class OracleSolver:
     """Oracle restatement of ``solveVofEqu`` (reconstruct + advect)."""
 
+    ORIENTATIONS = {"LS": 0, "isoAlphaGrad": 0, "alphaGrad": 1}
+
     def __init__(self, omesh: OracleMesh, mixedCellTol=1e-8, nAlphaBounds=10, snapTol=0.0, clip=True,
-                 splitWarpedFace=False, threads=1):
+                 splitWarpedFace=False, threads=1, orientationMethod="LS"):
         self.om = omesh
         self.h = lib().or_solver_new(omesh.h, mixedCellTol, nAlphaBounds, snapTol, int(clip),
                                      int(splitWarpedFace), threads)
+        lib().or_solver_set_orientation(self.h, self.ORIENTATIONS[orientationMethod])
 
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:

@@ -921,6 +921,7 @@ struct Solver {
     double snapTol = 0.0;
     bool clip = true;
     bool splitWarpedFace = false;
+    int orientation = 0;  // 0: isoAlphaGrad (LS), 1: alphaGrad
     int nThreads = 1;
     // fields
     std::vector<double> alpha, alphaOld, phi, dVf, alphaPhi;
@@ -989,6 +990,35 @@ struct Solver {
         }
     }
 
+    // reconstruction.C:74-82: interfaceN = -fvc::grad(alpha1) with the
+    // Gauss linear scheme, normalised.  fvc::grad restates OpenFOAM v2312's
+    // gaussGrad::gradf (faces in ascending order: += Sf*ssf on the owner, -=
+    // on the neighbour, then the boundary faces of non-empty patches, then
+    // /= V) over linear interpolation with surfaceInterpolation::makeWeights
+    // weights; OpenFOAM is a dependency of the reference, not part of it.
+    // alpha is zeroGradient on the boundary.
+    void calcInterfaceNFromRegAlphaGrad() {
+        std::vector<V3> g(m->nC, VZERO);
+        for (int f = 0; f < m->nIF; ++f) {
+            const int o = m->own[f], n = m->nei[f];
+            const double sfdOwn = std::fabs(dot(m->Sf[f], m->Cf[f] - m->C[o]));
+            const double sfdNei = std::fabs(dot(m->Sf[f], m->C[n] - m->Cf[f]));
+            const double w = sfdNei / (sfdOwn + sfdNei);
+            const double ssf = w * (alpha[o] - alpha[n]) + alpha[n];
+            const V3 v = ssf * m->Sf[f];
+            g[o] += v;
+            g[n] -= v;
+        }
+        for (int f = m->nIF; f < m->nF; ++f) {
+            if (m->boundaryEmpty(f)) continue;
+            g[m->own[f]] += alpha[m->own[f]] * m->Sf[f];
+        }
+        for (int c = 0; c < m->nC; ++c) {
+            const V3 n = -(g[c] / m->V[c]);
+            interfaceN[c] = n / (mag(n) + SMALL);
+        }
+    }
+
     // reconstruction.C:680-722
     void reconstruct() {
         const auto t0 = std::chrono::steady_clock::now();
@@ -997,7 +1027,8 @@ struct Solver {
         nCutPts.assign(mixedCells.size(), 0);
         faceClass.assign(mixedCells.size(), 0);
         if (!mixedCells.empty()) {
-            calcInterfaceNFromIsoAlphaGrad();
+            if (orientation == 1) calcInterfaceNFromRegAlphaGrad();
+            else calcInterfaceNFromIsoAlphaGrad();
             const int nMixed = (int)mixedCells.size();
 #pragma omp parallel num_threads(nThreads)
             {
@@ -1462,6 +1493,9 @@ void* or_solver_new(void* mesh, double mixedCellTol, int nAlphaBounds, double sn
 }
 
 void or_solver_free(void* h) { delete (Solver*)h; }
+
+// 0: isoAlphaGrad / LS, 1: alphaGrad
+void or_solver_set_orientation(void* h, int method) { ((Solver*)h)->orientation = method; }
 
 void or_set_alpha(void* h, const double* a) {
     Solver* s = (Solver*)h;

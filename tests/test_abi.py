@@ -70,7 +70,8 @@ def test_dict_reconstruction_scheme_alias_and_errors():
     with pytest.raises(g.SvpError) as e:
         g.parse_dict("orientationMethod nonsense;")
     assert e.value.code == -2                             # FatalError in the reference
-    for m in ("alphaGrad", "isoRDF", "RDF"):
+    assert g.parse_dict("orientationMethod alphaGrad;").orientation_method == b"alphaGrad"
+    for m in ("isoRDF", "RDF"):
         with pytest.raises(g.SvpError) as e:
             g.parse_dict(f"orientationMethod {m};")
         assert e.value.code == -3

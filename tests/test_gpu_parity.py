@@ -31,7 +31,8 @@ def _pair(mesh, oracle_lib, **dict_kw):
     om = oracle_lib.OracleMesh(mesh)
     kw = dict(mixedCellTol=dict_kw.get("mixedCellTol", 1e-8), nAlphaBounds=dict_kw.get("nAlphaBounds", 10),
               snapTol=dict_kw.get("snapTol", 0.0), clip=dict_kw.get("clip", True),
-              splitWarpedFace=dict_kw.get("splitWarpedFace", False))
+              splitWarpedFace=dict_kw.get("splitWarpedFace", False),
+              orientationMethod=dict_kw.get("orientationMethod", "LS"))
     ors = oracle_lib.OracleSolver(om, **kw)
     return gpu, om, ors
 
@@ -206,4 +207,26 @@ def test_parity_kahn_schedule(oracle_lib, monkeypatch):
     _say("parity", mesh.name, "Kahn schedule")
     exact, gpu, _ = _run_parity(mesh, oracle_lib, a0, 8, nAlphaBounds=3, clip=False)
     assert any(p[2] > 0 for p in _bound_stats(gpu)), "the Kahn schedule reports its rounds"
+    assert exact
+
+
+def test_parity_alphagrad_perturbed_hex(oracle_lib):
+    """orientationMethod alphaGrad (Gauss-linear gradient normal) on the
+    heavy-bounding perturbed hex case."""
+    import geometricvofext_amd as g
+    mesh = g.hex_box(24, perturb=0.2, seed=11)
+    om = oracle_lib.OracleMesh(mesh)
+    a0 = ramp_alpha(om.C, [((0.35, 0.35, 0.35), 0.15)], 1.5 / 24)
+    _say("parity", mesh.name, "alphaGrad")
+    exact, _, _ = _run_parity(mesh, oracle_lib, a0, 8, nAlphaBounds=3, clip=False, orientationMethod="alphaGrad")
+    assert exact
+
+
+def test_parity_alphagrad_tets(oracle_lib):
+    import geometricvofext_amd as g
+    mesh = g.tet_box(10, perturb=0.1, seed=5)
+    om = oracle_lib.OracleMesh(mesh)
+    a0 = ramp_alpha(om.C, [((0.45, 0.5, 0.5), 0.22)], 1.5 / 10)
+    _say("parity", mesh.name, "alphaGrad")
+    exact, _, _ = _run_parity(mesh, oracle_lib, a0, 5, orientationMethod="alphaGrad")
     assert exact
