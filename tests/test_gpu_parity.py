@@ -58,7 +58,7 @@ def _compare_interface(gpu, ors, step):
     return exact, len(gi["cells"])
 
 
-def _run_parity(mesh, oracle_lib, alpha0, n_steps, flow="deformation", **dict_kw):
+def _run_parity(mesh, oracle_lib, alpha0, n_steps, flow="deformation", sources=None, **dict_kw):
     gpu, om, ors = _pair(mesh, oracle_lib, **dict_kw)
     _compare_geometry(gpu, om)
     if flow == "deformation":
@@ -93,8 +93,12 @@ def _run_parity(mesh, oracle_lib, alpha0, n_steps, flow="deformation", **dict_kw
         exact, nm = _compare_interface(gpu, ors, step)
         all_exact &= exact
         n_ic.append(nm)
-        gpu.advect(dt)
-        ors.advect(dt)
+        if sources is None:
+            gpu.advect(dt)
+            ors.advect(dt)
+        else:
+            gpu.advect(dt, *sources)
+            ors.advect(dt, *sources)
         ga, oa = gpu.alpha(), ors.alpha()
         assert max_abs(ga, oa) <= TOL, f"step {step}: alpha off by {max_abs(ga, oa)}"
         gp, op = gpu.alpha_phi(), ors.alpha_phi()
@@ -229,4 +233,20 @@ def test_parity_alphagrad_tets(oracle_lib):
     a0 = ramp_alpha(om.C, [((0.45, 0.5, 0.5), 0.22)], 1.5 / 10)
     _say("parity", mesh.name, "alphaGrad")
     exact, _, _ = _run_parity(mesh, oracle_lib, a0, 5, orientationMethod="alphaGrad")
+    assert exact
+
+
+def test_parity_sources(oracle_lib):
+    """Non-zero implicit and explicit sources (advect(Sp, Su), as a
+    phase-change solver passes them: alphaSuSp.H) in the update and in the
+    bounding passes."""
+    import geometricvofext_amd as g
+    mesh = g.hex_box(20, perturb=0.2, seed=6)
+    om = oracle_lib.OracleMesh(mesh)
+    a0 = ramp_alpha(om.C, [((0.45, 0.5, 0.5), 0.22)], 1.5 / 20)
+    rng = np.random.default_rng(12)
+    Sp = -rng.uniform(0.0, 2.0, mesh.n_cells)
+    Su = rng.uniform(0.0, 0.5, mesh.n_cells) * (a0 > 0.5)
+    _say("parity", mesh.name, "Sp/Su sources")
+    exact, _, _ = _run_parity(mesh, oracle_lib, a0, 6, sources=(Sp, Su), nAlphaBounds=3, clip=False)
     assert exact
