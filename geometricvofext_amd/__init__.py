@@ -87,6 +87,7 @@ def lib():
     L.svp_get_alpha.argtypes = [vp, vp, C.c_int]
     L.svp_get_alpha_phi.argtypes = [vp, vp, C.c_int]
     L.svp_get_rho_phi.argtypes = [vp, dbl, dbl, vp, C.c_int]
+    L.svp_advect_sub_cycles.argtypes = [vp, dbl, C.c_int32, vp, vp, dbl, dbl, vp, C.c_int]
     L.svp_n_mixed.argtypes = [vp, C.POINTER(i32)]
     L.svp_get_interface.argtypes = [vp, vp, vp, vp, vp, vp, vp, C.c_int]
     L.svp_get_cut_class.argtypes = [vp, vp, vp, C.c_int]
@@ -224,6 +225,17 @@ class SolveVofEqu:
         sp = None if Sp is None else np.ascontiguousarray(Sp, dtype=np.float64)
         su = None if Su is None else np.ascontiguousarray(Su, dtype=np.float64)
         _check(lib().svp_advect(self.h, float(dt), _ptr(sp), _ptr(su), SVP_HOST))
+
+    def advect_sub_cycles(self, dt, n_sub, Sp=None, Su=None, rho1=1.0, rho2=0.0):
+        """The sub-cycled alpha step of interPlicFoam (alphaEqnSubCycle.H:30-60):
+        n_sub reconstruct + advect steps of dt/n_sub; returns rhoPhi summed
+        with weights (dt/n_sub)/dt (n_faces)."""
+        sp = None if Sp is None else np.ascontiguousarray(Sp, dtype=np.float64)
+        su = None if Su is None else np.ascontiguousarray(Su, dtype=np.float64)
+        a = np.zeros(self.mesh.n_faces)
+        _check(lib().svp_advect_sub_cycles(self.h, float(dt), int(n_sub), _ptr(sp), _ptr(su), float(rho1),
+                                           float(rho2), a.ctypes.data, SVP_HOST))
+        return a
 
     # outputs ----------------------------------------------------------------
     def alpha(self):

@@ -59,7 +59,10 @@ struct DBuf {
         reset();
         bytes = std::max<size_t>(n, 1) * sizeof(T);
         HIPCHECK(hipMalloc(&p, bytes));
-        HIPCHECK(hipMemset(p, 0, bytes));
+        // complete before return: the handles' work runs on non-blocking
+        // streams, which do not wait for the null stream
+        HIPCHECK(hipMemsetAsync(p, 0, bytes, nullptr));
+        HIPCHECK(hipStreamSynchronize(nullptr));
         return (T*)p;
     }
     template <class T>

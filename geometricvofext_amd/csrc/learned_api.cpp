@@ -39,8 +39,8 @@ void upload_packed(svp_normal_model& m, const float* packed) {
     std::vector<float> q;
     packed_to_padded(packed, q);
     HIPCHECK(hipMemcpy(m.pq.p, q.data(), sizeof(float) * N_PADDED, hipMemcpyHostToDevice));
-    HIPCHECK(hipMemset(m.m1.p, 0, sizeof(float) * N_PADDED));
-    HIPCHECK(hipMemset(m.m2.p, 0, sizeof(float) * N_PADDED));
+    HIPCHECK(hipMemsetAsync(m.m1.p, 0, sizeof(float) * N_PADDED, m.stream));
+    HIPCHECK(hipMemsetAsync(m.m2.p, 0, sizeof(float) * N_PADDED, m.stream));
     m.adamT = 0;
     launch_sync_bf16((const float*)m.pq.p, (uint16_t*)m.wb.p, m.stream);
     HIPCHECK(hipGetLastError());

@@ -281,6 +281,56 @@ public:
         return st_.alphaPhi;
     }
 
+    // getRhoPhi with constant densities (advection.H:329-343) of the last
+    // advect into the device buffer out.
+    void rhoPhi(double rho1, double rho2, double* out) {
+        launch_rho_phi(m_.nF, alphaPhi(), st_.phi, rho1, rho2, 0.0, out, 0, stream_);
+    }
+
+    // The alpha sub-step of interPlicFoam (alphaEqnSubCycle.H:30-60 with
+    // alphaEqn.H:9-20): nSub > 1 runs nSub reconstruct + advect steps of
+    // deltaT/nSub (Time::subCycle divides deltaT by nSub) and sums
+    // (subDeltaT/totalDeltaT) rhoPhi over them; nSub == 1 is one step whose
+    // rhoPhi is returned as is.  Sp/Su are held for every sub-step.  Returns
+    // the device buffer holding rhoPhi (n_faces).
+    const double* advectSubCycles(double dt, int nSub, const double* Sp, const double* Su, double rho1,
+                                  double rho2) {
+        if (nCases_ > 0) throw SvpError(SVP_E_ARG, "sub-cycles: not with a case batch");
+        if (nSub < 1) throw SvpError(SVP_E_ARG, "nAlphaSubCycles must be at least 1");
+        if (!(dt > 0.0) || !std::isfinite(dt)) throw SvpError(SVP_E_ARG, "advect: dt must be positive");
+        if (!std::isfinite(rho1) || !std::isfinite(rho2)) throw SvpError(SVP_E_ARG, "rho1/rho2 must be finite");
+        double* sum = rhoPhiSum_.p ? (double*)rhoPhiSum_.p : rhoPhiSum_.alloc<double>(m_.nF);
+        if (nSub == 1) {
+            reconstruct();
+            advect(dt, Sp, Su);
+            rhoPhi(rho1, rho2, sum);
+            return sum;
+        }
+        const double subDt = dt / nSub;
+        const double w = subDt / dt;
+        for (int k = 0; k < nSub; ++k) {
+            reconstruct();
+            advect(subDt, Sp, Su);
+            launch_rho_phi(m_.nF, alphaPhi(), st_.phi, rho1, rho2, w, sum, k == 0 ? 1 : 2, stream_);
+        }
+        return sum;
+    }
+
+    // Host sources are staged in device buffers kept by the handle.
+    void stageSources(const double*& Sp, const double*& Su, int where) {
+        if (where != SVP_HOST) return;
+        if (Sp) {
+            if (!harnessBufs_[1].p) allocHarness(harnessBufs_[1], m_.nC);
+            copyIn((double*)harnessBufs_[1].p, Sp, m_.nC, SVP_HOST);
+            Sp = (const double*)harnessBufs_[1].p;
+        }
+        if (Su) {
+            if (!harnessBufs_[2].p) allocHarness(harnessBufs_[2], m_.nC);
+            copyIn((double*)harnessBufs_[2].p, Su, m_.nC, SVP_HOST);
+            Su = (const double*)harnessBufs_[2].p;
+        }
+    }
+
     int nMixed() {
         int n = 0;
         HIPCHECK(hipMemcpyAsync(&n, st_.nMixed, sizeof(int), hipMemcpyDeviceToHost, stream_));
@@ -384,6 +434,7 @@ public:
     }
     double* allocHarness(DBuf& b, long n) { return b.alloc<double>(n); }
     DBuf harnessBufs_[3];
+    DBuf rhoPhiSum_;
 
 private:
     std::vector<std::unique_ptr<DBuf>> bufs_;
@@ -628,18 +679,18 @@ private:
         st_.pend = dalloc<int>(nC);
         st_.queue = dalloc<int>(2L * nC);
         st_.actIdx = dalloc<int>(nC);
-        HIPCHECK(hipMemset(st_.actIdx, 0xFF, sizeof(int) * nC));
+        HIPCHECK(hipMemsetAsync(st_.actIdx, 0xFF, sizeof(int) * nC, stream_));
         // correctedFaces order keys are cell * (max faces) + rank in 31 bits
         if ((long long)nC * (maxCellFaces_ <= NS_CELL_FACES ? NS_CELL_FACES : MAX_CELL_FACES) >= 0x7FFFFFFFll)
             throw SvpError(SVP_E_UNSUPPORTED, "mesh too large for 31-bit bounding order keys");
         st_.corrPos = dalloc<int>(nF);
-        HIPCHECK(hipMemset(st_.corrPos, 0xFF, sizeof(int) * nF));
+        HIPCHECK(hipMemsetAsync(st_.corrPos, 0xFF, sizeof(int) * nF, stream_));
         st_.touched = dalloc<int>(nC);
         st_.touchedFlag = dalloc<int>(nC);
         st_.boundStats = dalloc<int>(4 * MAX_BOUND_PASSES);
         st_.boundTotals = dalloc<unsigned long long>(5);
         st_.bcount = dalloc<int>(16);
-        HIPCHECK(hipMemset(st_.bcount, 0, 16 * sizeof(int)));
+        HIPCHECK(hipMemsetAsync(st_.bcount, 0, 16 * sizeof(int), stream_));
         st_.maxCellFaces = maxCellFaces_;
         // bounding rounds run grid-wide before the one-workgroup tail (tuning
         // knob SVP_BND_GRID_ROUNDS, 0..MAX_GRID_ROUNDS; results do not depend on it)
@@ -815,24 +866,18 @@ int svp_reconstruct(svp_handle* h) {
 int svp_advect(svp_handle* h, double dt, const double* Sp, const double* Su, int where) {
     return guarded([&] {
         solveVofEqu& s = H(h);
-        if (where == SVP_HOST && (Sp || Su)) {
-            // stage host sources in the harness buffers
-            double* dSp = nullptr;
-            double* dSu = nullptr;
-            if (Sp) {
-                if (!s.harnessBufs_[1].p) s.allocHarness(s.harnessBufs_[1], s.m_.nC);
-                dSp = (double*)s.harnessBufs_[1].p;
-                s.copyIn(dSp, Sp, s.m_.nC, SVP_HOST);
-            }
-            if (Su) {
-                if (!s.harnessBufs_[2].p) s.allocHarness(s.harnessBufs_[2], s.m_.nC);
-                dSu = (double*)s.harnessBufs_[2].p;
-                s.copyIn(dSu, Su, s.m_.nC, SVP_HOST);
-            }
-            s.advect(dt, dSp, dSu);
-        } else {
-            s.advect(dt, Sp, Su);
-        }
+        s.stageSources(Sp, Su, where);
+        s.advect(dt, Sp, Su);
+    });
+}
+
+int svp_advect_sub_cycles(svp_handle* h, double dt, int32_t n_sub, const double* Sp, const double* Su,
+                          double rho1, double rho2, double* rho_phi, int where) {
+    return guarded([&] {
+        solveVofEqu& s = H(h);
+        s.stageSources(Sp, Su, where);
+        const double* sum = s.advectSubCycles(dt, n_sub, Sp, Su, rho1, rho2);
+        s.copyOut(rho_phi, sum, s.m_.nF, where);
     });
 }
 
@@ -849,13 +894,15 @@ int svp_get_alpha_phi(svp_handle* h, double* ap, int where) {
 
 int svp_get_rho_phi(svp_handle* h, double rho1, double rho2, double* out, int where) {
     return guarded([&] {
+        if (!out) throw SvpError(SVP_E_ARG, "null output");
         solveVofEqu& s = H(h);
-        std::vector<double> ap(s.m_.nF), ph(s.m_.nF);
-        s.copyOut(ap.data(), s.alphaPhi(), s.m_.nF, SVP_HOST);
-        s.copyOut(ph.data(), s.st_.phi, s.m_.nF, SVP_HOST);
-        for (int f = 0; f < s.m_.nF; ++f) ap[f] = (rho1 - rho2) * ap[f] + rho2 * ph[f];
-        if (where == SVP_DEVICE) HIPCHECK(hipMemcpy(out, ap.data(), sizeof(double) * ap.size(), hipMemcpyHostToDevice));
-        else std::memcpy(out, ap.data(), sizeof(double) * ap.size());
+        if (where == SVP_DEVICE) {
+            s.rhoPhi(rho1, rho2, out);
+        } else {
+            if (!s.rhoPhiSum_.p) s.rhoPhiSum_.alloc<double>(s.m_.nF);
+            s.rhoPhi(rho1, rho2, (double*)s.rhoPhiSum_.p);
+            s.copyOut(out, (const double*)s.rhoPhiSum_.p, s.m_.nF, SVP_HOST);
+        }
     });
 }
 
@@ -1062,7 +1109,7 @@ int svp_harness_upload_base_flow(svp_handle* h, const double* U0, const double* 
         s.copyIn(s.U0_, U0, 3L * s.m_.nC, SVP_HOST);
         s.copyIn(s.phi0_, phi0, s.m_.nF, SVP_HOST);
         if (U0b) s.copyIn(s.U0b_, U0b, 3L * s.nB_, SVP_HOST);
-        else HIPCHECK(hipMemset(s.U0b_, 0, sizeof(double) * 3 * std::max(s.nB_, 1)));
+        else HIPCHECK(hipMemsetAsync(s.U0b_, 0, sizeof(double) * 3 * std::max(s.nB_, 1), s.stream_));
     });
 }
 

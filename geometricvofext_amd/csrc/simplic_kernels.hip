@@ -2245,4 +2245,22 @@ void launch_alpha_phi(const DevMesh& m, const DevState& st, double dt, hipStream
     k_alpha_phi<<<cdiv(m.nF, 256), 256, 0, s>>>(m, st, dt);
 }
 
+// advection::getRhoPhi with constant densities (advection.H:329-343):
+// rhoPhi = (rho1 - rho2) alphaPhi + rho2 phi per face.  mode 0 writes it,
+// mode 1 starts the sub-cycle sum rhoPhiSum = 0 + w rhoPhi and mode 2 adds
+// w rhoPhi to it (alphaEqnSubCycle.H:33-52, w = subDeltaT / totalDeltaT).
+__global__ void k_rho_phi(int nF, const double* __restrict__ ap, const double* __restrict__ phi, double drho,
+                          double rho2, double w, double* __restrict__ out, int mode) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nF) return;
+    const double rp = drho * ap[f] + rho2 * phi[f];
+    if (mode == 0) out[f] = rp;
+    else out[f] = (mode == 1 ? 0.0 : out[f]) + w * rp;
+}
+
+void launch_rho_phi(int nF, const double* ap, const double* phi, double rho1, double rho2, double w,
+                    double* out, int mode, hipStream_t s) {
+    if (nF > 0) k_rho_phi<<<cdiv(nF, 256), 256, 0, s>>>(nF, ap, phi, rho1 - rho2, rho2, w, out, mode);
+}
+
 }  // namespace svp

@@ -106,6 +106,18 @@ int svp_get_alpha(svp_handle* h, double* alpha, int where);
 int svp_get_alpha_phi(svp_handle* h, double* alpha_phi, int where);
 /* getRhoPhi = (rho1 - rho2) alphaPhi + rho2 phi (advection.H:329-343). */
 int svp_get_rho_phi(svp_handle* h, double rho1, double rho2, double* rho_phi, int where);
+/* The alpha sub-step of interPlicFoam-class solvers (alphaEqnSubCycle.H:30-60
+ * with alphaEqn.H:9-20; nAlphaSubCycles from alphaControls.H:3): n_sub > 1
+ * runs n_sub reconstruct + advect steps of dt/n_sub and returns
+ * rhoPhi = sum over them of (dt/n_sub)/dt * getRhoPhi(rho1, rho2); n_sub == 1
+ * is one reconstruct + advect whose getRhoPhi is returned.  Sp/Su (n_cells
+ * or NULL) are held for every sub-step; a caller whose sources change with
+ * alpha calls svp_advect per sub-step instead.  rho_phi (n_faces) may be
+ * NULL.  The PIMPLE outer-corrector averaging of phi and U
+ * (alphaEqnSubCycle.H:1-28, 62-67) stays with the caller (svp_set_flow,
+ * svp_get_alpha / svp_set_alpha). */
+int svp_advect_sub_cycles(svp_handle* h, double dt, int32_t n_sub, const double* Sp, const double* Su,
+                          double rho1, double rho2, double* rho_phi, int where);
 /* Number of mixed cells of the last reconstruct (reconstruction.C:675-676). */
 int svp_n_mixed(svp_handle* h, int32_t* n_mixed);
 /* Interface data of the last reconstruct per mixed cell (ascending cells):

@@ -67,6 +67,9 @@ def lib():
     L.or_get_alpha.argtypes = [vp, _dp]
     L.or_get_dvf.argtypes = [vp, _dp]
     L.or_get_alpha_phi.argtypes = [vp, _dp]
+    L.or_get_rho_phi.argtypes = [vp, C.c_double, C.c_double, _dp]
+    L.or_advect_sub_cycles.argtypes = [vp, C.c_double, C.c_int, C.c_void_p, C.c_void_p, C.c_double,
+                                       C.c_double, _dp]
     L.or_n_mixed.argtypes = [vp]
     L.or_n_mixed.restype = C.c_int
     L.or_get_mixed.argtypes = [vp, _ip, _ip, _ip, _dp, _dp, _dp, _dp, _dp]
@@ -214,6 +217,20 @@ class OracleSolver:
     def alpha_phi(self):
         a = np.zeros(self.om.mesh.n_faces)
         lib().or_get_alpha_phi(self.h, a)
+        return a
+
+    def rho_phi(self, rho1, rho2):
+        a = np.zeros(self.om.mesh.n_faces)
+        lib().or_get_rho_phi(self.h, rho1, rho2, a)
+        return a
+
+    def advect_sub_cycles(self, dt, n_sub, Sp=None, Su=None, rho1=1.0, rho2=0.0):
+        """Sub-cycled alpha step (alphaEqnSubCycle.H:30-60); returns rhoPhi."""
+        sp = None if Sp is None else _c(Sp, np.float64)
+        su = None if Su is None else _c(Su, np.float64)
+        a = np.zeros(self.om.mesh.n_faces)
+        lib().or_advect_sub_cycles(self.h, dt, int(n_sub), None if sp is None else sp.ctypes.data,
+                                   None if su is None else su.ctypes.data, rho1, rho2, a)
         return a
 
     def mixed(self):

@@ -1529,6 +1529,38 @@ void or_get_alpha_phi(void* h, double* a) {
     Solver* s = (Solver*)h;
     std::memcpy(a, s->alphaPhi.data(), sizeof(double) * s->m->nF);
 }
+// advection::getRhoPhi with constant densities (advection.H:329-343).
+void or_get_rho_phi(void* h, double rho1, double rho2, double* out) {
+    Solver* s = (Solver*)h;
+    const double drho = rho1 - rho2;
+    for (int f = 0; f < s->m->nF; ++f) out[f] = drho * s->alphaPhi[f] + rho2 * s->phi[f];
+}
+
+// alphaEqnSubCycle.H:30-60 with alphaEqn.H:9-20.  Time::subCycle divides
+// deltaT by the number of sub-cycles; every sub-cycle adds
+// (subDeltaT/totalDeltaT) rhoPhi to a sum that starts at zero.
+void or_advect_sub_cycles(void* h, double dt, int nSub, const double* Sp, const double* Su, double rho1,
+                          double rho2, double* rhoPhi) {
+    Solver* s = (Solver*)h;
+    const int nF = s->m->nF;
+    if (nSub <= 1) {
+        s->reconstruct();
+        s->advect(dt, Sp, Su);
+        or_get_rho_phi(h, rho1, rho2, rhoPhi);
+        return;
+    }
+    const double subDt = dt / nSub;
+    const double w = subDt / dt;
+    std::vector<double> sum(nF, 0.0), rp(nF);
+    for (int k = 0; k < nSub; ++k) {
+        s->reconstruct();
+        s->advect(subDt, Sp, Su);
+        or_get_rho_phi(h, rho1, rho2, rp.data());
+        for (int f = 0; f < nF; ++f) sum[f] += w * rp[f];
+    }
+    std::memcpy(rhoPhi, sum.data(), sizeof(double) * nF);
+}
+
 int or_n_mixed(void* h) { return (int)((Solver*)h)->mixedCells.size(); }
 void or_get_cut_class(void* h, int* nCutPts, uint64_t* faceClass) {
     Solver* s = (Solver*)h;

@@ -250,3 +250,30 @@ def test_parity_sources(oracle_lib):
     _say("parity", mesh.name, "Sp/Su sources")
     exact, _, _ = _run_parity(mesh, oracle_lib, a0, 6, sources=(Sp, Su), nAlphaBounds=3, clip=False)
     assert exact
+
+
+def test_parity_sub_cycles_rho_phi(oracle_lib):
+    """svp_advect_sub_cycles (alphaEqnSubCycle.H:30-60) and svp_get_rho_phi
+    (advection.H:329-343) against the oracle: alpha and the summed rhoPhi
+    after every outer step, with and without sources."""
+    import geometricvofext_amd as g
+    mesh = g.hex_box(20, perturb=0.2, seed=8)
+    gpu, om, ors = _pair(mesh, oracle_lib, nAlphaBounds=3)
+    U, phi = om.deformation_field()
+    a0 = ramp_alpha(om.C, [((0.5, 0.45, 0.5), 0.2)], 1.5 / 20)
+    gpu.set_alpha(a0)
+    ors.set_alpha(a0)
+    gpu.set_flow(phi, U)
+    ors.set_flow(phi, U)
+    rng = np.random.default_rng(4)
+    Sp = -rng.uniform(0.0, 1.0, mesh.n_cells)
+    Su = rng.uniform(0.0, 0.2, mesh.n_cells) * (a0 > 0.5)
+    dt = 0.5 / np.max(om.courant(phi, a0))
+    for step, (n_sub, src) in enumerate(((1, None), (3, None), (2, (Sp, Su)), (4, None))):
+        _say("sub-cycles", mesh.name, "step", step, "n_sub", n_sub, "sources", src is not None)
+        args = () if src is None else src
+        rg = gpu.advect_sub_cycles(dt * n_sub, n_sub, *args, rho1=1000.0, rho2=1.2)
+        ro = ors.advect_sub_cycles(dt * n_sub, n_sub, *args, rho1=1000.0, rho2=1.2)
+        assert bits_equal(gpu.alpha(), ors.alpha()), f"step {step}: alpha off by {max_abs(gpu.alpha(), ors.alpha())}"
+        assert bits_equal(rg, ro), f"step {step}: rhoPhi off by {max_abs(rg, ro)}"
+        assert bits_equal(gpu.rho_phi(998.2, 0.6), ors.rho_phi(998.2, 0.6))
