@@ -90,6 +90,7 @@ def lib():
     L.svp_advect_sub_cycles.argtypes = [vp, dbl, C.c_int32, vp, vp, dbl, dbl, vp, C.c_int]
     L.svp_n_mixed.argtypes = [vp, C.POINTER(i32)]
     L.svp_get_interface.argtypes = [vp, vp, vp, vp, vp, vp, vp, C.c_int]
+    L.svp_get_plic_surface.argtypes = [vp, vp, vp, vp, vp, vp, C.c_int]
     L.svp_get_cut_class.argtypes = [vp, vp, vp, C.c_int]
     L.svp_get_un0.argtypes = [vp, vp, C.c_int]
     L.svp_get_geometry.argtypes = [vp, vp, vp, vp, vp, vp, vp]
@@ -257,6 +258,19 @@ class SolveVofEqu:
         n = C.c_int32()
         _check(lib().svp_n_mixed(self.h, C.byref(n)))
         return n.value
+
+    def plic_surface(self):
+        """PLIC polygons of the last reconstruct (reconstruction::interface,
+        reconstruction.C:787-835): dict(points (n, 3), offsets (n_faces + 1),
+        cells (n_faces)); polygon f is points[offsets[f]:offsets[f + 1]]."""
+        nf, npt = C.c_int32(), C.c_int32()
+        _check(lib().svp_get_plic_surface(self.h, C.addressof(nf), C.addressof(npt), None, None, None, SVP_HOST))
+        cells = np.zeros(max(nf.value, 1), np.int32)
+        offs = np.zeros(nf.value + 1, np.int32)
+        pts = np.zeros(3 * max(npt.value, 1))
+        _check(lib().svp_get_plic_surface(self.h, C.addressof(nf), C.addressof(npt), cells.ctypes.data,
+                                          offs.ctypes.data, pts.ctypes.data, SVP_HOST))
+        return dict(points=pts.reshape(-1, 3)[:npt.value], offsets=offs, cells=cells[:nf.value])
 
     def interface(self):
         n = self.n_mixed()

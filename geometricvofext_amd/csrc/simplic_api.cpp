@@ -316,6 +316,56 @@ public:
         return sum;
     }
 
+    // reconstruction::interface (reconstruction.C:787-835) of the last
+    // reconstruct: sizes always; the polygons when all three outputs are given.
+    void plicSurface(int32_t* nFaces, int32_t* nPts, int32_t* cells, int32_t* offs, double* pts, int where) {
+        if (!nFaces || !nPts) throw SvpError(SVP_E_ARG, "svp_get_plic_surface: null size output");
+        const bool want = cells || offs || pts;
+        if (want && !(cells && offs && pts))
+            throw SvpError(SVP_E_ARG, "svp_get_plic_surface: give face_cells, face_offsets and points, or none");
+        checkDeviceErrors();
+        const int k = nMixed();
+        int tot[2] = {0, 0};
+        if (k > 0) {
+            if (plicIdx_.bytes < sizeof(int) * (2L * k + 2)) plicIdx_.alloc<int>(2L * k + 2);
+            int* ptOff = (int*)plicIdx_.p;
+            int* faceIdx = ptOff + k;
+            int* totals = faceIdx + k;
+            launch_plic_surface(m_, st_, k, 0, ptOff, faceIdx, totals, nullptr, nullptr, nullptr, stream_);
+            HIPCHECK(hipGetLastError());
+            HIPCHECK(hipMemcpyAsync(tot, totals, sizeof(tot), hipMemcpyDeviceToHost, stream_));
+            HIPCHECK(hipStreamSynchronize(stream_));
+            checkDeviceErrors();
+        }
+        *nPts = tot[0];
+        *nFaces = tot[1];
+        if (!want) return;
+        int32_t* dCells = cells;
+        int32_t* dOffs = offs;
+        double* dPts = pts;
+        if (where != SVP_DEVICE) {
+            const size_t need = sizeof(double) * 3 * std::max(tot[0], 1) + sizeof(int) * (2L * tot[1] + 1) + 64;
+            if (plicOut_.bytes < need) plicOut_.alloc<char>(need);
+            dPts = (double*)plicOut_.p;
+            dCells = (int32_t*)(dPts + 3L * std::max(tot[0], 1));
+            dOffs = dCells + tot[1];
+        }
+        if (k > 0) {
+            int* ptOff = (int*)plicIdx_.p;
+            launch_plic_surface(m_, st_, k, 1, ptOff, ptOff + k, nullptr, dPts, dCells, dOffs, stream_);
+            HIPCHECK(hipGetLastError());
+        }
+        // the closing offset: face_offsets[n_faces] = n_points
+        HIPCHECK(hipMemcpyAsync(dOffs + tot[1], &tot[0], sizeof(int), hipMemcpyHostToDevice, stream_));
+        HIPCHECK(hipStreamSynchronize(stream_));
+        checkDeviceErrors();
+        if (where != SVP_DEVICE) {
+            copyOut(pts, dPts, 3L * tot[0], SVP_HOST);
+            copyOutT(cells, dCells, tot[1], SVP_HOST);
+            copyOutT(offs, dOffs, tot[1] + 1L, SVP_HOST);
+        }
+    }
+
     // Host sources are staged in device buffers kept by the handle.
     void stageSources(const double*& Sp, const double*& Su, int where) {
         if (where != SVP_HOST) return;
@@ -435,6 +485,7 @@ public:
     double* allocHarness(DBuf& b, long n) { return b.alloc<double>(n); }
     DBuf harnessBufs_[3];
     DBuf rhoPhiSum_;
+    DBuf plicIdx_, plicOut_;
 
 private:
     std::vector<std::unique_ptr<DBuf>> bufs_;
@@ -904,6 +955,11 @@ int svp_get_rho_phi(svp_handle* h, double rho1, double rho2, double* out, int wh
             s.copyOut(out, (const double*)s.rhoPhiSum_.p, s.m_.nF, SVP_HOST);
         }
     });
+}
+
+int svp_get_plic_surface(svp_handle* h, int32_t* n_faces, int32_t* n_points, int32_t* face_cells,
+                         int32_t* face_offsets, double* points, int where) {
+    return guarded([&] { H(h).plicSurface(n_faces, n_points, face_cells, face_offsets, points, where); });
 }
 
 int svp_n_mixed(svp_handle* h, int32_t* n) {

@@ -881,6 +881,110 @@ void launch_find_signed_distance(const DevMesh& m, const DevState& st, const Par
 }
 
 // ---------------------------------------------------------------------------
+// PLIC surface: reconstruction::interface (reconstruction.C:787-835) with
+// cutCell::interfacePoints (cutCell.C:545-608), one lane per mixed cell
+// ---------------------------------------------------------------------------
+// Vector::normalise of OpenFOAM: zero below ROOTVSMALL, else divided by its
+// magnitude component by component
+__device__ inline V3 normalised(V3 v) {
+    const double s = mag(v);
+    return s < SVP_ROOTVSMALL ? VZERO : v / s;
+}
+
+// The plane crossings of the cut faces (face order, crossing order) sorted
+// by their angle about the interface centre, in a frame with z along the
+// interface area vector and x towards the first crossing; a crossing whose
+// angle lies within 1e-8 of the previous one in that order is dropped.  The
+// insertion sort is stable, like sortedOrder.  Returns the polygon size.
+template <int K>
+__device__ inline int interface_points(const FaceCut* fc, int nF, const CellCut& cut, V3 (&q)[K]) {
+    V3 raw[K];
+    double ang[K];
+    int ord[K];
+    int n = 0;
+    for (int f = 0; f < nF; ++f) {
+        if (fc[f].status != 0) continue;
+        for (int k = 0; k < fc[f].nIf && n < K; ++k) raw[n++] = fc[f].ifPts[k];
+    }
+    if (n == 0) return 0;
+    const V3 C = cut.ifCentre;
+    const V3 zhat = cut.ifArea / mag(cut.ifArea);
+    V3 xhat = raw[0] - C;
+    xhat = xhat - dot(xhat, zhat) * zhat;
+    xhat = normalised(xhat);
+    const V3 yhat = normalised(cross(zhat, xhat));
+    for (int k = 0; k < n; ++k) {
+        const V3 r = raw[k] - C;
+        ang[k] = atan2(dot(r, yhat), dot(r, xhat));
+        ord[k] = k;
+    }
+    for (int a = 1; a < n; ++a) {
+        const int t = ord[a];
+        const double ta = ang[t];
+        int b = a - 1;
+        while (b >= 0 && ang[ord[b]] > ta) {
+            ord[b + 1] = ord[b];
+            --b;
+        }
+        ord[b + 1] = t;
+    }
+    int out = 0;
+    q[out++] = raw[ord[0]];
+    for (int k = 1; k < n; ++k)
+        if (fabs(ang[ord[k]] - ang[ord[k - 1]]) > 1e-8) q[out++] = raw[ord[k]];
+    return out;
+}
+
+// Count mode (pts == nullptr): ptOff[i] = polygon size of mixed cell i,
+// faceIdx[i] = 1 if it has a polygon.  Write mode: ptOff and faceIdx hold the
+// exclusive scans of those; the polygon goes to pts[ptOff[i]..], its cell and
+// first point to faceCells / faceOff at faceIdx[i].  Every mixed cell is cut
+// again through its mesh faces (no warped-face split), as the reference does.
+template <class Poly>
+__global__ void __launch_bounds__(64) k_plic_polygons(DevMesh m, DevState st, int nMixed, int* ptOff, int* faceIdx,
+                                                      double* pts, int* faceCells, int* faceOff) {
+    constexpr int K = Poly::maxFaces * MAX_IF_POINTS;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nMixed; i += gridDim.x * blockDim.x) {
+        const int c = st.mixed[i];
+        Poly P;
+        FaceCut fc[Poly::maxFaces];
+        CellCut cut;
+        V3 q[K];
+        int n = 0;
+        if (!build_cell_poly(m, c, false, P)) {
+            atomicOr(st.errFlag, ERR_CELL_TOO_LARGE);
+        } else {
+            if (!calcSubCell(P, ld3(m.C, c), m.V[c], ld3(st.ifN, i), st.ifD[i], fc, cut))
+                atomicOr(st.errFlag, ERR_TOO_MANY_CROSSINGS);
+            if (cut.status == 0) n = interface_points(fc, P.nFaces, cut, q);
+        }
+        if (!pts) {
+            ptOff[i] = n;
+            faceIdx[i] = n > 0 ? 1 : 0;
+            continue;
+        }
+        if (n == 0) continue;
+        const int o = ptOff[i], fi = faceIdx[i];
+        faceCells[fi] = c;
+        faceOff[fi] = o;
+        for (int k = 0; k < n; ++k) st3(pts, o + k, q[k]);
+    }
+}
+
+void launch_plic_surface(const DevMesh& m, const DevState& st, int nMixed, int pass, int* ptOff, int* faceIdx,
+                         int* totals, double* pts, int* faceCells, int* faceOff, hipStream_t s) {
+    if (nMixed <= 0) return;
+    const int grid = std::min(1024, cdiv(nMixed, 64));
+    if (pass == 0) {
+        k_plic_polygons<CellPolyNS><<<grid, 64, 0, s>>>(m, st, nMixed, ptOff, faceIdx, nullptr, nullptr, nullptr);
+        k_scan_counts<<<1, 1024, 0, s>>>(ptOff, nMixed, totals, nullptr);
+        k_scan_counts<<<1, 1024, 0, s>>>(faceIdx, nMixed, totals + 1, nullptr);
+    } else {
+        k_plic_polygons<CellPolyNS><<<grid, 64, 0, s>>>(m, st, nMixed, ptOff, faceIdx, pts, faceCells, faceOff);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Advection
 // ---------------------------------------------------------------------------
 

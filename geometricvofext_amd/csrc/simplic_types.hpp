@@ -170,6 +170,10 @@ enum Stage : int {
 void launch_advect_stage(int stage, const DevMesh& m, const DevState& st, const Params& p, double dt,
                          const double* Sp, const double* Su, hipStream_t s);
 void launch_alpha_phi(const DevMesh& m, const DevState& st, double dt, hipStream_t s);
+// pass 0: polygon sizes and their scans (totals[0] points, totals[1] faces);
+// pass 1: the polygons
+void launch_plic_surface(const DevMesh& m, const DevState& st, int nMixed, int pass, int* ptOff, int* faceIdx,
+                         int* totals, double* pts, int* faceCells, int* faceOff, hipStream_t s);
 void launch_rho_phi(int nF, const double* ap, const double* phi, double rho1, double rho2, double w,
                     double* out, int mode, hipStream_t s);
 int detect_blocks(int nC);

@@ -125,6 +125,18 @@ int svp_n_mixed(svp_handle* h, int32_t* n_mixed);
  * vector S (3).  Any output pointer may be NULL. */
 int svp_get_interface(svp_handle* h, int32_t* cells, int32_t* status, double* n, double* d,
                       double* C, double* S, int where);
+/* PLIC surface of the last reconstruct (reconstruction::interface,
+ * reconstruction.C:787-835, returned as a plicSurface, plicSurface.H:58-100):
+ * every mixed cell is cut again by its plane through its mesh faces, and each
+ * cell the plane cuts gives one polygon, cells ascending.  Its points are the
+ * plane crossings of the cut faces ordered by angle about the interface
+ * centre, near-equal angles (1e-8) merged (cutCell::interfacePoints,
+ * cutCell.C:545-608).  n_faces / n_points are always written; face_cells
+ * (n_faces), face_offsets (n_faces + 1, polygon f is points
+ * [face_offsets[f], face_offsets[f+1])) and points (3 n_points) are filled
+ * when all three are non-NULL (all NULL: size query). */
+int svp_get_plic_surface(svp_handle* h, int32_t* n_faces, int32_t* n_points, int32_t* face_cells,
+                         int32_t* face_offsets, double* points, int where);
 /* Face-cut classification of the final plane cut per mixed cell: 2 bits per
  * cell face ((status+1) << 2*k, cells() face order) and the number of plane
  * crossings (cut vertices) on its cut faces. */

@@ -68,6 +68,7 @@ def lib():
     L.or_get_dvf.argtypes = [vp, _dp]
     L.or_get_alpha_phi.argtypes = [vp, _dp]
     L.or_get_rho_phi.argtypes = [vp, C.c_double, C.c_double, _dp]
+    L.or_plic_surface.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_void_p, C.c_void_p, C.c_void_p]
     L.or_advect_sub_cycles.argtypes = [vp, C.c_double, C.c_int, C.c_void_p, C.c_void_p, C.c_double,
                                        C.c_double, _dp]
     L.or_n_mixed.argtypes = [vp]
@@ -232,6 +233,18 @@ class OracleSolver:
         lib().or_advect_sub_cycles(self.h, dt, int(n_sub), None if sp is None else sp.ctypes.data,
                                    None if su is None else su.ctypes.data, rho1, rho2, a)
         return a
+
+    def plic_surface(self):
+        """PLIC polygons of the last reconstruct (reconstruction.C:787-835):
+        points (n, 3), face offsets (n_faces + 1) and the cell of each face."""
+        nf, npt = C.c_int(), C.c_int()
+        lib().or_plic_surface(self.h, C.byref(nf), C.byref(npt), None, None, None)
+        cells = np.zeros(max(nf.value, 1), np.int32)
+        offs = np.zeros(nf.value + 1, np.int32)
+        pts = np.zeros(3 * max(npt.value, 1))
+        lib().or_plic_surface(self.h, C.byref(nf), C.byref(npt), cells.ctypes.data, offs.ctypes.data,
+                              pts.ctypes.data)
+        return dict(points=pts.reshape(-1, 3)[:npt.value], offsets=offs, cells=cells[:nf.value])
 
     def mixed(self):
         n = lib().or_n_mixed(self.h)

@@ -25,6 +25,10 @@
 //   extendMarkedCells / setDownwindFaces / netFlux / applyBruteForceBounding
 //                                           src/SimPLIC/advection/advection.C:54-82,224-308
 //   limitFlux / boundFlux / advect          src/SimPLIC/advection/advectionTemplates.C:118-418
+//   interfacePoints / interface (PLIC)     src/SimPLIC/cut/cutCell/cutCell.C:545-608,
+//                                           src/SimPLIC/reconstruction/reconstruction.C:787-835
+//   getRhoPhi, sub-cycled alpha step        src/SimPLIC/advection/advection.H:329-343,
+//                                           applications/solvers/interPlicFoam/alphaEqnSubCycle.H:30-60
 //   harness (U, phi, Courant numbers)       applications/test/plicVofAdvectionFoam/updateU.H:1-70,
 //                                           alphaCourantNo.H:87-107
 //
@@ -686,6 +690,42 @@ struct CutCell {
         } else {
             fullySubmerged = false;
         }
+    }
+
+    // Vector::normalise of OpenFOAM: zero below ROOTVSMALL, else divided by
+    // its magnitude component by component
+    static V3 normalised(V3 v) {
+        const double s = mag(v);
+        return s < ROOTVSMALL ? VZERO : v / s;
+    }
+
+    // cutCell.C:545-608: the plane crossings of the cut faces of the last
+    // calcSubCell, ordered by angle about the interface centre in a frame
+    // with z along the interface area vector and x towards the first
+    // crossing; a crossing whose angle lies within 1e-8 of the previous one
+    // in that order is dropped.  sortedOrder is a stable sort.
+    std::vector<V3> interfacePoints() const {
+        std::vector<V3> out;
+        if (cellStatus != 0) return out;
+        const V3 zhat = interfaceArea / mag(interfaceArea);
+        V3 xhat = interfaceEdges[0][0] - interfaceCentre;
+        xhat = xhat - dot(xhat, zhat) * zhat;
+        xhat = normalised(xhat);
+        const V3 yhat = normalised(cross(zhat, xhat));
+        std::vector<V3> pts;
+        std::vector<double> ang;
+        for (const auto& edgePoints : interfaceEdges)
+            for (const V3& p : edgePoints) {
+                pts.push_back(p);
+                ang.push_back(std::atan2(dot(p - interfaceCentre, yhat), dot(p - interfaceCentre, xhat)));
+            }
+        std::vector<int> order(pts.size());
+        std::iota(order.begin(), order.end(), 0);
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return ang[a] < ang[b]; });
+        out.push_back(pts[order[0]]);
+        for (size_t k = 1; k < order.size(); ++k)
+            if (std::fabs(ang[order[k]] - ang[order[k - 1]]) > 1e-8) out.push_back(pts[order[k]]);
+        return out;
     }
 
     // cutCell.C:343-542
@@ -1559,6 +1599,35 @@ void or_advect_sub_cycles(void* h, double dt, int nSub, const double* Sp, const 
         for (int f = 0; f < nF; ++f) sum[f] += w * rp[f];
     }
     std::memcpy(rhoPhi, sum.data(), sizeof(double) * nF);
+}
+
+// reconstruction::interface (reconstruction.C:787-835): every mixed cell of
+// the last reconstruct is cut again by its plane (n, d) through its mesh faces
+// (no warped-face split); one polygon of interfacePoints per cell the plane
+// cuts, cells ascending.  With pts == NULL only the sizes are returned.
+void or_plic_surface(void* h, int* nFaces, int* nPts, int* cells, int* offsets, double* pts) {
+    Solver* s = (Solver*)h;
+    CutCell cc;
+    cc.m = s->m;
+    int nf = 0, np = 0;
+    for (int c : s->mixedCells) {
+        if (cc.calcSubCell(c, s->interfaceN[c], s->interfaceD[c], false) != 0) continue;
+        const std::vector<V3> poly = cc.interfacePoints();
+        if (pts) {
+            cells[nf] = c;
+            offsets[nf] = np;
+            for (size_t k = 0; k < poly.size(); ++k) {
+                pts[3 * (np + k)] = poly[k].x;
+                pts[3 * (np + k) + 1] = poly[k].y;
+                pts[3 * (np + k) + 2] = poly[k].z;
+            }
+        }
+        nf++;
+        np += (int)poly.size();
+    }
+    if (pts) offsets[nf] = np;
+    *nFaces = nf;
+    *nPts = np;
 }
 
 int or_n_mixed(void* h) { return (int)((Solver*)h)->mixedCells.size(); }
