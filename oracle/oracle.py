@@ -68,6 +68,9 @@ def lib():
     L.or_get_dvf.argtypes = [vp, _dp]
     L.or_get_alpha_phi.argtypes = [vp, _dp]
     L.or_get_rho_phi.argtypes = [vp, C.c_double, C.c_double, _dp]
+    L.or_solver_set_rdf.argtypes = [vp, C.c_double, C.c_double, C.c_int]
+    L.or_rdf_iterations_run.argtypes = [vp]
+    L.or_rdf_iterations_run.restype = C.c_int
     L.or_plic_surface.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_void_p, C.c_void_p, C.c_void_p]
     L.or_advect_sub_cycles.argtypes = [vp, C.c_double, C.c_int, C.c_void_p, C.c_void_p, C.c_double,
                                        C.c_double, _dp]
@@ -174,14 +177,19 @@ def ls_grad(pos, val):
 class OracleSolver:
     """Oracle restatement of ``solveVofEqu`` (reconstruct + advect)."""
 
-    ORIENTATIONS = {"LS": 0, "isoAlphaGrad": 0, "alphaGrad": 1}
+    ORIENTATIONS = {"LS": 0, "isoAlphaGrad": 0, "alphaGrad": 1, "isoRDF": 2, "RDF": 2}
 
     def __init__(self, omesh: OracleMesh, mixedCellTol=1e-8, nAlphaBounds=10, snapTol=0.0, clip=True,
-                 splitWarpedFace=False, threads=1, orientationMethod="LS"):
+                 splitWarpedFace=False, threads=1, orientationMethod="LS", tol=1e-6, relTol=0.1, iterations=5):
         self.om = omesh
         self.h = lib().or_solver_new(omesh.h, mixedCellTol, nAlphaBounds, snapTol, int(clip),
                                      int(splitWarpedFace), threads)
         lib().or_solver_set_orientation(self.h, self.ORIENTATIONS[orientationMethod])
+        lib().or_solver_set_rdf(self.h, float(tol), float(relTol), int(iterations))
+
+    def rdf_iterations_run(self):
+        """Iterations of the last isoRDF reconstruct."""
+        return lib().or_rdf_iterations_run(self.h)
 
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:

@@ -961,7 +961,9 @@ struct Solver {
     double snapTol = 0.0;
     bool clip = true;
     bool splitWarpedFace = false;
-    int orientation = 0;  // 0: isoAlphaGrad (LS), 1: alphaGrad
+    int orientation = 0;  // 0: isoAlphaGrad (LS), 1: alphaGrad, 2: isoRDF
+    double rdfTol = 1e-6, rdfRelTol = 0.1;
+    int rdfIterations = 5;
     int nThreads = 1;
     // fields
     std::vector<double> alpha, alphaOld, phi, dVf, alphaPhi;
@@ -1059,6 +1061,156 @@ struct Solver {
         }
     }
 
+    // isoInterfaceGrad (reconstruction.C:144-191): the polyDegree1 least
+    // squares gradient of a cell field over the cell-point-cell stencil of
+    // every mixed cell (not normalised)
+    void isoInterfaceGrad(const std::vector<double>& field, std::vector<V3>& g) const {
+        const int nMixed = (int)mixedCells.size();
+        for (int i = 0; i < nMixed; ++i) {
+            const int celli = mixedCells[i];
+            std::vector<V3> cellCentre;
+            std::vector<double> values;
+            for (int k : m->stencil[celli]) {
+                cellCentre.push_back(m->C[k] - m->C[celli]);
+                values.push_back(field[k]);
+            }
+            g[i] = lsGrad(cellCentre, values);
+        }
+    }
+
+    // Vector::normalise(tol) of OpenFOAM, in place: zero below tol, else
+    // divided by its magnitude component by component
+    static V3& normaliseInPlace(V3& v, double tol) {
+        const double s = mag(v);
+        if (s < tol) v = VZERO;
+        else v = v / s;
+        return v;
+    }
+
+    // reconstruction.C:194-405 (isoRDF, also selected as RDF) with OpenFOAM
+    // v2312's reconstructedDistanceFunction: markCellsNearSurf(mixed, 1) and
+    // constructRDF.  OpenFOAM is a dependency of the reference, not part of
+    // it; updateContactAngle changes nothing without contact-angle patches,
+    // which this mesh description does not have.
+    int rdfIterationsRun = 0;
+    void calcInterfaceNFromIsoRDF() {
+        const double TSMALL_ = 10.0 * SMALL;
+        const int nMixed = (int)mixedCells.size();
+        const int nC = m->nC;
+        std::vector<char> isMixed(nC, 0), next(nC, 0);
+        for (int c : mixedCells) isMixed[c] = 1;
+        // level 0: interface cells, level 1: the cells sharing a point with one
+        for (int c : mixedCells)
+            for (int k : m->stencil[c]) next[k] = 1;
+        std::vector<V3> g(nMixed, VZERO);
+        isoInterfaceGrad(alpha, g);
+        std::fill(interfaceC.begin(), interfaceC.end(), VZERO);
+        std::fill(interfaceN.begin(), interfaceN.end(), VZERO);
+        std::vector<char> tooCoarse(nC, 0);
+        std::vector<double> rdf(nC, 0.0);
+        std::vector<double> normalRes(nMixed), avgAngle(nMixed);
+        rdfIterationsRun = 0;
+        for (int iter = 0; iter < rdfIterations; ++iter) {
+            rdfIterationsRun = iter + 1;
+            for (int i = 0; i < nMixed; ++i) {
+                const int c = mixedCells[i];
+                interfaceN[c] = -normaliseInPlace(g[i], SMALL);
+                if (tooCoarse[c]) continue;
+                double d = interfaceD[c];
+                V3 C = interfaceC[c], S = interfaceS[c];
+                CutCell cc;
+                cc.m = m;
+                cc.cutFace.m = m;
+                int nc = 0;
+                cellStatus[i] = cc.findSignedDistance(c, alpha[c], interfaceN[c], splitWarpedFace, d, C, S, &nc);
+                interfaceD[c] = d;
+                interfaceC[c] = C;
+                interfaceS[c] = S;
+            }
+            // constructRDF(nextToInterface, interfaceC, -interfaceN)
+            for (int c = 0; c < nC; ++c) {
+                if (!next[c]) {
+                    rdf[c] = 0.0;
+                    continue;
+                }
+                const V3 nrm = -interfaceN[c];
+                if (mag(nrm) != 0.0) {
+                    const V3 n = (-nrm) / mag(nrm);
+                    rdf[c] = dot(interfaceC[c] - m->C[c], n);
+                } else {
+                    double averageWeight = 0.0, avgDist = 0.0;
+                    for (int k : m->stencil[c]) {
+                        V3 n = -(-interfaceN[k]);
+                        if (mag(n) != 0.0) {
+                            n = n / mag(n);
+                            V3 dv = interfaceC[k] - m->C[c];
+                            const double distToSurf = dot(dv, n);
+                            double weight = 0.0;
+                            if (mag(dv) != 0.0) {
+                                dv = dv / mag(dv);
+                                const double w = std::fabs(dot(dv, n));
+                                weight = w * w;
+                            } else {
+                                weight = 1.0;
+                            }
+                            averageWeight += weight;
+                            avgDist += distToSurf * weight;
+                        }
+                    }
+                    if (averageWeight != 0.0) rdf[c] = avgDist / averageWeight;
+                }
+            }
+            isoInterfaceGrad(rdf, g);
+            for (int i = 0; i < nMixed; ++i) {
+                const int c = mixedCells[i];
+                // the test normalises g[i] in place only when the first
+                // operand of the 'or' is false; its result is not used
+                if (!(mag(interfaceN[c]) < TSMALL_)) (void)normaliseInPlace(g[i], SMALL);
+                double avgDiffNormal = 0.0, weight = 0.0;
+                const V3 cellNormal = interfaceN[c];
+                const auto& stc = m->stencil[c];
+                for (size_t j = 0; j < stc.size(); ++j) {
+                    const V3 normal = interfaceN[stc[j]];
+                    if (mag(normal) >= TSMALL_ && j != 0) {
+                        const V3 n = normal / mag(normal);
+                        const double cosAngle = std::max(std::min(dot(cellNormal, n), 1.0), -1.0);
+                        avgDiffNormal += std::acos(cosAngle) * mag(normal);
+                        weight += mag(normal);
+                    }
+                }
+                if (weight != 0.0) avgDiffNormal /= weight;
+                else avgDiffNormal = 0.0;
+                const V3 newCellNormal = -normaliseInPlace(g[i], SMALL);
+                normalRes[i] = 1.0 - dot(cellNormal, newCellNormal);
+                avgAngle[i] = avgDiffNormal;
+            }
+            int resCounter = 0;
+            double avgRes = 0.0, avgNormRes = 0.0;
+            for (int i = 0; i < nMixed; ++i) {
+                const int c = mixedCells[i];
+                const double avgA = avgAngle[i];
+                if (avgA > 0.26 && iter > 0) {
+                    tooCoarse[c] = 1;
+                } else {
+                    avgRes += normalRes[i];
+                    const double discreteError = 0.01 * (avgA * avgA);
+                    const double normRes = discreteError != 0.0 ? normalRes[i] / std::max(discreteError, rdfTol)
+                                                                : normalRes[i] / rdfTol;
+                    avgNormRes += normRes;
+                    resCounter++;
+                }
+            }
+            if (resCounter == 0) {
+                resCounter = 1;
+                avgRes = 0.0;
+                avgNormRes = 0.0;
+            }
+            if (((avgNormRes / resCounter < rdfRelTol || avgRes / resCounter < rdfTol) && iter >= 1) ||
+                iter + 1 == rdfIterations)
+                break;
+        }
+    }
+
     // reconstruction.C:680-722
     void reconstruct() {
         const auto t0 = std::chrono::steady_clock::now();
@@ -1068,6 +1220,7 @@ struct Solver {
         faceClass.assign(mixedCells.size(), 0);
         if (!mixedCells.empty()) {
             if (orientation == 1) calcInterfaceNFromRegAlphaGrad();
+            else if (orientation == 2) calcInterfaceNFromIsoRDF();
             else calcInterfaceNFromIsoAlphaGrad();
             const int nMixed = (int)mixedCells.size();
 #pragma omp parallel num_threads(nThreads)
@@ -1534,8 +1687,16 @@ void* or_solver_new(void* mesh, double mixedCellTol, int nAlphaBounds, double sn
 
 void or_solver_free(void* h) { delete (Solver*)h; }
 
-// 0: isoAlphaGrad / LS, 1: alphaGrad
+// 0: isoAlphaGrad / LS, 1: alphaGrad, 2: isoRDF / RDF
 void or_solver_set_orientation(void* h, int method) { ((Solver*)h)->orientation = method; }
+// isoRDF keys tol, relTol, iterations (reconstruction.C:513-515)
+void or_solver_set_rdf(void* h, double tol, double relTol, int iterations) {
+    Solver* s = (Solver*)h;
+    s->rdfTol = tol;
+    s->rdfRelTol = relTol;
+    s->rdfIterations = iterations;
+}
+int or_rdf_iterations_run(void* h) { return ((Solver*)h)->rdfIterationsRun; }
 
 void or_set_alpha(void* h, const double* a) {
     Solver* s = (Solver*)h;
