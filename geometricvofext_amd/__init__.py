@@ -43,6 +43,9 @@ class SvpDict(C.Structure):
         ("split_warped_face", C.c_int32),
         ("write_plic_fields", C.c_int32),
         ("learned_residual_tol", C.c_double),
+        ("rdf_tol", C.c_double),
+        ("rdf_rel_tol", C.c_double),
+        ("rdf_iterations", C.c_int32),
     ]
 
 
@@ -91,6 +94,7 @@ def lib():
     L.svp_n_mixed.argtypes = [vp, C.POINTER(i32)]
     L.svp_get_interface.argtypes = [vp, vp, vp, vp, vp, vp, vp, C.c_int]
     L.svp_get_plic_surface.argtypes = [vp, vp, vp, vp, vp, vp, C.c_int]
+    L.svp_rdf_iterations.argtypes = [vp, vp]
     L.svp_get_cut_class.argtypes = [vp, vp, vp, C.c_int]
     L.svp_get_un0.argtypes = [vp, vp, C.c_int]
     L.svp_get_geometry.argtypes = [vp, vp, vp, vp, vp, vp, vp]
@@ -156,7 +160,8 @@ def make_dict(**kw) -> SvpDict:
     names = {"orientationMethod": "orientation_method", "mixedCellTol": "mixed_cell_tol",
              "nAlphaBounds": "n_alpha_bounds", "snapTol": "snap_tol", "clip": "clip",
              "splitWarpedFace": "split_warped_face", "writePlicFields": "write_plic_fields",
-             "learnedResidualTol": "learned_residual_tol"}
+             "learnedResidualTol": "learned_residual_tol", "tol": "rdf_tol", "relTol": "rdf_rel_tol",
+             "iterations": "rdf_iterations"}
     for k, v in kw.items():
         k = names.get(k, k)
         if k == "orientation_method":
@@ -253,6 +258,12 @@ class SolveVofEqu:
         a = np.zeros(self.mesh.n_faces)
         _check(lib().svp_get_rho_phi(self.h, float(rho1), float(rho2), a.ctypes.data, SVP_HOST))
         return a
+
+    def rdf_iterations(self) -> int:
+        """Iterations of the last isoRDF / RDF reconstruct."""
+        n = C.c_int32()
+        _check(lib().svp_rdf_iterations(self.h, C.byref(n)))
+        return n.value
 
     def n_mixed(self) -> int:
         n = C.c_int32()

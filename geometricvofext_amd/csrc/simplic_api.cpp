@@ -61,6 +61,7 @@ public:
         params_.splitWarpedFace = dict.split_warped_face;
         learned_.active = std::string(dict.orientation_method) == "learned";
         gaussNormal_ = std::string(dict.orientation_method) == "alphaGrad";
+        rdfNormal_ = std::string(dict.orientation_method) == "isoRDF" || std::string(dict.orientation_method) == "RDF";
         learned_.tol = dict.learned_residual_tol;
         buildMesh(mesh);
         allocState();
@@ -88,7 +89,8 @@ public:
             reconstructLearned();
         } else {
             stage(STAGE_NORMAL, [&] {
-                if (gaussNormal_) launch_grad_normal(m_, st_, stream_);
+                if (rdfNormal_) rdfNormals();
+                else if (gaussNormal_) launch_grad_normal(m_, st_, stream_);
                 else launch_ls_normal(m_, st_, stream_);
             });
             stage(STAGE_FSD, [&] { launch_find_signed_distance(m_, st_, params_, stream_); });
@@ -281,6 +283,58 @@ public:
         return st_.alphaPhi;
     }
 
+    // calcInterfaceNFromIsoRDF (reconstruction.C:194-405): the iterations run
+    // on the device; after each the host reads the three residual sums and
+    // applies the stopping rule.  The planes of the cells that are not too
+    // coarse are placed by the findSignedDistance kernels over a work list.
+    void rdfNormals() {
+        const long nC = m_.nC;
+        if (!rdf_.next) {
+            rdf_.next = dalloc<unsigned char>(nC);
+            rdf_.tooCoarse = dalloc<unsigned char>(nC);
+            rdf_.rdf = dalloc<double>(nC);
+            rdf_.cellN = dalloc<double>(3 * nC);
+            rdf_.cellC = dalloc<double>(3 * nC);
+            rdf_.g = dalloc<double>(3 * nC);
+            rdf_.res = dalloc<double>(2 * nC);
+            rdf_.counted = dalloc<int>(nC);
+            rdf_.wl = dalloc<int>(nC);
+            rdf_.wn = dalloc<int>(1);
+            rdf_.sums = dalloc<double>(3);
+        }
+        HIPCHECK(hipMemsetAsync(rdf_.next, 0, nC, stream_));
+        HIPCHECK(hipMemsetAsync(rdf_.tooCoarse, 0, nC, stream_));
+        HIPCHECK(hipMemsetAsync(rdf_.rdf, 0, sizeof(double) * nC, stream_));
+        HIPCHECK(hipMemsetAsync(rdf_.cellN, 0, sizeof(double) * 3 * nC, stream_));
+        HIPCHECK(hipMemsetAsync(rdf_.cellC, 0, sizeof(double) * 3 * nC, stream_));
+        launch_rdf_begin(m_, st_, rdf_, stream_);
+        const double tol = dict_.rdf_tol, relTol = dict_.rdf_rel_tol;
+        const int iterations = dict_.rdf_iterations;
+        rdfIterationsRun_ = 0;
+        for (int iter = 0; iter < iterations; ++iter) {
+            rdfIterationsRun_ = iter + 1;
+            HIPCHECK(hipMemsetAsync(rdf_.wn, 0, sizeof(int), stream_));
+            launch_rdf_normals(m_, st_, rdf_, stream_);
+            launch_find_signed_distance(m_, st_, params_, stream_, rdf_.wl, rdf_.wn);
+            launch_rdf_field(m_, st_, rdf_, stream_);
+            launch_rdf_residual(m_, st_, rdf_, iter, tol, stream_);
+            HIPCHECK(hipGetLastError());
+            double sums[3];
+            HIPCHECK(hipMemcpyAsync(sums, rdf_.sums, sizeof(sums), hipMemcpyDeviceToHost, stream_));
+            HIPCHECK(hipStreamSynchronize(stream_));
+            double avgRes = sums[0], avgNormRes = sums[1];
+            long resCounter = (long)sums[2];
+            if (resCounter == 0) {
+                resCounter = 1;
+                avgRes = 0.0;
+                avgNormRes = 0.0;
+            }
+            if (((avgNormRes / resCounter < relTol || avgRes / resCounter < tol) && iter >= 1) ||
+                iter + 1 == iterations)
+                break;
+        }
+    }
+
     // getRhoPhi with constant densities (advection.H:329-343) of the last
     // advect into the device buffer out.
     void rhoPhi(double rho1, double rho2, double* out) {
@@ -365,6 +419,8 @@ public:
             copyOutT(offs, dOffs, tot[1] + 1L, SVP_HOST);
         }
     }
+
+    int rdfIterationsRun() const { return rdfIterationsRun_; }
 
     // Host sources are staged in device buffers kept by the handle.
     void stageSources(const double*& Sp, const double*& Su, int where) {
@@ -495,6 +551,9 @@ private:
     double reconTime_ = 0.0, advectTime_ = 0.0;
     bool reconstructed_ = false;
     bool gaussNormal_ = false;  // orientationMethod alphaGrad
+    bool rdfNormal_ = false;    // orientationMethod isoRDF / RDF
+    RdfBufs rdf_{};
+    int rdfIterationsRun_ = 0;
     bool alphaPhiValid_ = false;
     int maxCellFaces_ = 0;
     int profMask_ = 0;
@@ -767,6 +826,7 @@ private:
                                                     : (void*)dalloc<BndRec<MAX_CELL_FACES>>(nC);
         st_.un0Fused = (!dict_.split_warped_face && maxCellFaces_ <= 8 && m_.maxCellPoints <= NS_CELL_POINTS &&
                         m_.triQuadFaces) ? 1 : 0;
+        st_.fsdKeep = rdfNormal_ ? 1 : 0;
         st_.minmax = dalloc<double>(4);
         st_.errFlag = dalloc<int>(1);
         st_.nIcTotal = dalloc<unsigned long long>(1);
@@ -803,6 +863,9 @@ void svp_dict_defaults(svp_dict* d) {
     d->split_warped_face = 0;
     d->write_plic_fields = 0;
     d->learned_residual_tol = 0.25;
+    d->rdf_tol = 1e-6;
+    d->rdf_rel_tol = 0.1;
+    d->rdf_iterations = 5;
 }
 
 // OpenFOAM dictionary entries "key value;" with comments; sub-dictionaries
@@ -860,15 +923,15 @@ int svp_dict_parse(const char* text, svp_dict* out) {
             else if (key == "splitWarpedFace") d.split_warped_face = toBool(val);
             else if (key == "writePlicFields") d.write_plic_fields = toBool(val);
             else if (key == "learnedResidualTol") d.learned_residual_tol = std::stod(val);
+            else if (key == "tol") d.rdf_tol = std::stod(val);
+            else if (key == "relTol") d.rdf_rel_tol = std::stod(val);
+            else if (key == "iterations") d.rdf_iterations = std::stoi(val);
         }
         if (!method.empty()) {
             if (method == "LS" || method == "isoAlphaGrad" || method == "alphaGrad" || method == "learned" ||
-                method == "MLP") {
+                method == "MLP" || method == "isoRDF" || method == "RDF") {
                 std::snprintf(d.orientation_method, sizeof d.orientation_method, "%s",
                               method == "MLP" ? "learned" : method.c_str());
-            } else if (method == "isoRDF" || method == "RDF") {
-                throw SvpError(SVP_E_UNSUPPORTED,
-                               "orientationMethod " + method + " is not provided (LS/isoAlphaGrad/alphaGrad are)");
             } else {
                 throw SvpError(SVP_E_DICT, "Invalid orientationMethod " + method +
                                                ". Valid methods are: alphaGrad isoAlphaGrad isoRDF LS RDF learned");
@@ -885,13 +948,12 @@ int svp_create(const svp_mesh* mesh, const svp_dict* dict, int device, svp_handl
         if (dict) d = *dict;
         else svp_dict_defaults(&d);
         const std::string m(d.orientation_method);
-        if (m != "LS" && m != "isoAlphaGrad" && m != "alphaGrad" && m != "learned") {
-            if (m == "isoRDF" || m == "RDF")
-                throw SvpError(SVP_E_UNSUPPORTED, "orientationMethod " + m + " is not provided");
+        if (m != "LS" && m != "isoAlphaGrad" && m != "alphaGrad" && m != "learned" && m != "isoRDF" && m != "RDF")
             throw SvpError(SVP_E_DICT, "Invalid orientationMethod " + m);
-        }
         if (d.n_alpha_bounds < 0 || !(d.mixed_cell_tol >= 0.0) || d.snap_tol < 0.0 || !(d.learned_residual_tol >= 0.0))
             throw SvpError(SVP_E_DICT, "invalid tolerance or bound count");
+        if ((m == "isoRDF" || m == "RDF") && (!(d.rdf_tol >= 0.0) || !(d.rdf_rel_tol >= 0.0) || d.rdf_iterations < 0))
+            throw SvpError(SVP_E_DICT, "invalid isoRDF tol, relTol or iterations");
         auto h = std::make_unique<svp_handle>();
         h->s = std::make_unique<solveVofEqu>(*mesh, d, device);
         *out = h.release();
@@ -954,6 +1016,13 @@ int svp_get_rho_phi(svp_handle* h, double rho1, double rho2, double* out, int wh
             s.rhoPhi(rho1, rho2, (double*)s.rhoPhiSum_.p);
             s.copyOut(out, (const double*)s.rhoPhiSum_.p, s.m_.nF, SVP_HOST);
         }
+    });
+}
+
+int svp_rdf_iterations(svp_handle* h, int32_t* n) {
+    return guarded([&] {
+        if (!n) throw SvpError(SVP_E_ARG, "null output");
+        *n = H(h).rdfIterationsRun();
     });
 }
 

@@ -393,6 +393,193 @@ void launch_grad_normal(const DevMesh& m, const DevState& st, hipStream_t s) {
 }
 
 // ---------------------------------------------------------------------------
+// isoRDF / RDF normals (reconstruction.C:194-405) with OpenFOAM's
+// reconstructedDistanceFunction (markCellsNearSurf with one layer,
+// constructRDF); one lane per mixed cell or per cell
+// ---------------------------------------------------------------------------
+// Vector::normalise(tol) of OpenFOAM
+__device__ inline V3 normalise_tol(V3 v, double tol) {
+    const double s = mag(v);
+    return s < tol ? VZERO : v / s;
+}
+
+__global__ void k_rdf_begin(DevMesh m, DevState st, RdfBufs r) {
+    const int nMixed = *st.nMixed;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nMixed; i += gridDim.x * blockDim.x) {
+        const int c = st.mixed[i];
+        // level 0 and level 1 of markCellsNearSurf: the stencil holds the cell
+        // and every cell sharing a point with it
+        for (int k = m.stOff[c]; k < m.stOff[c + 1]; ++k) r.next[m.st[k]] = 1;
+        st.ifD[i] = 0.0;
+        st3(st.ifN, i, VZERO);
+        st3(st.ifC, i, VZERO);
+        st3(st.ifS, i, VZERO);
+        st3(r.g, i, ls_gradient(m, st.alpha, c));
+    }
+}
+
+__global__ void k_rdf_normals(DevState st, RdfBufs r) {
+    const int nMixed = *st.nMixed;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nMixed; i += gridDim.x * blockDim.x) {
+        const int c = st.mixed[i];
+        const V3 gi = normalise_tol(ld3(r.g, i), SVP_SMALL);
+        st3(r.g, i, gi);
+        const V3 n = -gi;
+        st3(st.ifN, i, n);
+        st3(r.cellN, c, n);
+        if (!r.tooCoarse[c]) r.wl[atomicAdd(r.wn, 1)] = i;
+    }
+}
+
+__global__ void k_rdf_scatter_c(DevState st, RdfBufs r) {
+    const int nMixed = *st.nMixed;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nMixed; i += gridDim.x * blockDim.x)
+        st3(r.cellC, st.mixed[i], ld3(st.ifC, i));
+}
+
+// constructRDF(nextToInterface, interfaceC, -interfaceN): the signed distance
+// to the cell's own plane in an interface cell, else the average over the
+// stencil's planes weighted by the squared cosine between the plane normal
+// and the direction to the plane centre; cells away from the interface get 0
+// and a cell with no weight keeps its value.
+__global__ void k_rdf_construct(DevMesh m, RdfBufs r) {
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < m.nC; c += gridDim.x * blockDim.x) {
+        if (!r.next[c]) {
+            r.rdf[c] = 0.0;
+            continue;
+        }
+        const V3 cc = ld3(m.C, c);
+        const V3 nrm = -ld3(r.cellN, c);
+        if (mag(nrm) != 0.0) {
+            const V3 n = (-nrm) / mag(nrm);
+            r.rdf[c] = dot(ld3(r.cellC, c) - cc, n);
+            continue;
+        }
+        double averageWeight = 0.0, avgDist = 0.0;
+        for (int k = m.stOff[c]; k < m.stOff[c + 1]; ++k) {
+            const int o = m.st[k];
+            V3 n = -(-ld3(r.cellN, o));
+            if (mag(n) != 0.0) {
+                n = n / mag(n);
+                V3 dv = ld3(r.cellC, o) - cc;
+                const double distToSurf = dot(dv, n);
+                double weight;
+                if (mag(dv) != 0.0) {
+                    dv = dv / mag(dv);
+                    const double w = fabs(dot(dv, n));
+                    weight = w * w;
+                } else {
+                    weight = 1.0;
+                }
+                averageWeight += weight;
+                avgDist += distToSurf * weight;
+            }
+        }
+        if (averageWeight != 0.0) r.rdf[c] = avgDist / averageWeight;
+    }
+}
+
+__global__ void k_rdf_grad(DevMesh m, DevState st, RdfBufs r) {
+    const int nMixed = *st.nMixed;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nMixed; i += gridDim.x * blockDim.x)
+        st3(r.g, i, ls_gradient(m, r.rdf, st.mixed[i]));
+}
+
+// Residual of one mixed cell (reconstruction.C:276-330) and its share of
+// the averages (:333-357).  g is normalised in place as the reference does:
+// once inside the zero-normal test when the cell's normal is not zero, once
+// for the new normal.
+__global__ void k_rdf_residual(DevMesh m, DevState st, RdfBufs r, int iter, double tol) {
+    const int nMixed = *st.nMixed;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nMixed; i += gridDim.x * blockDim.x) {
+        const int c = st.mixed[i];
+        const V3 cellNormal = ld3(r.cellN, c);
+        V3 gi = ld3(r.g, i);
+        if (!(mag(cellNormal) < SVP_TSMALL)) gi = normalise_tol(gi, SVP_SMALL);
+        double avgDiffNormal = 0.0, weight = 0.0;
+        for (int k = m.stOff[c] + 1; k < m.stOff[c + 1]; ++k) {
+            const V3 normal = ld3(r.cellN, m.st[k]);
+            if (mag(normal) >= SVP_TSMALL) {
+                const V3 n = normal / mag(normal);
+                const double cosAngle = fmax(fmin(dot(cellNormal, n), 1.0), -1.0);
+                avgDiffNormal += acos(cosAngle) * mag(normal);
+                weight += mag(normal);
+            }
+        }
+        avgDiffNormal = weight != 0.0 ? avgDiffNormal / weight : 0.0;
+        gi = normalise_tol(gi, SVP_SMALL);
+        st3(r.g, i, gi);
+        const double normalRes = 1.0 - dot(cellNormal, -gi);
+        if (avgDiffNormal > 0.26 && iter > 0) {
+            r.tooCoarse[c] = 1;
+            r.res[2 * i] = 0.0;
+            r.res[2 * i + 1] = 0.0;
+            r.counted[i] = 0;
+        } else {
+            const double discreteError = 0.01 * (avgDiffNormal * avgDiffNormal);
+            const double normRes = discreteError != 0.0 ? normalRes / fmax(discreteError, tol) : normalRes / tol;
+            r.res[2 * i] = normalRes;
+            r.res[2 * i + 1] = normRes;
+            r.counted[i] = 1;
+        }
+    }
+}
+
+// sums of the residual terms, one workgroup (a fixed order: the sums only
+// feed the stopping test)
+__global__ void __launch_bounds__(1024) k_rdf_reduce(DevState st, RdfBufs r) {
+    __shared__ double s0[1024], s1[1024], s2[1024];
+    const int nMixed = *st.nMixed;
+    const int t = threadIdx.x;
+    double a = 0.0, b = 0.0, n = 0.0;
+    for (int i = t; i < nMixed; i += 1024) {
+        a += r.res[2 * i];
+        b += r.res[2 * i + 1];
+        n += (double)r.counted[i];
+    }
+    s0[t] = a;
+    s1[t] = b;
+    s2[t] = n;
+    __syncthreads();
+    for (int w = 512; w > 0; w >>= 1) {
+        if (t < w) {
+            s0[t] += s0[t + w];
+            s1[t] += s1[t + w];
+            s2[t] += s2[t + w];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        r.sums[0] = s0[0];
+        r.sums[1] = s1[0];
+        r.sums[2] = s2[0];
+    }
+}
+
+// the caller clears next, tooCoarse, rdf, cellN and cellC first
+void launch_rdf_begin(const DevMesh& m, const DevState& st, const RdfBufs& r, hipStream_t s) {
+    k_rdf_begin<<<1024, 64, 0, s>>>(m, st, r);
+}
+
+// the caller clears wn first
+void launch_rdf_normals(const DevMesh& m, const DevState& st, const RdfBufs& r, hipStream_t s) {
+    (void)m;
+    k_rdf_normals<<<1024, 256, 0, s>>>(st, r);
+}
+
+void launch_rdf_field(const DevMesh& m, const DevState& st, const RdfBufs& r, hipStream_t s) {
+    k_rdf_scatter_c<<<1024, 256, 0, s>>>(st, r);
+    k_rdf_construct<<<std::min(4096, cdiv(m.nC, 256)), 256, 0, s>>>(m, r);
+    k_rdf_grad<<<1024, 64, 0, s>>>(m, st, r);
+}
+
+void launch_rdf_residual(const DevMesh& m, const DevState& st, const RdfBufs& r, int iter, double tol,
+                         hipStream_t s) {
+    k_rdf_residual<<<1024, 64, 0, s>>>(m, st, r, iter, tol);
+    k_rdf_reduce<<<1, 1024, 0, s>>>(st, r);
+}
+
+// ---------------------------------------------------------------------------
 // Local polyhedron of a cell (cutCell.C:140-236)
 // ---------------------------------------------------------------------------
 template <class Poly>
@@ -590,10 +777,13 @@ __global__ void __launch_bounds__(64) k_find_signed_distance(DevMesh m, DevState
         st.status[i] = o.status;
         st.nCutPts[i] = o.nCutPts;
         st.faceClass[i] = o.faceClass;
-        // interfaceD/C/S stay zero where the reference returns before writing
-        st.ifD[i] = o.wrote ? o.d : 0.0;
-        st3(st.ifC, i, o.wrote ? o.C : VZERO);
-        st3(st.ifS, i, o.wrote ? o.S : VZERO);
+        // interfaceD/C/S stay as they were where the reference returns before
+        // writing: zero after initialize, the last value inside the isoRDF loop
+        if (o.wrote || !st.fsdKeep) {
+            st.ifD[i] = o.wrote ? o.d : 0.0;
+            st3(st.ifC, i, o.wrote ? o.C : VZERO);
+            st3(st.ifS, i, o.wrote ? o.S : VZERO);
+        }
     }
 }
 
@@ -640,9 +830,11 @@ __global__ void __launch_bounds__(64, SVP_FSD_WAVES) k_fsd_group(DevMesh m, DevS
                 st.status[i] = (int)sgn(0.5 - alphaI);
                 st.nCutPts[i] = 0;
                 st.faceClass[i] = 0ull;
-                st.ifD[i] = 0.0;
-                st3(st.ifC, i, VZERO);
-                st3(st.ifS, i, VZERO);
+                if (!st.fsdKeep) {
+                    st.ifD[i] = 0.0;
+                    st3(st.ifC, i, VZERO);
+                    st3(st.ifS, i, VZERO);
+                }
                 st.un0[i] = 0.0;
             }
             continue;

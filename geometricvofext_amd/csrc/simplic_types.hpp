@@ -83,6 +83,7 @@ struct DevState {
     unsigned long long* boundTotals;  // [5] running sums: candidates, active, rounds, corrected faces, passes
     int* bcount;        // [16] per-pass counters of the bounding kernels
     int un0Fused;       // plane placement by 8-lane groups, which also computes Un0
+    int fsdKeep;        // 1: where findSignedDistance writes no d/C/S they keep their values (isoRDF)
     int gridRounds;     // bounding rounds run as grid-wide kernels before the one-workgroup tail
     int bndGrid;        // workgroups of the grid-stride bounding kernels
     int bndFlow;        // workgroups of the one-launch dataflow pass, 0 = rounds and tail
@@ -153,6 +154,32 @@ void launch_ls_normal(const DevMesh& m, const DevState& st, hipStream_t s, const
                       const int* wn = nullptr);
 // alphaGrad normal: minus the Gauss-linear gradient of alpha, normalised
 void launch_grad_normal(const DevMesh& m, const DevState& st, hipStream_t s);
+
+// isoRDF work buffers (reconstruction.C:194-405)
+struct RdfBufs {
+    unsigned char* next;       // nC: nextToInterface (mixed cells and the cells sharing a point with one)
+    unsigned char* tooCoarse;  // nC
+    double* rdf;               // nC: reconstructed distance function
+    double* cellN;             // 3*nC: interfaceN per cell
+    double* cellC;             // 3*nC: interfaceC per cell
+    double* g;                 // 3*nC: per mixed cell, the least-squares gradient (interfaceNormal)
+    double* res;               // 2*nC: per mixed cell, its normalRes and normRes terms
+    int* counted;              // nC: per mixed cell, 1 when it enters the residual averages
+    int* wl;                   // nC: mixed cells that are not too coarse
+    int* wn;                   // [1]
+    double* sums;              // [3]: avgRes, avgNormRes and resCounter before the division
+};
+// (next, tooCoarse, rdf, cellN and cellC cleared by the caller) zero d/C/S/n
+// of the mixed cells, mark nextToInterface, g = least-squares gradient of alpha
+void launch_rdf_begin(const DevMesh& m, const DevState& st, const RdfBufs& r, hipStream_t s);
+// normals from g (in place), and the work list of cells that are not too
+// coarse (wn cleared by the caller)
+void launch_rdf_normals(const DevMesh& m, const DevState& st, const RdfBufs& r, hipStream_t s);
+// constructRDF from the planes, then g = least-squares gradient of it
+void launch_rdf_field(const DevMesh& m, const DevState& st, const RdfBufs& r, hipStream_t s);
+// residuals, too-coarse marks and their sums
+void launch_rdf_residual(const DevMesh& m, const DevState& st, const RdfBufs& r, int iter, double tol,
+                         hipStream_t s);
 void launch_find_signed_distance(const DevMesh& m, const DevState& st, const Params& p, hipStream_t s,
                                  const int* wl = nullptr, const int* wn = nullptr);
 // pipeline stages of one step, in launch order (also the profiler's slots)

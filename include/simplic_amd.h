@@ -57,7 +57,7 @@ typedef struct {
 /* The fvSolution solvers{"alpha.*"{...}} keys the reference reads
  * (reconstruction.C:502-516, advection.C:455-457). */
 typedef struct {
-    char orientation_method[32];   /* "isoAlphaGrad" (default), "LS", or "learned" (simplic_learned.h) */
+    char orientation_method[32];   /* "isoAlphaGrad" (default), "LS", "alphaGrad", "isoRDF", "RDF" or "learned" (simplic_learned.h) */
     double mixed_cell_tol;         /* 1e-8 */
     int32_t n_alpha_bounds;        /* 10 */
     double snap_tol;               /* 0 */
@@ -65,6 +65,9 @@ typedef struct {
     int32_t split_warped_face;     /* 0 */
     int32_t write_plic_fields;     /* 0 */
     double learned_residual_tol;   /* learnedResidualTol, "learned" method only: 0.25 (>= 1: no fallback test) */
+    double rdf_tol;                /* tol, isoRDF only: 1e-6 (reconstruction.C:513) */
+    double rdf_rel_tol;            /* relTol, isoRDF only: 0.1 (reconstruction.C:514) */
+    int32_t rdf_iterations;        /* iterations, isoRDF only: 5 (reconstruction.C:515) */
 } svp_dict;
 
 /* Defaults of every key (reconstruction.C:502-516, advection.C:455-457). */
@@ -74,8 +77,11 @@ void svp_dict_defaults(svp_dict* d);
  * C/C++ comments allowed).  Unknown keys are ignored, as OpenFOAM does.
  * `reconstructionScheme` is accepted as an alias of `orientationMethod`.
  * Returns SVP_E_DICT for an unknown orientation method
- * (reconstruction.C:610-625) and SVP_E_UNSUPPORTED for isoRDF / RDF.
- * alphaGrad is the Gauss-linear gradient normal (reconstruction.C:74-82).
+ * (reconstruction.C:610-625).  alphaGrad is the Gauss-linear gradient normal
+ * (reconstruction.C:74-82); isoRDF / RDF the reconstructed-distance-function
+ * normal (reconstruction.C:194-405) with keys tol, relTol and iterations;
+ * contact-angle boundaries (updateContactAngle) are not part of this mesh
+ * description, so its boundary correction is not applied.
  * "learned" (alias "MLP") selects the learned normal of simplic_learned.h. */
 int svp_dict_parse(const char* text, svp_dict* out);
 
@@ -118,6 +124,10 @@ int svp_get_rho_phi(svp_handle* h, double rho1, double rho2, double* rho_phi, in
  * svp_get_alpha / svp_set_alpha). */
 int svp_advect_sub_cycles(svp_handle* h, double dt, int32_t n_sub, const double* Sp, const double* Su,
                           double rho1, double rho2, double* rho_phi, int where);
+/* Iterations the last isoRDF / RDF reconstruct ran (the loop of
+ * reconstruction.C:236-400); no counterpart in the reference, which does not
+ * report it.  0 for the other orientation methods. */
+int svp_rdf_iterations(svp_handle* h, int32_t* n);
 /* Number of mixed cells of the last reconstruct (reconstruction.C:675-676). */
 int svp_n_mixed(svp_handle* h, int32_t* n_mixed);
 /* Interface data of the last reconstruct per mixed cell (ascending cells):

@@ -72,9 +72,11 @@ def test_dict_reconstruction_scheme_alias_and_errors():
     assert e.value.code == -2                             # FatalError in the reference
     assert g.parse_dict("orientationMethod alphaGrad;").orientation_method == b"alphaGrad"
     for m in ("isoRDF", "RDF"):
-        with pytest.raises(g.SvpError) as e:
-            g.parse_dict(f"orientationMethod {m};")
-        assert e.value.code == -3
+        d = g.parse_dict(f"orientationMethod {m}; tol 1e-7; relTol 0.05; iterations 8;")
+        assert d.orientation_method == m.encode()
+        assert d.rdf_tol == 1e-7 and d.rdf_rel_tol == 0.05 and d.rdf_iterations == 8
+    d = g.parse_dict("orientationMethod RDF;")
+    assert d.rdf_tol == 1e-6 and d.rdf_rel_tol == 0.1 and d.rdf_iterations == 5   # reconstruction.C:513-515
     with pytest.raises(g.SvpError):
         g.parse_dict("clip maybe;")
 

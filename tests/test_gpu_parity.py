@@ -310,3 +310,35 @@ def test_parity_plic_surface(oracle_lib):
         gpu.reconstruct()
         ors.reconstruct()
         _compare_plic(gpu, ors, mesh.name)
+
+
+def test_parity_rdf(oracle_lib):
+    """orientationMethod RDF (isoRDF, reconstruction.C:194-405) against the
+    oracle on perturbed hex and tets, with a sphere of two cells' radius whose
+    normals turn by more than 15 degrees between neighbours (too-coarse
+    cells), over deformation steps; then the iteration count of one more
+    reconstruct."""
+    import geometricvofext_amd as g
+    for mesh, n in ((g.hex_box(20, perturb=0.2, seed=11), 20), (g.tet_box(8, perturb=0.1, seed=12), 8)):
+        _say("RDF", mesh.name)
+        om = oracle_lib.OracleMesh(mesh)
+        a0 = ramp_alpha(om.C, [((0.5, 0.5, 0.4), 0.22), ((0.3, 0.7, 0.75), 2.0 / n)], 1.2 / n)
+        _, gpu, ors = _run_parity(mesh, oracle_lib, a0, 3, nAlphaBounds=3, orientationMethod="RDF")
+        gpu.reconstruct()
+        ors.reconstruct()
+        exact, nm = _compare_interface(gpu, ors, "final")
+        assert gpu.rdf_iterations() == ors.rdf_iterations_run() >= 2
+        _say(f"{mesh.name}: {nm} mixed cells, {gpu.rdf_iterations()} RDF iterations, bit-exact={exact}")
+    # tight tolerances: every iteration runs, with too-coarse cells skipped
+    mesh = g.hex_box(16, perturb=0.15, seed=13)
+    om = oracle_lib.OracleMesh(mesh)
+    a0 = ramp_alpha(om.C, [((0.5, 0.5, 0.5), 0.25), ((0.25, 0.25, 0.7), 2.0 / 16)], 1.2 / 16)
+    gpu, _, _ = _pair(mesh, oracle_lib, orientationMethod="RDF", iterations=6, tol=1e-15, relTol=1e-15)
+    ors = oracle_lib.OracleSolver(om, orientationMethod="RDF", iterations=6, tol=1e-15, relTol=1e-15)
+    gpu.set_alpha(a0)
+    ors.set_alpha(a0)
+    gpu.reconstruct()
+    ors.reconstruct()
+    exact, nm = _compare_interface(gpu, ors, "six iterations")
+    assert gpu.rdf_iterations() == ors.rdf_iterations_run() == 6
+    _say(f"{mesh.name}: {nm} mixed cells, 6 RDF iterations, bit-exact={exact}")
