@@ -95,6 +95,7 @@ def lib():
     L.svp_get_interface.argtypes = [vp, vp, vp, vp, vp, vp, vp, C.c_int]
     L.svp_get_plic_surface.argtypes = [vp, vp, vp, vp, vp, vp, C.c_int]
     L.svp_rdf_iterations.argtypes = [vp, vp]
+    L.svp_map_alpha_field.argtypes = [vp, vp, vp, dbl, dbl, C.c_int]
     L.svp_get_cut_class.argtypes = [vp, vp, vp, C.c_int]
     L.svp_get_un0.argtypes = [vp, vp, C.c_int]
     L.svp_get_geometry.argtypes = [vp, vp, vp, vp, vp, vp, vp]
@@ -258,6 +259,17 @@ class SolveVofEqu:
         a = np.zeros(self.mesh.n_faces)
         _check(lib().svp_get_rho_phi(self.h, float(rho1), float(rho2), a.ctypes.data, SVP_HOST))
         return a
+
+    def map_alpha_field(self, n, d, lower_refine_level, upper_refine_level):
+        """mapAlphaField (reconstruction.C:725-784): cells with alpha in the
+        refine levels take the volume fraction of the plane (n, d) given per
+        cell (n: (n_cells, 3), d: (n_cells,))."""
+        n = np.ascontiguousarray(np.asarray(n, dtype=np.float64).reshape(-1))
+        d = np.ascontiguousarray(d, dtype=np.float64)
+        if n.size != 3 * self.mesh.n_cells or d.size != self.mesh.n_cells:
+            raise ValueError("n must be (n_cells, 3) and d (n_cells,)")
+        _check(lib().svp_map_alpha_field(self.h, n.ctypes.data, d.ctypes.data, float(lower_refine_level),
+                                         float(upper_refine_level), SVP_HOST))
 
     def rdf_iterations(self) -> int:
         """Iterations of the last isoRDF / RDF reconstruct."""

@@ -422,6 +422,29 @@ public:
 
     int rdfIterationsRun() const { return rdfIterationsRun_; }
 
+    // reconstruction::mapAlphaField (reconstruction.C:725-784) after a mesh
+    // change: n (3 per cell) and d (per cell) are the interface fields the
+    // caller mapped onto this mesh.
+    void mapAlphaField(const double* n, const double* d, double lower, double upper, int where) {
+        if (!n || !d) throw SvpError(SVP_E_ARG, "svp_map_alpha_field: null n or d");
+        if (!std::isfinite(lower) || !std::isfinite(upper)) throw SvpError(SVP_E_ARG, "refine levels must be finite");
+        const double* dn = n;
+        const double* dd = d;
+        if (where != SVP_DEVICE) {
+            if (mapBuf_.bytes < sizeof(double) * 4 * (size_t)m_.nC) mapBuf_.alloc<double>(4L * m_.nC);
+            double* b = (double*)mapBuf_.p;
+            copyIn(b, n, 3L * m_.nC, SVP_HOST);
+            copyIn(b + 3L * m_.nC, d, m_.nC, SVP_HOST);
+            dn = b;
+            dd = b + 3L * m_.nC;
+        }
+        launch_map_alpha(m_, st_, dn, dd, lower, upper, stream_);
+        HIPCHECK(hipGetLastError());
+        HIPCHECK(hipStreamSynchronize(stream_));
+        checkDeviceErrors();
+        reconstructed_ = false;
+    }
+
     // Host sources are staged in device buffers kept by the handle.
     void stageSources(const double*& Sp, const double*& Su, int where) {
         if (where != SVP_HOST) return;
@@ -541,7 +564,7 @@ public:
     double* allocHarness(DBuf& b, long n) { return b.alloc<double>(n); }
     DBuf harnessBufs_[3];
     DBuf rhoPhiSum_;
-    DBuf plicIdx_, plicOut_;
+    DBuf plicIdx_, plicOut_, mapBuf_;
 
 private:
     std::vector<std::unique_ptr<DBuf>> bufs_;
@@ -1017,6 +1040,11 @@ int svp_get_rho_phi(svp_handle* h, double rho1, double rho2, double* out, int wh
             s.copyOut(out, (const double*)s.rhoPhiSum_.p, s.m_.nF, SVP_HOST);
         }
     });
+}
+
+int svp_map_alpha_field(svp_handle* h, const double* n, const double* d, double lower_refine_level,
+                        double upper_refine_level, int where) {
+    return guarded([&] { H(h).mapAlphaField(n, d, lower_refine_level, upper_refine_level, where); });
 }
 
 int svp_rdf_iterations(svp_handle* h, int32_t* n) {

@@ -1163,6 +1163,32 @@ __global__ void __launch_bounds__(64) k_plic_polygons(DevMesh m, DevState st, in
     }
 }
 
+// reconstruction::mapAlphaField (reconstruction.C:725-784): every cell whose
+// alpha lies in [lower, upper] takes the volume fraction of its plane (n, d)
+// cut through its mesh faces (no warped-face split); n and d are per cell.
+__global__ void __launch_bounds__(64) k_map_alpha(DevMesh m, DevState st, const double* __restrict__ n,
+                                                  const double* __restrict__ d, double lower, double upper) {
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < m.nC; c += gridDim.x * blockDim.x) {
+        const double a = st.alpha[c];
+        if (!(a >= lower && a <= upper)) continue;
+        CellPolyNS P;
+        FaceCut fc[CellPolyNS::maxFaces];
+        CellCut cut;
+        if (!build_cell_poly(m, c, false, P)) {
+            atomicOr(st.errFlag, ERR_CELL_TOO_LARGE);
+            continue;
+        }
+        if (!calcSubCell(P, ld3(m.C, c), m.V[c], ld3(n, c), d[c], fc, cut))
+            atomicOr(st.errFlag, ERR_TOO_MANY_CROSSINGS);
+        st.alpha[c] = cut.VOF;
+    }
+}
+
+void launch_map_alpha(const DevMesh& m, const DevState& st, const double* n, const double* d, double lower,
+                      double upper, hipStream_t s) {
+    if (m.nC > 0) k_map_alpha<<<std::min(4096, cdiv(m.nC, 64)), 64, 0, s>>>(m, st, n, d, lower, upper);
+}
+
 void launch_plic_surface(const DevMesh& m, const DevState& st, int nMixed, int pass, int* ptOff, int* faceIdx,
                          int* totals, double* pts, int* faceCells, int* faceOff, hipStream_t s) {
     if (nMixed <= 0) return;

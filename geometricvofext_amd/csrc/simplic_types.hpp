@@ -197,6 +197,8 @@ enum Stage : int {
 void launch_advect_stage(int stage, const DevMesh& m, const DevState& st, const Params& p, double dt,
                          const double* Sp, const double* Su, hipStream_t s);
 void launch_alpha_phi(const DevMesh& m, const DevState& st, double dt, hipStream_t s);
+void launch_map_alpha(const DevMesh& m, const DevState& st, const double* n, const double* d, double lower,
+                      double upper, hipStream_t s);
 // pass 0: polygon sizes and their scans (totals[0] points, totals[1] faces);
 // pass 1: the polygons
 void launch_plic_surface(const DevMesh& m, const DevState& st, int nMixed, int pass, int* ptOff, int* faceIdx,

@@ -124,6 +124,16 @@ int svp_get_rho_phi(svp_handle* h, double rho1, double rho2, double* rho_phi, in
  * svp_get_alpha / svp_set_alpha). */
 int svp_advect_sub_cycles(svp_handle* h, double dt, int32_t n_sub, const double* Sp, const double* Su,
                           double rho1, double rho2, double* rho_phi, int where);
+/* solveVofEqu::mapAlphaField (solveVofEqu.C:102-105 -> reconstruction.C:725-784)
+ * after a mesh refinement: every cell whose alpha lies in
+ * [lower_refine_level, upper_refine_level] (dynamicRefineFvMeshCoeffs) takes
+ * the volume fraction of the plane n.x + d = 0 in it, cut through its mesh
+ * faces.  n (3 per cell) and d (per cell) are interfaceN / interfaceD as the
+ * caller's mesh mapper carried them onto this mesh; the reference's own
+ * guards (dynamicRefineFvMesh, mesh changing, mapAlphaField true) are the
+ * caller's. */
+int svp_map_alpha_field(svp_handle* h, const double* n, const double* d, double lower_refine_level,
+                        double upper_refine_level, int where);
 /* Iterations the last isoRDF / RDF reconstruct ran (the loop of
  * reconstruction.C:236-400); no counterpart in the reference, which does not
  * report it.  0 for the other orientation methods. */

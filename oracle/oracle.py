@@ -68,6 +68,7 @@ def lib():
     L.or_get_dvf.argtypes = [vp, _dp]
     L.or_get_alpha_phi.argtypes = [vp, _dp]
     L.or_get_rho_phi.argtypes = [vp, C.c_double, C.c_double, _dp]
+    L.or_map_alpha_field.argtypes = [vp, _dp, _dp, C.c_double, C.c_double]
     L.or_solver_set_rdf.argtypes = [vp, C.c_double, C.c_double, C.c_int]
     L.or_rdf_iterations_run.argtypes = [vp]
     L.or_rdf_iterations_run.restype = C.c_int
@@ -186,6 +187,11 @@ class OracleSolver:
                                      int(splitWarpedFace), threads)
         lib().or_solver_set_orientation(self.h, self.ORIENTATIONS[orientationMethod])
         lib().or_solver_set_rdf(self.h, float(tol), float(relTol), int(iterations))
+
+    def map_alpha_field(self, n, d, lower, upper):
+        """mapAlphaField (reconstruction.C:725-784) with per-cell n, d."""
+        lib().or_map_alpha_field(self.h, _c(np.asarray(n).reshape(-1), np.float64), _c(d, np.float64),
+                                 float(lower), float(upper))
 
     def rdf_iterations_run(self):
         """Iterations of the last isoRDF reconstruct."""

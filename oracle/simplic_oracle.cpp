@@ -27,6 +27,7 @@
 //   limitFlux / boundFlux / advect          src/SimPLIC/advection/advectionTemplates.C:118-418
 //   interfacePoints / interface (PLIC)     src/SimPLIC/cut/cutCell/cutCell.C:545-608,
 //                                           src/SimPLIC/reconstruction/reconstruction.C:787-835
+//   mapAlphaField                           src/SimPLIC/reconstruction/reconstruction.C:725-784
 //   getRhoPhi, sub-cycled alpha step        src/SimPLIC/advection/advection.H:329-343,
 //                                           applications/solvers/interPlicFoam/alphaEqnSubCycle.H:30-60
 //   harness (U, phi, Courant numbers)       applications/test/plicVofAdvectionFoam/updateU.H:1-70,
@@ -1789,6 +1790,22 @@ void or_plic_surface(void* h, int* nFaces, int* nPts, int* cells, int* offsets, 
     if (pts) offsets[nf] = np;
     *nFaces = nf;
     *nPts = np;
+}
+
+// reconstruction::mapAlphaField (reconstruction.C:725-784) with per-cell
+// n (3 per cell) and d: cells with lower <= alpha <= upper take the volume
+// fraction of their plane cut through the mesh faces.
+void or_map_alpha_field(void* h, const double* n, const double* d, double lower, double upper) {
+    Solver* s = (Solver*)h;
+    CutCell cc;
+    cc.m = s->m;
+    cc.cutFace.m = s->m;
+    for (int c = 0; c < s->m->nC; ++c) {
+        if (s->alpha[c] >= lower && s->alpha[c] <= upper) {
+            cc.calcSubCell(c, v3(n[3 * c], n[3 * c + 1], n[3 * c + 2]), d[c], false);
+            s->alpha[c] = cc.VOF;
+        }
+    }
 }
 
 int or_n_mixed(void* h) { return (int)((Solver*)h)->mixedCells.size(); }

@@ -342,3 +342,28 @@ def test_parity_rdf(oracle_lib):
     exact, nm = _compare_interface(gpu, ors, "six iterations")
     assert gpu.rdf_iterations() == ors.rdf_iterations_run() == 6
     _say(f"{mesh.name}: {nm} mixed cells, 6 RDF iterations, bit-exact={exact}")
+
+
+def test_parity_map_alpha_field(oracle_lib):
+    """svp_map_alpha_field (reconstruction.C:725-784) against the oracle with
+    a random plane per cell through the cell's neighbourhood, on perturbed
+    hex and tets; cells outside the refine levels keep their alpha."""
+    import geometricvofext_amd as g
+    for mesh in (g.hex_box(16, perturb=0.2, seed=21), g.tet_box(6, perturb=0.1, seed=22)):
+        _say("mapAlphaField", mesh.name)
+        gpu, om, ors = _pair(mesh, oracle_lib)
+        rng = np.random.default_rng(9)
+        a0 = rng.uniform(-0.2, 1.2, mesh.n_cells)
+        n = rng.normal(size=(mesh.n_cells, 3))
+        n /= np.linalg.norm(n, axis=1)[:, None]
+        n[::17] = 0.0                                    # zero normals take the uncut branches
+        d = -np.sum(n * om.C, axis=1) + rng.uniform(-0.5, 0.5, mesh.n_cells) / 16
+        gpu.set_alpha(a0)
+        ors.set_alpha(a0)
+        gpu.map_alpha_field(n, d, 0.01, 0.99)
+        ors.map_alpha_field(n, d, 0.01, 0.99)
+        ga, oa = gpu.alpha(), ors.alpha()
+        keep = ~((a0 >= 0.01) & (a0 <= 0.99))
+        assert np.array_equal(ga[keep], a0[keep])
+        assert max_abs(ga, oa) <= TOL, f"{mesh.name}: mapped alpha off by {max_abs(ga, oa)}"
+        _say(f"{mesh.name}: {int((~keep).sum())} cells mapped, bit-exact={bits_equal(ga, oa)}")
