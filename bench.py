@@ -176,14 +176,14 @@ def pmc_summary(copy_fetch_dir, copy_write_dir, run_fetch_dir, run_write_dir):
     return out
 
 
-def build_case(wl):
+def build_case(wl, alpha_dict=ALPHA_DICT):
     import geometricvofext_amd as g
     if wl["kind"] == "hex":
         mesh = g.hex_box(wl["N"], perturb=wl["perturb"], seed=wl["seed"])
     else:
         mesh = g.tet_box(wl["N"], perturb=wl["perturb"], seed=wl["seed"])
     from geometricvofext_amd import cases
-    solver = g.SolveVofEqu(mesh, ALPHA_DICT)
+    solver = g.SolveVofEqu(mesh, dict(alpha_dict))
     shapes = wl["shapes"]
     if isinstance(shapes, str):
         shapes = cases.random_spheres(int(shapes.split(":")[1]))
@@ -361,13 +361,14 @@ def device_sync(solver):
         pass
 
 
-def cpu_baseline(mesh, alpha0, seq, U0, phi0, n_steps, threads):
+def cpu_baseline(mesh, alpha0, seq, U0, phi0, n_steps, threads, alpha_dict=ALPHA_DICT):
     """The oracle (CPU restatement of the reference path) on the same mesh,
     initial field and time steps, first n_steps steps."""
     from oracle import oracle as O
     om = O.OracleMesh(mesh)
-    s = O.OracleSolver(om, mixedCellTol=ALPHA_DICT["mixedCellTol"], nAlphaBounds=ALPHA_DICT["nAlphaBounds"],
-                       snapTol=ALPHA_DICT["snapTol"], clip=ALPHA_DICT["clip"], threads=threads)
+    s = O.OracleSolver(om, mixedCellTol=alpha_dict["mixedCellTol"], nAlphaBounds=alpha_dict["nAlphaBounds"],
+                       snapTol=alpha_dict["snapTol"], clip=alpha_dict["clip"], threads=threads,
+                       orientationMethod=alpha_dict["orientationMethod"])
     s.set_alpha(alpha0)
     n_ic = 0
     for k, (_, dt, uf) in enumerate(seq[:n_steps]):
@@ -390,6 +391,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--stage", default="auto", help="stage timed for the roofline (auto = slowest)")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-normal comparison")
+    ap.add_argument("--orientation", default="LS", choices=["LS", "alphaGrad", "RDF"],
+                    help="interface normal method (default LS, the reference test case's)")
     ap.add_argument("--copy-probe", action="store_true", help="run only the counter-calibration copy")
     ap.add_argument("--pmc-summary", nargs=4, metavar=("COPY_FETCH", "COPY_WRITE", "RUN_FETCH", "RUN_WRITE"),
                     help="summarise rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of the copy probe and "
@@ -411,7 +414,8 @@ def main():
     import geometricvofext_amd as g
     wl = WORKLOADS[args.workload]
     t_setup = time.time()
-    mesh, solver, alpha0, U0, phi0 = build_case(wl)
+    alpha_dict = dict(ALPHA_DICT, orientationMethod=args.orientation)
+    mesh, solver, alpha0, U0, phi0 = build_case(wl, alpha_dict)
     seq = time_sequence(solver, alpha0, wl["end_time"], period=flow_period(wl))
     n_total = len(seq)
     K = args.steps if args.steps > 0 else n_total
@@ -493,7 +497,7 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": wl["desc"], "cells": mesh.n_cells, "faces": mesh.n_faces,
-                   "alpha_dict": ALPHA_DICT, "path": "plain LS"},
+                   "alpha_dict": alpha_dict, "path": f"plain {args.orientation}"},
         "roofline": {"bound": "hbm", "stage": stage, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_of(args.workload, stage),
                      "algorithmic_bytes_per_launch": st_bytes, "avg_launch_us": 1e6 * st_avg},
@@ -519,13 +523,13 @@ def main():
             pass
         threads = min(ncpu, 16)
         t_c = time.time()
-        nic_c, sec_c = cpu_baseline(mesh, alpha0, seq, U0, phi0, args.cpu_steps, threads)
+        nic_c, sec_c = cpu_baseline(mesh, alpha0, seq, U0, phi0, args.cpu_steps, threads, alpha_dict)
         res["cpu_baseline"] = {"value": nic_c / sec_c, "unit": "interface-cell updates/s", "cores": threads,
                                "kind": "port",
                                "sample": f"oracle on the same mesh, field and time steps, first "
                                          f"{args.cpu_steps} steps, {threads} OpenMP threads "
                                          f"({platform.processor() or platform.machine()})"}
-        nic_1, sec_1 = cpu_baseline(mesh, alpha0, seq, U0, phi0, max(2, args.cpu_steps // 2), 1)
+        nic_1, sec_1 = cpu_baseline(mesh, alpha0, seq, U0, phi0, max(2, args.cpu_steps // 2), 1, alpha_dict)
         res["cpu_baseline_serial"] = {"value": nic_1 / sec_1, "unit": "interface-cell updates/s", "cores": 1,
                                       "kind": "port", "sample": f"first {max(2, args.cpu_steps // 2)} steps"}
         print(f"[bench] cpu baseline {time.time() - t_c:.1f}s", file=sys.stderr, flush=True)

@@ -93,7 +93,7 @@ public:
                 else if (gaussNormal_) launch_grad_normal(m_, st_, stream_);
                 else launch_ls_normal(m_, st_, stream_);
             });
-            stage(STAGE_FSD, [&] { launch_find_signed_distance(m_, st_, params_, stream_); });
+            stage(STAGE_FSD, [&] { placePlanes(); });
         }
         HIPCHECK(hipEventRecord(b, stream_));
         HIPCHECK(hipGetLastError());
@@ -283,14 +283,18 @@ public:
         return st_.alphaPhi;
     }
 
-    // calcInterfaceNFromIsoRDF (reconstruction.C:194-405): the iterations run
-    // on the device; after each the host reads the three residual sums and
-    // applies the stopping rule.  The planes of the cells that are not too
-    // coarse are placed by the findSignedDistance kernels over a work list.
+    // calcInterfaceNFromIsoRDF (reconstruction.C:194-405): every iteration
+    // runs on the device, the stopping rule included, so the host queues
+    // `iterations` of them without waiting; those after the stop do nothing.
+    // The planes of the cells that are not too coarse are placed by the
+    // findSignedDistance kernels over a work list; the final placement of
+    // reconstruct then only has the too-coarse cells left (the others would
+    // get the same planes again from the same normals).
     void rdfNormals() {
         const long nC = m_.nC;
         if (!rdf_.next) {
-            rdf_.next = dalloc<unsigned char>(nC);
+            rdf_.next = dalloc<int>(nC);
+            rdf_.nextList = dalloc<int>(nC);
             rdf_.tooCoarse = dalloc<unsigned char>(nC);
             rdf_.rdf = dalloc<double>(nC);
             rdf_.cellN = dalloc<double>(3 * nC);
@@ -299,40 +303,25 @@ public:
             rdf_.res = dalloc<double>(2 * nC);
             rdf_.counted = dalloc<int>(nC);
             rdf_.wl = dalloc<int>(nC);
-            rdf_.wn = dalloc<int>(1);
-            rdf_.sums = dalloc<double>(3);
+            rdf_.wl2 = dalloc<int>(nC);
+            rdf_.ctl = dalloc<int>(RDF_CTL);
         }
-        HIPCHECK(hipMemsetAsync(rdf_.next, 0, nC, stream_));
-        HIPCHECK(hipMemsetAsync(rdf_.tooCoarse, 0, nC, stream_));
-        HIPCHECK(hipMemsetAsync(rdf_.rdf, 0, sizeof(double) * nC, stream_));
-        HIPCHECK(hipMemsetAsync(rdf_.cellN, 0, sizeof(double) * 3 * nC, stream_));
-        HIPCHECK(hipMemsetAsync(rdf_.cellC, 0, sizeof(double) * 3 * nC, stream_));
+        HIPCHECK(hipMemsetAsync(rdf_.ctl, 0, sizeof(int) * RDF_CTL, stream_));
         launch_rdf_begin(m_, st_, rdf_, stream_);
-        const double tol = dict_.rdf_tol, relTol = dict_.rdf_rel_tol;
         const int iterations = dict_.rdf_iterations;
-        rdfIterationsRun_ = 0;
-        for (int iter = 0; iter < iterations; ++iter) {
-            rdfIterationsRun_ = iter + 1;
-            HIPCHECK(hipMemsetAsync(rdf_.wn, 0, sizeof(int), stream_));
-            launch_rdf_normals(m_, st_, rdf_, stream_);
-            launch_find_signed_distance(m_, st_, params_, stream_, rdf_.wl, rdf_.wn);
-            launch_rdf_field(m_, st_, rdf_, stream_);
-            launch_rdf_residual(m_, st_, rdf_, iter, tol, stream_);
-            HIPCHECK(hipGetLastError());
-            double sums[3];
-            HIPCHECK(hipMemcpyAsync(sums, rdf_.sums, sizeof(sums), hipMemcpyDeviceToHost, stream_));
-            HIPCHECK(hipStreamSynchronize(stream_));
-            double avgRes = sums[0], avgNormRes = sums[1];
-            long resCounter = (long)sums[2];
-            if (resCounter == 0) {
-                resCounter = 1;
-                avgRes = 0.0;
-                avgNormRes = 0.0;
-            }
-            if (((avgNormRes / resCounter < relTol || avgRes / resCounter < tol) && iter >= 1) ||
-                iter + 1 == iterations)
-                break;
-        }
+        for (int iter = 0; iter < iterations; ++iter)
+            launch_rdf_iteration(m_, st_, params_, rdf_, iter, iterations, dict_.rdf_tol, dict_.rdf_rel_tol,
+                                 stream_);
+        launch_rdf_end(m_, st_, rdf_, stream_);
+        HIPCHECK(hipGetLastError());
+    }
+
+    // final plane placement of reconstruct (reconstruction.C:711-716)
+    void placePlanes() {
+        if (rdfNormal_ && dict_.rdf_iterations > 0)
+            launch_find_signed_distance(m_, st_, params_, stream_, rdf_.wl2, rdf_.ctl + RDF_WN2);
+        else
+            launch_find_signed_distance(m_, st_, params_, stream_);
     }
 
     // getRhoPhi with constant densities (advection.H:329-343) of the last
@@ -420,7 +409,13 @@ public:
         }
     }
 
-    int rdfIterationsRun() const { return rdfIterationsRun_; }
+    int rdfIterationsRun() {
+        if (!rdfNormal_ || !rdf_.ctl) return 0;
+        int n = 0;
+        HIPCHECK(hipMemcpyAsync(&n, rdf_.ctl + RDF_ITERS, sizeof(int), hipMemcpyDeviceToHost, stream_));
+        HIPCHECK(hipStreamSynchronize(stream_));
+        return n;
+    }
 
     // reconstruction::mapAlphaField (reconstruction.C:725-784) after a mesh
     // change: n (3 per cell) and d (per cell) are the interface fields the
@@ -576,7 +571,6 @@ private:
     bool gaussNormal_ = false;  // orientationMethod alphaGrad
     bool rdfNormal_ = false;    // orientationMethod isoRDF / RDF
     RdfBufs rdf_{};
-    int rdfIterationsRun_ = 0;
     bool alphaPhiValid_ = false;
     int maxCellFaces_ = 0;
     int profMask_ = 0;

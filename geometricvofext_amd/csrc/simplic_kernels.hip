@@ -403,13 +403,19 @@ __device__ inline V3 normalise_tol(V3 v, double tol) {
     return s < tol ? VZERO : v / s;
 }
 
+// The buffers are clean (zero) between reconstructs: the first allocation
+// zeroes them and k_rdf_end clears what one reconstruct touched, all of it
+// inside the nextToInterface cells.  ctl holds the small counters.
 __global__ void k_rdf_begin(DevMesh m, DevState st, RdfBufs r) {
     const int nMixed = *st.nMixed;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nMixed; i += gridDim.x * blockDim.x) {
         const int c = st.mixed[i];
         // level 0 and level 1 of markCellsNearSurf: the stencil holds the cell
         // and every cell sharing a point with it
-        for (int k = m.stOff[c]; k < m.stOff[c + 1]; ++k) r.next[m.st[k]] = 1;
+        for (int k = m.stOff[c]; k < m.stOff[c + 1]; ++k) {
+            const int o = m.st[k];
+            if (atomicExch(&r.next[o], 1) == 0) r.nextList[atomicAdd(&r.ctl[RDF_NNEXT], 1)] = o;
+        }
         st.ifD[i] = 0.0;
         st3(st.ifN, i, VZERO);
         st3(st.ifC, i, VZERO);
@@ -419,6 +425,7 @@ __global__ void k_rdf_begin(DevMesh m, DevState st, RdfBufs r) {
 }
 
 __global__ void k_rdf_normals(DevState st, RdfBufs r) {
+    if (r.ctl[RDF_DONE]) return;
     const int nMixed = *st.nMixed;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nMixed; i += gridDim.x * blockDim.x) {
         const int c = st.mixed[i];
@@ -427,27 +434,27 @@ __global__ void k_rdf_normals(DevState st, RdfBufs r) {
         const V3 n = -gi;
         st3(st.ifN, i, n);
         st3(r.cellN, c, n);
-        if (!r.tooCoarse[c]) r.wl[atomicAdd(r.wn, 1)] = i;
+        if (!r.tooCoarse[c]) r.wl[atomicAdd(&r.ctl[RDF_WN], 1)] = i;
     }
 }
 
 __global__ void k_rdf_scatter_c(DevState st, RdfBufs r) {
+    if (r.ctl[RDF_DONE]) return;
     const int nMixed = *st.nMixed;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nMixed; i += gridDim.x * blockDim.x)
         st3(r.cellC, st.mixed[i], ld3(st.ifC, i));
 }
 
-// constructRDF(nextToInterface, interfaceC, -interfaceN): the signed distance
-// to the cell's own plane in an interface cell, else the average over the
-// stencil's planes weighted by the squared cosine between the plane normal
-// and the direction to the plane centre; cells away from the interface get 0
-// and a cell with no weight keeps its value.
+// constructRDF(nextToInterface, interfaceC, -interfaceN) over the
+// nextToInterface cells (the others hold 0): the signed distance to the
+// cell's own plane in an interface cell, else the average over the stencil's
+// planes weighted by the squared cosine between the plane normal and the
+// direction to the plane centre; a cell with no weight keeps its value.
 __global__ void k_rdf_construct(DevMesh m, RdfBufs r) {
-    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < m.nC; c += gridDim.x * blockDim.x) {
-        if (!r.next[c]) {
-            r.rdf[c] = 0.0;
-            continue;
-        }
+    if (r.ctl[RDF_DONE]) return;
+    const int nNext = r.ctl[RDF_NNEXT];
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < nNext; j += gridDim.x * blockDim.x) {
+        const int c = r.nextList[j];
         const V3 cc = ld3(m.C, c);
         const V3 nrm = -ld3(r.cellN, c);
         if (mag(nrm) != 0.0) {
@@ -480,6 +487,7 @@ __global__ void k_rdf_construct(DevMesh m, RdfBufs r) {
 }
 
 __global__ void k_rdf_grad(DevMesh m, DevState st, RdfBufs r) {
+    if (r.ctl[RDF_DONE]) return;
     const int nMixed = *st.nMixed;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nMixed; i += gridDim.x * blockDim.x)
         st3(r.g, i, ls_gradient(m, r.rdf, st.mixed[i]));
@@ -490,6 +498,7 @@ __global__ void k_rdf_grad(DevMesh m, DevState st, RdfBufs r) {
 // once inside the zero-normal test when the cell's normal is not zero, once
 // for the new normal.
 __global__ void k_rdf_residual(DevMesh m, DevState st, RdfBufs r, int iter, double tol) {
+    if (r.ctl[RDF_DONE]) return;
     const int nMixed = *st.nMixed;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nMixed; i += gridDim.x * blockDim.x) {
         const int c = st.mixed[i];
@@ -525,9 +534,14 @@ __global__ void k_rdf_residual(DevMesh m, DevState st, RdfBufs r, int iter, doub
     }
 }
 
-// sums of the residual terms, one workgroup (a fixed order: the sums only
-// feed the stopping test)
-__global__ void __launch_bounds__(1024) k_rdf_reduce(DevState st, RdfBufs r) {
+// Sums of the residual terms in one workgroup (a fixed order: the sums only
+// feed the stopping test), then the stopping rule of reconstruction.C:359-399
+// on the device: it sets the done flag that turns the remaining iterations'
+// kernels into no-ops.  Also counts the iteration and clears the work-list
+// count for the next one.
+__global__ void __launch_bounds__(1024) k_rdf_reduce(DevState st, RdfBufs r, int iter, int iterations, double tol,
+                                                     double relTol) {
+    if (r.ctl[RDF_DONE]) return;
     __shared__ double s0[1024], s1[1024], s2[1024];
     const int nMixed = *st.nMixed;
     const int t = threadIdx.x;
@@ -550,33 +564,61 @@ __global__ void __launch_bounds__(1024) k_rdf_reduce(DevState st, RdfBufs r) {
         __syncthreads();
     }
     if (t == 0) {
-        r.sums[0] = s0[0];
-        r.sums[1] = s1[0];
-        r.sums[2] = s2[0];
+        double avgRes = s0[0], avgNormRes = s1[0];
+        long long resCounter = (long long)s2[0];
+        if (resCounter == 0) {
+            resCounter = 1;
+            avgRes = 0.0;
+            avgNormRes = 0.0;
+        }
+        const bool stop = ((avgNormRes / resCounter < relTol || avgRes / resCounter < tol) && iter >= 1) ||
+                          iter + 1 == iterations;
+        r.ctl[RDF_ITERS] = iter + 1;
+        r.ctl[RDF_WN] = 0;
+        if (stop) r.ctl[RDF_DONE] = 1;
     }
 }
 
-// the caller clears next, tooCoarse, rdf, cellN and cellC first
+// after the iterations: the work list of the cells that are (or became) too
+// coarse, whose planes the final placement still has to compute
+__global__ void k_rdf_coarse_list(DevState st, RdfBufs r) {
+    const int nMixed = *st.nMixed;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nMixed; i += gridDim.x * blockDim.x)
+        if (r.tooCoarse[st.mixed[i]]) r.wl2[atomicAdd(&r.ctl[RDF_WN2], 1)] = i;
+}
+
+// clear what this reconstruct touched (all inside the nextToInterface cells)
+__global__ void k_rdf_end(RdfBufs r) {
+    const int nNext = r.ctl[RDF_NNEXT];
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < nNext; j += gridDim.x * blockDim.x) {
+        const int c = r.nextList[j];
+        r.next[c] = 0;
+        r.tooCoarse[c] = 0;
+        r.rdf[c] = 0.0;
+        st3(r.cellN, c, VZERO);
+        st3(r.cellC, c, VZERO);
+    }
+}
+
 void launch_rdf_begin(const DevMesh& m, const DevState& st, const RdfBufs& r, hipStream_t s) {
     k_rdf_begin<<<1024, 64, 0, s>>>(m, st, r);
 }
 
-// the caller clears wn first
-void launch_rdf_normals(const DevMesh& m, const DevState& st, const RdfBufs& r, hipStream_t s) {
-    (void)m;
+void launch_rdf_iteration(const DevMesh& m, const DevState& st, const Params& p, const RdfBufs& r, int iter,
+                          int iterations, double tol, double relTol, hipStream_t s) {
     k_rdf_normals<<<1024, 256, 0, s>>>(st, r);
-}
-
-void launch_rdf_field(const DevMesh& m, const DevState& st, const RdfBufs& r, hipStream_t s) {
+    launch_find_signed_distance(m, st, p, s, r.wl, r.ctl + RDF_WN);
     k_rdf_scatter_c<<<1024, 256, 0, s>>>(st, r);
-    k_rdf_construct<<<std::min(4096, cdiv(m.nC, 256)), 256, 0, s>>>(m, r);
+    k_rdf_construct<<<1024, 64, 0, s>>>(m, r);
     k_rdf_grad<<<1024, 64, 0, s>>>(m, st, r);
+    k_rdf_residual<<<1024, 64, 0, s>>>(m, st, r, iter, tol);
+    k_rdf_reduce<<<1, 1024, 0, s>>>(st, r, iter, iterations, tol, relTol);
 }
 
-void launch_rdf_residual(const DevMesh& m, const DevState& st, const RdfBufs& r, int iter, double tol,
-                         hipStream_t s) {
-    k_rdf_residual<<<1024, 64, 0, s>>>(m, st, r, iter, tol);
-    k_rdf_reduce<<<1, 1024, 0, s>>>(st, r);
+void launch_rdf_end(const DevMesh& m, const DevState& st, const RdfBufs& r, hipStream_t s) {
+    (void)m;
+    k_rdf_coarse_list<<<1024, 256, 0, s>>>(st, r);
+    k_rdf_end<<<1024, 256, 0, s>>>(r);
 }
 
 // ---------------------------------------------------------------------------

@@ -155,9 +155,11 @@ void launch_ls_normal(const DevMesh& m, const DevState& st, hipStream_t s, const
 // alphaGrad normal: minus the Gauss-linear gradient of alpha, normalised
 void launch_grad_normal(const DevMesh& m, const DevState& st, hipStream_t s);
 
-// isoRDF work buffers (reconstruction.C:194-405)
+// isoRDF work buffers (reconstruction.C:194-405); zero between reconstructs
+enum { RDF_NNEXT = 0, RDF_DONE = 1, RDF_ITERS = 2, RDF_WN = 3, RDF_WN2 = 4, RDF_CTL = 8 };
 struct RdfBufs {
-    unsigned char* next;       // nC: nextToInterface (mixed cells and the cells sharing a point with one)
+    int* next;                 // nC: nextToInterface (mixed cells and the cells sharing a point with one)
+    int* nextList;             // nC: those cells, ctl[RDF_NNEXT] of them
     unsigned char* tooCoarse;  // nC
     double* rdf;               // nC: reconstructed distance function
     double* cellN;             // 3*nC: interfaceN per cell
@@ -165,21 +167,20 @@ struct RdfBufs {
     double* g;                 // 3*nC: per mixed cell, the least-squares gradient (interfaceNormal)
     double* res;               // 2*nC: per mixed cell, its normalRes and normRes terms
     int* counted;              // nC: per mixed cell, 1 when it enters the residual averages
-    int* wl;                   // nC: mixed cells that are not too coarse
-    int* wn;                   // [1]
-    double* sums;              // [3]: avgRes, avgNormRes and resCounter before the division
+    int* wl;                   // nC: mixed cells placed this iteration (not too coarse)
+    int* wl2;                  // nC: mixed cells too coarse at the end
+    int* ctl;                  // [RDF_CTL]: next count, done flag, iterations run, wl and wl2 counts
 };
-// (next, tooCoarse, rdf, cellN and cellC cleared by the caller) zero d/C/S/n
-// of the mixed cells, mark nextToInterface, g = least-squares gradient of alpha
+// (ctl cleared by the caller) mark nextToInterface and list it, zero d/C/S/n
+// of the mixed cells, g = least-squares gradient of alpha
 void launch_rdf_begin(const DevMesh& m, const DevState& st, const RdfBufs& r, hipStream_t s);
-// normals from g (in place), and the work list of cells that are not too
-// coarse (wn cleared by the caller)
-void launch_rdf_normals(const DevMesh& m, const DevState& st, const RdfBufs& r, hipStream_t s);
-// constructRDF from the planes, then g = least-squares gradient of it
-void launch_rdf_field(const DevMesh& m, const DevState& st, const RdfBufs& r, hipStream_t s);
-// residuals, too-coarse marks and their sums
-void launch_rdf_residual(const DevMesh& m, const DevState& st, const RdfBufs& r, int iter, double tol,
-                         hipStream_t s);
+// one iteration: normals, plane placement of the cells that are not too
+// coarse, distance function, its gradient, residuals and the stopping rule
+// (a no-op once the rule has stopped the loop)
+void launch_rdf_iteration(const DevMesh& m, const DevState& st, const Params& p, const RdfBufs& r, int iter,
+                          int iterations, double tol, double relTol, hipStream_t s);
+// the too-coarse work list, then clear the buffers for the next reconstruct
+void launch_rdf_end(const DevMesh& m, const DevState& st, const RdfBufs& r, hipStream_t s);
 void launch_find_signed_distance(const DevMesh& m, const DevState& st, const Params& p, hipStream_t s,
                                  const int* wl = nullptr, const int* wn = nullptr);
 // pipeline stages of one step, in launch order (also the profiler's slots)
