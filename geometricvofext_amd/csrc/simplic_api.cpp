@@ -817,6 +817,7 @@ private:
         st_.boundStats = dalloc<int>(4 * MAX_BOUND_PASSES);
         st_.boundTotals = dalloc<unsigned long long>(5);
         st_.bcount = dalloc<int>(16);
+        st_.bndLook = dalloc<unsigned long long>((nC + 63) / 64 / 1024 + 1);
         HIPCHECK(hipMemsetAsync(st_.bcount, 0, 16 * sizeof(int), stream_));
         st_.maxCellFaces = maxCellFaces_;
         // bounding rounds run grid-wide before the one-workgroup tail (tuning

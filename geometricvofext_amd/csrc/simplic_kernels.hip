@@ -6,6 +6,8 @@
 // (compiled with -ffp-contract=off); each kernel cites the reference lines it
 // follows.  Control flow that the reference runs on the host (mixed-cell count,
 // bounding passes) stays on the device, so a step never waits for the host.
+#include <atomic>
+
 #include "simplic_device.hpp"
 #include "simplic_types.hpp"
 
@@ -1905,11 +1907,20 @@ __device__ inline void bound_finish(const BndRec<MF>& R, const double (&cin)[MF]
         if (k < nf && down(k) && writable(k)) out(k, R.fid[k], cor[k]);
 }
 
-// Ascending candidates of the pass from the bitmap: per-block popcounts,
-// one-workgroup scan of the block counts, then every block writes its
-// candidates (coalesced word reads throughout).  With a zero bound count the
-// pass is off (no candidates).  The scan also clears the per-pass counters.
+// Ascending candidates of the pass from the bitmap in one launch: every
+// block counts its 1024 words, publishes the count and looks back over the
+// blocks before it for their prefix (a decoupled look-back), then writes its
+// candidates.  Blocks take tickets in the order they start, so a block only
+// waits for blocks that are already running and each of those waits only on
+// its own predecessors.  The published words carry the launch's epoch, which
+// makes any word of an earlier launch read as not yet published.  With a
+// zero bound count the pass is off (no candidates).  Ticket 0 also clears
+// the per-pass counters.
 constexpr int BND_WORDS_PER_BLOCK = 1024;
+constexpr int BND_LIST_TICKET = 15;
+static_assert(BND_LIST_TICKET > BND_PROGRESS, "list ticket outside the cleared counters");
+constexpr unsigned long long LOOK_AGG = 1ull << 30, LOOK_PREFIX = 2ull << 30;
+constexpr int LOOK_SPIN_LIMIT = 1 << 22;
 
 // any cell out of bounds: sum of the spread counters (each slot is one cache
 // line, so the update kernel's blocks do not queue on a single address)
@@ -1920,51 +1931,63 @@ __device__ inline bool oob_any(const DevState& st) {
     return n != 0;
 }
 
-__global__ void __launch_bounds__(256) k_bnd_count(DevState st, int nWords) {
-    __shared__ int lds[4];
-    if (!oob_any(st)) return;
-    int c = 0;
-    for (int w = blockIdx.x * BND_WORDS_PER_BLOCK + threadIdx.x;
-         w < min(nWords, (blockIdx.x + 1) * BND_WORDS_PER_BLOCK); w += 256)
-        c += __popcll(st.candBits[w]);
-    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
-    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = c;
-    __syncthreads();
-    if (threadIdx.x == 0) st.blockCount[blockIdx.x] = lds[0] + lds[1] + lds[2] + lds[3];
-}
-
-__global__ void __launch_bounds__(1024) k_bnd_scan(DevState st, int nBlocks) {
+__global__ void __launch_bounds__(256) k_bnd_list(DevState st, int nWords, int nBlocks, unsigned epoch) {
     __shared__ int lds[16];
-    const int t = threadIdx.x;
-    if (t >= 1 && t <= BND_PROGRESS) st.bcount[t] = 0;
+    __shared__ int sTicket, sBase;
+    if (threadIdx.x == 0) {
+        const int t = atomicAdd(&st.bcount[BND_LIST_TICKET], 1);
+        // the last ticket of this launch: every block holds its own, so the
+        // counter is ready for the next launch
+        if (t == nBlocks - 1) atomicExch(&st.bcount[BND_LIST_TICKET], 0);
+        sTicket = t;
+    }
+    __syncthreads();
+    const int b = sTicket;
+    if (b == 0 && threadIdx.x >= 1 && threadIdx.x <= BND_PROGRESS) st.bcount[threadIdx.x] = 0;
     if (!oob_any(st)) {
-        if (t == 0) st.bcount[0] = 0;
+        if (b == 0 && threadIdx.x == 0) st.bcount[0] = 0;
         return;
     }
-    // nBlocks <= 1024 * per
-    const int per = (nBlocks + 1023) / 1024;
-    const int i0 = min(nBlocks, t * per), i1 = min(nBlocks, i0 + per);
-    int sum = 0;
-    for (int i = i0; i < i1; ++i) sum += st.blockCount[i];
-    int total;
-    int off = wg_scan1024(sum, lds, total);
-    for (int i = i0; i < i1; ++i) {
-        const int v = st.blockCount[i];
-        st.blockCount[i] = off;
-        off += v;
+    const int wBeg = b * BND_WORDS_PER_BLOCK, wEnd = min(nWords, wBeg + BND_WORDS_PER_BLOCK);
+    int cnt = 0;
+    for (int w = wBeg + threadIdx.x; w < wEnd; w += 256) cnt += __popcll(st.candBits[w]);
+    for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o);
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = cnt;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned long long total = (unsigned long long)(lds[0] + lds[1] + lds[2] + lds[3]);
+        const unsigned long long tag = (unsigned long long)epoch << 32;
+        unsigned long long excl = 0;
+        if (b > 0) {
+            __hip_atomic_store(&st.bndLook[b], tag | LOOK_AGG | total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            int j = b - 1, spins = 0;
+            while (j >= 0) {
+                const unsigned long long v =
+                    __hip_atomic_load(&st.bndLook[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if ((v >> 32) != epoch) {
+                    if (++spins > LOOK_SPIN_LIMIT) {
+                        atomicOr(st.errFlag, ERR_BOUND_STALL);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                    continue;
+                }
+                excl += v & (LOOK_AGG - 1);
+                if (v & LOOK_PREFIX) break;
+                --j;
+            }
+        }
+        __hip_atomic_store(&st.bndLook[b], tag | LOOK_PREFIX | (excl + total), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+        if (b == nBlocks - 1) st.bcount[0] = (int)(excl + total);
+        sBase = (int)excl;
     }
-    if (t == 0) st.bcount[0] = total;
-}
-
-__global__ void __launch_bounds__(256) k_bnd_write(DevState st, int nWords) {
-    __shared__ int lds[16];
-    if (st.bcount[0] == 0) return;
-    int base = st.blockCount[blockIdx.x];
+    __syncthreads();
+    int base = sBase;
     // the block's words in chunks of 256: a block scan of the popcounts per chunk
-    for (int w0 = blockIdx.x * BND_WORDS_PER_BLOCK; w0 < min(nWords, (blockIdx.x + 1) * BND_WORDS_PER_BLOCK);
-         w0 += 256) {
+    for (int w0 = wBeg; w0 < wEnd; w0 += 256) {
         const int w = w0 + threadIdx.x;
-        unsigned long long bits = (w < nWords) ? st.candBits[w] : 0ull;
+        unsigned long long bits = (w < wEnd) ? st.candBits[w] : 0ull;
         const int c = __popcll(bits);
         // exclusive scan over the 256 threads: wave scans, then the 4 wave totals
         const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -2522,6 +2545,14 @@ __global__ void __launch_bounds__(256) k_bnd_reset(DevMesh m, DevState st, int p
     }
 }
 
+// epochs of the candidate-list launches, never 0 (the cleared look-back words)
+static unsigned next_list_epoch() {
+    static std::atomic<unsigned> epoch{0};
+    unsigned e = ++epoch;
+    while (e == 0) e = ++epoch;
+    return e;
+}
+
 template <int MF>
 void launch_bound_pass(const DevMesh& m, const DevState& st, int pass, double dt, const double* Sp,
                        const double* Su, hipStream_t s) {
@@ -2530,9 +2561,7 @@ void launch_bound_pass(const DevMesh& m, const DevState& st, int pass, double dt
     // one thread per cell of the mesh
     const int grid = std::max(1, std::min(st.bndGrid, cdiv(m.nC, 256)));
     const int nbw = cdiv(nWords, BND_WORDS_PER_BLOCK);
-    k_bnd_count<<<nbw, 256, 0, s>>>(st, nWords);
-    k_bnd_scan<<<1, 1024, 0, s>>>(st, nbw);
-    k_bnd_write<<<nbw, 256, 0, s>>>(st, nWords);
+    k_bnd_list<<<nbw, 256, 0, s>>>(st, nWords, nbw, next_list_epoch());
     k_bnd_prep<MF><<<grid, 256, 0, s>>>(m, st, dt, Sp, Su);
     if (st.bndFlow) {
         k_bnd_flow<MF><<<std::max(1, std::min(st.bndFlow, cdiv(m.nC, flow_chunk<MF>()))), flow_threads<MF>(), 0, s>>>(

@@ -82,6 +82,7 @@ struct DevState {
     int* boundStats;    // [4*maxPasses]: candidates, active, rounds, corrected faces
     unsigned long long* boundTotals;  // [5] running sums: candidates, active, rounds, corrected faces, passes
     int* bcount;        // [16] per-pass counters of the bounding kernels
+    unsigned long long* bndLook;  // per candidate-list block: epoch, look-back flag and count
     int un0Fused;       // plane placement by 8-lane groups, which also computes Un0
     int fsdKeep;        // 1: where findSignedDistance writes no d/C/S they keep their values (isoRDF)
     int gridRounds;     // bounding rounds run as grid-wide kernels before the one-workgroup tail
