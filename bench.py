@@ -109,8 +109,8 @@ def step_bytes(n_cells, n_faces, n_ic, cell_kind):
 
 # kernels of each pipeline stage, for the HBM traffic read from profiler counters
 STAGE_KERNELS = {
-    "detect": ("k_detect_flags", "k_scan_counts", "k_compact_flags", "k_zero_int"),
-    "normal": ("k_ls_normal",),
+    "detect": ("k_detect_list",),
+    "normal": ("k_ls_normal", "k_grad_normal", "k_rdf_"),
     "findSignedDistance": ("k_fsd_group", "k_find_signed_distance"),
     "upwind": ("k_upwind",),
     "interfaceFlux": ("k_interface_flux", "k_un0"),

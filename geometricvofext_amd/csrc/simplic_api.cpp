@@ -252,7 +252,8 @@ public:
             std::snprintf(buf, sizeof buf,
                           "device error bits 0x%x (1: cell exceeds cut limits, 2: more than %d plane "
                           "crossings on a face, 4: bounding candidate overflow, 8: bounding work overflow, "
-                          "16: bounding dependency wait without progress)",
+                          "16: bounding wait without progress, 32: mixed-cell detection wait without "
+                          "progress)",
                           err, MAX_IF_POINTS);
             throw SvpError(SVP_E_DEVICE, buf);
         }
@@ -787,7 +788,6 @@ private:
         st_.alphaPhi = dalloc<double>(nF);
         st_.corr = dalloc<double>(nF);
         st_.mixedFlag = dalloc<unsigned char>(nC);
-        st_.blockCount = dalloc<int>(detect_blocks(nC));
         st_.nMixed = dalloc<int>(1);
         st_.mixed = dalloc<int>(nC);
         st_.status = dalloc<int>(nC);
@@ -818,6 +818,8 @@ private:
         st_.boundTotals = dalloc<unsigned long long>(5);
         st_.bcount = dalloc<int>(16);
         st_.bndLook = dalloc<unsigned long long>((nC + 63) / 64 / 1024 + 1);
+        st_.detLook = dalloc<unsigned long long>(detect_blocks(nC));
+        st_.detTicket = dalloc<int>(1);
         HIPCHECK(hipMemsetAsync(st_.bcount, 0, 16 * sizeof(int), stream_));
         st_.maxCellFaces = maxCellFaces_;
         // bounding rounds run grid-wide before the one-workgroup tail (tuning

@@ -109,6 +109,79 @@ void launch_geometry(const DevMesh& m, hipStream_t s) {
 // ---------------------------------------------------------------------------
 int detect_blocks(int nC) { return cdiv(nC, DETECT_BLOCK); }
 
+// Ordered compaction in one launch (decoupled look-back): every block takes a
+// ticket in the order blocks start, publishes its total, and sums the totals
+// of the blocks before it until it meets a published prefix.  A block only
+// waits for blocks that took earlier tickets, which are already running, and
+// each of those waits only on its own predecessors.  The published words
+// carry the launch's epoch, so a word of an earlier launch reads as not yet
+// published.  Counts stay below 2^30.
+constexpr unsigned long long LOOK_AGG = 1ull << 30, LOOK_PREFIX = 2ull << 30;
+constexpr int LOOK_SPIN_LIMIT = 1 << 22;
+
+// epochs of the look-back launches, never 0 (the cleared look-back words)
+static unsigned next_list_epoch() {
+    static std::atomic<unsigned> epoch{0};
+    unsigned e = ++epoch;
+    while (e == 0) e = ++epoch;
+    return e;
+}
+
+// the block's ticket (thread 0 only); the last ticket of the launch resets
+// the counter for the next launch
+__device__ inline int take_ticket(int* counter, int nBlocks) {
+    const int t = atomicAdd(counter, 1);
+    if (t == nBlocks - 1) atomicExch(counter, 0);
+    return t;
+}
+
+// Wave 0 of the block (all 64 lanes): publish this block's total and return
+// the sum of the totals of blocks 0..b-1 (the same value in every lane).  The
+// lanes read 64 predecessors at once; the walk stops at the nearest published
+// prefix and waits only while a word before it is unpublished.  A wait
+// without progress for about 0.1 s sets errBit.
+__device__ inline unsigned long long wave_sum_u64(unsigned long long v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+__device__ inline unsigned long long look_back(unsigned long long* look, int b, unsigned epoch,
+                                               unsigned long long total, int* errFlag, int errBit) {
+    const int lane = threadIdx.x & 63;
+    const unsigned long long tag = (unsigned long long)epoch << 32;
+    unsigned long long excl = 0;
+    if (b > 0) {
+        if (lane == 0)
+            __hip_atomic_store(&look[b], tag | LOOK_AGG | total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        int j0 = b - 1, spins = 0;
+        while (true) {
+            const int j = j0 - lane;
+            // before block 0: a published prefix of zero
+            const unsigned long long v =
+                j >= 0 ? __hip_atomic_load(&look[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : (tag | LOOK_PREFIX);
+            const bool ready = (v >> 32) == epoch;
+            const unsigned long long notReady = __ballot(!ready);
+            const unsigned long long isPrefix = __ballot(ready && (v & LOOK_PREFIX));
+            const int p = isPrefix ? __ffsll((long long)isPrefix) - 1 : 64;
+            const unsigned long long need = p >= 63 ? ~0ull : ((1ull << (p + 1)) - 1ull);
+            if (notReady & need) {
+                if (++spins > LOOK_SPIN_LIMIT) {
+                    if (lane == 0) atomicOr(errFlag, errBit);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+                continue;
+            }
+            excl += wave_sum_u64(lane <= p ? (v & (LOOK_AGG - 1)) : 0ull);
+            if (p < 64) break;
+            j0 -= 64;
+        }
+    }
+    if (lane == 0)
+        __hip_atomic_store(&look[b], tag | LOOK_PREFIX | (excl + total), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return excl;
+}
+
 // exclusive prefix of a per-lane flag within a DETECT_BLOCK block; returns the
 // block total in *total (LDS-resident)
 __device__ inline int block_prefix(bool flag, int* lds, int& total) {
@@ -189,36 +262,78 @@ __global__ void __launch_bounds__(1024) k_scan_counts(int* counts, int nb, int* 
 // reconstruction::initialize mixed-cell detection (reconstruction.C:634-677,
 // isAMixedCell reconstruction.H:281-288): ascending list of mixed cells
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(DETECT_BLOCK) k_detect_flags(DevMesh m, DevState st, double tol) {
-    __shared__ int lds[DETECT_BLOCK / 64];
-    const int c = blockIdx.x * DETECT_BLOCK + threadIdx.x;
-    bool flag = false;
-    if (c < m.nC) {
-        const double a = st.alpha[c];
-        // cells of an idle case of a batch are not reconstructed
-        flag = (tol < a) && (a < 1.0 - tol) && (!st.caseOf || st.dtCase[st.caseOf[c]] > 0.0);
-        st.mixedFlag[c] = flag ? 1 : 0;
-    }
-    int total;
-    block_prefix(flag, lds, total);
-    if (threadIdx.x == 0) st.blockCount[blockIdx.x] = total;
-}
+// Detection in one launch: flags, a look-back over the blocks, and the
+// ascending mixed-cell list; the last ticket writes the count.  A block of
+// 1024 threads takes 8 chunks of 1024 cells (thread t holds cell t of every
+// chunk, so the reads are coalesced and the flags stay in registers); few,
+// large blocks keep the look-back chain short.
+constexpr int DETECT_LIST_THREADS = 1024, DETECT_LIST_CHUNKS = 8;
+constexpr int DETECT_LIST_CELLS = DETECT_LIST_THREADS * DETECT_LIST_CHUNKS;
 
-__global__ void __launch_bounds__(DETECT_BLOCK) k_compact_flags(int n, const unsigned char* flag,
-                                                                const int* blockOff, int* out) {
-    __shared__ int lds[DETECT_BLOCK / 64];
-    const int c = blockIdx.x * DETECT_BLOCK + threadIdx.x;
-    const bool f = (c < n) && flag[c];
-    int total;
-    const int pre = block_prefix(f, lds, total);
-    if (f) out[blockOff[blockIdx.x] + pre] = c;
+__global__ void __launch_bounds__(DETECT_LIST_THREADS) k_detect_list(DevMesh m, DevState st, double tol,
+                                                                     int nBlocks, unsigned epoch) {
+    constexpr int NW = DETECT_LIST_THREADS / 64;
+    __shared__ int cnt[DETECT_LIST_CHUNKS][NW];
+    __shared__ int sTicket, sBase;
+    if (threadIdx.x == 0) sTicket = take_ticket(st.detTicket, nBlocks);
+    __syncthreads();
+    const int b = sTicket;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int c0 = b * DETECT_LIST_CELLS + threadIdx.x;
+    double a[DETECT_LIST_CHUNKS];
+#pragma unroll
+    for (int k = 0; k < DETECT_LIST_CHUNKS; ++k) {
+        const int c = c0 + k * DETECT_LIST_THREADS;
+        a[k] = c < m.nC ? st.alpha[c] : 0.0;
+    }
+    unsigned flags = 0;
+    int inWave[DETECT_LIST_CHUNKS];
+#pragma unroll
+    for (int k = 0; k < DETECT_LIST_CHUNKS; ++k) {
+        const int c = c0 + k * DETECT_LIST_THREADS;
+        bool flag = false;
+        if (c < m.nC) {
+            // cells of an idle case of a batch are not reconstructed
+            flag = (tol < a[k]) && (a[k] < 1.0 - tol) && (!st.caseOf || st.dtCase[st.caseOf[c]] > 0.0);
+            st.mixedFlag[c] = flag ? 1 : 0;
+        }
+        flags |= flag ? (1u << k) : 0u;
+        const unsigned long long bal = __ballot(flag);
+        inWave[k] = __popcll(bal & ((1ull << lane) - 1ull));
+        if (lane == 0) cnt[k][wave] = __popcll(bal);
+    }
+    __syncthreads();
+    int total = 0;
+    for (int k = 0; k < DETECT_LIST_CHUNKS; ++k)
+        for (int w = 0; w < NW; ++w) total += cnt[k][w];
+    if (threadIdx.x < 64) {
+        const unsigned long long excl =
+            look_back(st.detLook, b, epoch, (unsigned long long)total, st.errFlag, ERR_DETECT_STALL);
+        if (threadIdx.x == 0) {
+            if (b == nBlocks - 1) {
+                *st.nMixed = (int)(excl + total);
+                *st.nIcTotal += excl + total;
+            }
+            sBase = (int)excl;
+        }
+    }
+    __syncthreads();
+    int off = sBase;
+#pragma unroll
+    for (int k = 0; k < DETECT_LIST_CHUNKS; ++k) {
+        int pre = 0, tk = 0;
+        for (int w = 0; w < NW; ++w) {
+            pre += w < wave ? cnt[k][w] : 0;
+            tk += cnt[k][w];
+        }
+        if (flags & (1u << k)) st.mixed[off + pre + inWave[k]] = c0 + k * DETECT_LIST_THREADS;
+        off += tk;
+    }
 }
 
 void launch_detect(const DevMesh& m, const DevState& st, const Params& p, hipStream_t s) {
-    const int nb = detect_blocks(m.nC);
-    k_detect_flags<<<nb, DETECT_BLOCK, 0, s>>>(m, st, p.mixedCellTol);
-    k_scan_counts<<<1, 1024, 0, s>>>(st.blockCount, nb, st.nMixed, st.nIcTotal);
-    k_compact_flags<<<nb, DETECT_BLOCK, 0, s>>>(m.nC, st.mixedFlag, st.blockCount, st.mixed);
+    const int nb = cdiv(m.nC, DETECT_LIST_CELLS);
+    k_detect_list<<<nb, DETECT_LIST_THREADS, 0, s>>>(m, st, p.mixedCellTol, nb, next_list_epoch());
 }
 
 // ---------------------------------------------------------------------------
@@ -1254,6 +1369,9 @@ void launch_plic_surface(const DevMesh& m, const DevState& st, int nMixed, int p
 // lambda*(aP - aN) + aN with lambda = pos0(phi); zeroGradient boundary value
 // is the owner value; empty patches carry nothing.
 __global__ void __launch_bounds__(256) k_upwind(DevMesh m, DevState st, double dtStep) {
+    // the out-of-bounds counters of the update that follows start at zero
+    if (blockIdx.x == 0)
+        for (int k = threadIdx.x; k < OOB_SLOTS * OOB_STRIDE; k += blockDim.x) st.oobCount[k] = 0;
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= m.nF) return;
     const double dt = cell_dt(st, m.own[f], dtStep);
@@ -1919,9 +2037,6 @@ __device__ inline void bound_finish(const BndRec<MF>& R, const double (&cin)[MF]
 constexpr int BND_WORDS_PER_BLOCK = 1024;
 constexpr int BND_LIST_TICKET = 15;
 static_assert(BND_LIST_TICKET > BND_PROGRESS, "list ticket outside the cleared counters");
-constexpr unsigned long long LOOK_AGG = 1ull << 30, LOOK_PREFIX = 2ull << 30;
-constexpr int LOOK_SPIN_LIMIT = 1 << 22;
-
 // any cell out of bounds: sum of the spread counters (each slot is one cache
 // line, so the update kernel's blocks do not queue on a single address)
 __device__ inline bool oob_any(const DevState& st) {
@@ -1934,13 +2049,7 @@ __device__ inline bool oob_any(const DevState& st) {
 __global__ void __launch_bounds__(256) k_bnd_list(DevState st, int nWords, int nBlocks, unsigned epoch) {
     __shared__ int lds[16];
     __shared__ int sTicket, sBase;
-    if (threadIdx.x == 0) {
-        const int t = atomicAdd(&st.bcount[BND_LIST_TICKET], 1);
-        // the last ticket of this launch: every block holds its own, so the
-        // counter is ready for the next launch
-        if (t == nBlocks - 1) atomicExch(&st.bcount[BND_LIST_TICKET], 0);
-        sTicket = t;
-    }
+    if (threadIdx.x == 0) sTicket = take_ticket(&st.bcount[BND_LIST_TICKET], nBlocks);
     __syncthreads();
     const int b = sTicket;
     if (b == 0 && threadIdx.x >= 1 && threadIdx.x <= BND_PROGRESS) st.bcount[threadIdx.x] = 0;
@@ -1954,33 +2063,13 @@ __global__ void __launch_bounds__(256) k_bnd_list(DevState st, int nWords, int n
     for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o);
     if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = cnt;
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (threadIdx.x < 64) {
         const unsigned long long total = (unsigned long long)(lds[0] + lds[1] + lds[2] + lds[3]);
-        const unsigned long long tag = (unsigned long long)epoch << 32;
-        unsigned long long excl = 0;
-        if (b > 0) {
-            __hip_atomic_store(&st.bndLook[b], tag | LOOK_AGG | total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            int j = b - 1, spins = 0;
-            while (j >= 0) {
-                const unsigned long long v =
-                    __hip_atomic_load(&st.bndLook[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                if ((v >> 32) != epoch) {
-                    if (++spins > LOOK_SPIN_LIMIT) {
-                        atomicOr(st.errFlag, ERR_BOUND_STALL);
-                        break;
-                    }
-                    __builtin_amdgcn_s_sleep(1);
-                    continue;
-                }
-                excl += v & (LOOK_AGG - 1);
-                if (v & LOOK_PREFIX) break;
-                --j;
-            }
+        const unsigned long long excl = look_back(st.bndLook, b, epoch, total, st.errFlag, ERR_BOUND_STALL);
+        if (threadIdx.x == 0) {
+            if (b == nBlocks - 1) st.bcount[0] = (int)(excl + total);
+            sBase = (int)excl;
         }
-        __hip_atomic_store(&st.bndLook[b], tag | LOOK_PREFIX | (excl + total), __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
-        if (b == nBlocks - 1) st.bcount[0] = (int)(excl + total);
-        sBase = (int)excl;
     }
     __syncthreads();
     int base = sBase;
@@ -2545,14 +2634,6 @@ __global__ void __launch_bounds__(256) k_bnd_reset(DevMesh m, DevState st, int p
     }
 }
 
-// epochs of the candidate-list launches, never 0 (the cleared look-back words)
-static unsigned next_list_epoch() {
-    static std::atomic<unsigned> epoch{0};
-    unsigned e = ++epoch;
-    while (e == 0) e = ++epoch;
-    return e;
-}
-
 template <int MF>
 void launch_bound_pass(const DevMesh& m, const DevState& st, int pass, double dt, const double* Sp,
                        const double* Su, hipStream_t s) {
@@ -2604,7 +2685,6 @@ void launch_advect_stage(int stage, const DevMesh& m, const DevState& st, const 
     const int nb = detect_blocks(m.nC);
     switch (stage) {
         case STAGE_UPWIND:
-            k_zero_int<<<1, 256, 0, s>>>(st.oobCount, OOB_SLOTS * OOB_STRIDE);
             k_upwind<<<cdiv(m.nF, 256), 256, 0, s>>>(m, st, dt);
             break;
         case STAGE_IFLUX:

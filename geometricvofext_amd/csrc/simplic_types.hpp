@@ -56,7 +56,6 @@ struct DevState {
     double* alphaPhi;   // nF
     double* corr;       // nF, bounding corrections (kept zero between passes)
     unsigned char* mixedFlag;   // nC
-    int* blockCount;    // per detection block
     int* nMixed;        // [1]
     int* mixed;         // capacity nC
     int* status;
@@ -83,6 +82,8 @@ struct DevState {
     unsigned long long* boundTotals;  // [5] running sums: candidates, active, rounds, corrected faces, passes
     int* bcount;        // [16] per-pass counters of the bounding kernels
     unsigned long long* bndLook;  // per candidate-list block: epoch, look-back flag and count
+    unsigned long long* detLook;  // per detection block: the same
+    int* detTicket;               // [1] detection block tickets
     int un0Fused;       // plane placement by 8-lane groups, which also computes Un0
     int fsdKeep;        // 1: where findSignedDistance writes no d/C/S they keep their values (isoRDF)
     int gridRounds;     // bounding rounds run as grid-wide kernels before the one-workgroup tail
@@ -119,6 +120,7 @@ enum : int {
     ERR_TOO_MANY_CROSSINGS = 2,
     ERR_BOUND_SCHEDULE = 4,
     ERR_BOUND_STALL = 16,
+    ERR_DETECT_STALL = 32,
 };
 constexpr int MAX_BOUND_PASSES = 64;
 constexpr int MAX_GRID_ROUNDS = 8;
