@@ -443,7 +443,7 @@ __device__ inline void gatherFace(const Poly& P, int f, V3* buf, int& n) {
 // cutCell::calcInterfaceCentreAndArea (cutCell.C:37-100) over face records
 // in cell-face order; the orientation test is against the zero vector, which
 // is what subCellCentre_ holds at that point of the reference.
-__device__ inline void interfaceCentreAndArea(const FaceCut* fc, int nF, V3& ifC, V3& ifS) {
+__device__ __attribute__((always_inline)) inline void interfaceCentreAndArea(const FaceCut* fc, int nF, V3& ifC, V3& ifS) {
     V3 fCentre = VZERO;
     int nEdgePoints = 0;
     for (int f = 0; f < nF; ++f) {
@@ -480,7 +480,7 @@ __device__ inline void interfaceCentreAndArea(const FaceCut* fc, int nF, V3& ifC
 
 // cutCell::calcSubCell (cutCell.C:343-542) combining face records fc[0..nF)
 // produced in cell-face order.  Cc, Vc: cell centre and volume.
-__device__ inline void combineSubCell(const FaceCut* fc, int nF, V3 Cc, double Vc, CellCut& out) {
+__device__ __attribute__((always_inline)) inline void combineSubCell(const FaceCut* fc, int nF, V3 Cc, double Vc, CellCut& out) {
     bool fullySubmerged = true, fullyEmpty = true;
     int nSubmergedFaces = 0;
     for (int f = 0; f < nF; ++f) {

@@ -946,16 +946,16 @@ __global__ void __launch_bounds__(64) k_find_signed_distance(DevMesh m, DevState
     }
 }
 
-// findSignedDistance with one 8-lane group per mixed cell and one lane per
-// cell face (cells of at most 8 faces and 16 points, faces as stored).  A
+// findSignedDistance with one G-lane group per mixed cell and one lane per
+// cell face (cells of at most G faces and 16 points, faces as stored; G = 8,
+// or 4 on meshes of tetrahedra, which places 16 cells per wave).  A
 // plane cut = every lane cuts its face (points in registers), writes the face
 // record to LDS, and every lane of the group combines the records in face
 // order, so all lanes hold the same VOF and follow the same bracketing and
 // Newton steps as the one-lane version.  The vertex levels are sorted by rank
 // in LDS.  One wave per workgroup: LDS traffic of a wave is in order.
-constexpr int FSD_G = 8;
 
-__device__ inline void fsd_face_cut(int fsz, const V3 (&q4)[4], const V3 (&q3)[3], V3 n, double dist,
+__device__ __attribute__((always_inline)) inline void fsd_face_cut(int fsz, const V3 (&q4)[4], const V3 (&q3)[3], V3 n, double dist,
                                     FaceCut& fc, bool& ok) {
     if (fsz == 4) ok &= calcSubFaceN<4>(q4, n, dist, fc);
     else ok &= calcSubFaceN<3>(q3, n, dist, fc);
@@ -970,13 +970,16 @@ __device__ inline void fsd_face_cut(int fsz, const V3 (&q4)[4], const V3 (&q3)[3
 #define SVP_FLUX_WAVES 1
 #endif
 
+template <int G>
 __global__ void __launch_bounds__(64, SVP_FSD_WAVES) k_fsd_group(DevMesh m, DevState st, const int* wl, const int* wn) {
-    __shared__ FaceCut rec[64 / FSD_G][FSD_G];
-    __shared__ double vraw[64 / FSD_G][NS_CELL_POINTS];
-    __shared__ double vsrt[64 / FSD_G][NS_CELL_POINTS];
-    const int g = threadIdx.x / FSD_G, l = threadIdx.x % FSD_G;
+    static_assert(G >= 3 && 64 % G == 0 && NS_CELL_POINTS % G == 0, "group of at least 3 lanes dividing the wave");
+    constexpr int PPL = NS_CELL_POINTS / G;  // cell points per lane
+    __shared__ FaceCut rec[64 / G][G];
+    __shared__ double vraw[64 / G][NS_CELL_POINTS];
+    __shared__ double vsrt[64 / G][NS_CELL_POINTS];
+    const int g = threadIdx.x / G, l = threadIdx.x % G;
     const int nWork = wl ? *wn : *st.nMixed;
-    for (int wi = blockIdx.x * (64 / FSD_G) + g; wi < nWork; wi += gridDim.x * (64 / FSD_G)) {
+    for (int wi = blockIdx.x * (64 / G) + g; wi < nWork; wi += gridDim.x * (64 / G)) {
         const int i = wl ? wl[wi] : wi;
         const int c = st.mixed[i];
         const int fb = m.cOff[c], nF = m.cOff[c + 1] - fb;
@@ -1020,21 +1023,36 @@ __global__ void __launch_bounds__(64, SVP_FSD_WAVES) k_fsd_group(DevMesh m, DevS
         }
         // vertex levels -n.x of the cell points, sorted descending by rank
         const int pb = m.cpOff[c], np = m.cpOff[c + 1] - pb;
+        // (points past PPL per lane cannot occur: np <= NS_CELL_POINTS = PPL * G)
         double va = 0.0, vb = 0.0;
         if (l < np) va = -dot(normalI, ld3(m.pts, m.cPts[pb + l]));
-        if (l + FSD_G < np) vb = -dot(normalI, ld3(m.pts, m.cPts[pb + l + FSD_G]));
+        if (l + G < np) vb = -dot(normalI, ld3(m.pts, m.cPts[pb + l + G]));
         if (l < np) vraw[g][l] = va;
-        if (l + FSD_G < np) vraw[g][l + FSD_G] = vb;
+        if (l + G < np) vraw[g][l + G] = vb;
+        if constexpr (PPL > 2) {
+            for (int k = l + 2 * G; k < np; k += G) vraw[g][k] = -dot(normalI, ld3(m.pts, m.cPts[pb + k]));
+        }
         __builtin_amdgcn_wave_barrier();
         {
             int ra = 0, rb = 0;
             for (int j = 0; j < np; ++j) {
                 const double v = vraw[g][j];
                 ra += (v > va) || (v == va && j < l);
-                rb += (v > vb) || (v == vb && j < l + FSD_G);
+                rb += (v > vb) || (v == vb && j < l + G);
             }
             if (l < np) vsrt[g][ra] = va;
-            if (l + FSD_G < np) vsrt[g][rb] = vb;
+            if (l + G < np) vsrt[g][rb] = vb;
+            if constexpr (PPL > 2) {
+                for (int k = l + 2 * G; k < np; k += G) {
+                    const double vk = vraw[g][k];
+                    int rk = 0;
+                    for (int j = 0; j < np; ++j) {
+                        const double v = vraw[g][j];
+                        rk += (v > vk) || (v == vk && j < k);
+                    }
+                    vsrt[g][rk] = vk;
+                }
+            }
         }
         __builtin_amdgcn_wave_barrier();
         bool ok = true;
@@ -1181,15 +1199,15 @@ __global__ void __launch_bounds__(64, SVP_FSD_WAVES) k_fsd_group(DevMesh m, DevS
             }
             // first face (in cell-face order) with a containing tet, else the
             // tet of largest minimum weight, earliest on ties
-            const unsigned long long grpMask = 0xFFull << (g * FSD_G);
+            const unsigned long long grpMask = ((1ull << G) - 1ull) << (g * G);
             const unsigned long long foundMask = __ballot(found && l < nF) & grpMask;
             int src = -1;
             if (foundMask) {
-                src = __ffsll((long long)foundMask) - 1 - g * FSD_G;
+                src = __ffsll((long long)foundMask) - 1 - g * G;
             } else {
                 double best = -1e300;
                 for (int k = 0; k < nF; ++k) {
-                    const double v = __shfl(bestMin, g * FSD_G + k);
+                    const double v = __shfl(bestMin, g * G + k);
                     if (v > best) {
                         best = v;
                         src = k;
@@ -1199,9 +1217,9 @@ __global__ void __launch_bounds__(64, SVP_FSD_WAVES) k_fsd_group(DevMesh m, DevS
             double w[4];
             int pp[3];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) w[q] = __shfl(foundMask ? fw[q] : bw[q], g * FSD_G + src);
+            for (int q = 0; q < 4; ++q) w[q] = __shfl(foundMask ? fw[q] : bw[q], g * G + src);
 #pragma unroll
-            for (int q = 0; q < 3; ++q) pp[q] = __shfl(foundMask ? fp[q] : bp[q], g * FSD_G + src);
+            for (int q = 0; q < 3; ++q) pp[q] = __shfl(foundMask ? fp[q] : bp[q], g * G + src);
             // point values: lane q of the group evaluates point q
             V3 pu = VZERO;
             if (l < 3) {
@@ -1215,7 +1233,7 @@ __global__ void __launch_bounds__(64, SVP_FSD_WAVES) k_fsd_group(DevMesh m, DevS
             V3 pv[3];
 #pragma unroll
             for (int q = 0; q < 3; ++q)
-                pv[q] = v3(__shfl(pu.x, g * FSD_G + q), __shfl(pu.y, g * FSD_G + q), __shfl(pu.z, g * FSD_G + q));
+                pv[q] = v3(__shfl(pu.x, g * G + q), __shfl(pu.y, g * G + q), __shfl(pu.z, g * G + q));
             const V3 Ui = w[0] * ld3(st.U, c) + w[1] * pv[0] + w[2] * pv[1] + w[3] * pv[2];
             un0 = dot(Ui, normalI);
         }
@@ -1227,7 +1245,8 @@ __global__ void __launch_bounds__(64, SVP_FSD_WAVES) k_fsd_group(DevMesh m, DevS
 void launch_find_signed_distance(const DevMesh& m, const DevState& st, const Params& p, hipStream_t s,
                                  const int* wl, const int* wn) {
     if (p.splitWarpedFace) k_find_signed_distance<CellPolySP><<<1024, 64, 0, s>>>(m, st, 1, wl, wn);
-    else if (st.un0Fused) k_fsd_group<<<2048, 64, 0, s>>>(m, st, wl, wn);
+    else if (st.un0Fused && st.maxCellFaces <= 4) k_fsd_group<4><<<2048, 64, 0, s>>>(m, st, wl, wn);
+    else if (st.un0Fused) k_fsd_group<8><<<2048, 64, 0, s>>>(m, st, wl, wn);
     else k_find_signed_distance<CellPolyNS><<<1024, 64, 0, s>>>(m, st, 0, wl, wn);
 }
 
