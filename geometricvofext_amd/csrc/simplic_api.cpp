@@ -574,6 +574,7 @@ private:
     RdfBufs rdf_{};
     bool alphaPhiValid_ = false;
     int maxCellFaces_ = 0;
+    long cellFaceEntries_ = 0;  // size of the cell -> face CSR
     int profMask_ = 0;
     double profSec_[N_STAGES] = {};
     int64_t profCnt_[N_STAGES] = {};
@@ -667,6 +668,7 @@ private:
         for (int f = 0; f < nIF; ++f) cnt[nbr[f] + 1]++;
         std::vector<int> cOff(nC + 1, 0);
         for (int c = 0; c < nC; ++c) cOff[c + 1] = cOff[c] + cnt[c + 1];
+        cellFaceEntries_ = cOff[nC];
         std::vector<int> fill(cOff.begin(), cOff.end() - 1), cFaces(cOff[nC]), cFacesAsc(cOff[nC]);
         for (int f = 0; f < nF; ++f) cFaces[fill[own[f]]++] = f;
         for (int f = 0; f < nIF; ++f) cFaces[fill[nbr[f]]++] = f;
@@ -799,6 +801,8 @@ private:
         st_.ifS = dalloc<double>(3L * nC);
         st_.un0 = dalloc<double>(nC);
         st_.oobCount = dalloc<int>(OOB_SLOTS * OOB_STRIDE);
+        st_.fluxList = dalloc<int>(cellFaceEntries_);
+        st_.fluxCount = dalloc<int>(1);
         st_.candCount = dalloc<int>(1);
         st_.candBits = dalloc<unsigned long long>((nC + 63) / 64);
         st_.cand = dalloc<int>(nC);

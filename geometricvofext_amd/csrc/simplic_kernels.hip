@@ -1388,9 +1388,12 @@ void launch_plic_surface(const DevMesh& m, const DevState& st, int nMixed, int p
 // lambda*(aP - aN) + aN with lambda = pos0(phi); zeroGradient boundary value
 // is the owner value; empty patches carry nothing.
 __global__ void __launch_bounds__(256) k_upwind(DevMesh m, DevState st, double dtStep) {
-    // the out-of-bounds counters of the update that follows start at zero
-    if (blockIdx.x == 0)
+    // the out-of-bounds counters of the update and the flux work-list count
+    // of the interface flux that follow start at zero
+    if (blockIdx.x == 0) {
         for (int k = threadIdx.x; k < OOB_SLOTS * OOB_STRIDE; k += blockDim.x) st.oobCount[k] = 0;
+        if (threadIdx.x == 0) *st.fluxCount = 0;
+    }
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= m.nF) return;
     const double dt = cell_dt(st, m.own[f], dtStep);
@@ -1710,25 +1713,64 @@ __global__ void __launch_bounds__(64) k_un0(DevMesh m, DevState st) {
     }
 }
 
-// timeIntegratedFlux with one thread per (cut cell, cell face): each downwind
-// face's volume is independent of the others
-template <bool TRIQUAD>
-__global__ void __launch_bounds__(64, SVP_FLUX_WAVES) k_interface_flux_faces(DevMesh m, DevState st, double dtStep, int slots) {
+// The (cut cell, cell face) pairs whose face is downwind of the cell, packed
+// as i * slots + k: about half of a cut cell's faces, so the flux kernel
+// below runs on full waves.  Wave-aggregated appends (the order does not
+// matter: each face is downwind of one cell and gets one volume).  The count
+// is cleared by the upwind kernel.
+__global__ void __launch_bounds__(256) k_flux_list(DevMesh m, DevState st, int slots) {
     const int nMixed = *st.nMixed;
     const long total = (long)nMixed * slots;
-    for (long w = (long)blockIdx.x * blockDim.x + threadIdx.x; w < total; w += (long)gridDim.x * blockDim.x) {
-        const int i = (int)(w / slots), k = (int)(w % slots);
-        if (st.status[i] != 0) continue;
+    const int lane = threadIdx.x & 63;
+    const long stride = (long)gridDim.x * blockDim.x;
+    for (long w0 = (long)blockIdx.x * blockDim.x + (threadIdx.x & ~63); w0 < total; w0 += stride) {
+        const long w = w0 + lane;
+        bool take = false;
+        if (w < total) {
+            const int i = (int)(w / slots), k = (int)(w % slots);
+            if (st.status[i] == 0) {
+                const int c = st.mixed[i];
+                const int fb = m.cOff[c];
+                if (k < m.cOff[c + 1] - fb) {
+                    const int f = m.cFaces[fb + k];
+                    const double ph = st.phi[f];
+                    if (f < m.nIF) take = (m.own[f] == c) ? (ph >= 0.0) : (ph < 0.0);
+                    else take = !m.fEmpty[f] && (ph >= 0.0);
+                }
+            }
+        }
+        const unsigned long long bal = __ballot(take);
+        int base = 0;
+        if (lane == 0 && bal) base = atomicAdd(st.fluxCount, __popcll(bal));
+        base = __shfl(base, 0);
+        if (take) st.fluxList[base + __popcll(bal & ((1ull << lane) - 1ull))] = (int)w;
+    }
+}
+
+// timeIntegratedFlux with one thread per (cut cell, downwind face): each
+// downwind face's volume is independent of the others.  LISTED takes the
+// pairs from the work list; otherwise every (cut cell, face slot) pair is a
+// thread and the upwind faces drop out (cheaper for tetrahedra, whose light
+// faces do not repay the list).
+template <bool TRIQUAD, bool LISTED>
+__global__ void __launch_bounds__(64, SVP_FLUX_WAVES) k_interface_flux_faces(DevMesh m, DevState st, double dtStep, int slots) {
+    const long nWork = LISTED ? (long)*st.fluxCount : (long)*st.nMixed * slots;
+    for (long wi = (long)blockIdx.x * blockDim.x + threadIdx.x; wi < nWork; wi += (long)gridDim.x * blockDim.x) {
+        const int w = LISTED ? st.fluxList[wi] : (int)wi;
+        const int i = w / slots, k = w % slots;
+        if (!LISTED && st.status[i] != 0) continue;
         const int c = st.mixed[i];
         const int fb = m.cOff[c];
-        if (k >= m.cOff[c + 1] - fb) continue;
+        if (!LISTED && k >= m.cOff[c + 1] - fb) continue;
         const double dt = cell_dt(st, c, dtStep);
         const int f = m.cFaces[fb + k];
         const double ph = st.phi[f];
-        bool down;
-        if (f < m.nIF) down = (m.own[f] == c) ? (ph >= 0.0) : (ph < 0.0);
-        else down = !m.fEmpty[f] && (ph >= 0.0);
-        if (!down) continue;
+        if (!LISTED) {
+            bool down;
+            if (f < m.nIF) down = (m.own[f] == c) ? (ph >= 0.0) : (ph < 0.0);
+            else down = !m.fEmpty[f] && (ph >= 0.0);
+            if (!down) continue;
+        }
         const V3 n = ld3(st.ifN, i);
         const double d = st.ifD[i], Un0 = st.un0[i];
         const int s0 = m.fOff[f], fsz = m.fOff[f + 1] - s0;
@@ -2708,8 +2750,14 @@ void launch_advect_stage(int stage, const DevMesh& m, const DevState& st, const 
             break;
         case STAGE_IFLUX:
             if (!st.un0Fused) k_un0<<<512, 64, 0, s>>>(m, st);
-            if (m.triQuadFaces) k_interface_flux_faces<true><<<2048, 64, 0, s>>>(m, st, dt, st.maxCellFaces);
-            else k_interface_flux_faces<false><<<2048, 64, 0, s>>>(m, st, dt, st.maxCellFaces);
+            if (st.maxCellFaces > 4) {
+                k_flux_list<<<1024, 256, 0, s>>>(m, st, st.maxCellFaces);
+                if (m.triQuadFaces) k_interface_flux_faces<true, true><<<2048, 64, 0, s>>>(m, st, dt, st.maxCellFaces);
+                else k_interface_flux_faces<false, true><<<2048, 64, 0, s>>>(m, st, dt, st.maxCellFaces);
+            } else {
+                if (m.triQuadFaces) k_interface_flux_faces<true, false><<<2048, 64, 0, s>>>(m, st, dt, st.maxCellFaces);
+                else k_interface_flux_faces<false, false><<<2048, 64, 0, s>>>(m, st, dt, st.maxCellFaces);
+            }
             break;
         case STAGE_UPDATE:
             k_update<<<nb, DETECT_BLOCK, 0, s>>>(m, st, dt, Sp, Su);

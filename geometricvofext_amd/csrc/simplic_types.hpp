@@ -68,6 +68,8 @@ struct DevState {
     double* un0;
     // bounding bookkeeping (limitFlux/boundFlux)
     int* oobCount;      // cells with alpha-1 > aTol or alpha < -aTol, spread over OOB_SLOTS counters
+    int* fluxList;      // (cut cell, downwind face) pairs of the interface flux, cell-face CSR size
+    int* fluxCount;     // [1]
     int* candCount;     // [1] candidates of the last pass (diagnostics)
     unsigned long long* candBits;  // nC bits: alpha outside [-aTol,1+aTol] and needBounding
     int* cand;          // nC, ascending candidates of the current pass
