@@ -947,8 +947,9 @@ __global__ void __launch_bounds__(64) k_find_signed_distance(DevMesh m, DevState
 }
 
 // findSignedDistance with one G-lane group per mixed cell and one lane per
-// cell face (cells of at most G faces and 16 points, faces as stored; G = 8,
-// or 4 on meshes of tetrahedra, which places 16 cells per wave).  A
+// cell face (cells of at most G faces and 16 points, faces as stored; G is
+// the mesh's largest face count: 4 for tetrahedra (16 cells per wave), 6 for
+// hexahedra (10 cells per wave, 4 lanes idle), else 8).  A
 // plane cut = every lane cuts its face (points in registers), writes the face
 // record to LDS, and every lane of the group combines the records in face
 // order, so all lanes hold the same VOF and follow the same bracketing and
@@ -972,14 +973,16 @@ __device__ __attribute__((always_inline)) inline void fsd_face_cut(int fsz, cons
 
 template <int G>
 __global__ void __launch_bounds__(64, SVP_FSD_WAVES) k_fsd_group(DevMesh m, DevState st, const int* wl, const int* wn) {
-    static_assert(G >= 3 && 64 % G == 0 && NS_CELL_POINTS % G == 0, "group of at least 3 lanes dividing the wave");
-    constexpr int PPL = NS_CELL_POINTS / G;  // cell points per lane
-    __shared__ FaceCut rec[64 / G][G];
-    __shared__ double vraw[64 / G][NS_CELL_POINTS];
-    __shared__ double vsrt[64 / G][NS_CELL_POINTS];
+    static_assert(G >= 3 && G <= 8, "groups of 3 to 8 lanes");
+    constexpr int NG = 64 / G;                         // groups per wave; lanes past NG * G idle
+    constexpr int PPL = (NS_CELL_POINTS + G - 1) / G;  // cell points per lane
+    __shared__ FaceCut rec[NG][G];
+    __shared__ double vraw[NG][NS_CELL_POINTS];
+    __shared__ double vsrt[NG][NS_CELL_POINTS];
     const int g = threadIdx.x / G, l = threadIdx.x % G;
+    if (g >= NG) return;
     const int nWork = wl ? *wn : *st.nMixed;
-    for (int wi = blockIdx.x * (64 / G) + g; wi < nWork; wi += gridDim.x * (64 / G)) {
+    for (int wi = blockIdx.x * NG + g; wi < nWork; wi += gridDim.x * NG) {
         const int i = wl ? wl[wi] : wi;
         const int c = st.mixed[i];
         const int fb = m.cOff[c], nF = m.cOff[c + 1] - fb;
@@ -1246,6 +1249,7 @@ void launch_find_signed_distance(const DevMesh& m, const DevState& st, const Par
                                  const int* wl, const int* wn) {
     if (p.splitWarpedFace) k_find_signed_distance<CellPolySP><<<1024, 64, 0, s>>>(m, st, 1, wl, wn);
     else if (st.un0Fused && st.maxCellFaces <= 4) k_fsd_group<4><<<2048, 64, 0, s>>>(m, st, wl, wn);
+    else if (st.un0Fused && st.maxCellFaces <= 6) k_fsd_group<6><<<2048, 64, 0, s>>>(m, st, wl, wn);
     else if (st.un0Fused) k_fsd_group<8><<<2048, 64, 0, s>>>(m, st, wl, wn);
     else k_find_signed_distance<CellPolyNS><<<1024, 64, 0, s>>>(m, st, 0, wl, wn);
 }
