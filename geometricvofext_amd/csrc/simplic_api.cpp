@@ -111,7 +111,7 @@ public:
         HIPCHECK(hipEventRecord(a, stream_));
         std::swap(st_.alpha, st_.alphaOld);
         for (int sg = STAGE_UPWIND; sg <= STAGE_SNAPCLIP; ++sg)
-            stage(sg, [&] { launch_advect_stage(sg, m_, st_, params_, dt, Sp, Su, stream_); });
+            stage(sg, [&] { launch_advect_stage(sg, m_, st_, params_, dt, Sp, Su, &bndPassSeq_, stream_); });
         HIPCHECK(hipEventRecord(b, stream_));
         HIPCHECK(hipGetLastError());
         pendingAdvect_.push_back({a, b});
@@ -575,6 +575,7 @@ private:
     bool alphaPhiValid_ = false;
     int maxCellFaces_ = 0;
     long cellFaceEntries_ = 0;  // size of the cell -> face CSR
+    int bndPassSeq_ = 0;        // bounding passes launched so far (candidate-list half)
     int profMask_ = 0;
     double profSec_[N_STAGES] = {};
     int64_t profCnt_[N_STAGES] = {};
@@ -805,7 +806,7 @@ private:
         st_.fluxCount = dalloc<int>(1);
         st_.candCount = dalloc<int>(1);
         st_.candBits = dalloc<unsigned long long>((nC + 63) / 64);
-        st_.cand = dalloc<int>(nC);
+        st_.cand = dalloc<int>(2L * nC);
         st_.active = dalloc<unsigned char>(nC);
         st_.pend = dalloc<int>(nC);
         st_.queue = dalloc<int>(2L * nC);
@@ -820,11 +821,11 @@ private:
         st_.touchedFlag = dalloc<int>(nC);
         st_.boundStats = dalloc<int>(4 * MAX_BOUND_PASSES);
         st_.boundTotals = dalloc<unsigned long long>(5);
-        st_.bcount = dalloc<int>(16);
+        st_.bcount = dalloc<int>(32);
         st_.bndLook = dalloc<unsigned long long>((nC + 63) / 64 / 1024 + 1);
         st_.detLook = dalloc<unsigned long long>(detect_blocks(nC));
         st_.detTicket = dalloc<int>(1);
-        HIPCHECK(hipMemsetAsync(st_.bcount, 0, 16 * sizeof(int), stream_));
+        HIPCHECK(hipMemsetAsync(st_.bcount, 0, 32 * sizeof(int), stream_));
         st_.maxCellFaces = maxCellFaces_;
         // bounding rounds run grid-wide before the one-workgroup tail (tuning
         // knob SVP_BND_GRID_ROUNDS, 0..MAX_GRID_ROUNDS; results do not depend on it)

@@ -2102,6 +2102,8 @@ __device__ inline void bound_finish(const BndRec<MF>& R, const double (&cin)[MF]
 constexpr int BND_WORDS_PER_BLOCK = 1024;
 constexpr int BND_LIST_TICKET = 15;
 static_assert(BND_LIST_TICKET > BND_PROGRESS, "list ticket outside the cleared counters");
+// candidates of the previous pass, whose leftovers the next list clears
+constexpr int BND_PREV_COUNT = 16;
 // any cell out of bounds: sum of the spread counters (each slot is one cache
 // line, so the update kernel's blocks do not queue on a single address)
 __device__ inline bool oob_any(const DevState& st) {
@@ -2111,12 +2113,27 @@ __device__ inline bool oob_any(const DevState& st) {
     return n != 0;
 }
 
-__global__ void __launch_bounds__(256) k_bnd_list(DevState st, int nWords, int nBlocks, unsigned epoch) {
+__global__ void __launch_bounds__(256) k_bnd_list(DevMesh m, DevState st, int nWords, int nBlocks, unsigned epoch,
+                                                  const int* candPrev) {
     __shared__ int lds[16];
     __shared__ int sTicket, sBase;
     if (threadIdx.x == 0) sTicket = take_ticket(&st.bcount[BND_LIST_TICKET], nBlocks);
     __syncthreads();
     const int b = sTicket;
+    // what the previous pass (of this step or the last one) left: the
+    // corrections and correctedFaces keys on the faces of its active cells
+    // and their candidate index, back to 0 / -1 before this pass's records
+    const int nPrev = st.bcount[BND_PREV_COUNT];
+    for (int i = b * blockDim.x + threadIdx.x; i < nPrev; i += nBlocks * blockDim.x) {
+        if (!st.active[i]) continue;
+        const int c = candPrev[i];
+        for (int k = m.cOff[c]; k < m.cOff[c + 1]; ++k) {
+            const int f = m.cFaces[k];
+            st.corr[f] = 0.0;
+            st.corrPos[f] = -1;
+        }
+        st.actIdx[c] = -1;
+    }
     if (b == 0 && threadIdx.x >= 1 && threadIdx.x <= BND_PROGRESS) st.bcount[threadIdx.x] = 0;
     if (!oob_any(st)) {
         if (b == 0 && threadIdx.x == 0) st.bcount[0] = 0;
@@ -2587,9 +2604,27 @@ __global__ void __launch_bounds__(flow_threads<MF>()) k_bnd_flow(DevMesh m, DevS
 // and the dVf of every corrected face (the first-loop faces of the active
 // candidates)
 template <int MF>
-__global__ void __launch_bounds__(256) k_bnd_apply(DevMesh m, DevState st) {
+__global__ void __launch_bounds__(256) k_bnd_apply(DevMesh m, DevState st, int pass) {
     const double aTol = 100.0 * SVP_SMALL;
     const int nT = st.bcount[2], nCand = st.bcount[0];
+    // pass statistics, and the candidate count for the next list's clearing
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        st.bcount[BND_PREV_COUNT] = nCand;
+        if (pass < MAX_BOUND_PASSES && nCand > 0) {
+            int nAct = 0;
+            for (int r = 0; r <= st.gridRounds; ++r) nAct += st.bcount[3 + r];
+            st.boundStats[4 * pass + 0] = nCand;
+            st.boundStats[4 * pass + 1] = nAct;
+            st.boundStats[4 * pass + 2] = st.bcount[BND_CTRL - 1];
+            st.boundStats[4 * pass + 3] = st.bcount[1];
+            st.boundTotals[0] += (unsigned long long)nCand;
+            st.boundTotals[1] += (unsigned long long)nAct;
+            st.boundTotals[2] += (unsigned long long)st.bcount[BND_CTRL - 1];
+            st.boundTotals[3] += (unsigned long long)st.bcount[1];
+            st.boundTotals[4] += 1ull;
+            *st.candCount = nCand;
+        }
+    }
     for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < nT + nCand; idx += gridDim.x * blockDim.x) {
         if (idx >= nT) {
             const int i = idx - nT;
@@ -2669,45 +2704,15 @@ __global__ void __launch_bounds__(256) k_bnd_apply(DevMesh m, DevState st) {
     }
 }
 
-__global__ void __launch_bounds__(256) k_bnd_reset(DevMesh m, DevState st, int pass) {
-    const int nCand = st.bcount[0];
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nCand; i += gridDim.x * blockDim.x) {
-        if (!st.active[i]) continue;
-        const int c = st.cand[i];
-        for (int k = m.cOff[c]; k < m.cOff[c + 1]; ++k) {
-            const int f = m.cFaces[k];
-            if (face_downwind_of(m, st.phi, f, c)) {
-                st.corr[f] = 0.0;
-                st.corrPos[f] = -1;
-            }
-        }
-        st.actIdx[c] = -1;
-    }
-    if (blockIdx.x == 0 && threadIdx.x == 0 && pass < MAX_BOUND_PASSES && nCand > 0) {
-        int nAct = 0;
-        for (int r = 0; r <= st.gridRounds; ++r) nAct += st.bcount[3 + r];
-        st.boundStats[4 * pass + 0] = nCand;
-        st.boundStats[4 * pass + 1] = nAct;
-        st.boundStats[4 * pass + 2] = st.bcount[BND_CTRL - 1];
-        st.boundStats[4 * pass + 3] = st.bcount[1];
-        st.boundTotals[0] += (unsigned long long)nCand;
-        st.boundTotals[1] += (unsigned long long)nAct;
-        st.boundTotals[2] += (unsigned long long)st.bcount[BND_CTRL - 1];
-        st.boundTotals[3] += (unsigned long long)st.bcount[1];
-        st.boundTotals[4] += 1ull;
-        *st.candCount = nCand;
-    }
-}
-
 template <int MF>
 void launch_bound_pass(const DevMesh& m, const DevState& st, int pass, double dt, const double* Sp,
-                       const double* Su, hipStream_t s) {
+                       const double* Su, const int* candPrev, hipStream_t s) {
     const int nWords = (m.nC + 63) / 64;
     // candidate-sized work: a modest grid-stride grid launches faster than
     // one thread per cell of the mesh
     const int grid = std::max(1, std::min(st.bndGrid, cdiv(m.nC, 256)));
     const int nbw = cdiv(nWords, BND_WORDS_PER_BLOCK);
-    k_bnd_list<<<nbw, 256, 0, s>>>(st, nWords, nbw, next_list_epoch());
+    k_bnd_list<<<nbw, 256, 0, s>>>(m, st, nWords, nbw, next_list_epoch(), candPrev);
     k_bnd_prep<MF><<<grid, 256, 0, s>>>(m, st, dt, Sp, Su);
     if (st.bndFlow) {
         k_bnd_flow<MF><<<std::max(1, std::min(st.bndFlow, cdiv(m.nC, flow_chunk<MF>()))), flow_threads<MF>(), 0, s>>>(
@@ -2717,16 +2722,22 @@ void launch_bound_pass(const DevMesh& m, const DevState& st, int pass, double dt
         for (int r = 0; r < st.gridRounds; ++r) k_bnd_round<MF><<<grid, 256, 0, s>>>(m, st, r, dt);
         k_bnd_tail<MF><<<1, TAIL_THREADS, 0, s>>>(m, st, dt);
     }
-    k_bnd_apply<MF><<<grid, 256, 0, s>>>(m, st);
-    k_bnd_reset<<<grid, 256, 0, s>>>(m, st, pass);
+    k_bnd_apply<MF><<<grid, 256, 0, s>>>(m, st, pass);
 }
 
+// The candidate list alternates between the two halves of st.cand by the
+// handle's running pass count, so a pass's list kernel can clear what the
+// previous pass left (its candidates are still in the other half).
 void launch_bound(const DevMesh& m, const DevState& st, const Params& p, double dt, const double* Sp,
-                  const double* Su, hipStream_t s) {
+                  const double* Su, int* passSeq, hipStream_t s) {
     const int nPasses = std::min(p.nAlphaBounds, MAX_BOUND_PASSES);
     for (int pass = 0; pass < nPasses; ++pass) {
-        if (st.maxCellFaces <= NS_CELL_FACES) launch_bound_pass<NS_CELL_FACES>(m, st, pass, dt, Sp, Su, s);
-        else launch_bound_pass<MAX_CELL_FACES>(m, st, pass, dt, Sp, Su, s);
+        DevState sp = st;
+        const int seq = (*passSeq)++;
+        sp.cand = st.cand + (long)(seq & 1) * m.nC;
+        const int* candPrev = st.cand + (long)((seq + 1) & 1) * m.nC;
+        if (st.maxCellFaces <= NS_CELL_FACES) launch_bound_pass<NS_CELL_FACES>(m, sp, pass, dt, Sp, Su, candPrev, s);
+        else launch_bound_pass<MAX_CELL_FACES>(m, sp, pass, dt, Sp, Su, candPrev, s);
     }
 }
 
@@ -2746,7 +2757,7 @@ __global__ void k_zero_int(int* p, int n) {
 }
 
 void launch_advect_stage(int stage, const DevMesh& m, const DevState& st, const Params& p, double dt,
-                         const double* Sp, const double* Su, hipStream_t s) {
+                         const double* Sp, const double* Su, int* passSeq, hipStream_t s) {
     const int nb = detect_blocks(m.nC);
     switch (stage) {
         case STAGE_UPWIND:
@@ -2767,7 +2778,7 @@ void launch_advect_stage(int stage, const DevMesh& m, const DevState& st, const 
             k_update<<<nb, DETECT_BLOCK, 0, s>>>(m, st, dt, Sp, Su);
             break;
         case STAGE_BOUND:
-            launch_bound(m, st, p, dt, Sp, Su, s);
+            launch_bound(m, st, p, dt, Sp, Su, passSeq, s);
             break;
         case STAGE_SNAPCLIP:
             if (p.snapTol > 0.0 || p.clip)

@@ -72,7 +72,7 @@ struct DevState {
     int* fluxCount;     // [1]
     int* candCount;     // [1] candidates of the last pass (diagnostics)
     unsigned long long* candBits;  // nC bits: alpha outside [-aTol,1+aTol] and needBounding
-    int* cand;          // nC, ascending candidates of the current pass
+    int* cand;          // 2*nC: ascending candidates of a pass, halves alternating by pass
     unsigned char* active;  // nC, per candidate: writes corrections this pass
     int* pend;          // nC, per candidate: active upwind predecessors not yet done
     int* queue;         // 2*nC, Kahn frontiers
@@ -82,7 +82,7 @@ struct DevState {
     int* touchedFlag;   // nC
     int* boundStats;    // [4*maxPasses]: candidates, active, rounds, corrected faces
     unsigned long long* boundTotals;  // [5] running sums: candidates, active, rounds, corrected faces, passes
-    int* bcount;        // [16] per-pass counters of the bounding kernels
+    int* bcount;        // [32] per-pass counters of the bounding kernels
     unsigned long long* bndLook;  // per candidate-list block: epoch, look-back flag and count
     unsigned long long* detLook;  // per detection block: the same
     int* detTicket;               // [1] detection block tickets
@@ -200,8 +200,10 @@ enum Stage : int {
     STAGE_SNAPCLIP = 7,
     N_STAGES = 8,
 };
+// passSeq: the handle's running count of bounding passes (selects the
+// candidate-list half of each pass)
 void launch_advect_stage(int stage, const DevMesh& m, const DevState& st, const Params& p, double dt,
-                         const double* Sp, const double* Su, hipStream_t s);
+                         const double* Sp, const double* Su, int* passSeq, hipStream_t s);
 void launch_alpha_phi(const DevMesh& m, const DevState& st, double dt, hipStream_t s);
 void launch_map_alpha(const DevMesh& m, const DevState& st, const double* n, const double* d, double lower,
                       double upper, hipStream_t s);
