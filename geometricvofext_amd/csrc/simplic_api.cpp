@@ -822,7 +822,8 @@ private:
         st_.boundStats = dalloc<int>(4 * MAX_BOUND_PASSES);
         st_.boundTotals = dalloc<unsigned long long>(5);
         st_.bcount = dalloc<int>(32);
-        st_.bndLook = dalloc<unsigned long long>((nC + 63) / 64 / 1024 + 1);
+        // one word per candidate-list block (blocks of at least 64 bitmap words)
+        st_.bndLook = dalloc<unsigned long long>((nC + 63) / 64 / 64 + 1);
         st_.detLook = dalloc<unsigned long long>(detect_blocks(nC));
         st_.detTicket = dalloc<int>(1);
         HIPCHECK(hipMemsetAsync(st_.bcount, 0, 32 * sizeof(int), stream_));
@@ -833,6 +834,13 @@ private:
         if (const char* e = std::getenv("SVP_BND_GRID_ROUNDS")) {
             const int r = std::atoi(e);
             if (r >= 0 && r <= MAX_GRID_ROUNDS) st_.gridRounds = r;
+        }
+        // bitmap words per candidate-list block (tuning knob SVP_BND_LIST_WORDS;
+        // results do not depend on it)
+        st_.bndListWords = 256;
+        if (const char* e = std::getenv("SVP_BND_LIST_WORDS")) {
+            const int w = std::atoi(e);
+            if (w >= 64 && w <= 4096) st_.bndListWords = w;
         }
         st_.bndGrid = 128;
         if (const char* e = std::getenv("SVP_BND_GRID")) {

@@ -1719,16 +1719,21 @@ __global__ void __launch_bounds__(64) k_un0(DevMesh m, DevState st) {
 
 // The (cut cell, cell face) pairs whose face is downwind of the cell, packed
 // as i * slots + k: about half of a cut cell's faces, so the flux kernel
-// below runs on full waves.  Wave-aggregated appends (the order does not
-// matter: each face is downwind of one cell and gets one volume).  The count
+// below runs on full waves.  Block-aggregated appends (the order does not
+// matter: each face is downwind of one cell and gets one volume), one atomic
+// per block and round (a single counter serialises its atomics).  The count
 // is cleared by the upwind kernel.
 __global__ void __launch_bounds__(256) k_flux_list(DevMesh m, DevState st, int slots) {
     const int nMixed = *st.nMixed;
     const long total = (long)nMixed * slots;
     const int lane = threadIdx.x & 63;
     const long stride = (long)gridDim.x * blockDim.x;
-    for (long w0 = (long)blockIdx.x * blockDim.x + (threadIdx.x & ~63); w0 < total; w0 += stride) {
-        const long w = w0 + lane;
+    __shared__ int wcnt[4], sBase;
+    const int wave = threadIdx.x >> 6;
+    // block-uniform trip count: every wave of the block takes part in the
+    // block's single atomic per round
+    for (long b0 = (long)blockIdx.x * blockDim.x; b0 < total; b0 += stride) {
+        const long w = b0 + threadIdx.x;
         bool take = false;
         if (w < total) {
             const int i = (int)(w / slots), k = (int)(w % slots);
@@ -1744,10 +1749,17 @@ __global__ void __launch_bounds__(256) k_flux_list(DevMesh m, DevState st, int s
             }
         }
         const unsigned long long bal = __ballot(take);
-        int base = 0;
-        if (lane == 0 && bal) base = atomicAdd(st.fluxCount, __popcll(bal));
-        base = __shfl(base, 0);
-        if (take) st.fluxList[base + __popcll(bal & ((1ull << lane) - 1ull))] = (int)w;
+        if (lane == 0) wcnt[wave] = __popcll(bal);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const int tot = wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3];
+            sBase = tot ? atomicAdd(st.fluxCount, tot) : 0;
+        }
+        __syncthreads();
+        int pre = sBase;
+        for (int k = 0; k < wave; ++k) pre += wcnt[k];
+        if (take) st.fluxList[pre + __popcll(bal & ((1ull << lane) - 1ull))] = (int)w;
+        __syncthreads();
     }
 }
 
@@ -2091,7 +2103,7 @@ __device__ inline void bound_finish(const BndRec<MF>& R, const double (&cin)[MF]
 }
 
 // Ascending candidates of the pass from the bitmap in one launch: every
-// block counts its 1024 words, publishes the count and looks back over the
+// block counts its st.bndListWords words, publishes the count and looks back over the
 // blocks before it for their prefix (a decoupled look-back), then writes its
 // candidates.  Blocks take tickets in the order they start, so a block only
 // waits for blocks that are already running and each of those waits only on
@@ -2099,7 +2111,6 @@ __device__ inline void bound_finish(const BndRec<MF>& R, const double (&cin)[MF]
 // makes any word of an earlier launch read as not yet published.  With a
 // zero bound count the pass is off (no candidates).  Ticket 0 also clears
 // the per-pass counters.
-constexpr int BND_WORDS_PER_BLOCK = 1024;
 constexpr int BND_LIST_TICKET = 15;
 static_assert(BND_LIST_TICKET > BND_PROGRESS, "list ticket outside the cleared counters");
 // candidates of the previous pass, whose leftovers the next list clears
@@ -2122,24 +2133,28 @@ __global__ void __launch_bounds__(256) k_bnd_list(DevMesh m, DevState st, int nW
     const int b = sTicket;
     // what the previous pass (of this step or the last one) left: the
     // corrections and correctedFaces keys on the faces of its active cells
-    // and their candidate index, back to 0 / -1 before this pass's records
-    const int nPrev = st.bcount[BND_PREV_COUNT];
-    for (int i = b * blockDim.x + threadIdx.x; i < nPrev; i += nBlocks * blockDim.x) {
-        if (!st.active[i]) continue;
-        const int c = candPrev[i];
-        for (int k = m.cOff[c]; k < m.cOff[c + 1]; ++k) {
-            const int f = m.cFaces[k];
-            st.corr[f] = 0.0;
-            st.corrPos[f] = -1;
+    // and their candidate index, back to 0 / -1 before this pass's records.
+    // Done last, off the look-back chain.
+    auto clearPrevious = [&]() {
+        const int nPrev = st.bcount[BND_PREV_COUNT];
+        for (int i = b * blockDim.x + threadIdx.x; i < nPrev; i += nBlocks * blockDim.x) {
+            if (!st.active[i]) continue;
+            const int c = candPrev[i];
+            for (int k = m.cOff[c]; k < m.cOff[c + 1]; ++k) {
+                const int f = m.cFaces[k];
+                st.corr[f] = 0.0;
+                st.corrPos[f] = -1;
+            }
+            st.actIdx[c] = -1;
         }
-        st.actIdx[c] = -1;
-    }
+    };
     if (b == 0 && threadIdx.x >= 1 && threadIdx.x <= BND_PROGRESS) st.bcount[threadIdx.x] = 0;
     if (!oob_any(st)) {
         if (b == 0 && threadIdx.x == 0) st.bcount[0] = 0;
+        clearPrevious();
         return;
     }
-    const int wBeg = b * BND_WORDS_PER_BLOCK, wEnd = min(nWords, wBeg + BND_WORDS_PER_BLOCK);
+    const int wBeg = b * st.bndListWords, wEnd = min(nWords, wBeg + st.bndListWords);
     int cnt = 0;
     for (int w = wBeg + threadIdx.x; w < wEnd; w += 256) cnt += __popcll(st.candBits[w]);
     for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o);
@@ -2181,6 +2196,7 @@ __global__ void __launch_bounds__(256) k_bnd_list(DevMesh m, DevState st, int nW
         base += tot;
         __syncthreads();
     }
+    clearPrevious();
 }
 
 template <int MF>
@@ -2711,7 +2727,7 @@ void launch_bound_pass(const DevMesh& m, const DevState& st, int pass, double dt
     // candidate-sized work: a modest grid-stride grid launches faster than
     // one thread per cell of the mesh
     const int grid = std::max(1, std::min(st.bndGrid, cdiv(m.nC, 256)));
-    const int nbw = cdiv(nWords, BND_WORDS_PER_BLOCK);
+    const int nbw = cdiv(nWords, st.bndListWords);
     k_bnd_list<<<nbw, 256, 0, s>>>(m, st, nWords, nbw, next_list_epoch(), candPrev);
     k_bnd_prep<MF><<<grid, 256, 0, s>>>(m, st, dt, Sp, Su);
     if (st.bndFlow) {

@@ -91,6 +91,7 @@ struct DevState {
     int gridRounds;     // bounding rounds run as grid-wide kernels before the one-workgroup tail
     int bndGrid;        // workgroups of the grid-stride bounding kernels
     int bndFlow;        // workgroups of the one-launch dataflow pass, 0 = rounds and tail
+    int bndListWords;   // bitmap words per block of the candidate list (64..4096)
     void* bndRec;       // nC records BndRec<MF> of the active bounding candidates
     int maxCellFaces;
     // batch of independent cases in one mesh: case of every cell and the
