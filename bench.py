@@ -113,7 +113,7 @@ STAGE_KERNELS = {
     "normal": ("k_ls_normal", "k_grad_normal", "k_rdf_"),
     "findSignedDistance": ("k_fsd_group", "k_find_signed_distance"),
     "upwind": ("k_upwind",),
-    "interfaceFlux": ("k_interface_flux", "k_un0"),
+    "interfaceFlux": ("k_interface_flux", "k_un0", "k_flux_list"),
     "update": ("k_update",),
     "bounding": ("k_bnd_",),
     "snapClip": ("k_snap_clip",),
