@@ -1,7 +1,7 @@
 #!/usr/bin/env python3
 """Benchmark of the SimPLIC alpha transport on one MI355X.
 
-Metric (BASELINE.json): interface-cell updates/s = sum over steps of the mixed
+Metric: interface-cell updates/s = sum over steps of the mixed
 cell count N_ic / sum over steps of (t_reconstruct + t_advect), both times
 measured with HIP events on the library's stream.  A step is one
 reconstruct() + advect() of the deformation test (applications/test/
@@ -406,8 +406,9 @@ def main():
         return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this build measures one GPU; the alpha transport shards by spatial domain "
-              "decomposition (see DESIGN.md) but no multi-GPU path is built, so --gpus must be 1.",
+        print("bench.py measures one GPU; the alpha transport shards by spatial domain "
+              "decomposition (halo exchange of alpha, fluxes and bounding corrections) but no multi-GPU "
+              "path is built, so --gpus must be 1.",
               file=sys.stderr)
         sys.exit(2)
 
