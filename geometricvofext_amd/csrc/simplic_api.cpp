@@ -1192,7 +1192,7 @@ int svp_batch_define(svp_handle* h, int32_t n_cases, const int32_t* cell_offsets
         // switched off: the first GPU run of a batch stalled without an error
         // and the cause is not identified yet
         if (!std::getenv("SVP_ENABLE_BATCH"))
-            throw SvpError(SVP_E_UNSUPPORTED, "svp_batch_define: batched cases are disabled in this build");
+            throw SvpError(SVP_E_UNSUPPORTED, "svp_batch_define: batched cases are switched off (their first GPU test stalled; cause not identified)");
         H(h).batchDefine(n_cases, cell_offsets);
     });
 }
