@@ -507,6 +507,8 @@ def main():
         "n_ic_mean": n_ic_mean,
         "wall_s_timed_region": wall,
         "stage_ms_per_step": {k: 1e3 * v[0] / max(v[1], 1) for k, v in stages.items()},
+        "bounding_per_pass": dict(zip(("candidates", "active", "rounds", "corrected_faces"),
+                                      [(b - a) / max(bt1[4] - bt0[4], 1) for a, b in zip(bt0[:4], bt1[:4])])),
         "t_end": seq[K - 1][0],
     }
     if K == n_total:
