@@ -567,7 +567,7 @@ __global__ void k_rdf_scatter_c(DevState st, RdfBufs r) {
 // cell's own plane in an interface cell, else the average over the stencil's
 // planes weighted by the squared cosine between the plane normal and the
 // direction to the plane centre; a cell with no weight keeps its value.
-__global__ void k_rdf_construct(DevMesh m, RdfBufs r) {
+__global__ void __launch_bounds__(64) k_rdf_construct(DevMesh m, RdfBufs r) {
     if (r.ctl[RDF_DONE]) return;
     const int nNext = r.ctl[RDF_NNEXT];
     for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < nNext; j += gridDim.x * blockDim.x) {
@@ -580,23 +580,40 @@ __global__ void k_rdf_construct(DevMesh m, RdfBufs r) {
             continue;
         }
         double averageWeight = 0.0, avgDist = 0.0;
-        for (int k = m.stOff[c]; k < m.stOff[c + 1]; ++k) {
-            const int o = m.st[k];
-            V3 n = -(-ld3(r.cellN, o));
-            if (mag(n) != 0.0) {
-                n = n / mag(n);
-                V3 dv = ld3(r.cellC, o) - cc;
-                const double distToSurf = dot(dv, n);
-                double weight;
-                if (mag(dv) != 0.0) {
-                    dv = dv / mag(dv);
-                    const double w = fabs(dot(dv, n));
-                    weight = w * w;
-                } else {
-                    weight = 1.0;
+        // stencil entries in batches of 8 with every load issued first (past
+        // the end: the last entry again, left out of the sums); the sums run
+        // in stencil order
+        constexpr int SB = 8;
+        const int kb = m.stOff[c], ke = m.stOff[c + 1];
+        for (int k0 = kb; k0 < ke; k0 += SB) {
+            int o[SB];
+#pragma unroll
+            for (int j = 0; j < SB; ++j) o[j] = m.st[min(k0 + j, ke - 1)];
+            V3 nb[SB], cb[SB];
+#pragma unroll
+            for (int j = 0; j < SB; ++j) {
+                nb[j] = ld3(r.cellN, o[j]);
+                cb[j] = ld3(r.cellC, o[j]);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int j = 0; j < SB; ++j) {
+                V3 n = -(-nb[j]);
+                if (k0 + j < ke && mag(n) != 0.0) {
+                    n = n / mag(n);
+                    V3 dv = cb[j] - cc;
+                    const double distToSurf = dot(dv, n);
+                    double weight;
+                    if (mag(dv) != 0.0) {
+                        dv = dv / mag(dv);
+                        const double w = fabs(dot(dv, n));
+                        weight = w * w;
+                    } else {
+                        weight = 1.0;
+                    }
+                    averageWeight += weight;
+                    avgDist += distToSurf * weight;
                 }
-                averageWeight += weight;
-                avgDist += distToSurf * weight;
             }
         }
         if (averageWeight != 0.0) r.rdf[c] = avgDist / averageWeight;
@@ -614,7 +631,7 @@ __global__ void k_rdf_grad(DevMesh m, DevState st, RdfBufs r) {
 // the averages (:333-357).  g is normalised in place as the reference does:
 // once inside the zero-normal test when the cell's normal is not zero, once
 // for the new normal.
-__global__ void k_rdf_residual(DevMesh m, DevState st, RdfBufs r, int iter, double tol) {
+__global__ void __launch_bounds__(64) k_rdf_residual(DevMesh m, DevState st, RdfBufs r, int iter, double tol) {
     if (r.ctl[RDF_DONE]) return;
     const int nMixed = *st.nMixed;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nMixed; i += gridDim.x * blockDim.x) {
@@ -623,13 +640,23 @@ __global__ void k_rdf_residual(DevMesh m, DevState st, RdfBufs r, int iter, doub
         V3 gi = ld3(r.g, i);
         if (!(mag(cellNormal) < SVP_TSMALL)) gi = normalise_tol(gi, SVP_SMALL);
         double avgDiffNormal = 0.0, weight = 0.0;
-        for (int k = m.stOff[c] + 1; k < m.stOff[c + 1]; ++k) {
-            const V3 normal = ld3(r.cellN, m.st[k]);
-            if (mag(normal) >= SVP_TSMALL) {
-                const V3 n = normal / mag(normal);
-                const double cosAngle = fmax(fmin(dot(cellNormal, n), 1.0), -1.0);
-                avgDiffNormal += acos(cosAngle) * mag(normal);
-                weight += mag(normal);
+        // the stencil past the cell itself, in batches of 8 with the loads first
+        constexpr int SB = 8;
+        const int kb = m.stOff[c] + 1, ke = m.stOff[c + 1];
+        for (int k0 = kb; k0 < ke; k0 += SB) {
+            V3 nb[SB];
+#pragma unroll
+            for (int j = 0; j < SB; ++j) nb[j] = ld3(r.cellN, m.st[min(k0 + j, ke - 1)]);
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int j = 0; j < SB; ++j) {
+                const V3 normal = nb[j];
+                if (k0 + j < ke && mag(normal) >= SVP_TSMALL) {
+                    const V3 n = normal / mag(normal);
+                    const double cosAngle = fmax(fmin(dot(cellNormal, n), 1.0), -1.0);
+                    avgDiffNormal += acos(cosAngle) * mag(normal);
+                    weight += mag(normal);
+                }
             }
         }
         avgDiffNormal = weight != 0.0 ? avgDiffNormal / weight : 0.0;
