@@ -393,3 +393,21 @@ def test_parity_degenerate_fields(oracle_lib, method):
         assert np.array_equal(gs["cells"], os_["cells"]) and np.array_equal(gs["offsets"], os_["offsets"])
         if name == "empty":
             assert gpu.n_mixed() == 0 and len(gs["cells"]) == 0 and gs["offsets"].tolist() == [0]
+
+
+def test_parity_rdf_split_warped_faces(oracle_lib):
+    """isoRDF with splitWarpedFace: the plane placement of every iteration
+    runs the general-polyhedron kernel over the work list, keeping the planes
+    of too-coarse cells."""
+    import geometricvofext_amd as g
+    mesh = g.hex_box(12, perturb=0.25, seed=41)
+    om = oracle_lib.OracleMesh(mesh)
+    a0 = ramp_alpha(om.C, [((0.5, 0.5, 0.5), 0.26), ((0.25, 0.3, 0.7), 2.0 / 12)], 1.2 / 12)
+    _say("RDF split", mesh.name)
+    exact, gpu, ors = _run_parity(mesh, oracle_lib, a0, 3, nAlphaBounds=3, orientationMethod="RDF",
+                                  splitWarpedFace=True)
+    gpu.reconstruct()
+    ors.reconstruct()
+    _compare_interface(gpu, ors, "final")
+    assert gpu.rdf_iterations() == ors.rdf_iterations_run()
+    assert exact
