@@ -344,6 +344,30 @@ def time_sequence(solver, alpha0, end_time, max_steps=None, period=6.0):
     return seq
 
 
+def host_io_rate(solver, alpha0, seq, U0, phi0):
+    """Steps as a host-side solver drives them through SVP_HOST buffers: alpha,
+    phi and U copied in, reconstruct + advect, alpha and alphaPhi copied out,
+    every step.  Wall time with the copies included (PCIe-inclusive); not the
+    benchmark's value, which has the fields resident in HBM."""
+    solver.set_alpha(alpha0)
+    a = np.array(alpha0, dtype=np.float64)
+    n_ic = 0
+    device_sync(solver)
+    t0 = time.perf_counter()
+    for (_, dt, uf) in seq:
+        solver.set_alpha(a)
+        solver.set_flow(phi0 * uf, U0 * uf)
+        solver.reconstruct()
+        n_ic += solver.n_mixed()
+        solver.advect(dt)
+        a = solver.alpha()
+        solver.alpha_phi()
+    wall = time.perf_counter() - t0
+    return {"steps": len(seq), "ms_per_step": 1e3 * wall / len(seq), "updates_per_s": n_ic / wall,
+            "note": "host buffers: alpha, phi, U in and alpha, alphaPhi out over PCIe each step; "
+                    "flow scaling on the host (numpy) included"}
+
+
 def run_steps(solver, seq):
     for (_, dt, uf) in seq:
         solver.scale_flow(uf)
@@ -393,6 +417,9 @@ def main():
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-normal comparison")
     ap.add_argument("--orientation", default="LS", choices=["LS", "alphaGrad", "RDF"],
                     help="interface normal method (default LS, the reference test case's)")
+    ap.add_argument("--host-io", type=int, default=0, metavar="STEPS",
+                    help="also time STEPS steps with host buffers (alpha, phi and U in, alpha and "
+                         "alphaPhi out over PCIe every step, as a host-side solver would)")
     ap.add_argument("--copy-probe", action="store_true", help="run only the counter-calibration copy")
     ap.add_argument("--pmc-summary", nargs=4, metavar=("COPY_FETCH", "COPY_WRITE", "RUN_FETCH", "RUN_WRITE"),
                     help="summarise rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of the copy probe and "
@@ -514,6 +541,10 @@ def main():
     if K == n_total:
         res["accuracy"] = {"E_s_vs_t0": harness.shape_error(alpha_end, alpha0, V),
                            "E_v": harness.volume_error(alpha_end, V, float(np.sum(alpha0 * V)))}
+
+    if args.host_io > 0:
+        res["host_io"] = host_io_rate(solver, alpha0, seq[:args.host_io], U0, phi0)
+        say("host-buffer steps done")
 
     if not args.no_learned:
         res["learned_vs_plain"] = learned_comparison(say)
