@@ -182,23 +182,6 @@ __device__ inline unsigned long long look_back(unsigned long long* look, int b, 
     return excl;
 }
 
-// exclusive prefix of a per-lane flag within a DETECT_BLOCK block; returns the
-// block total in *total (LDS-resident)
-__device__ inline int block_prefix(bool flag, int* lds, int& total) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const unsigned long long bal = __ballot(flag);
-    const int inWave = __popcll(bal & ((1ull << lane) - 1ull));
-    if (lane == 0) lds[wave] = __popcll(bal);
-    __syncthreads();
-    int off = 0, tot = 0;
-    for (int w = 0; w < DETECT_BLOCK / 64; ++w) {
-        if (w < wave) off += lds[w];
-        tot += lds[w];
-    }
-    total = tot;
-    return off + inWave;
-}
-
 // workgroup exclusive scan of one int per thread (blockDim.x == 1024)
 // exclusive scan over a workgroup of 1024 threads: wave scans by shuffles,
 // then a scan of the 16 wave totals; lds needs 16 ints
@@ -1858,10 +1841,12 @@ __device__ inline bool need_bounding(const DevMesh& m, const unsigned char* mixe
 // [-aTol, 1+aTol] for the bounding loop and writes the bounding candidates
 // (outside the bounds and next to the interface) as a bitmap, one 64-bit word
 // per wave.
-__global__ void __launch_bounds__(DETECT_BLOCK) k_update(DevMesh m, DevState st, double dtStep,
+constexpr int UPDATE_BLOCK = 256;
+
+__global__ void __launch_bounds__(UPDATE_BLOCK) k_update(DevMesh m, DevState st, double dtStep,
                                                          const double* Sp, const double* Su) {
     const double aTol = 100.0 * SVP_SMALL;
-    const int c = blockIdx.x * DETECT_BLOCK + threadIdx.x;
+    const int c = blockIdx.x * UPDATE_BLOCK + threadIdx.x;
     bool cand = false, oob = false;
     const double dt = c < m.nC ? cell_dt(st, c, dtStep) : 1.0;
     if (c < m.nC && dt == 0.0) {
@@ -1913,7 +1898,9 @@ __device__ inline bool face_downwind_of(const DevMesh& m, const double* phi, int
 // limitFlux (advectionTemplates.C:118-215), one pass = the kernels below.
 // The pass runs only while cells lie outside [-aTol, 1+aTol] (bound count of
 // the update kernel, kept current by the apply kernel):
-//   count/scan/write  ascending candidates from the bitmap
+//   list              ascending candidates from the bitmap (one launch), and
+//                     the clearing of what the previous pass left
+//                     (dVfCorrectionValues = 0 for its active cells)
 //   prep              every candidate's first redistribution loop, which
 //                     decides whether it writes corrections and which faces
 //                     open correctedFaces; the record of each writing cell
@@ -1923,8 +1910,8 @@ __device__ inline bool face_downwind_of(const DevMesh& m, const double* phi, int
 //   scatter/round/tail  the same as Kahn rounds: grid-wide kernels, then the
 //                     remaining rounds inside one workgroup
 //   apply             corrections applied per cell in correctedFaces order,
-//                     dVf of the corrected faces, bitmap and bound count
-//   reset             dVfCorrectionValues = 0 for the next pass
+//                     dVf of the corrected faces, bitmap and bound count,
+//                     pass statistics
 // bcount: [0] candidates, [1] corrected faces (prep), [2] touched cells,
 // [3+r] frontier size of round r (r <= st.gridRounds; [3] = active cells of
 // the dataflow pass), [BND_CTRL-1] rounds, then the dataflow ticket and
@@ -2801,7 +2788,6 @@ __global__ void k_zero_int(int* p, int n) {
 
 void launch_advect_stage(int stage, const DevMesh& m, const DevState& st, const Params& p, double dt,
                          const double* Sp, const double* Su, int* passSeq, hipStream_t s) {
-    const int nb = detect_blocks(m.nC);
     switch (stage) {
         case STAGE_UPWIND:
             k_upwind<<<cdiv(m.nF, 256), 256, 0, s>>>(m, st, dt);
@@ -2818,7 +2804,7 @@ void launch_advect_stage(int stage, const DevMesh& m, const DevState& st, const 
             }
             break;
         case STAGE_UPDATE:
-            k_update<<<nb, DETECT_BLOCK, 0, s>>>(m, st, dt, Sp, Su);
+            k_update<<<cdiv(m.nC, UPDATE_BLOCK), UPDATE_BLOCK, 0, s>>>(m, st, dt, Sp, Su);
             break;
         case STAGE_BOUND:
             launch_bound(m, st, p, dt, Sp, Su, passSeq, s);
