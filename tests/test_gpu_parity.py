@@ -411,3 +411,28 @@ def test_parity_rdf_split_warped_faces(oracle_lib):
     _compare_interface(gpu, ors, "final")
     assert gpu.rdf_iterations() == ors.rdf_iterations_run()
     assert exact
+
+
+def test_entry_point_errors():
+    """Argument errors of the coupling and export entry points come back as
+    codes (the reference would abort): no sub-cycles, partial PLIC outputs, a
+    null mapped field, and invalid isoRDF keys at creation."""
+    import ctypes as C
+    import geometricvofext_amd as g
+    mesh = g.hex_box(6)
+    s = g.SolveVofEqu(mesh, dict(orientationMethod="LS"))
+    s.set_alpha(np.full(mesh.n_cells, 0.5))
+    L = g.lib()
+    assert L.svp_advect_sub_cycles(s.h, 0.01, 0, None, None, 1.0, 1.0, None, 0) == -1
+    assert L.svp_advect_sub_cycles(s.h, -0.01, 2, None, None, 1.0, 1.0, None, 0) == -1
+    s.reconstruct()
+    nf, npt = C.c_int32(), C.c_int32()
+    cells = np.zeros(mesh.n_cells, np.int32)
+    assert L.svp_get_plic_surface(s.h, C.addressof(nf), C.addressof(npt), cells.ctypes.data, None, None, 0) == -1
+    assert L.svp_get_plic_surface(s.h, None, C.addressof(npt), None, None, None, 0) == -1
+    assert L.svp_map_alpha_field(s.h, None, None, 0.0, 1.0, 0) == -1
+    with pytest.raises(g.SvpError) as e:
+        g.SolveVofEqu(mesh, dict(orientationMethod="RDF", iterations=-1))
+    assert e.value.code == -2
+    with pytest.raises(g.SvpError):
+        s.advect_sub_cycles(0.01, 0)
