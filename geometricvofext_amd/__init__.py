@@ -102,6 +102,7 @@ def lib():
     L.svp_timers.argtypes = [vp, C.POINTER(dbl), C.POINTER(dbl), C.POINTER(C.c_uint64)]
     L.svp_device_view_get.argtypes = [vp, C.POINTER(SvpDeviceView)]
     L.svp_synchronize.argtypes = [vp]
+    L.svp_test_set_error_bits.argtypes = [vp, i32]
     L.svp_harness_deformation.argtypes = [vp, vp, vp]
     L.svp_harness_rotation.argtypes = [vp, vp, vp, vp, vp]
     L.svp_harness_upload_base_flow.argtypes = [vp, vp, vp, vp]
@@ -333,6 +334,10 @@ class SolveVofEqu:
 
     def synchronize(self):
         _check(lib().svp_synchronize(self.h))
+
+    def set_error_bits(self, bits):
+        """Test hook: set device error bits as a failing kernel would."""
+        _check(lib().svp_test_set_error_bits(self.h, int(bits)))
 
     def device_view(self) -> SvpDeviceView:
         v = SvpDeviceView()

@@ -273,7 +273,18 @@ public:
         checkFinite(st_.U, 3L * m_.nC, "U");
     }
 
-    void getAlpha(double* out, int where) { copyOut(out, st_.alpha, m_.nC, where); }
+    // Host outputs wait for the stream anyway, so their calls also report
+    // the device error bits of the work before them; with device outputs the
+    // caller's svp_synchronize does.
+    void getAlpha(double* out, int where) {
+        copyOut(out, st_.alpha, m_.nC, where);
+        if (where != SVP_DEVICE) checkDeviceErrors();
+    }
+
+    void setErrorBits(int bits) {
+        HIPCHECK(hipMemcpyAsync(st_.errFlag, &bits, sizeof(int), hipMemcpyHostToDevice, stream_));
+        HIPCHECK(hipStreamSynchronize(stream_));
+    }
 
     const double* alphaPhi() {
         if (!alphaPhiValid_) {
@@ -1024,6 +1035,7 @@ int svp_advect_sub_cycles(svp_handle* h, double dt, int32_t n_sub, const double*
         s.stageSources(Sp, Su, where);
         const double* sum = s.advectSubCycles(dt, n_sub, Sp, Su, rho1, rho2);
         s.copyOut(rho_phi, sum, s.m_.nF, where);
+        if (where != SVP_DEVICE) s.checkDeviceErrors();
     });
 }
 
@@ -1035,6 +1047,7 @@ int svp_get_alpha_phi(svp_handle* h, double* ap, int where) {
     return guarded([&] {
         solveVofEqu& s = H(h);
         s.copyOut(ap, s.alphaPhi(), s.m_.nF, where);
+        if (where != SVP_DEVICE) s.checkDeviceErrors();
     });
 }
 
@@ -1048,6 +1061,7 @@ int svp_get_rho_phi(svp_handle* h, double rho1, double rho2, double* out, int wh
             if (!s.rhoPhiSum_.p) s.rhoPhiSum_.alloc<double>(s.m_.nF);
             s.rhoPhi(rho1, rho2, (double*)s.rhoPhiSum_.p);
             s.copyOut(out, (const double*)s.rhoPhiSum_.p, s.m_.nF, SVP_HOST);
+            s.checkDeviceErrors();
         }
     });
 }
@@ -1217,6 +1231,10 @@ int svp_learned_stats(svp_handle* h, int64_t* out4) {
 
 int svp_learned_residuals(svp_handle* h, double* out, int where) {
     return guarded([&] { H(h).learnedResiduals(out, where); });
+}
+
+int svp_test_set_error_bits(svp_handle* h, int32_t bits) {
+    return guarded([&] { H(h).setErrorBits(bits); });
 }
 
 const char* svp_last_error(void) { return g_err.c_str(); }

@@ -205,8 +205,17 @@ int svp_bound_totals(svp_handle* h, uint64_t* out5);
 int svp_profile_set(svp_handle* h, int mask);
 int svp_profile_get(svp_handle* h, double* seconds, int64_t* counts, int reset);
 
-/* Wait for all queued work of the handle. */
+/* Wait for all queued work of the handle and report the device error bits
+ * its kernels set (SVP_E_DEVICE; the bits are cleared).  svp_get_alpha,
+ * svp_get_alpha_phi, svp_get_rho_phi and svp_advect_sub_cycles report them
+ * too when their output is a host buffer (they wait for the stream anyway);
+ * with device outputs they do not wait, and svp_synchronize reports. */
 int svp_synchronize(svp_handle* h);
+
+/* Test hook, no counterpart in the reference: sets device error bits as a
+ * failing kernel would, so the error path of every reporting call can be
+ * exercised without making a kernel fail. */
+int svp_test_set_error_bits(svp_handle* h, int32_t bits);
 
 /* Message of the last failing call on this thread. */
 /* Batch of independent cases in one mesh (no reference counterpart: the

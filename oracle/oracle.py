@@ -86,6 +86,8 @@ def lib():
     L.or_point_u.argtypes = [vp, C.c_int, _dp]
     L.or_interpolate_u.argtypes = [vp, _dp, C.c_int, _dp]
     L.or_ls_grad.argtypes = [C.c_int, _dp, _dp, _dp]
+    L.or_set_mutation.argtypes = [C.c_int]
+    L.or_limit_flux.argtypes = [vp, C.c_double, _dp, _dp, _dp, _ip, C.c_int, _dp, _dp]
     L.or_deformation_field.argtypes = [vp, _dp, C.c_void_p, _dp]
     L.or_courant.argtypes = [vp, _dp, _dp, _dp]
     _lib = L
@@ -166,6 +168,23 @@ def time_integrated_area(pts, n, d, ptimes, un0, dt, magsf):
     pts = _c(np.asarray(pts).reshape(-1), np.float64)
     return lib().or_time_integrated_area(pts.size // 3, pts, _c(n, np.float64), d, _c(ptimes, np.float64),
                                          un0, dt, magsf)
+
+
+class mutation:
+    """Context manager that switches on one planted mistake of the oracle
+    (1: weighted least squares, 2: wrong interpolation tet, 3: boundFlux in
+    descending cell order), so that a pinning test can show its check fails."""
+
+    def __init__(self, k):
+        self.k = k
+
+    def __enter__(self):
+        lib().or_set_mutation(self.k)
+        return self
+
+    def __exit__(self, *exc):
+        lib().or_set_mutation(0)
+        return False
 
 
 def ls_grad(pos, val):
@@ -282,6 +301,16 @@ class OracleSolver:
         out = np.zeros(2)
         lib().or_timers(self.h, out)
         return out
+
+    def limit_flux(self, dt, alpha, alpha_old, dvf, mixed):
+        """limitFlux alone (advectionTemplates.C:118-215) on the given state;
+        phi from set_flow.  Returns the bounded alpha and dVf."""
+        a = np.zeros(self.om.mesh.n_cells)
+        v = np.zeros(self.om.mesh.n_faces)
+        mx = _c(mixed, np.int32)
+        lib().or_limit_flux(self.h, dt, _c(alpha, np.float64), _c(alpha_old, np.float64), _c(dvf, np.float64),
+                            mx, int(mx.size), a, v)
+        return a, v
 
     def bound_log(self):
         out = np.zeros(64)

@@ -56,6 +56,13 @@
 
 namespace oracle {
 
+// Planted mistakes for the pinning tests (tests/test_oracle_pins.py): a test
+// switches one on to show that its check fails for a plausible wrong variant.
+// 0 = none (the restatement); 1 = least squares weighted by 1/(|d|^2+1e-3);
+// 2 = cell-point interpolation in the worst tet instead of the one holding the
+// position; 3 = boundFlux walking the cells in descending order.
+static int g_mutation = 0;
+
 // OpenFOAM scalar constants (double precision build)
 static const double SMALL = 1e-15;
 static const double VSMALL = 1e-300;
@@ -938,9 +945,10 @@ static V3 lsGrad(const std::vector<V3>& pos, const std::vector<double>& val) {
     double B[4] = {};
     for (size_t s = 0; s < pos.size(); ++s) {
         const double t[4] = {1.0, pos[s].x, pos[s].y, pos[s].z};
+        const double w = g_mutation == 1 ? 1.0 / (magSqr(pos[s]) + 1e-3) : 1.0;
         for (int r = 0; r < 4; ++r) {
-            for (int c = 0; c < 4; ++c) A[r][c] += t[r] * t[c];
-            B[r] += t[r] * val[s];
+            for (int c = 0; c < 4; ++c) A[r][c] += g_mutation == 1 ? w * t[r] * t[c] : t[r] * t[c];
+            B[r] += g_mutation == 1 ? w * t[r] * val[s] : t[r] * val[s];
         }
     }
     double invA[4][4];
@@ -1295,8 +1303,17 @@ struct Solver {
                     w[3] = 1.0 - w[0] - w[1] - w[2];
                 }
                 const double mn = std::min(std::min(w[0], w[1]), std::min(w[2], w[3]));
-                if (mn + tol > 0.0) {
+                if (mn + tol > 0.0 && g_mutation != 2) {
                     return w[0] * U[celli] + w[1] * pointU(p0) + w[2] * pointU(p1) + w[3] * pointU(p2);
+                }
+                if (g_mutation == 2) {
+                    // planted mistake: the tet of the smallest minimum weight
+                    if (bestMin == -1e300 || -mn > bestMin) {
+                        bestMin = -mn;
+                        for (int q = 0; q < 4; ++q) bw[q] = w[q];
+                        btri[0] = p0; btri[1] = p1; btri[2] = p2;
+                    }
+                    continue;
                 }
                 if (mn > bestMin) {
                     bestMin = mn;
@@ -1393,7 +1410,8 @@ struct Solver {
         // corrections, and the depth of the write-dependency chain
         std::vector<int> depth(m->nC, 0);
         int nCandStat = 0, nWriters = 0, maxDepth = 0;
-        for (int celli = 0; celli < m->nC; ++celli) {
+        for (int cellk = 0; cellk < m->nC; ++cellk) {
+            const int celli = g_mutation == 3 ? m->nC - 1 - cellk : cellk;
             if (!nextToInterface[celli]) continue;
             if (!(alpha[celli] < -aTol || alpha[celli] > 1.0 + aTol)) continue;
             nCandStat++;
@@ -1869,6 +1887,23 @@ void or_interpolate_u(void* h, const double* pos, int cell, double* out) {
     Solver* s = (Solver*)h;
     const V3 u = s->interpolateU(v3(pos[0], pos[1], pos[2]), cell);
     out[0] = u.x; out[1] = u.y; out[2] = u.z;
+}
+
+void or_set_mutation(int k) { g_mutation = k; }
+
+// limitFlux (advectionTemplates.C:118-215) alone on a given state (exposed for
+// tests): alpha after the update, alpha.oldTime(), dVf and the mixed cells;
+// phi comes from or_set_flow.  Writes the bounded alpha and dVf.
+void or_limit_flux(void* h, double dt, const double* alpha, const double* alphaOld, const double* dVf,
+                   const int* mixed, int nMixed, double* alphaOut, double* dVfOut) {
+    Solver* s = (Solver*)h;
+    s->alpha.assign(alpha, alpha + s->m->nC);
+    s->alphaOld.assign(alphaOld, alphaOld + s->m->nC);
+    s->dVf.assign(dVf, dVf + s->m->nF);
+    s->mixedCells.assign(mixed, mixed + nMixed);
+    s->limitFlux(dt, nullptr, nullptr);
+    std::copy(s->alpha.begin(), s->alpha.end(), alphaOut);
+    std::copy(s->dVf.begin(), s->dVf.end(), dVfOut);
 }
 
 // LS gradient of arbitrary stencil data (exposed for tests)

@@ -134,6 +134,26 @@ def test_parity_perturbed_hex(oracle_lib):
     assert exact
 
 
+@pytest.mark.parametrize("snap_tol,clip", [(1e-12, True), (1e-6, False), (1e-4, True)])
+def test_parity_snap_and_clip(oracle_lib, snap_tol, clip):
+    """applyBruteForceBounding (advection.C:291-308): the production
+    dictionary of the reference's tutorials (snapTol 1e-12; clip true) and
+    larger snap tolerances that snap many cells every step."""
+    import geometricvofext_amd as g
+    mesh = g.hex_box(20, perturb=0.2, seed=17)
+    om = oracle_lib.OracleMesh(mesh)
+    a0 = ramp_alpha(om.C, [((0.4, 0.4, 0.45), 0.2)], 1.5 / 20)
+    _say("parity snap", snap_tol, clip, mesh.name)
+    exact, gpu, _ = _run_parity(mesh, oracle_lib, a0, 8, nAlphaBounds=3, snapTol=snap_tol, clip=clip)
+    a = gpu.alpha()
+    if clip:
+        assert a.min() >= 0.0 and a.max() <= 1.0
+    if snap_tol >= 1e-6:
+        mixed = (a > 0.0) & (a < 1.0)
+        assert np.all(a[mixed] >= snap_tol) and np.all(a[mixed] <= 1.0 - snap_tol)
+    assert exact
+
+
 def test_parity_uniform_fixture_sphere(oracle_lib):
     """64^3 deformation case started from the reference's own t=0 exact field."""
     import geometricvofext_amd as g
