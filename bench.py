@@ -387,24 +387,63 @@ def device_sync(solver):
         pass
 
 
-def cpu_baseline(mesh, alpha0, seq, U0, phi0, n_steps, threads, alpha_dict=ALPHA_DICT):
-    """The oracle (CPU restatement of the reference path) on the same mesh,
-    initial field and time steps, first n_steps steps."""
+def cpu_samples(solver, alpha0, seq, n_samples):
+    """Start states for the CPU baseline: the GPU field at n_samples steps
+    spread evenly over the whole run, each with that step's dt and uFactor, so
+    the sample sees the interface sizes of the whole run and not only of its
+    first steps."""
+    idx = sorted({int(round(x)) for x in np.linspace(0, len(seq) - 1, n_samples)})
+    solver.set_alpha(alpha0)
+    out, k = [], 0
+    for j in idx:
+        run_steps(solver, seq[k:j])
+        k = j
+        out.append((j, solver.alpha(), seq[j][1], seq[j][2]))
+    return out
+
+
+def cpu_threads():
+    """OpenMP threads of the CPU baseline: the CPU share the host gives this
+    job (OMP_NUM_THREADS when set, else the cores this process may run on),
+    with the machine's core count and model for the record."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except Exception:
+        avail = os.cpu_count() or 1
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    threads = int(env) if env.isdigit() and int(env) > 0 else avail
+    model = platform.processor() or platform.machine()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return min(threads, avail), {"nproc": os.cpu_count(), "affinity": avail, "model": model}
+
+
+def cpu_baseline(mesh, samples, U0, phi0, threads, alpha_dict=ALPHA_DICT, om=None):
+    """The oracle (CPU restatement of the reference path) on the same mesh:
+    one reconstruct + advect from each sampled GPU state with that step's
+    flow and dt, timed by the oracle's own reconstruct/advect timers."""
     from oracle import oracle as O
-    om = O.OracleMesh(mesh)
+    om = om or O.OracleMesh(mesh)
     s = O.OracleSolver(om, mixedCellTol=alpha_dict["mixedCellTol"], nAlphaBounds=alpha_dict["nAlphaBounds"],
                        snapTol=alpha_dict["snapTol"], clip=alpha_dict["clip"], threads=threads,
                        orientationMethod=alpha_dict["orientationMethod"])
-    s.set_alpha(alpha0)
     n_ic = 0
-    for k, (_, dt, uf) in enumerate(seq[:n_steps]):
+    for k, (j, a, dt, uf) in enumerate(samples):
+        s.set_alpha(a)
         s.set_flow(phi0 * uf, U0 * uf)
         s.reconstruct()
         n_ic += len(s.mixed()["cells"])
         s.advect(dt)
-        print(f"[bench] cpu baseline ({threads} threads) step {k + 1}/{n_steps}", file=sys.stderr, flush=True)
+        print(f"[bench] cpu baseline ({threads} threads) sample {k + 1}/{len(samples)} (step {j})",
+              file=sys.stderr, flush=True)
     rec, adv = s.timers()
-    return n_ic, rec + adv
+    return n_ic, rec + adv, om
 
 
 def main():
@@ -413,7 +452,8 @@ def main():
     ap.add_argument("--steps", type=int, default=0, help="timed steps (0 = run to t=T)")
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="hex128p", choices=sorted(WORKLOADS))
-    ap.add_argument("--cpu-steps", type=int, default=8, help="steps of the CPU baseline sample")
+    ap.add_argument("--cpu-steps", type=int, default=8,
+                    help="sampled steps of the CPU baseline, spread over the run")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--stage", default="auto", help="stage timed for the roofline (auto = slowest)")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-normal comparison")
@@ -552,22 +592,22 @@ def main():
         res["learned_vs_plain"] = learned_comparison(say)
 
     if not args.no_cpu_baseline:
-        ncpu = os.cpu_count() or 1
-        try:
-            ncpu = len(os.sched_getaffinity(0))
-        except Exception:
-            pass
-        threads = min(ncpu, 16)
+        threads, host = cpu_threads()
         t_c = time.time()
-        nic_c, sec_c = cpu_baseline(mesh, alpha0, seq, U0, phi0, args.cpu_steps, threads, alpha_dict)
+        samples = cpu_samples(solver, alpha0, seq, args.cpu_steps)
+        say(f"cpu baseline: {len(samples)} start states from the GPU run")
+        nic_c, sec_c, om = cpu_baseline(mesh, samples, U0, phi0, threads, alpha_dict)
+        steps_txt = ", ".join(str(j) for j, *_ in samples)
         res["cpu_baseline"] = {"value": nic_c / sec_c, "unit": "interface-cell updates/s", "cores": threads,
-                               "kind": "port",
-                               "sample": f"oracle on the same mesh, field and time steps, first "
-                                         f"{args.cpu_steps} steps, {threads} OpenMP threads "
-                                         f"({platform.processor() or platform.machine()})"}
-        nic_1, sec_1 = cpu_baseline(mesh, alpha0, seq, U0, phi0, max(2, args.cpu_steps // 2), 1, alpha_dict)
+                               "kind": "port", "host": host,
+                               "sample": f"oracle on the same mesh and flow, one reconstruct + advect from each "
+                                         f"of {len(samples)} GPU states spread over the run (steps {steps_txt} "
+                                         f"of {n_total}), {threads} OpenMP threads = the job's CPU share"}
+        half = samples[::2]
+        nic_1, sec_1, _ = cpu_baseline(mesh, half, U0, phi0, 1, alpha_dict, om)
         res["cpu_baseline_serial"] = {"value": nic_1 / sec_1, "unit": "interface-cell updates/s", "cores": 1,
-                                      "kind": "port", "sample": f"first {max(2, args.cpu_steps // 2)} steps"}
+                                      "kind": "port",
+                                      "sample": f"the same, every second state ({len(half)} steps), 1 thread"}
         print(f"[bench] cpu baseline {time.time() - t_c:.1f}s", file=sys.stderr, flush=True)
     print(json.dumps(res), flush=True)
 
