@@ -94,6 +94,7 @@ def lib():
     L.svp_n_mixed.argtypes = [vp, C.POINTER(i32)]
     L.svp_get_interface.argtypes = [vp, vp, vp, vp, vp, vp, vp, C.c_int]
     L.svp_get_plic_surface.argtypes = [vp, vp, vp, vp, vp, vp, C.c_int]
+    L.svp_get_plic_surface_device.argtypes = [vp, vp, vp, vp, vp, vp]
     L.svp_rdf_iterations.argtypes = [vp, vp]
     L.svp_map_alpha_field.argtypes = [vp, vp, vp, dbl, dbl, C.c_int]
     L.svp_get_cut_class.argtypes = [vp, vp, vp, C.c_int]

@@ -12,6 +12,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -41,6 +42,36 @@ void launch_init_alpha(const DevMesh& m, const void* shapes, int nShapes, int ma
 }  // namespace svp
 
 using namespace svp::api;
+
+// Scratch of svp_get_plic_surface_device, the PLIC export that allocates
+// nothing: the point count and polygon index of every mixed cell (scanned in
+// place into offsets), then the two totals.  A file-scope device array, so
+// every GPU the library runs on holds its own copy until the process ends.
+// In host code its name is only a stand-in: kernels get the address that
+// hipGetSymbolAddress returns for the current device, taken once per device
+// and kept.  Every call on a GPU shares that GPU's copy (calls that overlap on
+// one GPU race); each call writes all it reads, so nothing is cleared.
+static __device__ __align__(256) int scratch[2 * SVP_PLIC_NOALLOC_MAX_CELLS + 2];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratchMu;
+int* g_scratchAddr[kMaxDevices] = {};
+
+// address of this device's scratch (the caller has made `device` current)
+int* scratch_address(int device) {
+    if (device < 0 || device >= kMaxDevices) throw SvpError(SVP_E_ARG, "device ordinal out of range");
+    std::lock_guard<std::mutex> lock(g_scratchMu);
+    if (!g_scratchAddr[device]) {
+        void* p = nullptr;
+        HIPCHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) throw SvpError(SVP_E_HIP, "hipGetSymbolAddress returned no address for the PLIC scratch");
+        g_scratchAddr[device] = static_cast<int*>(p);
+    }
+    return g_scratchAddr[device];
+}
+
+}  // namespace
 
 namespace gve {
 namespace SimPLIC {
@@ -419,6 +450,44 @@ public:
             copyOutT(cells, dCells, tot[1], SVP_HOST);
             copyOutT(offs, dOffs, tot[1] + 1L, SVP_HOST);
         }
+    }
+
+    // The same export into the caller's device buffers, with the static
+    // scratch in place of plicIdx_ and the closing offset written by a kernel:
+    // no device or pinned allocation.  Mixed-cell counts above
+    // SVP_PLIC_NOALLOC_MAX_CELLS are refused.
+    void plicSurfaceNoAlloc(int32_t* nFaces, int32_t* nPts, int32_t* cells, int32_t* offs, double* pts) {
+        if (!nFaces || !nPts) throw SvpError(SVP_E_ARG, "svp_get_plic_surface_device: null size output");
+        const bool want = cells || offs || pts;
+        if (want && !(cells && offs && pts))
+            throw SvpError(SVP_E_ARG,
+                           "svp_get_plic_surface_device: give face_cells, face_offsets and points, or none");
+        int* ptOff = scratch_address(device_);
+        checkDeviceErrors();
+        const int k = nMixed();
+        if (k > SVP_PLIC_NOALLOC_MAX_CELLS)
+            throw SvpError(SVP_E_ARG, "svp_get_plic_surface_device: " + std::to_string(k) +
+                                          " mixed cells exceed the scratch of " +
+                                          std::to_string(SVP_PLIC_NOALLOC_MAX_CELLS) +
+                                          " cells (svp_get_plic_surface has no limit)");
+        int* faceIdx = ptOff + k;
+        int* totals = faceIdx + k;
+        int tot[2] = {0, 0};
+        if (k > 0) {
+            launch_plic_surface(m_, st_, k, 0, ptOff, faceIdx, totals, nullptr, nullptr, nullptr, stream_);
+            HIPCHECK(hipGetLastError());
+            HIPCHECK(hipMemcpyAsync(tot, totals, sizeof(tot), hipMemcpyDeviceToHost, stream_));
+            HIPCHECK(hipStreamSynchronize(stream_));
+            checkDeviceErrors();
+        }
+        *nPts = tot[0];
+        *nFaces = tot[1];
+        if (!want) return;
+        if (k > 0) launch_plic_surface(m_, st_, k, 1, ptOff, faceIdx, nullptr, pts, cells, offs, stream_);
+        launch_plic_close(offs, k > 0 ? totals : nullptr, stream_);
+        HIPCHECK(hipGetLastError());
+        HIPCHECK(hipStreamSynchronize(stream_));
+        checkDeviceErrors();
     }
 
     int rdfIterationsRun() {
@@ -1081,6 +1150,11 @@ int svp_rdf_iterations(svp_handle* h, int32_t* n) {
 int svp_get_plic_surface(svp_handle* h, int32_t* n_faces, int32_t* n_points, int32_t* face_cells,
                          int32_t* face_offsets, double* points, int where) {
     return guarded([&] { H(h).plicSurface(n_faces, n_points, face_cells, face_offsets, points, where); });
+}
+
+int svp_get_plic_surface_device(svp_handle* h, int32_t* n_faces, int32_t* n_points, int32_t* face_cells,
+                                int32_t* face_offsets, double* points) {
+    return guarded([&] { H(h).plicSurfaceNoAlloc(n_faces, n_points, face_cells, face_offsets, points); });
 }
 
 int svp_n_mixed(svp_handle* h, int32_t* n) {

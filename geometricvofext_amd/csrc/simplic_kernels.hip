@@ -1394,6 +1394,19 @@ void launch_plic_surface(const DevMesh& m, const DevState& st, int nMixed, int p
     }
 }
 
+// the closing polygon offset face_offsets[n_faces] = n_points, from the
+// totals of the size pass (totals[0] points, totals[1] polygons); without
+// totals (no mixed cell) the one offset face_offsets[0] = 0
+__global__ void k_plic_close(int* faceOff, const int* totals) {
+    if (threadIdx.x != 0) return;
+    if (totals) faceOff[totals[1]] = totals[0];
+    else faceOff[0] = 0;
+}
+
+void launch_plic_close(int* faceOff, const int* totals, hipStream_t s) {
+    k_plic_close<<<1, 64, 0, s>>>(faceOff, totals);
+}
+
 // ---------------------------------------------------------------------------
 // Advection
 // ---------------------------------------------------------------------------

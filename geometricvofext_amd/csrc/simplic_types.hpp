@@ -212,6 +212,7 @@ void launch_map_alpha(const DevMesh& m, const DevState& st, const double* n, con
 // pass 1: the polygons
 void launch_plic_surface(const DevMesh& m, const DevState& st, int nMixed, int pass, int* ptOff, int* faceIdx,
                          int* totals, double* pts, int* faceCells, int* faceOff, hipStream_t s);
+void launch_plic_close(int* faceOff, const int* totals, hipStream_t s);
 void launch_rho_phi(int nF, const double* ap, const double* phi, double rho1, double rho2, double w,
                     double* out, int mode, hipStream_t s);
 int detect_blocks(int nC);

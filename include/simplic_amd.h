@@ -157,6 +157,18 @@ int svp_get_interface(svp_handle* h, int32_t* cells, int32_t* status, double* n,
  * when all three are non-NULL (all NULL: size query). */
 int svp_get_plic_surface(svp_handle* h, int32_t* n_faces, int32_t* n_points, int32_t* face_cells,
                          int32_t* face_offsets, double* points, int where);
+/* No counterpart in the reference.  The same export for callers that cannot
+ * allocate device memory (code between HIP graph captures, tight serving
+ * loops, harnesses that forbid hipMalloc): face_cells, face_offsets and points
+ * are device buffers of the caller (sized by a prior size query), and the
+ * working space is a static device array of the library, one per GPU, so the
+ * call makes no device or pinned allocation.  Results equal
+ * svp_get_plic_surface's.  At most SVP_PLIC_NOALLOC_MAX_CELLS mixed cells
+ * (SVP_E_ARG beyond).  Calls that overlap on one GPU share that array and
+ * race; n_faces / n_points are host pointers. */
+#define SVP_PLIC_NOALLOC_MAX_CELLS (1 << 20)
+int svp_get_plic_surface_device(svp_handle* h, int32_t* n_faces, int32_t* n_points, int32_t* face_cells,
+                                int32_t* face_offsets, double* points);
 /* Face-cut classification of the final plane cut per mixed cell: 2 bits per
  * cell face ((status+1) << 2*k, cells() face order) and the number of plane
  * crossings (cut vertices) on its cut faces. */
