@@ -14,7 +14,7 @@ import os
 
 import numpy as np
 
-from .mesh import PolyMesh, hex_box, tet_box  # noqa: F401
+from .mesh import PolyMesh, agglomerated_box, brick_box, hex_box, refined_hex_box, tet_box  # noqa: F401
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libsimplic_amd.so")
@@ -46,6 +46,7 @@ class SvpDict(C.Structure):
         ("rdf_tol", C.c_double),
         ("rdf_rel_tol", C.c_double),
         ("rdf_iterations", C.c_int32),
+        ("grad_scheme", C.c_char * 32),
     ]
 
 
@@ -95,6 +96,18 @@ def lib():
     L.svp_get_interface.argtypes = [vp, vp, vp, vp, vp, vp, vp, C.c_int]
     L.svp_get_plic_surface.argtypes = [vp, vp, vp, vp, vp, vp, C.c_int]
     L.svp_get_plic_surface_device.argtypes = [vp, vp, vp, vp, vp, vp]
+    L.svp_get_plic_fields.argtypes = [vp, vp, vp, vp, vp, vp, C.c_int]
+    L.svp_batch_create.argtypes = [vp, i32, C.POINTER(vp)]
+    L.svp_batch_destroy.argtypes = [vp]
+    L.svp_batch_step.argtypes = [vp, vp]
+    L.svp_batch_tune.argtypes = [vp, vp, vp]
+    L.svp_batch_set_shapes.argtypes = [vp, vp]
+    L.svp_batch_timers.argtypes = [vp, C.POINTER(dbl), C.POINTER(C.c_uint64)]
+    L.svp_batch_profile_set.argtypes = [vp, C.c_int]
+    L.svp_batch_profile_get.argtypes = [vp, vp, vp, C.c_int]
+    L.svp_batch_courant.argtypes = [vp, vp, vp]
+    L.svp_batch_scale_flow.argtypes = [vp, vp]
+    L.svp_alpha_mapping_time.argtypes = [vp, C.POINTER(dbl)]
     L.svp_rdf_iterations.argtypes = [vp, vp]
     L.svp_map_alpha_field.argtypes = [vp, vp, vp, dbl, dbl, C.c_int]
     L.svp_get_cut_class.argtypes = [vp, vp, vp, C.c_int]
@@ -125,10 +138,6 @@ def lib():
     L.svp_nm_cube_fractions.argtypes = [vp, i64, vp, C.c_int]
     L.svp_bound_stats.argtypes = [vp, vp, C.c_int]
     L.svp_bound_totals.argtypes = [vp, vp]
-    L.svp_batch_define.argtypes = [vp, i32, vp]
-    L.svp_batch_set_dt.argtypes = [vp, vp, C.c_int]
-    L.svp_harness_scale_flow_batch.argtypes = [vp, vp]
-    L.svp_harness_courant_batch.argtypes = [vp, vp, vp]
     L.svp_set_normal_model.argtypes = [vp, vp]
     L.svp_learned_stats.argtypes = [vp, vp]
     L.svp_learned_residuals.argtypes = [vp, vp, C.c_int]
@@ -164,10 +173,10 @@ def make_dict(**kw) -> SvpDict:
              "nAlphaBounds": "n_alpha_bounds", "snapTol": "snap_tol", "clip": "clip",
              "splitWarpedFace": "split_warped_face", "writePlicFields": "write_plic_fields",
              "learnedResidualTol": "learned_residual_tol", "tol": "rdf_tol", "relTol": "rdf_rel_tol",
-             "iterations": "rdf_iterations"}
+             "iterations": "rdf_iterations", "gradScheme": "grad_scheme", "grad(alpha1)": "grad_scheme"}
     for k, v in kw.items():
         k = names.get(k, k)
-        if k == "orientation_method":
+        if k in ("orientation_method", "grad_scheme"):
             v = v.encode()
         elif isinstance(v, bool):
             v = int(v)
@@ -297,6 +306,24 @@ class SolveVofEqu:
                                           offs.ctypes.data, pts.ctypes.data, SVP_HOST))
         return dict(points=pts.reshape(-1, 3)[:npt.value], offsets=offs, cells=cells[:nf.value])
 
+    def plic_fields(self):
+        """The reference's full-mesh interface fields after the last
+        reconstruct and the face flatness (reconstruction.C:448-471, 520-584):
+        dict(interfaceN (n_cells, 3), interfaceD, interfaceC, interfaceS,
+        zeta (n_faces)); zero outside the mixed cells."""
+        nc, nf = self.mesh.n_cells, self.mesh.n_faces
+        N, D, Cc, S, z = np.zeros(3 * nc), np.zeros(nc), np.zeros(3 * nc), np.zeros(3 * nc), np.zeros(nf)
+        _check(lib().svp_get_plic_fields(self.h, N.ctypes.data, D.ctypes.data, Cc.ctypes.data, S.ctypes.data,
+                                         z.ctypes.data, SVP_HOST))
+        return dict(interfaceN=N.reshape(nc, 3), interfaceD=D, interfaceC=Cc.reshape(nc, 3),
+                    interfaceS=S.reshape(nc, 3), zeta=z)
+
+    def alpha_mapping_time(self):
+        """GPU seconds spent in map_alpha_field (alphaMappingTime)."""
+        t = C.c_double()
+        _check(lib().svp_alpha_mapping_time(self.h, C.byref(t)))
+        return t.value
+
     def interface(self):
         n = self.n_mixed()
         k = max(n, 1)
@@ -418,25 +445,6 @@ class SolveVofEqu:
         _check(lib().svp_bound_totals(self.h, out.ctypes.data))
         return [int(v) for v in out]
 
-    # batch of independent cases (svp_batch_define)
-    def batch_define(self, cell_offsets):
-        off = np.ascontiguousarray(cell_offsets, dtype=np.int32)
-        _check(lib().svp_batch_define(self.h, int(off.size - 1), off.ctypes.data))
-        self.n_cases = int(off.size - 1)
-
-    def batch_set_dt(self, dt):
-        dt = np.ascontiguousarray(dt, dtype=np.float64)
-        _check(lib().svp_batch_set_dt(self.h, dt.ctypes.data, SVP_HOST))
-
-    def scale_flow_batch(self, factors):
-        f = np.ascontiguousarray(factors, dtype=np.float64)
-        _check(lib().svp_harness_scale_flow_batch(self.h, f.ctypes.data))
-
-    def courant_batch(self):
-        co = np.zeros(self.n_cases)
-        aco = np.zeros(self.n_cases)
-        _check(lib().svp_harness_courant_batch(self.h, co.ctypes.data, aco.ctypes.data))
-        return co, aco
 
     # learned orientation method (include/simplic_learned.h)
     def set_normal_model(self, model):
@@ -455,3 +463,73 @@ class SolveVofEqu:
 
 
 from .learned import NormalModel, cube_fractions  # noqa: E402,F401
+
+
+class CaseBatch:
+    """Independent :class:`SolveVofEqu` handles advanced together
+    (``svp_batch_*``): one launch per stage for all cases, each case with
+    its own time step; every case's results equal its handle stepping alone.
+    Close the batch before using the handles on their own again."""
+
+    SHAPE_NAMES = ("normal", "findSignedDistance", "fluxList", "interfaceFlux", "boundRecords", "boundFlow",
+                   "boundApply")
+
+    def __init__(self, solvers):
+        self.solvers = list(solvers)
+        arr = (C.c_void_p * len(self.solvers))(*[sv.h for sv in self.solvers])
+        self.h = C.c_void_p()
+        _check(lib().svp_batch_create(arr, len(self.solvers), C.byref(self.h)))
+
+    def close(self):
+        if getattr(self, "h", None):
+            _check(lib().svp_batch_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.svp_batch_destroy(self.h)
+            self.h = None
+
+    def _dt(self, dt):
+        dt = np.ascontiguousarray(dt, dtype=np.float64)
+        assert dt.shape == (len(self.solvers),)
+        return dt
+
+    def step(self, dt):
+        """One step of every case with dt[k] > 0 (dt[k] <= 0: case left as is)."""
+        d = self._dt(dt)
+        _check(lib().svp_batch_step(self.h, d.ctypes.data))
+
+    def tune(self, dt):
+        d = self._dt(dt)
+        out = np.zeros(7, np.int32)
+        _check(lib().svp_batch_tune(self.h, d.ctypes.data, out.ctypes.data))
+        return dict(zip(self.SHAPE_NAMES, out.tolist()))
+
+    def set_shapes(self, shapes):
+        a = np.ascontiguousarray([shapes[k] for k in self.SHAPE_NAMES], dtype=np.int32)
+        _check(lib().svp_batch_set_shapes(self.h, a.ctypes.data))
+
+    def timers(self):
+        t, n = C.c_double(), C.c_uint64()
+        _check(lib().svp_batch_timers(self.h, C.byref(t), C.byref(n)))
+        return t.value, n.value
+
+    def courant(self):
+        n = len(self.solvers)
+        co, aco = np.zeros(n), np.zeros(n)
+        _check(lib().svp_batch_courant(self.h, co.ctypes.data, aco.ctypes.data))
+        return co, aco
+
+    def scale_flow(self, factors):
+        f = np.ascontiguousarray(factors, dtype=np.float64)
+        _check(lib().svp_batch_scale_flow(self.h, f.ctypes.data))
+
+    def profile_set(self, mask):
+        _check(lib().svp_batch_profile_set(self.h, int(mask)))
+
+    def profile_get(self, reset=False):
+        sec = np.zeros(8)
+        cnt = np.zeros(8, np.int64)
+        _check(lib().svp_batch_profile_get(self.h, sec.ctypes.data, cnt.ctypes.data, int(reset)))
+        return {name: (float(sec[k]), int(cnt[k])) for k, name in enumerate(SolveVofEqu.STAGES)}

@@ -8,30 +8,30 @@ namespace svp {
 static inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
 
 // updateU.H:117-127: phi *= uFactor; U *= uFactor
-__global__ void k_scale_flow(int nC, int nF, int nB, const double* U0, const double* phi0,
-                             const double* U0b, double factor, double* U, double* phi, double* Ub) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ inline void scale_flow_at(int i, int nC, int nF, int nB, const double* U0, const double* phi0,
+                                     const double* U0b, double factor, double* U, double* phi, double* Ub) {
     if (i < nF) phi[i] = phi0[i] * factor;
     if (i < 3 * nC) U[i] = U0[i] * factor;
     if (i < 3 * nB) Ub[i] = U0b ? U0b[i] * factor : 0.0;
 }
 
-// batch: each case scaled by its own factor (faces by their owner's case)
-__global__ void k_scale_flow_batch(DevMesh m, const unsigned short* caseOf, const double* factors,
-                                   const double* U0, const double* phi0, const double* U0b, double* U,
-                                   double* phi, double* Ub) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int nB = m.nF - m.nIF;
-    if (i < m.nF) phi[i] = phi0[i] * factors[caseOf[m.own[i]]];
-    if (i < 3 * m.nC) U[i] = U0[i] * factors[caseOf[i / 3]];
-    if (i < 3 * nB) Ub[i] = U0b ? U0b[i] * factors[caseOf[m.own[m.nIF + i / 3]]] : 0.0;
+__global__ void k_scale_flow(int nC, int nF, int nB, const double* U0, const double* phi0,
+                             const double* U0b, double factor, double* U, double* phi, double* Ub) {
+    scale_flow_at(blockIdx.x * blockDim.x + threadIdx.x, nC, nF, nB, U0, phi0, U0b, factor, U, phi, Ub);
 }
 
-void launch_scale_flow_batch(const DevMesh& m, const unsigned short* caseOf, const double* factors,
-                             const double* U0, const double* phi0, const double* U0b, double* U, double* phi,
-                             double* Ub, hipStream_t s) {
-    const long n = std::max<long>(m.nF, 3L * m.nC);
-    k_scale_flow_batch<<<cdiv(n, 256), 256, 0, s>>>(m, caseOf, factors, U0, phi0, U0b, U, phi, Ub);
+// batch: every case's base flow scaled by its own factor (blockIdx.y = case)
+__global__ void kb_scale_flow(const BatchCase* __restrict__ bc, const double* __restrict__ factors) {
+    const BatchCase& c = bc[blockIdx.y];
+    const DevState& st = c.st[0];
+    scale_flow_at(blockIdx.x * blockDim.x + threadIdx.x, c.m.nC, c.m.nF, c.nB, c.U0, c.phi0, c.U0b,
+                  factors[blockIdx.y], st.U, st.phi, st.Ub);
+}
+
+void launch_batch_scale_flow(const BatchCase* bc, const BatchStepInfo* info, const double* factors, int nCases,
+                             long maxN, hipStream_t s) {
+    (void)info;
+    kb_scale_flow<<<dim3(cdiv(maxN, 256), nCases), 256, 0, s>>>(bc, factors);
 }
 
 void launch_scale_flow(int nC, int nF, int nB, const double* U0, const double* phi0, const double* U0b,
@@ -43,8 +43,7 @@ void launch_scale_flow(int nC, int nF, int nB, const double* U0, const double* p
 // fvc::surfaceSum(mag(phi)) in ascending face order, then max of sum/V and of
 // (pos0(alpha-0.01)*pos0(0.99-alpha)*sum)/V.  All values are >= 0, so the max
 // is an integer max over the IEEE bit patterns.  Grid-stride blocks reduce in
-// LDS and issue one atomic pair per block (per case in a batch, where a block
-// holding cells of several cases falls back to one atomic pair per wave).
+// LDS and issue one atomic pair per block.
 __device__ inline void courant_cell(const DevMesh& m, const double* phi, const double* alpha, int c, double& r,
                                     double& ra) {
     double s = 0.0;
@@ -61,35 +60,9 @@ __device__ inline void courant_cell(const DevMesh& m, const double* phi, const d
 
 __device__ inline unsigned long long umax64(unsigned long long a, unsigned long long b) { return a > b ? a : b; }
 
-__global__ void __launch_bounds__(256) k_courant(DevMesh m, const double* phi, const double* alpha,
-                                                 const unsigned short* caseOf, unsigned long long* out) {
+__device__ __attribute__((always_inline)) inline void k_courant_body(const DevMesh& m, const double* phi,
+                                                                    const double* alpha, unsigned long long* out) {
     __shared__ unsigned long long lr[4], lra[4];
-    if (caseOf) {
-        // block-uniform loop; a wave whose cells share one case reduces first
-        for (int c0 = blockIdx.x * 256; c0 < m.nC; c0 += gridDim.x * 256) {
-            const int c = c0 + threadIdx.x;
-            const bool valid = c < m.nC;
-            double r = 0.0, ra = 0.0;
-            if (valid) courant_cell(m, phi, alpha, c, r, ra);
-            const int k = valid ? caseOf[c] : -1;
-            unsigned long long br = __double_as_longlong(r), bra = __double_as_longlong(ra);
-            const int k0 = __shfl(k, 0);
-            if (__all(!valid || k == k0)) {
-                for (int o = 32; o > 0; o >>= 1) {
-                    br = umax64(br, __shfl_xor(br, o));
-                    bra = umax64(bra, __shfl_xor(bra, o));
-                }
-                if ((threadIdx.x & 63) == 0 && k0 >= 0) {
-                    atomicMax(out + 2 * k0, br);
-                    atomicMax(out + 2 * k0 + 1, bra);
-                }
-            } else if (valid) {
-                atomicMax(out + 2 * k, br);
-                atomicMax(out + 2 * k + 1, bra);
-            }
-        }
-        return;
-    }
     unsigned long long br = 0ull, bra = 0ull;
     for (int c = blockIdx.x * 256 + threadIdx.x; c < m.nC; c += gridDim.x * 256) {
         double r, ra;
@@ -113,6 +86,20 @@ __global__ void __launch_bounds__(256) k_courant(DevMesh m, const double* phi, c
     }
 }
 
+__global__ void __launch_bounds__(256) k_courant(DevMesh m, const double* phi, const double* alpha,
+                                                 unsigned long long* out) {
+    k_courant_body(m, phi, alpha, out);
+}
+
+// batch: the Courant sums of case blockIdx.y into out[2 y], out[2 y + 1],
+// from its current alpha (view aPar)
+__global__ void __launch_bounds__(256) kb_courant(const BatchCase* __restrict__ bc,
+                                                  const BatchStepInfo* __restrict__ info, unsigned long long* out) {
+    const BatchCase& c = bc[blockIdx.y];
+    const DevState& st = c.st[info[blockIdx.y].aPar << 1];
+    k_courant_body(c.m, st.phi, st.alpha, out + 2 * blockIdx.y);
+}
+
 __global__ void k_zero_u64(unsigned long long* p, int n) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0ull;
 }
@@ -120,15 +107,9 @@ __global__ void k_zero_u64(unsigned long long* p, int n) {
 void launch_courant(const DevMesh& m, const double* phi, const double* alpha, unsigned long long* out,
                     hipStream_t s) {
     k_zero_u64<<<1, 64, 0, s>>>(out, 2);
-    k_courant<<<std::min(2048, cdiv(m.nC, 256)), 256, 0, s>>>(m, phi, alpha, nullptr, out);
+    k_courant<<<std::min(2048, cdiv(m.nC, 256)), 256, 0, s>>>(m, phi, alpha, out);
 }
 
-// per-case Courant numbers of a batch: out[2k], out[2k+1]
-void launch_courant_batch(const DevMesh& m, const double* phi, const double* alpha, const unsigned short* caseOf,
-                          int nCases, unsigned long long* out, hipStream_t s) {
-    k_zero_u64<<<cdiv(2 * nCases, 256), 256, 0, s>>>(out, 2 * nCases);
-    k_courant<<<std::min(4096, cdiv(m.nC, 256)), 256, 0, s>>>(m, phi, alpha, caseOf, out);
-}
 
 // ---------------------------------------------------------------------------
 // Initial alpha of a union of spheres / ellipsoids by adaptive tetrahedral
@@ -277,6 +258,12 @@ __global__ void k_count_nonfinite(const double* x, long n, int* count) {
 
 void launch_count_nonfinite(const double* x, long n, int* count, hipStream_t s) {
     if (n > 0) k_count_nonfinite<<<cdiv(n, 256), 256, 0, s>>>(x, n, count);
+}
+
+void launch_batch_courant(const BatchCase* bc, const BatchStepInfo* info, int nCases, int maxC,
+                          unsigned long long* out, hipStream_t s) {
+    k_zero_u64<<<cdiv(2L * nCases, 256), 256, 0, s>>>(out, 2 * nCases);
+    kb_courant<<<dim3(std::min(2048, cdiv(maxC, 256)), nCases), 256, 0, s>>>(bc, info, out);
 }
 
 }  // namespace svp

@@ -31,11 +31,6 @@ void launch_scale_flow(int nC, int nF, int nB, const double* U0, const double* p
                        double factor, double* U, double* phi, double* Ub, hipStream_t s);
 void launch_courant(const DevMesh& m, const double* phi, const double* alpha, unsigned long long* out,
                     hipStream_t s);
-void launch_courant_batch(const DevMesh& m, const double* phi, const double* alpha, const unsigned short* caseOf,
-                          int nCases, unsigned long long* out, hipStream_t s);
-void launch_scale_flow_batch(const DevMesh& m, const unsigned short* caseOf, const double* factors,
-                             const double* U0, const double* phi0, const double* U0b, double* U, double* phi,
-                             double* Ub, hipStream_t s);
 void launch_count_nonfinite(const double* x, long n, int* count, hipStream_t s);
 void launch_init_alpha(const DevMesh& m, const void* shapes, int nShapes, int maxDepth, double* alpha,
                        hipStream_t s);
@@ -71,6 +66,20 @@ int* scratch_address(int device) {
     return g_scratchAddr[device];
 }
 
+// grad(alpha1) scheme name with single spaces; "" when not one of the two
+// the alphaGrad normal provides
+std::string grad_scheme_name(const std::string& v) {
+    std::string out, tok;
+    for (char ch : v + " ") {
+        if (ch == ' ' || ch == '\t' || ch == '\n' || ch == '\r') {
+            if (!tok.empty()) out += (out.empty() ? "" : " ") + tok;
+            tok.clear();
+        } else {
+            tok += ch;
+        }
+    }
+    return (out == "Gauss linear" || out == "Gauss pointLinear") ? out : std::string();
+}
 }  // namespace
 
 namespace gve {
@@ -92,6 +101,12 @@ public:
         params_.splitWarpedFace = dict.split_warped_face;
         learned_.active = std::string(dict.orientation_method) == "learned";
         gaussNormal_ = std::string(dict.orientation_method) == "alphaGrad";
+        {
+            const std::string g = grad_scheme_name(std::string(dict.grad_scheme, strnlen(dict.grad_scheme, sizeof dict.grad_scheme)));
+            if (g.empty())
+                throw SvpError(SVP_E_UNSUPPORTED, "grad_scheme: this library provides Gauss linear and Gauss pointLinear");
+            pointLinear_ = g == "Gauss pointLinear";
+        }
         rdfNormal_ = std::string(dict.orientation_method) == "isoRDF" || std::string(dict.orientation_method) == "RDF";
         learned_.tol = dict.learned_residual_tol;
         buildMesh(mesh);
@@ -121,7 +136,7 @@ public:
         } else {
             stage(STAGE_NORMAL, [&] {
                 if (rdfNormal_) rdfNormals();
-                else if (gaussNormal_) launch_grad_normal(m_, st_, stream_);
+                else if (gaussNormal_) launch_grad_normal(m_, st_, pointLinear_, stream_);
                 else launch_ls_normal(m_, st_, stream_);
             });
             stage(STAGE_FSD, [&] { placePlanes(); });
@@ -135,8 +150,7 @@ public:
     // advection::advect (advectionTemplates.C:352-418); alpha1.oldTime() is
     // the field at the call, i.e. the double buffer swaps here.
     void advect(double dt, const double* Sp, const double* Su) {
-        if (nCases_ > 0) dt = 1.0;  // a batch takes every case's step from batchSetDt
-        else if (!(dt > 0.0) || !std::isfinite(dt)) throw SvpError(SVP_E_ARG, "advect: dt must be positive");
+        if (!(dt > 0.0) || !std::isfinite(dt)) throw SvpError(SVP_E_ARG, "advect: dt must be positive");
         if (!reconstructed_) reconstruct();
         hipEvent_t a = event(), b = event();
         HIPCHECK(hipEventRecord(a, stream_));
@@ -148,43 +162,6 @@ public:
         pendingAdvect_.push_back({a, b});
         lastDt_ = dt;
         alphaPhiValid_ = false;
-        reconstructed_ = false;
-    }
-
-    // Batch of independent cases in this mesh (cells [off[k], off[k+1]) form
-    // case k; the mesh must not connect cases).  Every case then advances with
-    // its own time step; a zero step leaves the case untouched.
-    void batchDefine(int nCases, const int32_t* off) {
-        if (nCases < 1 || nCases > 65535 || !off) throw SvpError(SVP_E_ARG, "svp_batch_define: 1..65535 cases");
-        if (off[0] != 0 || off[nCases] != m_.nC) throw SvpError(SVP_E_ARG, "svp_batch_define: offsets must span the cells");
-        std::vector<unsigned short> cs(m_.nC);
-        for (int k = 0; k < nCases; ++k) {
-            if (off[k + 1] < off[k]) throw SvpError(SVP_E_ARG, "svp_batch_define: offsets must ascend");
-            for (int c = off[k]; c < off[k + 1]; ++c) cs[c] = (unsigned short)k;
-        }
-        // no internal face may join two cases
-        std::vector<int32_t> own(m_.nIF), nbr(m_.nIF);
-        HIPCHECK(hipMemcpy(own.data(), m_.own, sizeof(int32_t) * m_.nIF, hipMemcpyDeviceToHost));
-        HIPCHECK(hipMemcpy(nbr.data(), m_.nbr, sizeof(int32_t) * m_.nIF, hipMemcpyDeviceToHost));
-        for (int f = 0; f < m_.nIF; ++f)
-            if (cs[own[f]] != cs[nbr[f]]) throw SvpError(SVP_E_MESH, "svp_batch_define: a face joins two cases");
-        st_.caseOf = dupload<unsigned short>(cs);
-        dtCase_ = dalloc<double>(nCases);
-        factorCase_ = dalloc<double>(nCases);
-        courantCase_ = dalloc<unsigned long long>(2 * nCases);
-        st_.dtCase = dtCase_;
-        nCases_ = nCases;
-        reconstructed_ = false;
-    }
-
-    void batchSetDt(const double* dt, int where) {
-        if (nCases_ == 0) throw SvpError(SVP_E_ARG, "svp_batch_set_dt: no batch defined");
-        if (!dt) throw SvpError(SVP_E_ARG, "svp_batch_set_dt: null input");
-        if (where != SVP_DEVICE)
-            for (int k = 0; k < nCases_; ++k)
-                if (!(dt[k] >= 0.0) || !std::isfinite(dt[k]))
-                    throw SvpError(SVP_E_ARG, "svp_batch_set_dt: steps must be finite and >= 0");
-        copyIn(dtCase_, dt, nCases_, where);
         reconstructed_ = false;
     }
 
@@ -381,7 +358,6 @@ public:
     // the device buffer holding rhoPhi (n_faces).
     const double* advectSubCycles(double dt, int nSub, const double* Sp, const double* Su, double rho1,
                                   double rho2) {
-        if (nCases_ > 0) throw SvpError(SVP_E_ARG, "sub-cycles: not with a case batch");
         if (nSub < 1) throw SvpError(SVP_E_ARG, "nAlphaSubCycles must be at least 1");
         if (!(dt > 0.0) || !std::isfinite(dt)) throw SvpError(SVP_E_ARG, "advect: dt must be positive");
         if (!std::isfinite(rho1) || !std::isfinite(rho2)) throw SvpError(SVP_E_ARG, "rho1/rho2 must be finite");
@@ -514,11 +490,53 @@ public:
             dn = b;
             dd = b + 3L * m_.nC;
         }
+        hipEvent_t a = event(), b = event();
+        HIPCHECK(hipEventRecord(a, stream_));
         launch_map_alpha(m_, st_, dn, dd, lower, upper, stream_);
+        HIPCHECK(hipEventRecord(b, stream_));
         HIPCHECK(hipGetLastError());
         HIPCHECK(hipStreamSynchronize(stream_));
+        float ms = 0.f;
+        HIPCHECK(hipEventElapsedTime(&ms, a, b));
+        mapTime_ += 1e-3 * ms;
+        freeEvents_.push_back(a);
+        freeEvents_.push_back(b);
         checkDeviceErrors();
         reconstructed_ = false;
+    }
+
+    // reconstruction::alphaMappingTime (reconstruction.H:396-399): GPU seconds
+    // spent in mapAlphaField
+    double alphaMappingTime() const { return mapTime_; }
+
+    // The full-mesh interface fields and the face flatness zeta of the
+    // reference (reconstruction.C:448-471, 520-584; written by the reference
+    // when writePlicFields is on).  Any output may be null.
+    void plicFields(double* N, double* D, double* C, double* S, double* zeta, int where) {
+        checkDeviceErrors();
+        const long nC = m_.nC;
+        if (N || D || C || S) {
+            double *dN = N, *dD = D, *dC = C, *dS = S;
+            if (where != SVP_DEVICE) {
+                if (plicFields_.bytes < sizeof(double) * 10 * (size_t)nC) plicFields_.alloc<double>(10L * nC);
+                double* b = (double*)plicFields_.p;
+                dN = N ? b : nullptr;
+                dD = D ? b + 3 * nC : nullptr;
+                dC = C ? b + 4 * nC : nullptr;
+                dS = S ? b + 7 * nC : nullptr;
+            }
+            launch_plic_fields(m_, st_, dN, dD, dC, dS, stream_);
+            HIPCHECK(hipGetLastError());
+            HIPCHECK(hipStreamSynchronize(stream_));
+            if (where != SVP_DEVICE) {
+                if (N) copyOut(N, dN, 3 * nC, SVP_HOST);
+                if (D) copyOut(D, dD, nC, SVP_HOST);
+                if (C) copyOut(C, dC, 3 * nC, SVP_HOST);
+                if (S) copyOut(S, dS, 3 * nC, SVP_HOST);
+            }
+        }
+        if (zeta) copyOut(zeta, m_.flat, m_.nF, where);
+        if (where == SVP_DEVICE) HIPCHECK(hipStreamSynchronize(stream_));
     }
 
     // Host sources are staged in device buffers kept by the handle.
@@ -564,6 +582,31 @@ public:
         nic = n;
     }
 
+    // Batched stepping (class caseBatch): whether this handle's step is the
+    // plain path a batch runs (least-squares normal, fast plane placement, no
+    // warped-face split), its view of the state, and the bookkeeping a batch
+    // step does for it.
+    bool batchable(std::string& why) const {
+        if (learned_.active || rdfNormal_ || gaussNormal_) {
+            why = "a batch runs the least-squares normal (orientationMethod LS / isoAlphaGrad)";
+            return false;
+        }
+        if (params_.splitWarpedFace || !st_.un0Fused) {
+            why = "a batch runs cells of at most 8 faces and 16 points without splitWarpedFace";
+            return false;
+        }
+        return true;
+    }
+    int batchGroup() const { return st_.maxCellFaces <= 4 ? 4 : (st_.maxCellFaces <= 6 ? 6 : 8); }
+    int passSeq() const { return bndPassSeq_; }
+    void afterBatchStep(double dt, int nPasses) {
+        std::swap(st_.alpha, st_.alphaOld);
+        bndPassSeq_ += nPasses;
+        lastDt_ = dt;
+        alphaPhiValid_ = false;
+        reconstructed_ = false;
+    }
+
     // accessors used by the C-ABI
     DevMesh m_{};
     DevState st_{};
@@ -580,11 +623,6 @@ public:
     unsigned long long* courant_ = nullptr;
     int* nonfinite_ = nullptr;
     std::vector<double> hostC_, hostCf_, hostSf_;
-    // batch of cases
-    int nCases_ = 0;
-    double* dtCase_ = nullptr;
-    double* factorCase_ = nullptr;
-    unsigned long long* courantCase_ = nullptr;
     // learned orientation method
     struct {
         bool active = false, ready = false;
@@ -640,16 +678,17 @@ public:
     double* allocHarness(DBuf& b, long n) { return b.alloc<double>(n); }
     DBuf harnessBufs_[3];
     DBuf rhoPhiSum_;
-    DBuf plicIdx_, plicOut_, mapBuf_;
+    DBuf plicIdx_, plicOut_, mapBuf_, plicFields_;
 
 private:
     std::vector<std::unique_ptr<DBuf>> bufs_;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> pendingRecon_, pendingAdvect_;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events_;
     std::vector<hipEvent_t> freeEvents_;
-    double reconTime_ = 0.0, advectTime_ = 0.0;
+    double reconTime_ = 0.0, advectTime_ = 0.0, mapTime_ = 0.0;
     bool reconstructed_ = false;
     bool gaussNormal_ = false;  // orientationMethod alphaGrad
+    bool pointLinear_ = false;  // alphaGrad with grad(alpha1) Gauss pointLinear
     bool rdfNormal_ = false;    // orientationMethod isoRDF / RDF
     RdfBufs rdf_{};
     bool alphaPhiValid_ = false;
@@ -762,7 +801,7 @@ private:
             for (int k = 0; k < e - b; ++k) cFacesAsc[b + k] = (own[srt[k]] == c) ? srt[k] : ~srt[k];
         }
         maxCellFaces_ = maxCellFaces;
-        const int faceLimit = dict_.split_warped_face ? MAX_CELL_FACES : NS_CELL_FACES;
+        const int faceLimit = dict_.split_warped_face ? MAX_CELL_FACES : GEN_CELL_FACES;
         if (maxCellFaces > faceLimit)
             throw SvpError(SVP_E_MESH, "a cell has " + std::to_string(maxCellFaces) + " faces (supported " +
                                            std::to_string(faceLimit) + ")");
@@ -779,7 +818,7 @@ private:
             maxCellPts = std::max(maxCellPts, (int)v.size());
             for (int p : v) pcCnt[p + 1]++;
         }
-        if (maxCellPts > (dict_.split_warped_face ? MAX_CELL_POINTS - 1 : NS_CELL_POINTS))
+        if (maxCellPts > (dict_.split_warped_face ? MAX_CELL_POINTS - 1 : GEN_CELL_POINTS))
             throw SvpError(SVP_E_MESH, "a cell has " + std::to_string(maxCellPts) + " points");
         std::vector<int> pcOff(nP + 1, 0);
         for (int p = 0; p < nP; ++p) pcOff[p + 1] = pcOff[p] + pcCnt[p + 1];
@@ -905,7 +944,8 @@ private:
         // one word per candidate-list block (blocks of at least 64 bitmap words)
         st_.bndLook = dalloc<unsigned long long>((nC + 63) / 64 / 64 + 1);
         st_.detLook = dalloc<unsigned long long>(detect_blocks(nC));
-        st_.detTicket = dalloc<int>(1);
+        st_.detTicket = dalloc<unsigned long long>(1);
+        st_.bndTicket = dalloc<unsigned long long>(1);
         HIPCHECK(hipMemsetAsync(st_.bcount, 0, 32 * sizeof(int), stream_));
         st_.maxCellFaces = maxCellFaces_;
         // bounding rounds run grid-wide before the one-workgroup tail (tuning
@@ -956,6 +996,387 @@ struct svp_handle {
 };
 
 using gve::SimPLIC::solveVofEqu;
+using namespace svp;
+
+// A batch of handles advanced together: every stage of a step is one launch
+// over all cases (blockIdx.y = case), each case running the single-case
+// kernel body on its own mesh and state.  While the batch exists its cases
+// work on the batch's stream, so calls on a case handle stay ordered with the
+// batch's steps.
+class caseBatch {
+public:
+    caseBatch(svp_handle* const* hs, int n) {
+        if (n < 1 || n > 65535 || !hs) throw SvpError(SVP_E_ARG, "svp_batch_create: 1..65535 cases");
+        for (int k = 0; k < n; ++k) {
+            if (!hs[k] || !hs[k]->s) throw SvpError(SVP_E_ARG, "svp_batch_create: null handle");
+            for (int j = 0; j < k; ++j)
+                if (hs[j] == hs[k]) throw SvpError(SVP_E_ARG, "svp_batch_create: a handle is given twice");
+            c_.push_back(hs[k]->s.get());
+        }
+        solveVofEqu& f = *c_[0];
+        device_ = f.device_;
+        HIPCHECK(hipSetDevice(device_));
+        params_ = f.params_;
+        G_ = f.batchGroup();
+        MF_ = f.st_.maxCellFaces <= NS_CELL_FACES ? NS_CELL_FACES : MAX_CELL_FACES;
+        triQuad_ = f.m_.triQuadFaces;
+        for (solveVofEqu* h : c_) {
+            std::string why;
+            if (h->device_ != device_) throw SvpError(SVP_E_ARG, "svp_batch_create: cases on different devices");
+            if (!h->batchable(why)) throw SvpError(SVP_E_UNSUPPORTED, "svp_batch_create: " + why);
+            const Params& q = h->params_;
+            if (q.mixedCellTol != params_.mixedCellTol || q.nAlphaBounds != params_.nAlphaBounds ||
+                q.snapTol != params_.snapTol || q.clip != params_.clip)
+                throw SvpError(SVP_E_ARG, "svp_batch_create: the cases' alpha dictionaries differ");
+            if (h->batchGroup() != G_ || (h->st_.maxCellFaces <= NS_CELL_FACES ? NS_CELL_FACES : MAX_CELL_FACES) != MF_ ||
+                h->m_.triQuadFaces != triQuad_)
+                throw SvpError(SVP_E_ARG, "svp_batch_create: the cases' cell types differ (hexahedra and "
+                                          "tetrahedra go to separate batches)");
+        }
+        HIPCHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+        // the cases' earlier work is done; from now on they use the batch's stream
+        std::vector<BatchCase> host(n);
+        for (int k = 0; k < n; ++k) {
+            solveVofEqu& h = *c_[k];
+            HIPCHECK(hipStreamSynchronize(h.stream_));
+            ownStream_.push_back(h.stream_);
+            h.stream_ = stream_;
+            BatchCase& b = host[k];
+            std::memset(&b, 0, sizeof b);
+            b.m = h.m_;
+            const int nC = h.m_.nC, S = h.passSeq();
+            for (int a = 0; a < 2; ++a)
+                for (int half = 0; half < 2; ++half) {
+                    DevState v = h.st_;
+                    if (a) std::swap(v.alpha, v.alphaOld);
+                    v.cand = h.st_.cand + (long)((S + half) & 1) * nC;
+                    b.st[(a << 1) | half] = v;
+                }
+            b.detBlocks = detect_list_blocks(nC);
+            b.listWords = (nC + 63) / 64;
+            b.listBlocks = (b.listWords + h.st_.bndListWords - 1) / h.st_.bndListWords;
+            b.nB = h.m_.nF - h.m_.nIF;
+            b.U0 = h.U0_;
+            b.phi0 = h.phi0_;
+            b.U0b = h.U0b_;
+            maxC_ = std::max(maxC_, nC);
+            maxF_ = std::max(maxF_, h.m_.nF);
+            maxDet_ = std::max(maxDet_, b.detBlocks);
+            maxList_ = std::max(maxList_, b.listBlocks);
+            maxN_ = std::max(maxN_, std::max<long>(h.m_.nF, 3L * std::max(nC, b.nB)));
+            haveFlow_ = haveFlow_ && h.U0_ != nullptr;
+        }
+        bc_.upload(host);
+        info_.alloc<BatchStepInfo>(n);
+        factors_.alloc<double>(n);
+        courant_.alloc<unsigned long long>(2L * n);
+        for (int r = 0; r < RING; ++r) {
+            HIPCHECK(hipHostMalloc((void**)&ring_[r], sizeof(BatchStepInfo) * n + sizeof(double) * n, 0));
+            HIPCHECK(hipEventCreateWithFlags(&ringEv_[r], hipEventDisableTiming));
+            HIPCHECK(hipEventRecord(ringEv_[r], stream_));
+        }
+        aPar_.assign(n, 0);
+        hPar_.assign(n, 0);
+        // launch shapes of the grid-stride stages (x per case; svp_batch_tune)
+        dims_.ls = 32;
+        dims_.fsd = 64;
+        dims_.fluxList = 8;
+        dims_.fluxFaces = 64;
+        dims_.prep = 8;
+        dims_.flow = 4;
+        dims_.apply = 8;
+        HIPCHECK(hipStreamSynchronize(stream_));
+    }
+
+    ~caseBatch() {
+        (void)hipSetDevice(device_);
+        (void)hipStreamSynchronize(stream_);
+        for (size_t k = 0; k < c_.size(); ++k) c_[k]->stream_ = ownStream_[k];
+        for (auto& e : pending_) {
+            (void)hipEventDestroy(e.first);
+            (void)hipEventDestroy(e.second);
+        }
+        for (int r = 0; r < RING; ++r) {
+            if (ring_[r]) (void)hipHostFree(ring_[r]);
+            if (ringEv_[r]) (void)hipEventDestroy(ringEv_[r]);
+        }
+        (void)hipStreamDestroy(stream_);
+    }
+
+    int size() const { return (int)c_.size(); }
+
+    // reconstruct + advect of every case with dt[k] > 0 (dt <= 0: the case is
+    // left as it is), one launch per stage for all of them
+    void step(const double* dt) {
+        if (!dt) throw SvpError(SVP_E_ARG, "svp_batch_step: null dt");
+        const int n = size();
+        for (int k = 0; k < n; ++k)
+            if (!std::isfinite(dt[k])) throw SvpError(SVP_E_ARG, "svp_batch_step: non-finite dt");
+        upload(dt, nullptr);
+        hipEvent_t a = event(), b = event();
+        HIPCHECK(hipEventRecord(a, stream_));
+        for (int sg = STAGE_DETECT; sg <= STAGE_SNAPCLIP; ++sg) {
+            profiled(sg, [&] {
+                launch_batch_stage(sg, (const BatchCase*)bc_.p, (const BatchStepInfo*)info_.p, n, maxC_, maxF_,
+                                   maxDet_, maxList_, dims_, params_, G_, MF_, triQuad_, stream_);
+            });
+        }
+        HIPCHECK(hipEventRecord(b, stream_));
+        HIPCHECK(hipGetLastError());
+        pending_.push_back({a, b});
+        const int nPasses = std::min(params_.nAlphaBounds, MAX_BOUND_PASSES);
+        for (int k = 0; k < n; ++k) {
+            if (!(dt[k] > 0.0)) continue;
+            aPar_[k] ^= 1;
+            hPar_[k] ^= (nPasses & 1);
+            c_[k]->afterBatchStep(dt[k], nPasses);
+        }
+        steps_++;
+    }
+
+    // harness: max over each case's cells of sum|phi|/V and of the same for
+    // 0.01 <= alpha <= 0.99 (svp_harness_courant per case)
+    void courant(double* co, double* aco) {
+        const int n = size();
+        upload(nullptr, nullptr);
+        launch_batch_courant((const BatchCase*)bc_.p, (const BatchStepInfo*)info_.p, n, maxC_,
+                             (unsigned long long*)courant_.p, stream_);
+        HIPCHECK(hipGetLastError());
+        std::vector<unsigned long long> r(2L * n);
+        HIPCHECK(hipMemcpyAsync(r.data(), courant_.p, sizeof(unsigned long long) * r.size(), hipMemcpyDeviceToHost,
+                                stream_));
+        HIPCHECK(hipStreamSynchronize(stream_));
+        for (int k = 0; k < n; ++k) {
+            double v[2];
+            std::memcpy(v, &r[2L * k], sizeof v);
+            if (co) co[k] = v[0];
+            if (aco) aco[k] = v[1];
+        }
+    }
+
+    // harness: each case's base flow times its own factor
+    void scaleFlow(const double* factors) {
+        if (!factors) throw SvpError(SVP_E_ARG, "svp_batch_scale_flow: null factors");
+        if (!haveFlow_) throw SvpError(SVP_E_ARG, "svp_batch_scale_flow: a case has no base flow uploaded");
+        upload(nullptr, factors);
+        launch_batch_scale_flow((const BatchCase*)bc_.p, (const BatchStepInfo*)info_.p, (const double*)factors_.p,
+                                size(), maxN_, stream_);
+        HIPCHECK(hipGetLastError());
+    }
+
+    // GPU seconds of the batch steps so far and the summed mixed-cell counts
+    // of the cases' reconstructs (their running totals)
+    void timers(double& sec, uint64_t& nic) {
+        HIPCHECK(hipStreamSynchronize(stream_));
+        for (auto& e : pending_) {
+            float ms = 0.f;
+            HIPCHECK(hipEventElapsedTime(&ms, e.first, e.second));
+            stepSec_ += 1e-3 * ms;
+            free_.push_back(e.first);
+            free_.push_back(e.second);
+        }
+        pending_.clear();
+        sec = stepSec_;
+        nic = 0;
+        for (solveVofEqu* h : c_) {
+            unsigned long long v = 0;
+            HIPCHECK(hipMemcpy(&v, h->st_.nIcTotal, sizeof v, hipMemcpyDeviceToHost));
+            nic += v;
+        }
+    }
+
+    // per-stage GPU time (the handle's stage numbering) of the steps since
+    // profiling was switched on
+    void profileSet(int mask) {
+        drainProfile();
+        profMask_ = mask;
+    }
+    void profileGet(double* sec, int64_t* cnt, bool reset) {
+        drainProfile();
+        for (int k = 0; k < N_STAGES; ++k) {
+            if (sec) sec[k] = profSec_[k];
+            if (cnt) cnt[k] = profCnt_[k];
+        }
+        if (reset) {
+            std::fill(profSec_, profSec_ + N_STAGES, 0.0);
+            std::fill(profCnt_, profCnt_ + N_STAGES, (int64_t)0);
+        }
+    }
+
+    // Launch-shape tuner: the x extent of each grid-stride stage is tried over
+    // a few values on the batch's current step (each trial restarts from the
+    // same fields), and the fastest kept.  Results do not depend on these
+    // shapes; only the stage times do.
+    void tune(const double* dt, int32_t* chosen) {
+        if (!dt) throw SvpError(SVP_E_ARG, "svp_batch_tune: null dt");
+        const int n = size();
+        // the fields every trial starts from
+        std::vector<DBuf> saved(n);
+        for (int k = 0; k < n; ++k) {
+            saved[k].alloc<double>(c_[k]->m_.nC);
+            HIPCHECK(hipMemcpyAsync(saved[k].p, c_[k]->st_.alpha, sizeof(double) * c_[k]->m_.nC,
+                                    hipMemcpyDeviceToDevice, stream_));
+        }
+        auto restore = [&] {
+            for (int k = 0; k < n; ++k)
+                HIPCHECK(hipMemcpyAsync(c_[k]->st_.alpha, saved[k].p, sizeof(double) * c_[k]->m_.nC,
+                                        hipMemcpyDeviceToDevice, stream_));
+        };
+        const int oldMask = profMask_;
+        auto trial = [&](int stage) {
+            profileSet(1 << stage);
+            step(dt);
+            double sec[N_STAGES];
+            profileGet(sec, nullptr, true);
+            restore();
+            return sec[stage];
+        };
+        struct Knob { int stage; int* v; std::vector<int> cand; };
+        std::vector<Knob> knobs = {{STAGE_NORMAL, &dims_.ls, {8, 16, 32, 64, 128}},
+                                   {STAGE_FSD, &dims_.fsd, {16, 32, 64, 128, 256}},
+                                   {STAGE_IFLUX, &dims_.fluxFaces, {16, 32, 64, 128, 256}},
+                                   {STAGE_IFLUX, &dims_.fluxList, {2, 4, 8, 16, 32}},
+                                   {STAGE_BOUND, &dims_.flow, {1, 2, 4, 8}},
+                                   {STAGE_BOUND, &dims_.prep, {2, 4, 8, 16, 32}},
+                                   {STAGE_BOUND, &dims_.apply, {2, 4, 8, 16, 32}}};
+        trial(STAGE_DETECT);  // warm-up
+        for (Knob& kb : knobs) {
+            int best = *kb.v;
+            double bestT = 1e30;
+            for (int v : kb.cand) {
+                *kb.v = v;
+                double t = 0.0;
+                for (int r = 0; r < 2; ++r) t += trial(kb.stage);
+                if (t < bestT) {
+                    bestT = t;
+                    best = v;
+                }
+            }
+            *kb.v = best;
+        }
+        profileSet(oldMask);
+        HIPCHECK(hipStreamSynchronize(stream_));
+        if (chosen) {
+            chosen[0] = dims_.ls;
+            chosen[1] = dims_.fsd;
+            chosen[2] = dims_.fluxList;
+            chosen[3] = dims_.fluxFaces;
+            chosen[4] = dims_.prep;
+            chosen[5] = dims_.flow;
+            chosen[6] = dims_.apply;
+        }
+    }
+
+    void setDims(const int32_t* d) {
+        if (!d) throw SvpError(SVP_E_ARG, "svp_batch_set_shapes: null");
+        for (int k = 0; k < 7; ++k)
+            if (d[k] < 1 || d[k] > 65535) throw SvpError(SVP_E_ARG, "svp_batch_set_shapes: 1..65535 per stage");
+        dims_.ls = d[0];
+        dims_.fsd = d[1];
+        dims_.fluxList = d[2];
+        dims_.fluxFaces = d[3];
+        dims_.prep = d[4];
+        dims_.flow = d[5];
+        dims_.apply = d[6];
+    }
+
+    int device_ = 0;
+
+private:
+    static constexpr int RING = 8;
+    std::vector<solveVofEqu*> c_;
+    std::vector<hipStream_t> ownStream_;
+    hipStream_t stream_ = nullptr;
+    Params params_{};
+    int G_ = 6, MF_ = NS_CELL_FACES, triQuad_ = 1;
+    int maxC_ = 0, maxF_ = 0, maxDet_ = 0, maxList_ = 0;
+    long maxN_ = 0;
+    bool haveFlow_ = true;
+    BatchDims dims_{};
+    DBuf bc_, info_, factors_, courant_;
+    char* ring_[RING] = {};
+    hipEvent_t ringEv_[RING] = {};
+    int ringPos_ = 0;
+    std::vector<int> aPar_, hPar_;
+    long steps_ = 0;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending_;
+    std::vector<hipEvent_t> free_, allEvents_;
+    double stepSec_ = 0.0;
+    int profMask_ = 0;
+    double profSec_[N_STAGES] = {};
+    int64_t profCnt_[N_STAGES] = {};
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> profPending_;
+
+    // the per-case step records (dt, buffer parities) and, when given, the
+    // flow factors, through a ring of pinned buffers: a buffer is refilled
+    // only after the copy that last read it has run
+    void upload(const double* dt, const double* factors) {
+        const int n = size();
+        const int r = ringPos_;
+        ringPos_ = (ringPos_ + 1) % RING;
+        HIPCHECK(hipEventSynchronize(ringEv_[r]));
+        BatchStepInfo* inf = (BatchStepInfo*)ring_[r];
+        double* fac = (double*)(ring_[r] + sizeof(BatchStepInfo) * n);
+        for (int k = 0; k < n; ++k) {
+            inf[k].dt = dt ? (dt[k] > 0.0 ? dt[k] : 0.0) : 0.0;
+            inf[k].aPar = aPar_[k];
+            inf[k].hPar = hPar_[k];
+            if (factors) fac[k] = factors[k];
+        }
+        HIPCHECK(hipMemcpyAsync(info_.p, inf, sizeof(BatchStepInfo) * n, hipMemcpyHostToDevice, stream_));
+        if (factors) HIPCHECK(hipMemcpyAsync(factors_.p, fac, sizeof(double) * n, hipMemcpyHostToDevice, stream_));
+        HIPCHECK(hipEventRecord(ringEv_[r], stream_));
+    }
+
+    hipEvent_t event() {
+        if (!free_.empty()) {
+            hipEvent_t e = free_.back();
+            free_.pop_back();
+            return e;
+        }
+        hipEvent_t e;
+        HIPCHECK(hipEventCreate(&e));
+        allEvents_.push_back(e);
+        return e;
+    }
+
+    template <class F>
+    void profiled(int sg, F&& launch) {
+        if (!(profMask_ & (1 << sg))) {
+            launch();
+            return;
+        }
+        hipEvent_t a = event(), b = event();
+        HIPCHECK(hipEventRecord(a, stream_));
+        launch();
+        HIPCHECK(hipEventRecord(b, stream_));
+        profPending_.push_back({sg, {a, b}});
+    }
+
+    void drainProfile() {
+        if (profPending_.empty()) return;
+        HIPCHECK(hipStreamSynchronize(stream_));
+        for (auto& e : profPending_) {
+            float ms = 0.f;
+            HIPCHECK(hipEventElapsedTime(&ms, e.second.first, e.second.second));
+            profSec_[e.first] += 1e-3 * ms;
+            profCnt_[e.first] += 1;
+            free_.push_back(e.second.first);
+            free_.push_back(e.second.second);
+        }
+        profPending_.clear();
+    }
+};
+
+struct svp_batch {
+    std::unique_ptr<caseBatch> b;
+};
+
+static caseBatch& B(svp_batch* b) {
+    if (!b || !b->b) throw SvpError(SVP_E_ARG, "null batch");
+    HIPCHECK(hipSetDevice(b->b->device_));
+    return *b->b;
+}
 
 static solveVofEqu& H(svp_handle* h) {
     if (!h || !h->s) throw SvpError(SVP_E_ARG, "null handle");
@@ -979,7 +1400,9 @@ void svp_dict_defaults(svp_dict* d) {
     d->rdf_tol = 1e-6;
     d->rdf_rel_tol = 0.1;
     d->rdf_iterations = 5;
+    std::snprintf(d->grad_scheme, sizeof d->grad_scheme, "Gauss linear");
 }
+
 
 // OpenFOAM dictionary entries "key value;" with comments; sub-dictionaries
 // are skipped.
@@ -1039,6 +1462,13 @@ int svp_dict_parse(const char* text, svp_dict* out) {
             else if (key == "tol") d.rdf_tol = std::stod(val);
             else if (key == "relTol") d.rdf_rel_tol = std::stod(val);
             else if (key == "iterations") d.rdf_iterations = std::stoi(val);
+            else if (key == "grad(alpha1)" || key == "gradScheme") {
+                const std::string g = grad_scheme_name(val);
+                if (g.empty())
+                    throw SvpError(SVP_E_UNSUPPORTED, "grad(alpha1) scheme '" + val +
+                                                          "': this library provides Gauss linear and Gauss pointLinear");
+                std::snprintf(d.grad_scheme, sizeof d.grad_scheme, "%s", g.c_str());
+            }
         }
         if (!method.empty()) {
             if (method == "LS" || method == "isoAlphaGrad" || method == "alphaGrad" || method == "learned" ||
@@ -1209,6 +1639,18 @@ int svp_get_geometry(svp_handle* h, double* Cf, double* Sf, double* magSf, doubl
     });
 }
 
+int svp_alpha_mapping_time(svp_handle* h, double* seconds) {
+    return guarded([&] {
+        if (!seconds) throw SvpError(SVP_E_ARG, "svp_alpha_mapping_time: null output");
+        *seconds = H(h).alphaMappingTime();
+    });
+}
+
+int svp_get_plic_fields(svp_handle* h, double* interface_n, double* interface_d, double* interface_c,
+                        double* interface_s, double* zeta, int where) {
+    return guarded([&] { H(h).plicFields(interface_n, interface_d, interface_c, interface_s, zeta, where); });
+}
+
 int svp_timers(svp_handle* h, double* rec, double* adv, uint64_t* nic) {
     return guarded([&] {
         double r, a;
@@ -1275,19 +1717,6 @@ int svp_synchronize(svp_handle* h) {
     });
 }
 
-int svp_batch_define(svp_handle* h, int32_t n_cases, const int32_t* cell_offsets) {
-    return guarded([&] {
-        // switched off: the first GPU run of a batch stalled without an error
-        // and the cause is not identified yet
-        if (!std::getenv("SVP_ENABLE_BATCH"))
-            throw SvpError(SVP_E_UNSUPPORTED, "svp_batch_define: batched cases are switched off (their first GPU test stalled; cause not identified)");
-        H(h).batchDefine(n_cases, cell_offsets);
-    });
-}
-
-int svp_batch_set_dt(svp_handle* h, const double* dt, int where) {
-    return guarded([&] { H(h).batchSetDt(dt, where); });
-}
 
 int svp_set_normal_model(svp_handle* h, const svp_normal_model* m) {
     return guarded([&] {
@@ -1309,6 +1738,61 @@ int svp_learned_residuals(svp_handle* h, double* out, int where) {
 
 int svp_test_set_error_bits(svp_handle* h, int32_t bits) {
     return guarded([&] { H(h).setErrorBits(bits); });
+}
+
+int svp_batch_create(svp_handle* const* cases, int32_t n_cases, svp_batch** out) {
+    return guarded([&] {
+        if (!out) throw SvpError(SVP_E_ARG, "svp_batch_create: null output");
+        *out = nullptr;
+        auto b = std::make_unique<svp_batch>();
+        b->b = std::make_unique<caseBatch>(cases, n_cases);
+        *out = b.release();
+    });
+}
+
+int svp_batch_destroy(svp_batch* b) {
+    return guarded([&] {
+        if (b && b->b) (void)hipSetDevice(b->b->device_);
+        delete b;
+    });
+}
+
+int svp_batch_step(svp_batch* b, const double* dt) {
+    return guarded([&] { B(b).step(dt); });
+}
+
+int svp_batch_tune(svp_batch* b, const double* dt, int32_t* shapes) {
+    return guarded([&] { B(b).tune(dt, shapes); });
+}
+
+int svp_batch_set_shapes(svp_batch* b, const int32_t* shapes) {
+    return guarded([&] { B(b).setDims(shapes); });
+}
+
+int svp_batch_timers(svp_batch* b, double* seconds, uint64_t* n_ic_total) {
+    return guarded([&] {
+        double s = 0.0;
+        uint64_t n = 0;
+        B(b).timers(s, n);
+        if (seconds) *seconds = s;
+        if (n_ic_total) *n_ic_total = n;
+    });
+}
+
+int svp_batch_profile_set(svp_batch* b, int mask) {
+    return guarded([&] { B(b).profileSet(mask); });
+}
+
+int svp_batch_profile_get(svp_batch* b, double* seconds, int64_t* counts, int reset) {
+    return guarded([&] { B(b).profileGet(seconds, counts, reset != 0); });
+}
+
+int svp_batch_courant(svp_batch* b, double* max_sum_phi_by_v, double* max_alpha_sum_phi_by_v) {
+    return guarded([&] { B(b).courant(max_sum_phi_by_v, max_alpha_sum_phi_by_v); });
+}
+
+int svp_batch_scale_flow(svp_batch* b, const double* factors) {
+    return guarded([&] { B(b).scaleFlow(factors); });
 }
 
 const char* svp_last_error(void) { return g_err.c_str(); }
@@ -1389,36 +1873,6 @@ int svp_harness_init_alpha(svp_handle* h, const double* shapes, int nShapes, int
         svp::launch_init_alpha(s.m_, ds, nShapes, maxDepth, s.st_.alpha, s.stream_);
         HIPCHECK(hipGetLastError());
         HIPCHECK(hipStreamSynchronize(s.stream_));
-    });
-}
-
-int svp_harness_scale_flow_batch(svp_handle* h, const double* factors) {
-    return guarded([&] {
-        solveVofEqu& s = H(h);
-        if (!s.U0_) throw SvpError(SVP_E_ARG, "base flow not uploaded");
-        if (s.nCases_ == 0 || !factors) throw SvpError(SVP_E_ARG, "svp_harness_scale_flow_batch: no batch / null");
-        s.copyIn(s.factorCase_, factors, s.nCases_, SVP_HOST);
-        svp::launch_scale_flow_batch(s.m_, s.st_.caseOf, s.factorCase_, s.U0_, s.phi0_, s.U0b_, s.st_.U, s.st_.phi,
-                                     s.st_.Ub, s.stream_);
-        HIPCHECK(hipGetLastError());
-    });
-}
-
-int svp_harness_courant_batch(svp_handle* h, double* co, double* aco) {
-    return guarded([&] {
-        solveVofEqu& s = H(h);
-        if (s.nCases_ == 0) throw SvpError(SVP_E_ARG, "svp_harness_courant_batch: no batch defined");
-        svp::launch_courant_batch(s.m_, s.st_.phi, s.st_.alpha, s.st_.caseOf, s.nCases_, s.courantCase_, s.stream_);
-        std::vector<unsigned long long> r(2L * s.nCases_);
-        HIPCHECK(hipMemcpyAsync(r.data(), s.courantCase_, sizeof(unsigned long long) * r.size(),
-                                hipMemcpyDeviceToHost, s.stream_));
-        HIPCHECK(hipStreamSynchronize(s.stream_));
-        for (int k = 0; k < s.nCases_; ++k) {
-            double v[2];
-            std::memcpy(v, &r[2L * k], sizeof v);
-            if (co) co[k] = v[0];
-            if (aco) aco[k] = v[1];
-        }
     });
 }
 

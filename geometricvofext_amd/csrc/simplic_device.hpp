@@ -32,6 +32,11 @@ constexpr int NS_CELL_POINTS = 16;
 constexpr int NS_FACE_IDX = 48;
 // cells with warped faces split into triangles around the face centre
 constexpr int SP_FACE_IDX = 96;
+// general polyhedra (no split) beyond the fast path: 2:1-refined hexahedra
+// (up to 24 faces, 20 points) and agglomerated or polyhedral cells
+constexpr int GEN_CELL_FACES = 24;
+constexpr int GEN_CELL_POINTS = 32;
+constexpr int GEN_FACE_IDX = GEN_CELL_FACES * MAX_FACE_POINTS;
 
 struct V3 {
     double x, y, z;
@@ -426,6 +431,7 @@ struct CellPolyT {
 };
 using CellPolyNS = CellPolyT<NS_CELL_FACES, NS_CELL_POINTS, NS_FACE_IDX>;
 using CellPolySP = CellPolyT<MAX_CELL_FACES, MAX_CELL_POINTS, SP_FACE_IDX>;
+using CellPolyGen = CellPolyT<GEN_CELL_FACES, GEN_CELL_POINTS, GEN_FACE_IDX>;
 
 // Result of cutting a cell (cutCell::calcSubCell)
 struct CellCut {

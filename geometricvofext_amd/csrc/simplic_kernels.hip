@@ -6,8 +6,6 @@
 // (compiled with -ffp-contract=off); each kernel cites the reference lines it
 // follows.  Control flow that the reference runs on the host (mixed-cell count,
 // bounding passes) stays on the device, so a step never waits for the host.
-#include <atomic>
-
 #include "simplic_device.hpp"
 #include "simplic_types.hpp"
 
@@ -119,20 +117,16 @@ int detect_blocks(int nC) { return cdiv(nC, DETECT_BLOCK); }
 constexpr unsigned long long LOOK_AGG = 1ull << 30, LOOK_PREFIX = 2ull << 30;
 constexpr int LOOK_SPIN_LIMIT = 1 << 22;
 
-// epochs of the look-back launches, never 0 (the cleared look-back words)
-static unsigned next_list_epoch() {
-    static std::atomic<unsigned> epoch{0};
-    unsigned e = ++epoch;
-    while (e == 0) e = ++epoch;
-    return e;
-}
-
-// the block's ticket (thread 0 only); the last ticket of the launch resets
-// the counter for the next launch
-__device__ inline int take_ticket(int* counter, int nBlocks) {
-    const int t = atomicAdd(counter, 1);
-    if (t == nBlocks - 1) atomicExch(counter, 0);
-    return t;
+// The block's ticket (thread 0 only).  The counter only grows, by nBlocks per
+// launch (every launch of a kernel on one counter has the same grid, and every
+// block takes one ticket), so ticket / nBlocks numbers the launch and
+// ticket % nBlocks is the block's place in it.  The launch number gives the
+// look-back epoch (1..2^32-1, never the 0 of cleared words) on the device, so
+// a launch replayed from a captured graph gets a fresh epoch as well.
+__device__ inline int take_ticket(unsigned long long* counter, int nBlocks, unsigned& epoch) {
+    const unsigned long long t = atomicAdd(counter, 1ull);
+    epoch = (unsigned)((t / (unsigned long long)nBlocks) % 0xFFFFFFFFull) + 1u;
+    return (int)(t % (unsigned long long)nBlocks);
 }
 
 // Wave 0 of the block (all 64 lanes): publish this block's total and return
@@ -253,14 +247,17 @@ __global__ void __launch_bounds__(1024) k_scan_counts(int* counts, int nb, int* 
 constexpr int DETECT_LIST_THREADS = 1024, DETECT_LIST_CHUNKS = 8;
 constexpr int DETECT_LIST_CELLS = DETECT_LIST_THREADS * DETECT_LIST_CHUNKS;
 
-__global__ void __launch_bounds__(DETECT_LIST_THREADS) k_detect_list(DevMesh m, DevState st, double tol,
-                                                                     int nBlocks, unsigned epoch) {
+int detect_list_blocks(int nC) { return cdiv(nC, DETECT_LIST_CELLS); }
+
+__device__ __attribute__((always_inline)) inline void k_detect_list_body(const DevMesh& m, const DevState& st, double tol, int nBlocks) {
     constexpr int NW = DETECT_LIST_THREADS / 64;
     __shared__ int cnt[DETECT_LIST_CHUNKS][NW];
     __shared__ int sTicket, sBase;
-    if (threadIdx.x == 0) sTicket = take_ticket(st.detTicket, nBlocks);
+    __shared__ unsigned sEpoch;
+    if (threadIdx.x == 0) sTicket = take_ticket(st.detTicket, nBlocks, sEpoch);
     __syncthreads();
     const int b = sTicket;
+    const unsigned epoch = sEpoch;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int c0 = b * DETECT_LIST_CELLS + threadIdx.x;
     double a[DETECT_LIST_CHUNKS];
@@ -276,8 +273,7 @@ __global__ void __launch_bounds__(DETECT_LIST_THREADS) k_detect_list(DevMesh m, 
         const int c = c0 + k * DETECT_LIST_THREADS;
         bool flag = false;
         if (c < m.nC) {
-            // cells of an idle case of a batch are not reconstructed
-            flag = (tol < a[k]) && (a[k] < 1.0 - tol) && (!st.caseOf || st.dtCase[st.caseOf[c]] > 0.0);
+            flag = (tol < a[k]) && (a[k] < 1.0 - tol);
             st.mixedFlag[c] = flag ? 1 : 0;
         }
         flags |= flag ? (1u << k) : 0u;
@@ -314,9 +310,13 @@ __global__ void __launch_bounds__(DETECT_LIST_THREADS) k_detect_list(DevMesh m, 
     }
 }
 
+__global__ void __launch_bounds__(DETECT_LIST_THREADS) k_detect_list(DevMesh m, DevState st, double tol, int nBlocks) {
+    k_detect_list_body(m, st, tol, nBlocks);
+}
+
 void launch_detect(const DevMesh& m, const DevState& st, const Params& p, hipStream_t s) {
     const int nb = cdiv(m.nC, DETECT_LIST_CELLS);
-    k_detect_list<<<nb, DETECT_LIST_THREADS, 0, s>>>(m, st, p.mixedCellTol, nb, next_list_epoch());
+    k_detect_list<<<nb, DETECT_LIST_THREADS, 0, s>>>(m, st, p.mixedCellTol, nb);
 }
 
 // ---------------------------------------------------------------------------
@@ -417,7 +417,7 @@ __device__ inline V3 ls_gradient(const DevMesh& m, const double* alpha, int c) {
 }
 
 // wl: optional list of mixed-cell positions (*wn entries) instead of all of them
-__global__ void __launch_bounds__(64, 1) k_ls_normal(DevMesh m, DevState st, const int* wl, const int* wn) {
+__device__ __attribute__((always_inline)) inline void k_ls_normal_body(const DevMesh& m, const DevState& st, const int* wl, const int* wn) {
     const int nWork = wl ? *wn : *st.nMixed;
     for (int wi = blockIdx.x * blockDim.x + threadIdx.x; wi < nWork; wi += gridDim.x * blockDim.x) {
         const int i = wl ? wl[wi] : wi;
@@ -426,6 +426,10 @@ __global__ void __launch_bounds__(64, 1) k_ls_normal(DevMesh m, DevState st, con
         n = n / (mag(n) + SVP_SMALL);
         st3(st.ifN, i, n);
     }
+}
+
+__global__ void __launch_bounds__(64, 1) k_ls_normal(DevMesh m, DevState st, const int* wl, const int* wn) {
+    k_ls_normal_body(m, st, wl, wn);
 }
 
 void launch_ls_normal(const DevMesh& m, const DevState& st, hipStream_t s, const int* wl, const int* wn) {
@@ -438,6 +442,46 @@ void launch_ls_normal(const DevMesh& m, const DevState& st, hipStream_t s, const
 // (owner) or subtract it (neighbour), boundary faces of non-empty patches
 // add Sf*alpha (zeroGradient), then the sum is divided by V.  Only mixed
 // cells need the normal.  Faces in batches of 8 with all loads in flight.
+// volPointInterpolation of alpha at point p: internal points over their
+// cells, boundary points over their boundary faces, whose zeroGradient value
+// is the owner's alpha (the point sources and weights of the U interpolation)
+__device__ inline double point_alpha(const DevMesh& m, const double* alpha, int p) {
+    double t = 0.0;
+    for (int k = m.psOff[p]; k < m.psOff[p + 1]; ++k) {
+        const int s = m.psSrc[k];
+        t += m.psW[k] * alpha[s >= 0 ? s : m.own[~s]];
+    }
+    return t;
+}
+
+// Gauss pointLinear face value of internal face f (OpenFOAM v2312
+// pointLinear::correction, selected by the NAG case's fvSchemes,
+// tutorials/test/plicVofOrientationFoam/NAG/system/fvSchemes:35): the fan of
+// triangles (pi, p_k, p_k-1), closed by (pi, p_0, p_last), around
+// pi = w C_own + (1 - w) C_nei with w the face weight at the owner cell's
+// index (as pointLinear indexes it); the area average of
+// (linear value + the two point values) / 3, returned as
+// linear + (average - linear).
+__device__ inline double point_linear_face_value(const DevMesh& m, const double* alpha, int f, double lin) {
+    const int o = m.own[f], n = m.nbr[f];
+    const double wq = m.linW[o];
+    const V3 pi = wq * ld3(m.C, o) + (1.0 - wq) * ld3(m.C, n);
+    const int b = m.fOff[f], np = m.fOff[f + 1] - b;
+    const int p0 = m.fPts[b], pl = m.fPts[b + np - 1];
+    double at = mag(0.5 * cross(ld3(m.pts, p0) - pi, ld3(m.pts, pl) - pi));
+    double sumAt = at;
+    double sumPsip = at * (1.0 / 3.0) * (lin + point_alpha(m, alpha, p0) + point_alpha(m, alpha, pl));
+    for (int k = 1; k < np; ++k) {
+        const int pk = m.fPts[b + k], pk1 = m.fPts[b + k - 1];
+        at = mag(0.5 * cross(ld3(m.pts, pk) - pi, ld3(m.pts, pk1) - pi));
+        sumAt += at;
+        sumPsip += at * (1.0 / 3.0) * (lin + point_alpha(m, alpha, pk) + point_alpha(m, alpha, pk1));
+    }
+    const double corr = sumPsip / sumAt - lin;
+    return lin + corr;
+}
+
+template <bool POINT_LINEAR>
 __global__ void __launch_bounds__(64, 1) k_grad_normal(DevMesh m, DevState st) {
     const int nMixed = *st.nMixed;
     constexpr int FB = 8;
@@ -472,13 +516,15 @@ __global__ void __launch_bounds__(64, 1) k_grad_normal(DevMesh m, DevState st) {
                 const int ef = ev[j];
                 if (ef >= 0) {
                     if (ef < m.nIF) {
-                        const double ssf = w[j] * (ac - ao[j]) + ao[j];
+                        double ssf = w[j] * (ac - ao[j]) + ao[j];
+                        if (POINT_LINEAR) ssf = point_linear_face_value(m, st.alpha, ef, ssf);
                         g = g + ssf * sf[j];
                     } else if (!em[j]) {
                         g = g + ac * sf[j];
                     }
                 } else {
-                    const double ssf = w[j] * (ao[j] - ac) + ac;
+                    double ssf = w[j] * (ao[j] - ac) + ac;
+                    if (POINT_LINEAR) ssf = point_linear_face_value(m, st.alpha, ~ef, ssf);
                     g = g - ssf * sf[j];
                 }
             }
@@ -488,8 +534,9 @@ __global__ void __launch_bounds__(64, 1) k_grad_normal(DevMesh m, DevState st) {
     }
 }
 
-void launch_grad_normal(const DevMesh& m, const DevState& st, hipStream_t s) {
-    k_grad_normal<<<1024, 64, 0, s>>>(m, st);
+void launch_grad_normal(const DevMesh& m, const DevState& st, bool pointLinear, hipStream_t s) {
+    if (pointLinear) k_grad_normal<true><<<1024, 64, 0, s>>>(m, st);
+    else k_grad_normal<false><<<1024, 64, 0, s>>>(m, st);
 }
 
 // ---------------------------------------------------------------------------
@@ -982,7 +1029,7 @@ __device__ __attribute__((always_inline)) inline void fsd_face_cut(int fsz, cons
 #endif
 
 template <int G>
-__global__ void __launch_bounds__(64, SVP_FSD_WAVES) k_fsd_group(DevMesh m, DevState st, const int* wl, const int* wn) {
+__device__ __attribute__((always_inline)) inline void k_fsd_group_body(const DevMesh& m, const DevState& st, const int* wl, const int* wn) {
     static_assert(G >= 3 && G <= 8, "groups of 3 to 8 lanes");
     constexpr int NG = 64 / G;                         // groups per wave; lanes past NG * G idle
     constexpr int PPL = (NS_CELL_POINTS + G - 1) / G;  // cell points per lane
@@ -1255,13 +1302,20 @@ __global__ void __launch_bounds__(64, SVP_FSD_WAVES) k_fsd_group(DevMesh m, DevS
     }
 }
 
+template <int G>
+__global__ void __launch_bounds__(64, SVP_FSD_WAVES) k_fsd_group(DevMesh m, DevState st, const int* wl, const int* wn) {
+    k_fsd_group_body<G>(m, st, wl, wn);
+}
+
 void launch_find_signed_distance(const DevMesh& m, const DevState& st, const Params& p, hipStream_t s,
                                  const int* wl, const int* wn) {
     if (p.splitWarpedFace) k_find_signed_distance<CellPolySP><<<1024, 64, 0, s>>>(m, st, 1, wl, wn);
     else if (st.un0Fused && st.maxCellFaces <= 4) k_fsd_group<4><<<2048, 64, 0, s>>>(m, st, wl, wn);
     else if (st.un0Fused && st.maxCellFaces <= 6) k_fsd_group<6><<<2048, 64, 0, s>>>(m, st, wl, wn);
     else if (st.un0Fused) k_fsd_group<8><<<2048, 64, 0, s>>>(m, st, wl, wn);
-    else k_find_signed_distance<CellPolyNS><<<1024, 64, 0, s>>>(m, st, 0, wl, wn);
+    else if (st.maxCellFaces <= NS_CELL_FACES && m.maxCellPoints <= NS_CELL_POINTS)
+        k_find_signed_distance<CellPolyNS><<<1024, 64, 0, s>>>(m, st, 0, wl, wn);
+    else k_find_signed_distance<CellPolyGen><<<1024, 64, 0, s>>>(m, st, 0, wl, wn);
 }
 
 // ---------------------------------------------------------------------------
@@ -1358,13 +1412,14 @@ __global__ void __launch_bounds__(64) k_plic_polygons(DevMesh m, DevState st, in
 // reconstruction::mapAlphaField (reconstruction.C:725-784): every cell whose
 // alpha lies in [lower, upper] takes the volume fraction of its plane (n, d)
 // cut through its mesh faces (no warped-face split); n and d are per cell.
+template <class Poly>
 __global__ void __launch_bounds__(64) k_map_alpha(DevMesh m, DevState st, const double* __restrict__ n,
                                                   const double* __restrict__ d, double lower, double upper) {
     for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < m.nC; c += gridDim.x * blockDim.x) {
         const double a = st.alpha[c];
         if (!(a >= lower && a <= upper)) continue;
-        CellPolyNS P;
-        FaceCut fc[CellPolyNS::maxFaces];
+        Poly P;
+        FaceCut fc[Poly::maxFaces];
         CellCut cut;
         if (!build_cell_poly(m, c, false, P)) {
             atomicOr(st.errFlag, ERR_CELL_TOO_LARGE);
@@ -1378,20 +1433,57 @@ __global__ void __launch_bounds__(64) k_map_alpha(DevMesh m, DevState st, const 
 
 void launch_map_alpha(const DevMesh& m, const DevState& st, const double* n, const double* d, double lower,
                       double upper, hipStream_t s) {
-    if (m.nC > 0) k_map_alpha<<<std::min(4096, cdiv(m.nC, 64)), 64, 0, s>>>(m, st, n, d, lower, upper);
+    if (m.nC <= 0) return;
+    if (st.maxCellFaces <= NS_CELL_FACES && m.maxCellPoints <= NS_CELL_POINTS)
+        k_map_alpha<CellPolyNS><<<std::min(4096, cdiv(m.nC, 64)), 64, 0, s>>>(m, st, n, d, lower, upper);
+    else
+        k_map_alpha<CellPolyGen><<<std::min(4096, cdiv(m.nC, 64)), 64, 0, s>>>(m, st, n, d, lower, upper);
 }
 
 void launch_plic_surface(const DevMesh& m, const DevState& st, int nMixed, int pass, int* ptOff, int* faceIdx,
                          int* totals, double* pts, int* faceCells, int* faceOff, hipStream_t s) {
     if (nMixed <= 0) return;
     const int grid = std::min(1024, cdiv(nMixed, 64));
+    const bool gen = st.maxCellFaces > NS_CELL_FACES || m.maxCellPoints > NS_CELL_POINTS;
     if (pass == 0) {
-        k_plic_polygons<CellPolyNS><<<grid, 64, 0, s>>>(m, st, nMixed, ptOff, faceIdx, nullptr, nullptr, nullptr);
+        if (gen) k_plic_polygons<CellPolyGen><<<grid, 64, 0, s>>>(m, st, nMixed, ptOff, faceIdx, nullptr, nullptr, nullptr);
+        else k_plic_polygons<CellPolyNS><<<grid, 64, 0, s>>>(m, st, nMixed, ptOff, faceIdx, nullptr, nullptr, nullptr);
         k_scan_counts<<<1, 1024, 0, s>>>(ptOff, nMixed, totals, nullptr);
         k_scan_counts<<<1, 1024, 0, s>>>(faceIdx, nMixed, totals + 1, nullptr);
     } else {
-        k_plic_polygons<CellPolyNS><<<grid, 64, 0, s>>>(m, st, nMixed, ptOff, faceIdx, pts, faceCells, faceOff);
+        if (gen) k_plic_polygons<CellPolyGen><<<grid, 64, 0, s>>>(m, st, nMixed, ptOff, faceIdx, pts, faceCells, faceOff);
+        else k_plic_polygons<CellPolyNS><<<grid, 64, 0, s>>>(m, st, nMixed, ptOff, faceIdx, pts, faceCells, faceOff);
     }
+}
+
+// The full-mesh interface fields of the reference (interfaceN/D/C/S,
+// reconstruction.C:520-570): zero in every cell (initialize,
+// reconstruction.C:638-641), then the values of the mixed cells from the
+// compact arrays.  Any output may be null.
+__global__ void k_plic_fields_zero(int nC, double* N, double* D, double* C, double* S) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < 3L * nC; i += (long)gridDim.x * blockDim.x) {
+        if (N) N[i] = 0.0;
+        if (C) C[i] = 0.0;
+        if (S) S[i] = 0.0;
+        if (D && i < nC) D[i] = 0.0;
+    }
+}
+
+__global__ void k_plic_fields_scatter(DevState st, double* N, double* D, double* C, double* S) {
+    const int k = *st.nMixed;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < k; i += gridDim.x * blockDim.x) {
+        const int c = st.mixed[i];
+        if (N) st3(N, c, ld3(st.ifN, i));
+        if (C) st3(C, c, ld3(st.ifC, i));
+        if (S) st3(S, c, ld3(st.ifS, i));
+        if (D) D[c] = st.ifD[i];
+    }
+}
+
+void launch_plic_fields(const DevMesh& m, const DevState& st, double* N, double* D, double* C, double* S,
+                        hipStream_t s) {
+    k_plic_fields_zero<<<std::min(4096, cdiv(3L * m.nC, 256)), 256, 0, s>>>(m.nC, N, D, C, S);
+    k_plic_fields_scatter<<<256, 256, 0, s>>>(st, N, D, C, S);
 }
 
 // the closing polygon offset face_offsets[n_faces] = n_points, from the
@@ -1411,10 +1503,11 @@ void launch_plic_close(int* faceOff, const int* totals, hipStream_t s) {
 // Advection
 // ---------------------------------------------------------------------------
 
+
 // dVf = upwind(phi).flux(alpha)*dt (advectionTemplates.C:371): face value
 // lambda*(aP - aN) + aN with lambda = pos0(phi); zeroGradient boundary value
 // is the owner value; empty patches carry nothing.
-__global__ void __launch_bounds__(256) k_upwind(DevMesh m, DevState st, double dtStep) {
+__device__ __attribute__((always_inline)) inline void k_upwind_body(const DevMesh& m, const DevState& st, double dtStep) {
     // the out-of-bounds counters of the update and the flux work-list count
     // of the interface flux that follow start at zero
     if (blockIdx.x == 0) {
@@ -1423,7 +1516,7 @@ __global__ void __launch_bounds__(256) k_upwind(DevMesh m, DevState st, double d
     }
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= m.nF) return;
-    const double dt = cell_dt(st, m.own[f], dtStep);
+    const double dt = dtStep;
     const double ph = st.phi[f];
     double v;
     if (f < m.nIF) {
@@ -1436,6 +1529,10 @@ __global__ void __launch_bounds__(256) k_upwind(DevMesh m, DevState st, double d
         v = (ph * st.alphaOld[m.own[f]]) * dt;
     }
     st.dVf[f] = v;
+}
+
+__global__ void __launch_bounds__(256) k_upwind(DevMesh m, DevState st, double dtStep) {
+    k_upwind_body(m, st, dtStep);
 }
 
 // volPointInterpolation value at a point
@@ -1746,7 +1843,7 @@ __global__ void __launch_bounds__(64) k_un0(DevMesh m, DevState st) {
 // matter: each face is downwind of one cell and gets one volume), one atomic
 // per block and round (a single counter serialises its atomics).  The count
 // is cleared by the upwind kernel.
-__global__ void __launch_bounds__(256) k_flux_list(DevMesh m, DevState st, int slots) {
+__device__ __attribute__((always_inline)) inline void k_flux_list_body(const DevMesh& m, const DevState& st, int slots) {
     const int nMixed = *st.nMixed;
     const long total = (long)nMixed * slots;
     const int lane = threadIdx.x & 63;
@@ -1786,13 +1883,17 @@ __global__ void __launch_bounds__(256) k_flux_list(DevMesh m, DevState st, int s
     }
 }
 
+__global__ void __launch_bounds__(256) k_flux_list(DevMesh m, DevState st, int slots) {
+    k_flux_list_body(m, st, slots);
+}
+
 // timeIntegratedFlux with one thread per (cut cell, downwind face): each
 // downwind face's volume is independent of the others.  LISTED takes the
 // pairs from the work list; otherwise every (cut cell, face slot) pair is a
 // thread and the upwind faces drop out (cheaper for tetrahedra, whose light
 // faces do not repay the list).
 template <bool TRIQUAD, bool LISTED>
-__global__ void __launch_bounds__(64, SVP_FLUX_WAVES) k_interface_flux_faces(DevMesh m, DevState st, double dtStep, int slots) {
+__device__ __attribute__((always_inline)) inline void k_interface_flux_faces_body(const DevMesh& m, const DevState& st, double dtStep, int slots) {
     const long nWork = LISTED ? (long)*st.fluxCount : (long)*st.nMixed * slots;
     for (long wi = (long)blockIdx.x * blockDim.x + threadIdx.x; wi < nWork; wi += (long)gridDim.x * blockDim.x) {
         const int w = LISTED ? st.fluxList[wi] : (int)wi;
@@ -1801,7 +1902,7 @@ __global__ void __launch_bounds__(64, SVP_FLUX_WAVES) k_interface_flux_faces(Dev
         const int c = st.mixed[i];
         const int fb = m.cOff[c];
         if (!LISTED && k >= m.cOff[c + 1] - fb) continue;
-        const double dt = cell_dt(st, c, dtStep);
+        const double dt = dtStep;
         const int f = m.cFaces[fb + k];
         const double ph = st.phi[f];
         if (!LISTED) {
@@ -1833,6 +1934,11 @@ __global__ void __launch_bounds__(64, SVP_FLUX_WAVES) k_interface_flux_faces(Dev
     }
 }
 
+template <bool TRIQUAD, bool LISTED>
+__global__ void __launch_bounds__(64, SVP_FLUX_WAVES) k_interface_flux_faces(DevMesh m, DevState st, double dtStep, int slots) {
+    k_interface_flux_faces_body<TRIQUAD, LISTED>(m, st, dtStep, slots);
+}
+
 __device__ inline bool loop_oob(double a, double aTol) { return (a - 1.0 > aTol) || (a < -aTol); }
 __device__ inline bool cell_oob(double a, double aTol) { return (a < -aTol) || (a > 1.0 + aTol); }
 
@@ -1856,15 +1962,12 @@ __device__ inline bool need_bounding(const DevMesh& m, const unsigned char* mixe
 // per wave.
 constexpr int UPDATE_BLOCK = 256;
 
-__global__ void __launch_bounds__(UPDATE_BLOCK) k_update(DevMesh m, DevState st, double dtStep,
-                                                         const double* Sp, const double* Su) {
+__device__ __attribute__((always_inline)) inline void k_update_body(const DevMesh& m, const DevState& st, double dtStep, const double* Sp, const double* Su) {
     const double aTol = 100.0 * SVP_SMALL;
     const int c = blockIdx.x * UPDATE_BLOCK + threadIdx.x;
     bool cand = false, oob = false;
-    const double dt = c < m.nC ? cell_dt(st, c, dtStep) : 1.0;
-    if (c < m.nC && dt == 0.0) {
-        st.alpha[c] = st.alphaOld[c];  // idle case of a batch: alpha unchanged
-    } else if (c < m.nC) {
+    const double dt = dtStep;
+    if (c < m.nC) {
         const double rDeltaT = 1.0 / dt;
         double sI = 0.0;
         for (int k = m.cOff[c]; k < m.cOff[c + 1]; ++k) {
@@ -1894,6 +1997,10 @@ __global__ void __launch_bounds__(UPDATE_BLOCK) k_update(DevMesh m, DevState st,
         if (word < (m.nC + 63) / 64) st.candBits[word] = bits;
         if (oobBits) atomicAdd(&st.oobCount[OOB_STRIDE * (blockIdx.x & (OOB_SLOTS - 1))], __popcll(oobBits));
     }
+}
+
+__global__ void __launch_bounds__(UPDATE_BLOCK) k_update(DevMesh m, DevState st, double dtStep, const double* Sp, const double* Su) {
+    k_update_body(m, st, dtStep, Sp, Su);
 }
 
 __device__ inline double face_value(const DevMesh& m, const double* f, int facei) {
@@ -2138,8 +2245,6 @@ __device__ inline void bound_finish(const BndRec<MF>& R, const double (&cin)[MF]
 // makes any word of an earlier launch read as not yet published.  With a
 // zero bound count the pass is off (no candidates).  Ticket 0 also clears
 // the per-pass counters.
-constexpr int BND_LIST_TICKET = 15;
-static_assert(BND_LIST_TICKET > BND_PROGRESS, "list ticket outside the cleared counters");
 // candidates of the previous pass, whose leftovers the next list clears
 constexpr int BND_PREV_COUNT = 16;
 // any cell out of bounds: sum of the spread counters (each slot is one cache
@@ -2151,13 +2256,14 @@ __device__ inline bool oob_any(const DevState& st) {
     return n != 0;
 }
 
-__global__ void __launch_bounds__(256) k_bnd_list(DevMesh m, DevState st, int nWords, int nBlocks, unsigned epoch,
-                                                  const int* candPrev) {
+__device__ __attribute__((always_inline)) inline void k_bnd_list_body(const DevMesh& m, const DevState& st, int nWords, int nBlocks, const int* candPrev) {
     __shared__ int lds[16];
     __shared__ int sTicket, sBase;
-    if (threadIdx.x == 0) sTicket = take_ticket(&st.bcount[BND_LIST_TICKET], nBlocks);
+    __shared__ unsigned sEpoch;
+    if (threadIdx.x == 0) sTicket = take_ticket(st.bndTicket, nBlocks, sEpoch);
     __syncthreads();
     const int b = sTicket;
+    const unsigned epoch = sEpoch;
     // what the previous pass (of this step or the last one) left: the
     // corrections and correctedFaces keys on the faces of its active cells
     // and their candidate index, back to 0 / -1 before this pass's records.
@@ -2226,14 +2332,17 @@ __global__ void __launch_bounds__(256) k_bnd_list(DevMesh m, DevState st, int nW
     clearPrevious();
 }
 
+__global__ void __launch_bounds__(256) k_bnd_list(DevMesh m, DevState st, int nWords, int nBlocks, const int* candPrev) {
+    k_bnd_list_body(m, st, nWords, nBlocks, candPrev);
+}
+
 template <int MF>
-__global__ void __launch_bounds__(256) k_bnd_prep(DevMesh m, DevState st, double dt, const double* Sp,
-                                                  const double* Su) {
+__device__ __attribute__((always_inline)) inline void k_bnd_prep_body(const DevMesh& m, const DevState& st, double dt, const double* Sp, const double* Su) {
     const int nCand = st.bcount[0];
     int nOpen = 0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nCand; i += gridDim.x * blockDim.x) {
         const int c = st.cand[i];
-        const int n = bound_record<MF>(m, st, c, cell_dt(st, c, dt), Sp, Su, ((BndRec<MF>*)st.bndRec)[i]);
+        const int n = bound_record<MF>(m, st, c, dt, Sp, Su, ((BndRec<MF>*)st.bndRec)[i]);
         st.active[i] = n > 0 ? 1 : 0;
         if (n > 0) st.actIdx[c] = i;
         nOpen += n;
@@ -2241,6 +2350,11 @@ __global__ void __launch_bounds__(256) k_bnd_prep(DevMesh m, DevState st, double
     // correctedFaces length of the pass (statistics)
     for (int o = 32; o > 0; o >>= 1) nOpen += __shfl_xor(nOpen, o);
     if ((threadIdx.x & 63) == 0 && nOpen) atomicAdd(&st.bcount[1], nOpen);
+}
+
+template <int MF>
+__global__ void __launch_bounds__(256) k_bnd_prep(DevMesh m, DevState st, double dt, const double* Sp, const double* Su) {
+    k_bnd_prep_body<MF>(m, st, dt, Sp, Su);
 }
 
 // correctedFaces keys and the cells they touch (both schedules), for candidate
@@ -2305,16 +2419,13 @@ __global__ void __launch_bounds__(256) k_bnd_scatter(DevMesh m, DevState st) {
     }
 }
 
-// boundFlux of candidate i from its record: its own case's time step in a batch
+// boundFlux of candidate i from its record
 template <int MF, class Out>
 __device__ inline void bound_candidate(const DevState& st, const BndRec<MF>& R, const double (&cin)[MF], int i,
                                        double dt, double rDeltaT, Out out) {
-    if (st.caseOf) {
-        const double dtc = st.dtCase[st.caseOf[st.cand[i]]];
-        bound_finish<MF>(R, cin, dtc, 1.0 / dtc, out);
-    } else {
-        bound_finish<MF>(R, cin, dt, rDeltaT, out);
-    }
+    (void)st;
+    (void)i;
+    bound_finish<MF>(R, cin, dt, rDeltaT, out);
 }
 
 // completes cell i of frontier r and releases its successors into frontier
@@ -2473,7 +2584,7 @@ struct FlowStall {
 };
 
 template <int MF>
-__global__ void __launch_bounds__(flow_threads<MF>()) k_bnd_flow(DevMesh m, DevState st, double dt) {
+__device__ __attribute__((always_inline)) inline void k_bnd_flow_body(const DevMesh& m, const DevState& st, double dt) {
     constexpr int CH = flow_chunk<MF>();
     // LDS slots [0, CH*MF): corrections written by the chunk's cells (slot
     // t*MF+k: cell t, face slot k); [CH*MF, 2*CH*MF): corrections read from
@@ -2560,9 +2671,7 @@ __global__ void __launch_bounds__(flow_threads<MF>()) k_bnd_flow(DevMesh m, DevS
                 src[k] = NS + t * MF + k;
             }
         }
-        // time step of the candidate (its case's in a batch)
-        double dtc = dt;
-        if (st.caseOf && active) dtc = st.dtCase[st.caseOf[st.cand[i]]];
+        const double dtc = dt;
         const double rdc = 1.0 / dtc;
         __syncthreads();
         FlowStall stall;
@@ -2643,11 +2752,16 @@ __global__ void __launch_bounds__(flow_threads<MF>()) k_bnd_flow(DevMesh m, DevS
     }
 }
 
+template <int MF>
+__global__ void __launch_bounds__(flow_threads<MF>()) k_bnd_flow(DevMesh m, DevState st, double dt) {
+    k_bnd_flow_body<MF>(m, st, dt);
+}
+
 // alpha of every touched cell in correctedFaces order (advectionTemplates.C:164-192)
 // and the dVf of every corrected face (the first-loop faces of the active
 // candidates)
 template <int MF>
-__global__ void __launch_bounds__(256) k_bnd_apply(DevMesh m, DevState st, int pass) {
+__device__ __attribute__((always_inline)) inline void k_bnd_apply_body(const DevMesh& m, const DevState& st, int pass) {
     const double aTol = 100.0 * SVP_SMALL;
     const int nT = st.bcount[2], nCand = st.bcount[0];
     // pass statistics, and the candidate count for the next list's clearing
@@ -2748,6 +2862,11 @@ __global__ void __launch_bounds__(256) k_bnd_apply(DevMesh m, DevState st, int p
 }
 
 template <int MF>
+__global__ void __launch_bounds__(256) k_bnd_apply(DevMesh m, DevState st, int pass) {
+    k_bnd_apply_body<MF>(m, st, pass);
+}
+
+template <int MF>
 void launch_bound_pass(const DevMesh& m, const DevState& st, int pass, double dt, const double* Sp,
                        const double* Su, const int* candPrev, hipStream_t s) {
     const int nWords = (m.nC + 63) / 64;
@@ -2755,7 +2874,7 @@ void launch_bound_pass(const DevMesh& m, const DevState& st, int pass, double dt
     // one thread per cell of the mesh
     const int grid = std::max(1, std::min(st.bndGrid, cdiv(m.nC, 256)));
     const int nbw = cdiv(nWords, st.bndListWords);
-    k_bnd_list<<<nbw, 256, 0, s>>>(m, st, nWords, nbw, next_list_epoch(), candPrev);
+    k_bnd_list<<<nbw, 256, 0, s>>>(m, st, nWords, nbw, candPrev);
     k_bnd_prep<MF><<<grid, 256, 0, s>>>(m, st, dt, Sp, Su);
     if (st.bndFlow) {
         k_bnd_flow<MF><<<std::max(1, std::min(st.bndFlow, cdiv(m.nC, flow_chunk<MF>()))), flow_threads<MF>(), 0, s>>>(
@@ -2785,14 +2904,17 @@ void launch_bound(const DevMesh& m, const DevState& st, const Params& p, double 
 }
 
 // applyBruteForceBounding (advection.C:291-308)
-__global__ void k_snap_clip(int nC, double* alpha, double snapTol, int clip, DevState st) {
+__device__ __attribute__((always_inline)) inline void k_snap_clip_body(int nC, double* alpha, double snapTol, int clip, const DevState& st) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nC) return;
-    if (st.caseOf && st.dtCase[st.caseOf[c]] == 0.0) return;  // idle case
     double a = alpha[c];
     if (snapTol > 0.0) a = a * pos0(a - snapTol) * neg0(a - (1.0 - snapTol)) + pos0(a - (1.0 - snapTol));
     if (clip) a = fmin(1.0, fmax(0.0, a));
     alpha[c] = a;
+}
+
+__global__ void k_snap_clip(int nC, double* alpha, double snapTol, int clip, DevState st) {
+    k_snap_clip_body(nC, alpha, snapTol, clip, st);
 }
 
 __global__ void k_zero_int(int* p, int n) {
@@ -2834,8 +2956,7 @@ void launch_advect_stage(int stage, const DevMesh& m, const DevState& st, const 
 __global__ void k_alpha_phi(DevMesh m, DevState st, double dtStep) {
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= m.nF) return;
-    const double dt = cell_dt(st, m.own[f], dtStep);
-    st.alphaPhi[f] = dt > 0.0 ? st.dVf[f] / dt : 0.0;
+    st.alphaPhi[f] = dtStep > 0.0 ? st.dVf[f] / dtStep : 0.0;
 }
 
 void launch_alpha_phi(const DevMesh& m, const DevState& st, double dt, hipStream_t s) {
@@ -2858,6 +2979,185 @@ __global__ void k_rho_phi(int nF, const double* __restrict__ ap, const double* _
 void launch_rho_phi(int nF, const double* ap, const double* phi, double rho1, double rho2, double w,
                     double* out, int mode, hipStream_t s) {
     if (nF > 0) k_rho_phi<<<cdiv(nF, 256), 256, 0, s>>>(nF, ap, phi, rho1 - rho2, rho2, w, out, mode);
+}
+
+// ---------------------------------------------------------------------------
+// Batched cases: independent handles advanced together, one launch per stage
+// with blockIdx.y = case.  Every case runs the single-case kernel body on its
+// own mesh and state (its own tickets, look-back words and counters), so its
+// results are those of its handle stepping alone; a case whose step is <= 0
+// returns at once.  Blocks past a case's own look-back grid return before
+// taking a ticket, so each case's ticket counter still grows by its own
+// block count per launch.
+// ---------------------------------------------------------------------------
+__device__ inline const DevState& batch_view(const BatchCase& c, const BatchStepInfo& si, int swapped, int pass) {
+    return c.st[((si.aPar ^ swapped) << 1) | (si.hPar ^ (pass & 1))];
+}
+
+__global__ void __launch_bounds__(DETECT_LIST_THREADS) kb_detect_list(const BatchCase* __restrict__ bc,
+                                                                      const BatchStepInfo* __restrict__ info,
+                                                                      double tol) {
+    const BatchCase& c = bc[blockIdx.y];
+    const BatchStepInfo si = info[blockIdx.y];
+    if (!(si.dt > 0.0) || (int)blockIdx.x >= c.detBlocks) return;
+    k_detect_list_body(c.m, batch_view(c, si, 0, 0), tol, c.detBlocks);
+}
+
+__global__ void __launch_bounds__(64, 1) kb_ls_normal(const BatchCase* __restrict__ bc,
+                                                      const BatchStepInfo* __restrict__ info) {
+    const BatchCase& c = bc[blockIdx.y];
+    const BatchStepInfo si = info[blockIdx.y];
+    if (!(si.dt > 0.0)) return;
+    k_ls_normal_body(c.m, batch_view(c, si, 0, 0), nullptr, nullptr);
+}
+
+template <int G>
+__global__ void __launch_bounds__(64, SVP_FSD_WAVES) kb_fsd_group(const BatchCase* __restrict__ bc,
+                                                                  const BatchStepInfo* __restrict__ info) {
+    const BatchCase& c = bc[blockIdx.y];
+    const BatchStepInfo si = info[blockIdx.y];
+    if (!(si.dt > 0.0)) return;
+    k_fsd_group_body<G>(c.m, batch_view(c, si, 0, 0), nullptr, nullptr);
+}
+
+__global__ void __launch_bounds__(256) kb_upwind(const BatchCase* __restrict__ bc,
+                                                 const BatchStepInfo* __restrict__ info) {
+    const BatchCase& c = bc[blockIdx.y];
+    const BatchStepInfo si = info[blockIdx.y];
+    if (!(si.dt > 0.0)) return;
+    k_upwind_body(c.m, batch_view(c, si, 1, 0), si.dt);
+}
+
+__global__ void __launch_bounds__(256) kb_flux_list(const BatchCase* __restrict__ bc,
+                                                    const BatchStepInfo* __restrict__ info) {
+    const BatchCase& c = bc[blockIdx.y];
+    const BatchStepInfo si = info[blockIdx.y];
+    if (!(si.dt > 0.0)) return;
+    const DevState& st = batch_view(c, si, 1, 0);
+    k_flux_list_body(c.m, st, st.maxCellFaces);
+}
+
+template <bool TRIQUAD, bool LISTED>
+__global__ void __launch_bounds__(64, SVP_FLUX_WAVES) kb_interface_flux_faces(const BatchCase* __restrict__ bc,
+                                                                              const BatchStepInfo* __restrict__ info) {
+    const BatchCase& c = bc[blockIdx.y];
+    const BatchStepInfo si = info[blockIdx.y];
+    if (!(si.dt > 0.0)) return;
+    const DevState& st = batch_view(c, si, 1, 0);
+    k_interface_flux_faces_body<TRIQUAD, LISTED>(c.m, st, si.dt, st.maxCellFaces);
+}
+
+__global__ void __launch_bounds__(UPDATE_BLOCK) kb_update(const BatchCase* __restrict__ bc,
+                                                          const BatchStepInfo* __restrict__ info) {
+    const BatchCase& c = bc[blockIdx.y];
+    const BatchStepInfo si = info[blockIdx.y];
+    if (!(si.dt > 0.0)) return;
+    k_update_body(c.m, batch_view(c, si, 1, 0), si.dt, nullptr, nullptr);
+}
+
+__global__ void __launch_bounds__(256) kb_bnd_list(const BatchCase* __restrict__ bc,
+                                                   const BatchStepInfo* __restrict__ info, int pass) {
+    const BatchCase& c = bc[blockIdx.y];
+    const BatchStepInfo si = info[blockIdx.y];
+    if (!(si.dt > 0.0) || (int)blockIdx.x >= c.listBlocks) return;
+    k_bnd_list_body(c.m, batch_view(c, si, 1, pass), c.listWords, c.listBlocks, batch_view(c, si, 1, pass + 1).cand);
+}
+
+template <int MF>
+__global__ void __launch_bounds__(256) kb_bnd_prep(const BatchCase* __restrict__ bc,
+                                                   const BatchStepInfo* __restrict__ info, int pass) {
+    const BatchCase& c = bc[blockIdx.y];
+    const BatchStepInfo si = info[blockIdx.y];
+    if (!(si.dt > 0.0)) return;
+    k_bnd_prep_body<MF>(c.m, batch_view(c, si, 1, pass), si.dt, nullptr, nullptr);
+}
+
+template <int MF>
+__global__ void __launch_bounds__(flow_threads<MF>()) kb_bnd_flow(const BatchCase* __restrict__ bc,
+                                                                  const BatchStepInfo* __restrict__ info, int pass) {
+    const BatchCase& c = bc[blockIdx.y];
+    const BatchStepInfo si = info[blockIdx.y];
+    if (!(si.dt > 0.0)) return;
+    k_bnd_flow_body<MF>(c.m, batch_view(c, si, 1, pass), si.dt);
+}
+
+template <int MF>
+__global__ void __launch_bounds__(256) kb_bnd_apply(const BatchCase* __restrict__ bc,
+                                                    const BatchStepInfo* __restrict__ info, int pass) {
+    const BatchCase& c = bc[blockIdx.y];
+    const BatchStepInfo si = info[blockIdx.y];
+    if (!(si.dt > 0.0)) return;
+    k_bnd_apply_body<MF>(c.m, batch_view(c, si, 1, pass), pass);
+}
+
+__global__ void kb_snap_clip(const BatchCase* __restrict__ bc, const BatchStepInfo* __restrict__ info,
+                             double snapTol, int clip) {
+    const BatchCase& c = bc[blockIdx.y];
+    const BatchStepInfo si = info[blockIdx.y];
+    if (!(si.dt > 0.0)) return;
+    const DevState& st = batch_view(c, si, 1, 0);
+    k_snap_clip_body(c.m.nC, st.alpha, snapTol, clip, st);
+}
+
+// pass = the pass of the step: its candidate-list half is the case's hPar
+// flipped per pass, and it indexes the pass statistics
+template <int MF>
+static void launch_batch_bound(const BatchCase* bc, const BatchStepInfo* info, int n, int maxListBlocks,
+                               const BatchDims& d, int nPasses, hipStream_t s) {
+    for (int pass = 0; pass < nPasses; ++pass) {
+        kb_bnd_list<<<dim3(maxListBlocks, n), 256, 0, s>>>(bc, info, pass);
+        kb_bnd_prep<MF><<<dim3(d.prep, n), 256, 0, s>>>(bc, info, pass);
+        kb_bnd_flow<MF><<<dim3(d.flow, n), flow_threads<MF>(), 0, s>>>(bc, info, pass);
+        kb_bnd_apply<MF><<<dim3(d.apply, n), 256, 0, s>>>(bc, info, pass);
+    }
+}
+
+void launch_batch_stage(int stage, const BatchCase* bc, const BatchStepInfo* info, int n, int maxC, int maxF,
+                        int maxDetBlocks, int maxListBlocks, const BatchDims& d, const Params& p, int G, int MF,
+                        int triQuad, hipStream_t s) {
+    switch (stage) {
+        case STAGE_DETECT:
+            kb_detect_list<<<dim3(maxDetBlocks, n), DETECT_LIST_THREADS, 0, s>>>(bc, info, p.mixedCellTol);
+            break;
+        case STAGE_NORMAL:
+            kb_ls_normal<<<dim3(d.ls, n), 64, 0, s>>>(bc, info);
+            break;
+        case STAGE_FSD:
+            if (G == 4) kb_fsd_group<4><<<dim3(d.fsd, n), 64, 0, s>>>(bc, info);
+            else if (G == 6) kb_fsd_group<6><<<dim3(d.fsd, n), 64, 0, s>>>(bc, info);
+            else kb_fsd_group<8><<<dim3(d.fsd, n), 64, 0, s>>>(bc, info);
+            break;
+        case STAGE_UPWIND:
+            kb_upwind<<<dim3(cdiv(maxF, 256), n), 256, 0, s>>>(bc, info);
+            break;
+        case STAGE_IFLUX:
+            if (G > 4) {
+                kb_flux_list<<<dim3(d.fluxList, n), 256, 0, s>>>(bc, info);
+                if (triQuad) kb_interface_flux_faces<true, true><<<dim3(d.fluxFaces, n), 64, 0, s>>>(bc, info);
+                else kb_interface_flux_faces<false, true><<<dim3(d.fluxFaces, n), 64, 0, s>>>(bc, info);
+            } else {
+                if (triQuad) kb_interface_flux_faces<true, false><<<dim3(d.fluxFaces, n), 64, 0, s>>>(bc, info);
+                else kb_interface_flux_faces<false, false><<<dim3(d.fluxFaces, n), 64, 0, s>>>(bc, info);
+            }
+            break;
+        case STAGE_UPDATE:
+            kb_update<<<dim3(cdiv(maxC, UPDATE_BLOCK), n), UPDATE_BLOCK, 0, s>>>(bc, info);
+            break;
+        case STAGE_BOUND: {
+            const int nPasses = std::min(p.nAlphaBounds, MAX_BOUND_PASSES);
+            if (MF <= NS_CELL_FACES)
+                launch_batch_bound<NS_CELL_FACES>(bc, info, n, maxListBlocks, d, nPasses, s);
+            else
+                launch_batch_bound<MAX_CELL_FACES>(bc, info, n, maxListBlocks, d, nPasses, s);
+            break;
+        }
+        case STAGE_SNAPCLIP:
+            if (p.snapTol > 0.0 || p.clip)
+                kb_snap_clip<<<dim3(cdiv(maxC, 256), n), 256, 0, s>>>(bc, info, p.snapTol, p.clip);
+            break;
+        default:
+            break;
+    }
 }
 
 }  // namespace svp

@@ -85,7 +85,8 @@ struct DevState {
     int* bcount;        // [32] per-pass counters of the bounding kernels
     unsigned long long* bndLook;  // per candidate-list block: epoch, look-back flag and count
     unsigned long long* detLook;  // per detection block: the same
-    int* detTicket;               // [1] detection block tickets
+    unsigned long long* detTicket;  // detection block tickets (grows by the block count per launch)
+    unsigned long long* bndTicket;  // candidate-list block tickets (the same)
     int un0Fused;       // plane placement by 8-lane groups, which also computes Un0
     int fsdKeep;        // 1: where findSignedDistance writes no d/C/S they keep their values (isoRDF)
     int gridRounds;     // bounding rounds run as grid-wide kernels before the one-workgroup tail
@@ -94,20 +95,11 @@ struct DevState {
     int bndListWords;   // bitmap words per block of the candidate list (64..4096)
     void* bndRec;       // nC records BndRec<MF> of the active bounding candidates
     int maxCellFaces;
-    // batch of independent cases in one mesh: case of every cell and the
-    // per-case time step (0 = the case is idle this step); null when the
-    // handle holds one case and the step's dt is passed by value
-    const unsigned short* caseOf;
-    const double* dtCase;
     double* minmax;     // [4]: min, max before bounding, min, max after
     int* errFlag;       // [1] device error bits
     unsigned long long* nIcTotal;  // [1] running sum of mixed-cell counts
 };
 
-// time step of cell c: its case's in a batch, else the step's
-__device__ inline double cell_dt(const DevState& st, int c, double dt) {
-    return st.caseOf ? st.dtCase[st.caseOf[c]] : dt;
-}
 
 struct Params {
     double mixedCellTol;
@@ -152,6 +144,38 @@ struct BndRec {
     double den;          // 1/dt (-Sp)
 };
 
+// One case of a batch of independent handles advanced together (one launch
+// per stage, blockIdx.y = case).  st holds the four views of the case's state
+// a step can need: bit 1 of the index swaps alpha and alphaOld (the double
+// buffer), bit 0 takes the other half of the candidate list.  View 0 is the
+// handle's state when the batch took it.
+struct BatchCase {
+    DevMesh m;
+    DevState st[4];
+    int detBlocks;        // blocks of the one-launch detection
+    int listBlocks;       // blocks of the one-launch candidate list
+    int listWords;        // candidate-bitmap words
+    int nB;               // boundary faces
+    // harness flow of the case (base fields scaled per step)
+    const double* U0;
+    const double* phi0;
+    const double* U0b;
+};
+
+// Per case and step: its time step (<= 0: the case is not advanced this
+// step) and the parities of its alpha buffers and candidate-list halves
+// relative to view 0.
+struct BatchStepInfo {
+    double dt;
+    int aPar;
+    int hPar;
+};
+
+// grid x dimension of each batched stage (the y dimension is the case count)
+struct BatchDims {
+    int ls, fsd, fluxList, fluxFaces, prep, flow, apply;
+};
+
 // launchers (simplic_kernels.hip)
 void launch_geometry(const DevMesh& m, hipStream_t s);
 void launch_detect(const DevMesh& m, const DevState& st, const Params& p, hipStream_t s);
@@ -159,7 +183,7 @@ void launch_detect(const DevMesh& m, const DevState& st, const Params& p, hipStr
 void launch_ls_normal(const DevMesh& m, const DevState& st, hipStream_t s, const int* wl = nullptr,
                       const int* wn = nullptr);
 // alphaGrad normal: minus the Gauss-linear gradient of alpha, normalised
-void launch_grad_normal(const DevMesh& m, const DevState& st, hipStream_t s);
+void launch_grad_normal(const DevMesh& m, const DevState& st, bool pointLinear, hipStream_t s);
 
 // isoRDF work buffers (reconstruction.C:194-405); zero between reconstructs
 enum { RDF_NNEXT = 0, RDF_DONE = 1, RDF_ITERS = 2, RDF_WN = 3, RDF_WN2 = 4, RDF_CTL = 8 };
@@ -213,8 +237,23 @@ void launch_map_alpha(const DevMesh& m, const DevState& st, const double* n, con
 void launch_plic_surface(const DevMesh& m, const DevState& st, int nMixed, int pass, int* ptOff, int* faceIdx,
                          int* totals, double* pts, int* faceCells, int* faceOff, hipStream_t s);
 void launch_plic_close(int* faceOff, const int* totals, hipStream_t s);
+// full-mesh interfaceN / interfaceD / interfaceC / interfaceS of the last
+// reconstruct (zero outside the mixed cells); null outputs are skipped
+void launch_plic_fields(const DevMesh& m, const DevState& st, double* N, double* D, double* C, double* S,
+                        hipStream_t s);
 void launch_rho_phi(int nF, const double* ap, const double* phi, double rho1, double rho2, double w,
                     double* out, int mode, hipStream_t s);
 int detect_blocks(int nC);
+int detect_list_blocks(int nC);   // blocks of the one-launch detection
+// batched stages (same order and kernels as one handle's reconstruct and
+// advect); G is the plane-placement lane group and MF the bounding record
+// size shared by all cases
+void launch_batch_stage(int stage, const BatchCase* bc, const BatchStepInfo* info, int nCases, int maxC, int maxF,
+                        int maxDetBlocks, int maxListBlocks, const BatchDims& d, const Params& p, int G, int MF,
+                        int triQuad, hipStream_t s);
+void launch_batch_courant(const BatchCase* bc, const BatchStepInfo* info, int nCases, int maxC,
+                          unsigned long long* out, hipStream_t s);
+void launch_batch_scale_flow(const BatchCase* bc, const BatchStepInfo* info, const double* factors, int nCases,
+                             long maxN, hipStream_t s);
 
 }  // namespace svp

@@ -296,6 +296,155 @@ def tet_box(N: int, perturb: float = 0.0, seed: int = 0, length: float = 1.0) ->
     )
 
 
+def agglomerated_box(N: int, group: np.ndarray, perturb: float = 0.0, seed: int = 0,
+                     name: str = "agglomerated") -> PolyMesh:
+    """Polyhedral mesh of the unit cube from an N^3 hex lattice whose cells are
+    merged into boxes: ``group`` (N^3, lattice cells i-fastest) names the cell
+    each lattice cell belongs to, and each group must be an axis-aligned box of
+    lattice cells.  The lattice faces between two cells (or between a cell
+    and the domain boundary) merge into one polygon, the loop of lattice
+    edges around them, so a face of a large cell next to smaller cells is
+    split as the small cells' faces are and keeps the edge midpoints they
+    share: the layout of a 2:1 hexRef8 refinement.  Cells take labels in the
+    order of their first lattice cell; internal faces are upper-triangular;
+    patches as ``hex_box``; unused lattice points (centres of merged faces,
+    cell interiors) are dropped.  ``perturb`` displaces interior lattice
+    points as in ``hex_box`` (faces become warped)."""
+    base = hex_box(N, perturb=perturb, seed=seed)
+    n1 = N + 1
+    group = np.asarray(group, dtype=np.int64).reshape(-1)
+    assert group.size == N ** 3
+    # cell labels by first lattice cell
+    _, first = np.unique(group, return_index=True)
+    order = np.argsort(first)
+    label_of = np.empty(group.max() + 1, dtype=np.int64)
+    label_of[np.unique(group)[order]] = np.arange(order.size)
+    cell = label_of[group]
+    nc = int(order.size)
+
+    def pidx(i, j, k):
+        return i + n1 * (j + n1 * k)
+
+    # lattice faces: (cell below/outside, cell above/outside, 4 points with
+    # the right-hand normal along +axis)
+    faces = []   # (owner cell, neighbour cell or -1 - patch, point loop) per lattice face
+    cid = lambda i, j, k: cell[i + N * (j + N * k)]
+    rng_ = range(N)
+    # the +axis quad of lattice face at (i,j,k) on the low side of cell (i,j,k) along the axis
+    def quad(axis, i, j, k):
+        if axis == 0:
+            return [pidx(i, j, k), pidx(i, j + 1, k), pidx(i, j + 1, k + 1), pidx(i, j, k + 1)]
+        if axis == 1:
+            return [pidx(i, j, k), pidx(i, j, k + 1), pidx(i + 1, j, k + 1), pidx(i + 1, j, k)]
+        return [pidx(i, j, k), pidx(i + 1, j, k), pidx(i + 1, j + 1, k), pidx(i, j + 1, k)]
+    patch_of = {(2, 1): 0, (1, 0): 1, (0, 0): 2, (1, 1): 3, (2, 0): 4, (0, 1): 5}   # (axis, high side)
+    merged = {}
+    for axis in range(3):
+        for k in range(N + (axis == 2)):
+            for j in range(N + (axis == 1)):
+                for i in range(N + (axis == 0)):
+                    lo = [i, j, k]
+                    lo[axis] -= 1
+                    a = cid(*lo) if lo[axis] >= 0 else None
+                    b = cid(i, j, k) if [i, j, k][axis] < N else None
+                    q = quad(axis, i, j, k)        # normal +axis: from a to b
+                    if a is not None and b is not None:
+                        if a == b:
+                            continue
+                        key = (min(a, b), max(a, b))
+                        loop = q if a < b else q[::-1]
+                    elif b is None:
+                        key = (a, -1 - patch_of[(axis, 1)])
+                        loop = q
+                    else:
+                        key = (b, -1 - patch_of[(axis, 0)])
+                        loop = q[::-1]
+                    merged.setdefault(key, []).append(loop)
+    # one polygon per key: the directed edges left after opposite pairs cancel
+    polys = {}
+    for key, loops in merged.items():
+        edges = {}
+        for L in loops:
+            for t in range(4):
+                u, v = L[t], L[(t + 1) % 4]
+                if (v, u) in edges:
+                    del edges[(v, u)]
+                else:
+                    edges[(u, v)] = True
+        nxt = {u: v for (u, v) in edges}
+        assert len(nxt) == len(edges), "merged faces must form one simple loop"
+        start = min(nxt)
+        poly = [start]
+        while nxt[poly[-1]] != start:
+            poly.append(nxt[poly[-1]])
+        assert len(poly) == len(edges), "merged faces must form one loop"
+        polys[key] = poly
+    internal = sorted(k for k in polys if k[1] >= 0)
+    bnd = sorted((k for k in polys if k[1] < 0), key=lambda k: (-1 - k[1], k[0]))
+    fl = [polys[k] for k in internal] + [polys[k] for k in bnd]
+    owner = np.array([k[0] for k in internal] + [k[0] for k in bnd], dtype=np.int64)
+    neighbour = np.array([k[1] for k in internal], dtype=np.int64)
+    used = np.unique(np.concatenate([np.asarray(f) for f in fl]))
+    relabel = np.full(base.points.shape[0], -1, dtype=np.int64)
+    relabel[used] = np.arange(used.size)
+    fpts = relabel[np.concatenate([np.asarray(f) for f in fl])]
+    offs = np.zeros(len(fl) + 1, dtype=np.int64)
+    np.cumsum([len(f) for f in fl], out=offs[1:])
+    names = ["top", "left", "back", "right", "bottom", "front"]
+    sizes = [sum(1 for k in bnd if -1 - k[1] == p) for p in range(6)]
+    starts = list(len(internal) + np.concatenate([[0], np.cumsum(sizes)[:-1]]))
+    m = PolyMesh(
+        points=np.ascontiguousarray(base.points[used]),
+        face_offsets=offs.astype(np.int32),
+        face_points=fpts.astype(np.int32),
+        owner=owner.astype(np.int32),
+        neighbour=neighbour.astype(np.int32),
+        n_cells=nc,
+        patch_names=names,
+        patch_start=np.array(starts, dtype=np.int32),
+        patch_size=np.array(sizes, dtype=np.int32),
+        patch_type=np.full(6, PATCH_WALL, dtype=np.int32),
+        name=name + (f"_p{perturb}_s{seed}" if perturb > 0 else ""),
+    )
+    m.check()
+    return m
+
+
+def refined_hex_box(N: int, lo=(0.25, 0.25, 0.25), hi=(0.75, 0.75, 0.75), perturb: float = 0.0,
+                    seed: int = 0) -> PolyMesh:
+    """N^3 hex mesh (N even in each refined direction's lattice) whose cells
+    with centres inside the box [lo, hi) are refined 2x2x2, as one hexRef8
+    level: cells next to the refined region keep the edge midpoints and have
+    their shared faces split in four (up to 24 faces and 20 points)."""
+    Nf = 2 * N
+    i, j, k = np.meshgrid(np.arange(Nf), np.arange(Nf), np.arange(Nf), indexing="ij")
+    i, j, k = (a.transpose(2, 1, 0).ravel() for a in (i, j, k))
+    ci, cj, ck = i // 2, j // 2, k // 2
+    cc = np.stack([(ci + 0.5) / N, (cj + 0.5) / N, (ck + 0.5) / N], axis=1)
+    refine = np.all((cc >= np.asarray(lo)) & (cc < np.asarray(hi)), axis=1)
+    coarse_id = ci + N * (cj + N * ck)
+    fine_id = N ** 3 + i + Nf * (j + Nf * k)
+    group = np.where(refine, fine_id, coarse_id)
+    return agglomerated_box(Nf, group, perturb=perturb, seed=seed, name=f"refined{N}")
+
+
+def brick_box(N: int, seed: int = 0, perturb: float = 0.0) -> PolyMesh:
+    """Polyhedral mesh of mixed boxes: the (2N)^3 lattice in 2x2x2 blocks,
+    each block (seeded) one cell, two 2x2x1 bricks along a random axis, or
+    eight lattice cells.  Cells have 6 to 16 faces."""
+    Nf = 2 * N
+    rng = np.random.default_rng(seed)
+    kind = rng.integers(0, 3, N ** 3)
+    axis = rng.integers(0, 3, N ** 3)
+    i, j, k = np.meshgrid(np.arange(Nf), np.arange(Nf), np.arange(Nf), indexing="ij")
+    i, j, k = (a.transpose(2, 1, 0).ravel() for a in (i, j, k))
+    b = (i // 2) + N * ((j // 2) + N * (k // 2))
+    half = np.choose(axis[b], [i % 2, j % 2, k % 2])
+    fine = i + Nf * (j + Nf * k)
+    group = np.where(kind[b] == 0, 8 * b, np.where(kind[b] == 1, 8 * b + 1 + half, 8 * N ** 3 + fine))
+    return agglomerated_box(Nf, group, perturb=perturb, seed=seed, name=f"bricks{N}_s{seed}")
+
+
 def union_mesh(meshes, name: str = "batch") -> tuple[PolyMesh, np.ndarray]:
     """Disjoint union of independent meshes as one polyMesh, for running many
     cases in the same launches.  Internal faces come case by case (each case's

@@ -8,6 +8,11 @@
 * ``write_series``: the ``<name>.vtp.series`` JSON index of such files over
   time that ``writeVtkSeries`` produces
   (applications/utilities/writeVtkSeries/writeVtkSeries.C:60-146).
+* ``write_plic_fields``: the fields the reference writes into each time
+  directory when ``writePlicFields`` is on (interfaceN, interfaceD,
+  interfaceC, interfaceS as volume fields and the face flatness zeta as a
+  surface field, src/SimPLIC/reconstruction/reconstruction.C:448-471,
+  520-584), as OpenFOAM ASCII field files.
 """
 from __future__ import annotations
 
@@ -104,3 +109,84 @@ def write_series(path, entries):
     out.append("  ]\n}\n")
     with open(path, "w") as f:
         f.write("".join(out))
+
+
+# name -> (OpenFOAM class, dimensions, value type) of the fields the
+# reference registers with writePlicFields (reconstruction.C:520-584)
+PLIC_FIELDS = {
+    "interfaceN": ("volVectorField", "[0 -1 0 0 0 0 0]", "vector"),
+    "interfaceD": ("volScalarField", "[0 1 0 0 0 0 0]", "scalar"),
+    "interfaceC": ("volVectorField", "[0 1 0 0 0 0 0]", "vector"),
+    "interfaceS": ("volVectorField", "[0 2 0 0 0 0 0]", "vector"),
+    "zeta": ("surfaceScalarField", "[0 0 0 0 0 0 0]", "scalar"),
+}
+
+
+def _foam_list(values, kind):
+    v = np.asarray(values, dtype=np.float64)
+    if kind == "vector":
+        v = v.reshape(-1, 3)
+        body = "\n".join("(%.17g %.17g %.17g)" % tuple(r) for r in v)
+        return f"nonuniform List<vector>\n{v.shape[0]}\n(\n{body}\n)\n"
+    body = "\n".join("%.17g" % x for x in v.reshape(-1))
+    return f"nonuniform List<scalar>\n{v.size}\n(\n{body}\n)\n"
+
+
+def write_foam_field(path, name, cls, dims, kind, internal, patches):
+    """One OpenFOAM ASCII field file; ``patches`` is a list of (patch name,
+    values or None), None giving a calculated patch of uniform zero."""
+    zero = "(0 0 0)" if kind == "vector" else "0"
+    lines = ["FoamFile", "{", "    version     2.0;", "    format      ascii;", f"    class       {cls};",
+             f'    location    "{os.path.basename(os.path.dirname(os.path.abspath(path)))}";',
+             f"    object      {name};", "}", "", f"dimensions      {dims};", "",
+             "internalField   " + _foam_list(internal, kind) + ";", "", "boundaryField", "{"]
+    for pname, vals in patches:
+        lines += [f"    {pname}", "    {", "        type            calculated;"]
+        if vals is None or np.asarray(vals).size == 0:
+            lines.append(f"        value           uniform {zero};")
+        else:
+            lines.append("        value           " + _foam_list(vals, kind) + ";")
+        lines.append("    }")
+    lines += ["}", ""]
+    with open(path, "w") as f:
+        f.write("\n".join(lines))
+
+
+def write_plic_fields(case_dir, time_name, mesh, fields):
+    """Write ``fields`` (``SolveVofEqu.plic_fields()``) into
+    ``<case_dir>/<time_name>/`` as the reference does with writePlicFields:
+    the four interface fields over the cells with calculated, zero boundary
+    patches, and zeta over the internal faces with its boundary face values
+    per patch.  Returns the paths written."""
+    d = os.path.join(case_dir, str(time_name))
+    os.makedirs(d, exist_ok=True)
+    out = []
+    nif = mesh.n_internal_faces
+    for name, (cls, dims, kind) in PLIC_FIELDS.items():
+        if name not in fields:
+            continue
+        v = np.asarray(fields[name], dtype=np.float64)
+        if name == "zeta":
+            internal = v[:nif]
+            patches = [(pn, v[int(st):int(st) + int(sz)])
+                       for pn, st, sz in zip(mesh.patch_names, mesh.patch_start, mesh.patch_size)]
+        else:
+            internal = v
+            patches = [(pn, None) for pn in mesh.patch_names]
+        path = os.path.join(d, name)
+        write_foam_field(path, name, cls, dims, kind, internal, patches)
+        out.append(path)
+    return out
+
+
+def read_foam_field(path):
+    """Internal field of an ASCII field file written by ``write_foam_field``
+    (n or n x 3 values)."""
+    text = open(path).read()
+    i = text.index("internalField")
+    head, rest = text[i:].split("(", 1)
+    n = int(head.split()[-1])
+    if "List<vector>" in head:
+        rows = rest.split(")", n)[:n]
+        return np.array([[float(t) for t in r.replace("(", " ").split()] for r in rows])
+    return np.array([float(t) for t in rest.split(")", 1)[0].split()[:n]])

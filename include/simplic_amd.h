@@ -68,6 +68,9 @@ typedef struct {
     double rdf_tol;                /* tol, isoRDF only: 1e-6 (reconstruction.C:513) */
     double rdf_rel_tol;            /* relTol, isoRDF only: 0.1 (reconstruction.C:514) */
     int32_t rdf_iterations;        /* iterations, isoRDF only: 5 (reconstruction.C:515) */
+    char grad_scheme[32];          /* alphaGrad only: the fvSchemes gradSchemes entry grad(alpha1) that
+                                      fvc::grad(alpha1_, "grad(alpha1)") reads (reconstruction.C:78):
+                                      "Gauss linear" (default) or "Gauss pointLinear" */
 } svp_dict;
 
 /* Defaults of every key (reconstruction.C:502-516, advection.C:455-457). */
@@ -77,8 +80,10 @@ void svp_dict_defaults(svp_dict* d);
  * C/C++ comments allowed).  Unknown keys are ignored, as OpenFOAM does.
  * `reconstructionScheme` is accepted as an alias of `orientationMethod`.
  * Returns SVP_E_DICT for an unknown orientation method
- * (reconstruction.C:610-625).  alphaGrad is the Gauss-linear gradient normal
- * (reconstruction.C:74-82); isoRDF / RDF the reconstructed-distance-function
+ * (reconstruction.C:610-625).  alphaGrad is the Gauss gradient normal
+ * (reconstruction.C:74-82) with the scheme of key "grad(alpha1)" (or
+ * "gradScheme"): Gauss linear or Gauss pointLinear, others SVP_E_UNSUPPORTED;
+ * isoRDF / RDF the reconstructed-distance-function
  * normal (reconstruction.C:194-405) with keys tol, relTol and iterations;
  * contact-angle boundaries (updateContactAngle) are not part of this mesh
  * description, so its boundary correction is not applied.
@@ -183,6 +188,19 @@ int svp_get_geometry(svp_handle* h, double* Cf, double* Sf, double* magSf, doubl
  * reconstructionTime/advectionTime, solveVofEqu.H:195-211) and the running sum
  * of mixed-cell counts over all reconstruct calls. */
 int svp_timers(svp_handle* h, double* reconstruction_s, double* advection_s, uint64_t* n_ic_total);
+/* solveVofEqu::alphaMappingTime (solveVofEqu.H:208-210 ->
+ * reconstruction.H:396-399): cumulative GPU seconds of svp_map_alpha_field. */
+int svp_alpha_mapping_time(svp_handle* h, double* seconds);
+/* The reference's full-mesh interface fields after the last reconstruct
+ * (reconstruction.C:520-570; zeroed every step by initialize,
+ * reconstruction.C:638-641, and filled in the mixed cells): interfaceN
+ * (3 n_cells), interfaceD (n_cells), interfaceC (3 n_cells), interfaceS
+ * (3 n_cells), and the face flatness zeta (n_faces, reconstruction.C:448-471).
+ * The reference writes them to disk when writePlicFields is on
+ * (svp_dict.write_plic_fields); here the caller writes them
+ * (geometricvofext_amd.plic_io.write_plic_fields).  Any output may be NULL. */
+int svp_get_plic_fields(svp_handle* h, double* interface_n, double* interface_d, double* interface_c,
+                        double* interface_s, double* zeta, int where);
 
 /* Device pointers of the resident fields, for zero-copy drivers. */
 typedef struct {
@@ -229,20 +247,43 @@ int svp_synchronize(svp_handle* h);
  * exercised without making a kernel fail. */
 int svp_test_set_error_bits(svp_handle* h, int32_t bits);
 
+/* Batch of independent cases advanced together (no counterpart in the
+ * reference, which runs one case per process: applications/test/
+ * plicVofAdvectionFoam/plicVof.H:13-57 is the loop each case follows).  The
+ * handles keep their own meshes and fields; svp_batch_step runs
+ * reconstruct + advect of every case with its own time step, and each stage
+ * of the step is one launch over all cases, each case running the kernels of
+ * a single handle on its own data, so every case's results equal those of its
+ * handle stepping alone.  The cases must share the alpha dictionary, use the
+ * least-squares normal and the same cell type (all hexahedra or all
+ * tetrahedra, at most 8 faces and 16 points, no splitWarpedFace); otherwise
+ * SVP_E_UNSUPPORTED / SVP_E_ARG.  While the batch exists its cases work on
+ * the batch's stream, so a handle may still be read (svp_get_alpha,
+ * svp_get_interface, ...) between steps; destroy the batch before using the
+ * handles on their own again.  A handle belongs to at most one batch. */
+typedef struct svp_batch svp_batch;
+int svp_batch_create(svp_handle* const* cases, int32_t n_cases, svp_batch** out);
+int svp_batch_destroy(svp_batch* b);
+/* One step of every case k with dt[k] > 0 (host array of n_cases); a case
+ * with dt[k] <= 0 is left as it is (a finished case). */
+int svp_batch_step(svp_batch* b, const double* dt);
+/* Launch shapes: the x extent per case of the stages whose kernels stride
+ * over their work (least-squares normal, plane placement, flux list,
+ * interface flux, bounding records, dataflow pass, apply), shapes[7] in that
+ * order.  svp_batch_tune times each over a few values on one step from the
+ * current fields (the fields are restored after every trial), keeps the
+ * fastest and writes them to shapes (may be NULL); results do not depend on
+ * the shapes.  svp_batch_set_shapes sets them directly. */
+int svp_batch_tune(svp_batch* b, const double* dt, int32_t* shapes);
+int svp_batch_set_shapes(svp_batch* b, const int32_t* shapes);
+/* GPU seconds of the batch's steps and the summed mixed-cell counts of the
+ * cases' reconstructs (their running totals, svp_timers' n_ic_total). */
+int svp_batch_timers(svp_batch* b, double* seconds, uint64_t* n_ic_total);
+/* Per-stage timing of batch steps (the stage numbering of svp_profile_set). */
+int svp_batch_profile_set(svp_batch* b, int mask);
+int svp_batch_profile_get(svp_batch* b, double* seconds, int64_t* counts, int reset);
+
 /* Message of the last failing call on this thread. */
-/* Batch of independent cases in one mesh (no reference counterpart: the
- * reference runs one case per process).  Cells [cell_offsets[k],
- * cell_offsets[k+1]) form case k; no internal face may join two cases
- * (SVP_E_MESH).  Build the mesh as the disjoint union of the cases with each
- * case's internal faces in its own order, then its boundary faces: every case
- * then gives bit-identical results to a separate handle.  Afterwards the dt
- * argument of svp_advect is ignored: each case advances by its entry of
- * svp_batch_set_dt, and a case with dt 0 is idle (not reconstructed, alpha
- * unchanged).  Cases share nAlphaBounds; a bounding pass only touches cases
- * with cells out of bounds, as each case on its own would. */
-int svp_batch_define(svp_handle* h, int32_t n_cases, const int32_t* cell_offsets);
-/* Per-case time steps (n_cases values, >= 0) for the next reconstruct/advect. */
-int svp_batch_set_dt(svp_handle* h, const double* dt, int where);
 
 const char* svp_last_error(void);
 

@@ -42,11 +42,11 @@ int svp_harness_init_alpha(svp_handle* h, const double* shapes, int n_shapes, in
  * current phi and alpha; multiply by 0.5*dt for the Courant numbers.
  * Synchronises with the handle's stream. */
 int svp_harness_courant(svp_handle* h, double* max_sum_phi_by_v, double* max_alpha_sum_phi_by_v);
+/* The same two maxima for every case of a batch (n_cases each), one launch. */
+int svp_batch_courant(svp_batch* b, double* max_sum_phi_by_v, double* max_alpha_sum_phi_by_v);
+/* Every case's uploaded base flow times its own factor (n_cases), one launch. */
+int svp_batch_scale_flow(svp_batch* b, const double* factors);
 
-/* Batch handles (svp_batch_define): each case's flow scaled by its own
- * factor (n_cases host values), and the two maxima per case (n_cases each). */
-int svp_harness_scale_flow_batch(svp_handle* h, const double* factors);
-int svp_harness_courant_batch(svp_handle* h, double* max_sum_phi_by_v, double* max_alpha_sum_phi_by_v);
 
 #ifdef __cplusplus
 }

@@ -60,6 +60,8 @@ def lib():
     L.or_solver_new.argtypes = [vp, C.c_double, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int]
     L.or_solver_free.argtypes = [vp]
     L.or_solver_set_orientation.argtypes = [vp, C.c_int]
+    L.or_solver_set_grad_scheme.argtypes = [vp, C.c_int]
+    L.or_grad_normals.argtypes = [vp, _dp, _dp]
     L.or_set_alpha.argtypes = [vp, _dp]
     L.or_set_flow.argtypes = [vp, _dp, _dp, C.c_void_p]
     L.or_reconstruct.argtypes = [vp]
@@ -200,12 +202,21 @@ class OracleSolver:
     ORIENTATIONS = {"LS": 0, "isoAlphaGrad": 0, "alphaGrad": 1, "isoRDF": 2, "RDF": 2}
 
     def __init__(self, omesh: OracleMesh, mixedCellTol=1e-8, nAlphaBounds=10, snapTol=0.0, clip=True,
-                 splitWarpedFace=False, threads=1, orientationMethod="LS", tol=1e-6, relTol=0.1, iterations=5):
+                 splitWarpedFace=False, threads=1, orientationMethod="LS", tol=1e-6, relTol=0.1, iterations=5,
+                 gradScheme="Gauss linear"):
         self.om = omesh
         self.h = lib().or_solver_new(omesh.h, mixedCellTol, nAlphaBounds, snapTol, int(clip),
                                      int(splitWarpedFace), threads)
         lib().or_solver_set_orientation(self.h, self.ORIENTATIONS[orientationMethod])
         lib().or_solver_set_rdf(self.h, float(tol), float(relTol), int(iterations))
+        lib().or_solver_set_grad_scheme(self.h, {"Gauss linear": 0, "Gauss pointLinear": 1}[" ".join(gradScheme.split())])
+
+    def grad_normals(self, alpha):
+        """The alphaGrad normal (normalised -grad(alpha) with the solver's
+        gradient scheme) of every cell for the given alpha."""
+        out = np.zeros(3 * self.om.mesh.n_cells)
+        lib().or_grad_normals(self.h, _c(alpha, np.float64), out)
+        return out.reshape(-1, 3)
 
     def map_alpha_field(self, n, d, lower, upper):
         """mapAlphaField (reconstruction.C:725-784) with per-cell n, d."""

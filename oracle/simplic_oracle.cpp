@@ -1048,14 +1048,69 @@ struct Solver {
     // /= V) over linear interpolation with surfaceInterpolation::makeWeights
     // weights; OpenFOAM is a dependency of the reference, not part of it.
     // alpha is zeroGradient on the boundary.
+    //
+    // gradScheme 1 is Gauss pointLinear (fvSchemes grad(alpha1), selected by
+    // tutorials/test/plicVofOrientationFoam/NAG/system/fvSchemes:35): the
+    // linear face value plus OpenFOAM v2312's pointLinear correction.  The
+    // face is fanned into triangles (pi, p_k, p_k-1), closing with (pi, p_0,
+    // p_last), around pi = w C_own + (1 - w) C_nei, where pointLinear takes w
+    // from the face weights at the index of the owner cell; each triangle adds
+    // its area times the mean of the linear face value and its two
+    // volPointInterpolation point values; the correction is that area average
+    // minus the linear value, and the face value is linear + correction.
+    // Boundary faces take no correction (non-coupled patches).
+    int gradScheme = 0;
+
+    double faceWeight(int f) const {
+        const int o = m->own[f], n = m->nei[f];
+        const double sfdOwn = std::fabs(dot(m->Sf[f], m->Cf[f] - m->C[o]));
+        const double sfdNei = std::fabs(dot(m->Sf[f], m->C[n] - m->Cf[f]));
+        return sfdNei / (sfdOwn + sfdNei);
+    }
+
+    // volPointInterpolation of alpha: internal points over their cells,
+    // boundary points over their boundary faces, whose zeroGradient value is
+    // the owner's alpha
+    double pointAlpha(int p) const {
+        double t = 0.0;
+        if (!m->isPatchPoint[p]) {
+            const auto& pc = m->pointCells[p];
+            for (size_t i = 0; i < pc.size(); ++i) t += m->pointW[p][i] * alpha[pc[i]];
+        } else {
+            const auto& pf = m->bPointFaces[p];
+            for (size_t i = 0; i < pf.size(); ++i) t += m->bPointW[p][i] * alpha[m->own[pf[i]]];
+        }
+        return t;
+    }
+
+    double pointLinearFaceValue(int f, double lin) const {
+        const int o = m->own[f], n = m->nei[f];
+        const double wq = g_mutation == 4 ? faceWeight(f) : faceWeight(o);
+        const V3 pi = wq * m->C[o] + (1.0 - wq) * m->C[n];
+        const int b = m->fOff[f], np = m->fOff[f + 1] - b;
+        auto triMag = [&](V3 a, V3 bb, V3 c) { return mag(0.5 * cross(bb - a, c - a)); };
+        const int p0 = m->fPts[b], pl = m->fPts[b + np - 1];
+        // planted mistake 5: the closing triangle (pi, p_0, p_last) left out
+        double at = g_mutation == 5 ? 0.0 : triMag(pi, m->points[p0], m->points[pl]);
+        double sumAt = at;
+        double sumPsip = at * (1.0 / 3.0) * (lin + pointAlpha(p0) + pointAlpha(pl));
+        for (int k = 1; k < np; ++k) {
+            const int pk = m->fPts[b + k], pk1 = m->fPts[b + k - 1];
+            at = triMag(pi, m->points[pk], m->points[pk1]);
+            sumAt += at;
+            sumPsip += at * (1.0 / 3.0) * (lin + pointAlpha(pk) + pointAlpha(pk1));
+        }
+        const double corr = sumPsip / sumAt - lin;
+        return lin + corr;
+    }
+
     void calcInterfaceNFromRegAlphaGrad() {
         std::vector<V3> g(m->nC, VZERO);
         for (int f = 0; f < m->nIF; ++f) {
             const int o = m->own[f], n = m->nei[f];
-            const double sfdOwn = std::fabs(dot(m->Sf[f], m->Cf[f] - m->C[o]));
-            const double sfdNei = std::fabs(dot(m->Sf[f], m->C[n] - m->Cf[f]));
-            const double w = sfdNei / (sfdOwn + sfdNei);
-            const double ssf = w * (alpha[o] - alpha[n]) + alpha[n];
+            const double w = faceWeight(f);
+            double ssf = w * (alpha[o] - alpha[n]) + alpha[n];
+            if (gradScheme == 1) ssf = pointLinearFaceValue(f, ssf);
             const V3 v = ssf * m->Sf[f];
             g[o] += v;
             g[n] -= v;
@@ -1708,6 +1763,20 @@ void or_solver_free(void* h) { delete (Solver*)h; }
 
 // 0: isoAlphaGrad / LS, 1: alphaGrad, 2: isoRDF / RDF
 void or_solver_set_orientation(void* h, int method) { ((Solver*)h)->orientation = method; }
+// gradient scheme of the alphaGrad normal: 0 Gauss linear, 1 Gauss pointLinear
+void or_solver_set_grad_scheme(void* h, int scheme) { ((Solver*)h)->gradScheme = scheme; }
+// the alphaGrad normal of every cell from the given alpha (tests)
+void or_grad_normals(void* h, const double* a, double* out) {
+    Solver* s = (Solver*)h;
+    s->alpha.assign(a, a + s->m->nC);
+    s->interfaceN.assign(s->m->nC, VZERO);
+    s->calcInterfaceNFromRegAlphaGrad();
+    for (int c = 0; c < s->m->nC; ++c) {
+        out[3 * c] = s->interfaceN[c].x;
+        out[3 * c + 1] = s->interfaceN[c].y;
+        out[3 * c + 2] = s->interfaceN[c].z;
+    }
+}
 // isoRDF keys tol, relTol, iterations (reconstruction.C:513-515)
 void or_solver_set_rdf(void* h, double tol, double relTol, int iterations) {
     Solver* s = (Solver*)h;

@@ -269,3 +269,52 @@ def test_learned_reconstruction_against_oracle(g, trained, oracle_lib):
         # least-squares normal (no full stencil, or fallback)
         assert np.array_equal(n, pitf["n"][k])
     assert n_kept == kept
+
+
+def test_learned_ellipsoid_deformation_to_end_time(g, trained, oracle_lib):
+    """The learned-normal workload (3 seeded ellipsoids, deformation to
+    t = 3) at 64^3: plain and learned paths to the end time.  Every 100 steps
+    the learned path's plane placement is checked against the oracle's
+    findSignedDistance on the GPU's normals (status and d bit-exact); at
+    t = T the learned shape error is at most the plain one and volume is
+    conserved."""
+    import bench
+    from geometricvofext_amd import cases, harness
+    m, _, _ = trained
+    mesh = g.hex_box(64)
+    shapes = cases.random_ellipsoids(bench.LEARNED_WORKLOAD["shapes_seed"])
+    om = oracle_lib.OracleMesh(mesh)
+    res = {}
+    seq = None
+    for method in ("LS", "learned"):
+        s = g.SolveVofEqu(mesh, dict(bench.ALPHA_DICT, orientationMethod=method))
+        if method == "learned":
+            s.set_normal_model(m)
+        s.init_alpha(shapes, max_depth=5)
+        a0 = s.alpha()
+        U0, phi0 = s.deformation_field()
+        s.upload_base_flow(U0, phi0)
+        if seq is None:
+            seq = bench.time_sequence(s, a0, 3.0)
+        s.set_alpha(a0)
+        checked = 0
+        for k, (_, dt, uf) in enumerate(seq):
+            s.scale_flow(uf)
+            s.reconstruct()
+            if method == "learned" and k % 100 == 0:
+                alpha = s.alpha()
+                itf = s.interface()
+                for j, c in enumerate(itf["cells"]):
+                    st, out = om.find_signed_distance(int(c), alpha[c], itf["n"][j])
+                    assert st == itf["status"][j], (k, int(c))
+                    assert out[0] == itf["d"][j] or (st != 0 and itf["d"][j] == 0.0), (k, int(c))
+                checked += len(itf["cells"])
+            s.advect(dt)
+        V = s.geometry()["V"]
+        a1 = s.alpha()
+        res[method] = (harness.shape_error(a1, a0, V), harness.volume_error(a1, V, float(np.sum(a0 * V))))
+        print(f"{method}: {len(seq)} steps, E_s {res[method][0]:.4f}, E_v {res[method][1]:.1e}"
+              + (f", {checked} plane placements checked against the oracle" if checked else ""), flush=True)
+        s.close()
+    assert abs(res["LS"][1]) < 1e-12 and abs(res["learned"][1]) < 1e-12
+    assert res["learned"][0] <= res["LS"][0]

@@ -32,7 +32,8 @@ def _pair(mesh, oracle_lib, **dict_kw):
     kw = dict(mixedCellTol=dict_kw.get("mixedCellTol", 1e-8), nAlphaBounds=dict_kw.get("nAlphaBounds", 10),
               snapTol=dict_kw.get("snapTol", 0.0), clip=dict_kw.get("clip", True),
               splitWarpedFace=dict_kw.get("splitWarpedFace", False),
-              orientationMethod=dict_kw.get("orientationMethod", "LS"))
+              orientationMethod=dict_kw.get("orientationMethod", "LS"),
+              gradScheme=dict_kw.get("gradScheme", "Gauss linear"))
     ors = oracle_lib.OracleSolver(om, **kw)
     return gpu, om, ors
 
@@ -258,6 +259,57 @@ def test_parity_alphagrad_tets(oracle_lib):
     assert exact
 
 
+@pytest.mark.parametrize("mesh_kind", ["hex", "tet"])
+def test_parity_alphagrad_point_linear(oracle_lib, mesh_kind):
+    """alphaGrad with grad(alpha1) Gauss pointLinear (the scheme of the
+    reference's NAG case, tutorials/test/plicVofOrientationFoam/NAG/system/
+    fvSchemes:35) on perturbed hex and tets, through the dictionary parser."""
+    import geometricvofext_amd as g
+    mesh = g.hex_box(20, perturb=0.2, seed=31) if mesh_kind == "hex" else g.tet_box(8, perturb=0.1, seed=32)
+    om = oracle_lib.OracleMesh(mesh)
+    a0 = ramp_alpha(om.C, [((0.45, 0.5, 0.55), 0.25)], 1.5 / 16)
+    d = g.parse_dict("orientationMethod alphaGrad; grad(alpha1) Gauss  pointLinear; nAlphaBounds 3;")
+    assert d.grad_scheme == b"Gauss pointLinear"
+    _say("parity", mesh.name, "alphaGrad Gauss pointLinear")
+    exact, _, _ = _run_parity(mesh, oracle_lib, a0, 5, nAlphaBounds=3, orientationMethod="alphaGrad",
+                              gradScheme="Gauss pointLinear")
+    assert exact
+
+
+@pytest.mark.parametrize("kind", ["refined", "bricks"])
+def test_parity_general_polyhedra(oracle_lib, kind):
+    """Meshes beyond the 8-face / 16-point fast path: a 2:1-refined hex mesh
+    (hexRef8 layout: coarse cells with the edge midpoints and split faces,
+    up to 20 points) and a mixed-brick polyhedral mesh (up to 16 faces), both
+    with displaced interior points (warped faces), over deformation steps
+    with heavy bounding; then mapAlphaField and the PLIC surface on them."""
+    import geometricvofext_amd as g
+    mesh = (g.refined_hex_box(10, perturb=0.15, seed=41) if kind == "refined"
+            else g.brick_box(8, seed=42, perturb=0.15))
+    om = oracle_lib.OracleMesh(mesh)
+    counts = np.bincount(mesh.owner, minlength=mesh.n_cells) + np.bincount(mesh.neighbour, minlength=mesh.n_cells)
+    _say("parity", mesh.name, mesh.n_cells, "cells, faces per cell up to", int(counts.max()))
+    a0 = ramp_alpha(om.C, [((0.47, 0.5, 0.52), 0.27)], 1.5 / 16)
+    exact, gpu, ors = _run_parity(mesh, oracle_lib, a0, 6, nAlphaBounds=3, clip=False)
+    assert exact
+    # mapAlphaField with a plane per cell
+    rng = np.random.default_rng(43)
+    n = rng.normal(size=(mesh.n_cells, 3))
+    n /= np.linalg.norm(n, axis=1)[:, None]
+    d = -np.sum(n * om.C, axis=1) + rng.uniform(-0.5, 0.5, mesh.n_cells) / 16
+    a = gpu.alpha()
+    ors.set_alpha(a)
+    gpu.map_alpha_field(n, d, 0.01, 0.99)
+    ors.map_alpha_field(n, d, 0.01, 0.99)
+    assert max_abs(gpu.alpha(), ors.alpha()) <= TOL
+    # PLIC polygons of a reconstruct
+    gpu.set_alpha(a)
+    ors.set_alpha(a)
+    gpu.reconstruct()
+    ors.reconstruct()
+    _compare_plic(gpu, ors, mesh.name)
+
+
 def test_parity_sources(oracle_lib):
     """Non-zero implicit and explicit sources (advect(Sp, Su), as a
     phase-change solver passes them: alphaSuSp.H) in the update and in the
@@ -389,6 +441,39 @@ def test_parity_map_alpha_field(oracle_lib):
         assert np.array_equal(ga[keep], a0[keep])
         assert max_abs(ga, oa) <= TOL, f"{mesh.name}: mapped alpha off by {max_abs(ga, oa)}"
         _say(f"{mesh.name}: {int((~keep).sum())} cells mapped, bit-exact={bits_equal(ga, oa)}")
+
+
+def test_parity_plic_fields_and_mapping_time(oracle_lib, tmp_path):
+    """svp_get_plic_fields: the full-mesh interfaceN/D/C/S of the reference
+    (zero outside the mixed cells, the oracle's values in them) and zeta = the
+    face flatness, written as OpenFOAM fields and read back; then
+    alphaMappingTime grows with svp_map_alpha_field."""
+    import geometricvofext_amd as g
+    from geometricvofext_amd import plic_io
+    mesh = g.hex_box(16, perturb=0.2, seed=23)
+    om = oracle_lib.OracleMesh(mesh)
+    a0 = ramp_alpha(om.C, [((0.5, 0.45, 0.5), 0.27)], 1.5 / 16)
+    _say("plic fields", mesh.name)
+    _, gpu, ors = _run_parity(mesh, oracle_lib, a0, 2, nAlphaBounds=3, writePlicFields=True)
+    gpu.reconstruct()
+    ors.reconstruct()
+    f = gpu.plic_fields()
+    oi = ors.mixed()
+    cells = oi["cells"]
+    rest = np.setdiff1d(np.arange(mesh.n_cells), cells)
+    for name, key in (("interfaceN", "n"), ("interfaceD", "d"), ("interfaceC", "C"), ("interfaceS", "S")):
+        assert max_abs(f[name][cells], oi[key]) <= TOL, name
+        assert not np.any(f[name][rest]), name
+    assert bits_equal(f["zeta"], om.flatness)
+    written = plic_io.write_plic_fields(str(tmp_path), "0.25", mesh, f)
+    assert len(written) == 5
+    back = plic_io.read_foam_field(str(tmp_path / "0.25" / "interfaceN"))
+    assert bits_equal(back, f["interfaceN"])
+    assert bits_equal(plic_io.read_foam_field(str(tmp_path / "0.25" / "zeta")), f["zeta"][:mesh.n_internal_faces])
+    assert gpu.alpha_mapping_time() == 0.0
+    n = np.tile([0.0, 0.0, 1.0], (mesh.n_cells, 1))
+    gpu.map_alpha_field(n, -np.full(mesh.n_cells, 0.5), 0.01, 0.99)
+    assert gpu.alpha_mapping_time() > 0.0
 
 
 @pytest.mark.parametrize("method", ["LS", "alphaGrad", "RDF"])

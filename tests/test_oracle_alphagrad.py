@@ -80,3 +80,93 @@ def test_perturbed_mesh_matches_face_sum():
     grad = gsum / V[:, None]
     expect = -grad / (np.linalg.norm(grad, axis=1)[:, None] + 1e-15)
     assert np.max(np.abs(n - expect[cells])) < 1e-10
+
+
+# --- Gauss pointLinear (fvSchemes grad(alpha1) of the reference's NAG case) ----
+
+def _uniform_point_linear_normals(N, alpha):
+    """Closed form of the pointLinear alphaGrad normal on a uniform N^3 mesh for
+    cells whose faces have interior points only: each square face splits into
+    four equal triangles about its centre, so its value is
+    lin / 3 + (2 / 3) mean(point values), every interior point value is the
+    mean of its 8 cells (equal inverse distances) and lin is the mean of the
+    two cells.  Returns (cell ids, normals) in natural i + N j + N^2 k order."""
+    a = alpha.reshape(N, N, N)                    # [k, j, i]
+    P = np.full((N + 1, N + 1, N + 1), np.nan)    # interior point values
+    P[1:N, 1:N, 1:N] = 0.125 * sum(a[dk:N - 1 + dk, dj:N - 1 + dj, di:N - 1 + di]
+                                   for dk in (0, 1) for dj in (0, 1) for di in (0, 1))
+    h = 1.0 / N
+    out, ids = [], []
+    for k in range(2, N - 2):
+        for j in range(2, N - 2):
+            for i in range(2, N - 2):
+                g = np.zeros(3)
+                for axis in range(3):
+                    for side in (0, 1):
+                        ijk = [i, j, k]
+                        nb = list(ijk)
+                        nb[axis] += 1 if side else -1
+                        lin = 0.5 * (a[ijk[2], ijk[1], ijk[0]] + a[nb[2], nb[1], nb[0]])
+                        # the face's 4 points: index along axis = cell index + side
+                        pts = []
+                        for u in (0, 1):
+                            for v in (0, 1):
+                                q = list(ijk)
+                                q[axis] += side
+                                o1, o2 = [x for x in range(3) if x != axis]
+                                q[o1] += u
+                                q[o2] += v
+                                pts.append(P[q[2], q[1], q[0]])
+                        val = lin / 3.0 + (2.0 / 3.0) * np.mean(pts)
+                        g[axis] += (1.0 if side else -1.0) * val * h * h
+                g /= h ** 3
+                n = -g
+                out.append(n / (np.linalg.norm(n) + 1e-15))
+                ids.append(i + N * j + N * N * k)
+    return np.array(ids), np.array(out)
+
+
+def _oracle_grad_normals(mesh, alpha, scheme):
+    om = O.OracleMesh(mesh)
+    s = O.OracleSolver(om, orientationMethod="alphaGrad", gradScheme=scheme)
+    return s.grad_normals(alpha)
+
+
+def test_point_linear_exact_for_linear_fields():
+    import geometricvofext_amd as g
+    N = 10
+    mesh = g.hex_box(N)
+    om = O.OracleMesh(mesh)
+    grad = np.array([0.3, -0.2, 0.1])
+    n = _oracle_grad_normals(mesh, 0.5 + (om.C - 0.5) @ grad, "Gauss pointLinear")
+    inner = np.all((om.C > 2.5 / N) & (om.C < 1.0 - 2.5 / N), axis=1)
+    assert np.max(np.abs(n[inner] + grad / np.linalg.norm(grad))) < 1e-12
+
+
+def test_point_linear_closed_form_uniform_mesh():
+    """On a uniform mesh with a random field the normal equals the closed form
+    above, and differs from Gauss linear (the correction is not zero)."""
+    import geometricvofext_amd as g
+    N = 8
+    mesh = g.hex_box(N)
+    alpha = np.random.default_rng(4).uniform(0.0, 1.0, mesh.n_cells)
+    ids, ref = _uniform_point_linear_normals(N, alpha)
+    got = _oracle_grad_normals(mesh, alpha, "Gauss pointLinear")[ids]
+    assert np.max(np.abs(got - ref)) < 1e-12
+    lin = _oracle_grad_normals(mesh, alpha, "Gauss linear")[ids]
+    assert np.max(np.abs(lin - ref)) > 1e-2
+
+
+def test_point_linear_pin_rejects_open_fan():
+    """A fan without its closing triangle (pi, p_0, p_last) breaks the closed
+    form.  (Which face weight pi takes, the owner cell's index or the
+    face's, cannot show on a uniform mesh, where all weights are 1/2: that
+    detail of OpenFOAM's pointLinear is unpinned here.)"""
+    import geometricvofext_amd as g
+    N = 8
+    mesh = g.hex_box(N)
+    alpha = np.random.default_rng(4).uniform(0.0, 1.0, mesh.n_cells)
+    ids, ref = _uniform_point_linear_normals(N, alpha)
+    with O.mutation(5):
+        got = _oracle_grad_normals(mesh, alpha, "Gauss pointLinear")[ids]
+    assert np.max(np.abs(got - ref)) > 1e-3

@@ -210,3 +210,72 @@ def test_interface_area_orientation_quirk(oracle_lib):
         ifc, ifs = out[5:8], out[8:11]
         assert ifs @ ifc >= 0.0
         assert abs(abs(ifs @ n) - np.linalg.norm(ifs)) < 1e-14
+
+
+def _cell_points(mesh, cell):
+    faces = [f for f in range(mesh.n_faces) if mesh.owner[f] == cell or
+             (f < mesh.n_internal_faces and mesh.neighbour[f] == cell)]
+    return faces, np.unique(np.concatenate([mesh.face(f) for f in faces]))
+
+
+def test_sub_cell_closed_form_on_refined_and_brick_cells(oracle_lib):
+    """Cells of a 2:1-refined hex mesh (a coarse cube whose faces carry the
+    edge midpoints, up to 9 faces and 20 points) and of a brick mesh (boxes
+    2h x 2h x h with split sides, up to 16 faces) cut by random planes: the
+    volume fraction is the Scardovelli-Zaleski closed form of the box, so the
+    collinear midpoints and split faces change nothing."""
+    rng = np.random.default_rng(7)
+    for mesh in (M.refined_hex_box(6), M.brick_box(4, seed=3)):
+        om = oracle_lib.OracleMesh(mesh)
+        counts = np.bincount(mesh.owner, minlength=mesh.n_cells) + \
+            np.bincount(mesh.neighbour, minlength=mesh.n_cells)
+        big = np.nonzero(counts > 6)[0]
+        assert big.size > 10
+        cut = 0
+        for cell in rng.choice(big, 12, replace=False):
+            _, pts = _cell_points(mesh, int(cell))
+            P = mesh.points[pts]
+            lo, hi = P.min(axis=0), P.max(axis=0)
+            ext = hi - lo
+            for _ in range(20):
+                n = rng.normal(size=3)
+                n /= np.linalg.norm(n)
+                if np.min(np.abs(n)) < 1e-3:
+                    continue
+                # plane n.(x - lo) + dloc = 0 through the box; scaled to the unit cube
+                lo_s, hi_s = float(np.minimum(n, 0) @ ext), float(np.maximum(n, 0) @ ext)
+                dloc = -(lo_s + rng.uniform(0.05, 0.95) * (hi_s - lo_s))
+                d = dloc - n @ lo
+                st, out = om.calc_sub_cell(int(cell), n, d)
+                m = n * ext
+                beta = -dloc - sum(mi for mi in m if mi < 0)
+                exact = cube_below(np.abs(m), beta)
+                assert abs(out[0] - exact) < 1e-12, (mesh.name, int(cell), out[0], exact)
+                cut += 1e-6 < exact < 1 - 1e-6
+        assert cut > 150
+
+
+def test_find_signed_distance_round_trip_refined_cells(oracle_lib):
+    """The round trip d* -> alpha* -> d on the coarse cells of a 2:1-refined
+    mesh: planar faces, so one cubic per vertex-level bracket, exact."""
+    mesh = M.refined_hex_box(6)
+    om = oracle_lib.OracleMesh(mesh)
+    counts = np.bincount(mesh.owner, minlength=mesh.n_cells) + np.bincount(mesh.neighbour, minlength=mesh.n_cells)
+    big = np.nonzero(counts > 6)[0]
+    rng = np.random.default_rng(9)
+    h = 1.0 / 6
+    n_checked = 0
+    for _ in range(150):
+        cell = int(rng.choice(big))
+        n = rng.normal(size=3)
+        n /= np.linalg.norm(n)
+        dstar = -(n @ om.C[cell]) + rng.uniform(-0.4, 0.4) * h
+        _, out = om.calc_sub_cell(cell, n, dstar)
+        a = out[0]
+        if not (1e-6 < a < 1 - 1e-6):
+            continue
+        st, res = om.find_signed_distance(cell, a, n)
+        assert st == 0
+        assert abs(res[0] - dstar) < 1e-12 * h + 1e-14
+        n_checked += 1
+    assert n_checked > 50

@@ -118,3 +118,33 @@ def test_vtp_round_trip(tmp_path):
     assert [e["time"] for e in doc["files"]] == [0.5, 1]
     with pytest.raises(ValueError):
         plic_io.write_vtp(path, ps["points"], ps["offsets"][:-1], None)
+
+
+def test_plic_fields_writer_round_trip(tmp_path):
+    """write_plic_fields writes the five writePlicFields files with the
+    reference's classes and dimensions (reconstruction.C:520-584); the
+    internal fields read back bit for bit and zeta carries its boundary face
+    values per patch."""
+    from geometricvofext_amd import mesh as M
+    from geometricvofext_amd import plic_io
+    mesh = M.hex_box(3)
+    rng = np.random.default_rng(2)
+    nc, nf = mesh.n_cells, mesh.n_faces
+    fields = dict(interfaceN=rng.normal(size=(nc, 3)), interfaceD=rng.normal(size=nc),
+                  interfaceC=rng.normal(size=(nc, 3)), interfaceS=rng.normal(size=(nc, 3)),
+                  zeta=rng.uniform(0.9, 1.0, nf))
+    paths = plic_io.write_plic_fields(str(tmp_path), "0.5", mesh, fields)
+    assert sorted(os.path.basename(p) for p in paths) == sorted(plic_io.PLIC_FIELDS)
+    for name, (cls, dims, kind) in plic_io.PLIC_FIELDS.items():
+        text = open(tmp_path / "0.5" / name).read()
+        assert f"class       {cls};" in text and f"dimensions      {dims};" in text
+        assert text.count("type            calculated;") == len(mesh.patch_names)
+        back = plic_io.read_foam_field(str(tmp_path / "0.5" / name))
+        ref = fields[name][:mesh.n_internal_faces] if name == "zeta" else fields[name]
+        assert np.array_equal(back, ref), name
+    z = open(tmp_path / "0.5" / "zeta").read()
+    first = mesh.patch_names[0]
+    s0, n0 = int(mesh.patch_start[0]), int(mesh.patch_size[0])
+    block = z[z.index(f"    {first}\n"):]
+    assert f"nonuniform List<scalar>\n{n0}\n(" in block
+    assert "%.17g" % fields["zeta"][s0] in block
