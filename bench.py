@@ -281,6 +281,108 @@ def learned_comparison(say):
     return out
 
 
+BATCH_WORKLOAD = dict(n_cases=256, seed=2025, nmin=16, nmax=48, tmin=0.5, tmax=1.0,
+                      desc="256 independent cases: uniform hex N^3, N ~ U{16..48} (seeded), 1-4 ellipsoids "
+                           "(semi-axes 0.05-0.2, random orientation), rotation or deformation to a seeded "
+                           "T in [0.5, 1], per-case Courant U(0.3, 0.5)")
+
+
+def batched_cases_run(say, oracle_cases=4):
+    """The batched workload run to every case's end time as one batch (one
+    launch per stage for all cases, launch shapes from the tuner): aggregate
+    interface-cell updates/s over the batch steps' GPU time, the same cases
+    stepped one handle at a time over the first steps for comparison, per-case
+    shape error at the end time, and the largest per-cell difference to the
+    oracle of a few cases run in lockstep."""
+    import geometricvofext_amd as g
+    from geometricvofext_amd import cases
+    from oracle import oracle as O
+    wl = BATCH_WORKLOAD
+    t0 = time.time()
+    specs = cases.batch_cases(wl["n_cases"], seed=wl["seed"], nmin=wl["nmin"], nmax=wl["nmax"], tmin=wl["tmin"],
+                              tmax=wl["tmax"])
+    from concurrent.futures import ThreadPoolExecutor
+    from geometricvofext_amd import harness
+    with ThreadPoolExecutor(8) as pool:
+        made = list(pool.map(lambda sp: cases.make_batch_case(sp, ALPHA_DICT), specs))
+    n_cells = int(sum(sp["N"] ** 3 for sp in specs))
+    say(f"batch: {len(specs)} cases, {n_cells} cells, set up in {time.time() - t0:.1f}s")
+    # the same cases one handle at a time over their first steps, then back
+    # to their initial state
+    seq_steps = 20
+    t_alone, nic_alone = 0.0, 0
+    for c, sp in zip(made, specs):
+        s = c["solver"]
+        r0, a0_, n0 = s.timers()
+        rt = harness.RunTime(end_time=sp["end_time"], max_co=sp["co"], max_alpha_co=sp["co"])
+        for _ in range(seq_steps):
+            co, aco = s.courant()
+            rt.set_delta_t(0.5 * co * rt.delta_t, 0.5 * aco * rt.delta_t)
+            rt.increment()
+            s.scale_flow(harness.u_factor(rt.value, rt.delta_t, c["period"]))
+            s.reconstruct()
+            s.advect(rt.delta_t)
+        r1, a1_, n1 = s.timers()
+        t_alone += (r1 - r0) + (a1_ - a0_)
+        nic_alone += n1 - n0
+        s.set_alpha(c["alpha0"])
+        s.scale_flow(0.0)
+    say(f"batch: the cases one handle at a time, {seq_steps} steps each: {nic_alone / t_alone / 1e6:.1f}M updates/s")
+    # oracle cases in lockstep: the smallest few
+    small = sorted(range(len(specs)), key=lambda k: specs[k]["N"])[:oracle_cases]
+    ors = {}
+    for k in small:
+        om = O.OracleMesh(made[k]["mesh"])
+        o = O.OracleSolver(om, nAlphaBounds=ALPHA_DICT["nAlphaBounds"], snapTol=ALPHA_DICT["snapTol"],
+                           clip=ALPHA_DICT["clip"], threads=16)
+        o.set_alpha(made[k]["alpha0"])
+        ors[k] = o
+    worst = {k: 0.0 for k in small}
+    b = g.CaseBatch([c["solver"] for c in made])
+    shapes = None
+    steps = 0
+    sec0 = nic0 = None
+    while any(c["rt"].run() for c in made):
+        co, aco = b.courant()
+        dt, fac = cases.batch_time_step(made, co, aco)
+        b.scale_flow(fac)
+        if shapes is None:
+            shapes = b.tune(dt)
+            sec0, nic0 = b.timers()
+            say(f"batch: tuned launch shapes {shapes}")
+        b.step(dt)
+        steps += 1
+        for k, o in ors.items():
+            if dt[k] > 0.0:
+                o.set_flow(made[k]["phi0"] * fac[k], made[k]["U0"] * fac[k])
+                o.reconstruct()
+                o.advect(dt[k])
+                worst[k] = max(worst[k], float(np.max(np.abs(made[k]["solver"].alpha() - o.alpha()))))
+        if steps % 250 == 0:
+            say(f"batch: step {steps}, {sum(c['rt'].run() for c in made)} cases running")
+    sec1, nic1 = b.timers()
+    hot = sec1 - sec0
+    b.close()
+    es = []
+    for c in made:
+        s = c["solver"]
+        V = s.geometry()["V"]
+        es.append(harness.shape_error(s.alpha(), c["alpha0"], V))
+    out = {"workload": wl["desc"], "cells": n_cells, "batch_steps": steps,
+           "updates_per_s": (nic1 - nic0) / hot, "ms_per_batch_step": 1e3 * hot / steps,
+           "n_ic_per_batch_step": (nic1 - nic0) / steps, "launch_shapes": shapes,
+           "one_handle_at_a_time": {"updates_per_s": nic_alone / t_alone, "steps_per_case": seq_steps},
+           "E_s_at_T": {"median": float(np.median(es)), "max": float(np.max(es))},
+           "oracle_lockstep": {"cases": [int(specs[k]["N"]) for k in small],
+                               "max_abs_alpha_diff": max(worst.values())}}
+    out["speedup_over_one_at_a_time"] = out["updates_per_s"] / out["one_handle_at_a_time"]["updates_per_s"]
+    say(f"batch: {steps} steps, {out['updates_per_s'] / 1e6:.1f}M updates/s, oracle lockstep max |diff| "
+        f"{out['oracle_lockstep']['max_abs_alpha_diff']:.1e}")
+    for c in made:
+        c["solver"].close()
+    return out
+
+
 def equal_error_comparison(out, shapes, wl, say):
     """The plain path on finer meshes of the same case: the time it needs to
     reach the learned path's L1 shape error at t=T (hot-path seconds of the
@@ -457,6 +559,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--stage", default="auto", help="stage timed for the roofline (auto = slowest)")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-normal comparison")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-cases run")
     ap.add_argument("--orientation", default="LS", choices=["LS", "alphaGrad", "RDF"],
                     help="interface normal method (default LS, the reference test case's)")
     ap.add_argument("--host-io", type=int, default=0, metavar="STEPS",
@@ -587,6 +690,9 @@ def main():
     if args.host_io > 0:
         res["host_io"] = host_io_rate(solver, alpha0, seq[:args.host_io], U0, phi0)
         say("host-buffer steps done")
+
+    if not args.no_batch:
+        res["batched"] = batched_cases_run(say)
 
     if not args.no_learned:
         res["learned_vs_plain"] = learned_comparison(say)

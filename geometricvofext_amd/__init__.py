@@ -471,8 +471,7 @@ class CaseBatch:
     its own time step; every case's results equal its handle stepping alone.
     Close the batch before using the handles on their own again."""
 
-    SHAPE_NAMES = ("normal", "findSignedDistance", "fluxList", "interfaceFlux", "boundRecords", "boundFlow",
-                   "boundApply")
+    SHAPE_NAMES = ("normal", "findSignedDistance", "fluxList", "interfaceFlux", "boundFlow", "boundApply")
 
     def __init__(self, solvers):
         self.solvers = list(solvers)
@@ -502,7 +501,7 @@ class CaseBatch:
 
     def tune(self, dt):
         d = self._dt(dt)
-        out = np.zeros(7, np.int32)
+        out = np.zeros(len(self.SHAPE_NAMES), np.int32)
         _check(lib().svp_batch_tune(self.h, d.ctypes.data, out.ctypes.data))
         return dict(zip(self.SHAPE_NAMES, out.tolist()))
 

@@ -118,19 +118,26 @@ void launch_courant(const DevMesh& m, const double* phi, const double* alpha, un
 // inside one (convex) shape is full, any other tet is refined 1:8 (edge
 // midpoints) down to maxDepth, where the linear interpolant of the shape
 // function is integrated exactly.  Shape k: g_k(x) = |R_k (x - c_k) / a_k|^2 - 1,
-// union f = min_k g_k.
+// union f = min_k g_k.  The miss test runs in each shape's own frame
+// q = R_k (x - c_k) / a_k, where the shape is the unit ball and a tet maps to
+// the tet of its mapped vertices: a tet misses the shape when the ball about
+// the mapped centroid through the farthest mapped vertex misses the unit
+// ball.
 // ---------------------------------------------------------------------------
 struct Shape {
     double c[3], a[3], R[9];   // centre, semi-axes, rows of the rotation
 };
 
-__device__ inline double shape_g(const Shape& s, V3 x) {
+// x in the shape's frame, where the shape is the unit ball
+__device__ inline V3 shape_q(const Shape& s, V3 x) {
     const double d0 = x.x - s.c[0], d1 = x.y - s.c[1], d2 = x.z - s.c[2];
-    const double q0 = (s.R[0] * d0 + s.R[1] * d1 + s.R[2] * d2) / s.a[0];
-    const double q1 = (s.R[3] * d0 + s.R[4] * d1 + s.R[5] * d2) / s.a[1];
-    const double q2 = (s.R[6] * d0 + s.R[7] * d1 + s.R[8] * d2) / s.a[2];
-    return q0 * q0 + q1 * q1 + q2 * q2 - 1.0;
+    return v3((s.R[0] * d0 + s.R[1] * d1 + s.R[2] * d2) / s.a[0], (s.R[3] * d0 + s.R[4] * d1 + s.R[5] * d2) / s.a[1],
+              (s.R[6] * d0 + s.R[7] * d1 + s.R[8] * d2) / s.a[2]);
 }
+
+__device__ inline double shape_gq(V3 q) { return q.x * q.x + q.y * q.y + q.z * q.z - 1.0; }
+
+__device__ inline double shape_g(const Shape& s, V3 x) { return shape_gq(shape_q(s, x)); }
 
 // volume fraction of {f < 0} for f linear on a tet with vertex values v[4]:
 // CDF of a linear function over a simplex, sum_i (-v_i)_+^3 / prod_j (v_j - v_i)
@@ -178,9 +185,10 @@ __global__ void k_init_alpha(DevMesh m, const Shape* shapes, int nShapes, int ma
     }
     bool touch = false;
     for (int s = 0; s < nShapes; ++s) {
+        // the cell's ball maps into the ball of radius R / (smallest semi-axis)
         const Shape& S = shapes[s];
-        const double big = fmax(S.a[0], fmax(S.a[1], S.a[2]));
-        if (mag(cc - v3(S.c[0], S.c[1], S.c[2])) - R <= big) touch = true;
+        const double small = fmin(S.a[0], fmin(S.a[1], S.a[2]));
+        if (mag(shape_q(S, cc)) - R / small <= 1.0) touch = true;
     }
     if (!touch) {
         alpha[c] = 0.0;
@@ -200,17 +208,20 @@ __global__ void k_init_alpha(DevMesh m, const Shape* shapes, int nShapes, int ma
                 const Tet t = stack[--sp];
                 const double vol = tet_vol(t.p[0], t.p[1], t.p[2], t.p[3]);
                 volAll += vol;
-                const V3 g = 0.25 * (t.p[0] + t.p[1] + t.p[2] + t.p[3]);
-                double rad = 0.0;
-                for (int i = 0; i < 4; ++i) rad = fmax(rad, mag(t.p[i] - g));
                 bool full = false, maybe = false;
                 for (int s = 0; s < nShapes && !full; ++s) {
                     const Shape& S = shapes[s];
+                    V3 q[4];
                     bool allIn = true;
-                    for (int i = 0; i < 4; ++i) allIn &= shape_g(S, t.p[i]) < 0.0;
+                    for (int i = 0; i < 4; ++i) {
+                        q[i] = shape_q(S, t.p[i]);
+                        allIn &= shape_gq(q[i]) < 0.0;
+                    }
                     if (allIn) full = true;
-                    const double big = fmax(S.a[0], fmax(S.a[1], S.a[2]));
-                    if (mag(g - v3(S.c[0], S.c[1], S.c[2])) - rad <= big) maybe = true;
+                    const V3 gq = 0.25 * (q[0] + q[1] + q[2] + q[3]);
+                    double rad = 0.0;
+                    for (int i = 0; i < 4; ++i) rad = fmax(rad, mag(q[i] - gq));
+                    if (mag(gq) - rad <= 1.0) maybe = true;
                 }
                 if (full) { volIn += vol; continue; }
                 if (!maybe) continue;

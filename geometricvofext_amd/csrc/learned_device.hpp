@@ -90,6 +90,71 @@ __host__ __device__ inline double cube_fraction_sorted(double m1, double m2, dou
     return (2.0 * a - m12) / (2.0 * m3);  // m12 <= a <= m3
 }
 
+// d cube_fraction_sorted / da: the area of the cut (same pieces and
+// arguments as cube_fraction_sorted)
+__host__ __device__ inline double cube_area_sorted(double m1, double m2, double m3, double a) {
+    const double m12 = m1 + m2;
+    if (a <= 0.0) return 0.0;
+    if (a < m1) return a * a / (2.0 * m1 * m2 * m3);
+    const double base = 6.0 * a - 3.0 * m1;
+    if (a < m2) return base / (6.0 * m2 * m3);
+    if (a < m12 && a < m3) {
+        const double b = a - m2;
+        return (base - 3.0 * b * b / m1) / (6.0 * m2 * m3);
+    }
+    if (m3 < m12) {
+        const double b = a - m2, c = a - m3;
+        return (base - 3.0 * (b * b + c * c) / m1) / (6.0 * m2 * m3);
+    }
+    return 1.0 / m3;
+}
+
+// Inverse of cube_fraction_sorted: the level a in [0, 1/2] at which the unit
+// cube holds the volume V in [0, 1/2] below m . xi = a (m1 <= m2 <= m3, sum
+// 1).  The first two pieces and the last linear one in closed form
+// (Scardovelli & Zaleski, J. Comput. Phys. 164 (2000): a = (6 m1 m2 m3 V)^(1/3)
+// and a = m1/2 + (2 m2 m3 V - m1^2/12)^(1/2)); the cubic pieces by Newton's
+// method inside the piece's bracket, bisecting when a step leaves it.
+__host__ __device__ inline double cube_level_sorted(double m1, double m2, double m3, double V) {
+    if (V <= 0.0) return 0.0;
+    const double m12 = m1 + m2;
+    const double V1 = cube_fraction_sorted(m1, m2, m3, m1);
+    if (V < V1) return cbrt(6.0 * m1 * m2 * m3 * V);
+    const double V2 = cube_fraction_sorted(m1, m2, m3, m2);
+    if (V < V2) return 0.5 * m1 + sqrt(fmax(2.0 * m2 * m3 * V - m1 * m1 / 12.0, 0.0));
+    double lo, hi, Vlo, Vhi;
+    if (m12 <= m3) {
+        // cubic on [m2, m12), linear from m12 on
+        const double V3 = cube_fraction_sorted(m1, m2, m3, m12);
+        if (V >= V3) return m3 * V + 0.5 * m12;
+        lo = m2; hi = m12; Vlo = V2; Vhi = V3;
+    } else {
+        // cubic on [m2, m3), another from m3 to the centre (m12 > 1/2 here)
+        const double V3 = cube_fraction_sorted(m1, m2, m3, m3);
+        if (V < V3) {
+            lo = m2; hi = m3; Vlo = V2; Vhi = V3;
+        } else {
+            lo = m3; hi = 0.5; Vlo = V3; Vhi = cube_fraction_sorted(m1, m2, m3, 0.5);
+        }
+    }
+    double a = (Vhi > Vlo) ? lo + (hi - lo) * ((V - Vlo) / (Vhi - Vlo)) : lo;
+    for (int it = 0; it < 12; ++it) {
+        const double f = cube_fraction_sorted(m1, m2, m3, a) - V;
+        if (f == 0.0) break;
+        if (f > 0.0) hi = a;
+        else lo = a;
+        const double fp = cube_area_sorted(m1, m2, m3, a);
+        double an = fp > 0.0 ? a - f / fp : 0.5 * (lo + hi);
+        if (fabs(an - a) <= 2e-16) {
+            a = an;
+            break;
+        }
+        if (!(an > lo && an < hi)) an = 0.5 * (lo + hi);
+        a = an;
+    }
+    return a;
+}
+
 __host__ __device__ inline double cube_fraction(double m0, double m1, double m2, double a) {
     const double s = m0 + m1 + m2;
     if (!(s > 0.0)) return a > 0.0 ? 1.0 : 0.0;
@@ -151,6 +216,22 @@ struct CubePlane {
         sm[0] = m0;
         sm[1] = m1;
         sm[2] = m2;
+    }
+    // the plane constant D at which [lo, lo + h]^3 holds the liquid fraction
+    // alpha on the side n . x + D < 0 (the inverse of fraction)
+    __host__ __device__ double offset(double alpha, double lox, double loy, double loz) const {
+        double a;
+        if (!(s > 0.0)) a = 0.0;
+        else if (alpha <= 0.0) a = 0.0;
+        else if (alpha >= 1.0) a = 1.0;
+        else if (alpha <= 0.5) a = cube_level_sorted(sm[0], sm[1], sm[2], alpha);
+        else a = 1.0 - cube_level_sorted(sm[0], sm[1], sm[2], 1.0 - alpha);
+        // fraction: rhs = -D - n . lo - (the reflected coefficients), a = rhs / s
+        double rhs = a * s;
+        if (neg[0] < 0.0) rhs += neg[0];
+        if (neg[1] < 0.0) rhs += neg[1];
+        if (neg[2] < 0.0) rhs += neg[2];
+        return -rhs - (nx * lox + ny * loy + nz * loz);
     }
     // liquid fraction of [lo, lo + h]^3 on the side n . x + D < 0
     __host__ __device__ double fraction(double D, double lox, double loy, double loz) const {

@@ -567,26 +567,23 @@ __global__ void __launch_bounds__(256) k_sync_bf16(const float* pq, uint16_t* wb
 // degenerate outputs are listed for the least-squares normal.
 // ---------------------------------------------------------------------------
 constexpr int INFER_WAVES = 4;
-constexpr int RESID_ROUNDS = 16;  // multisection rounds of the residual plane (bracket / 5 each)
 
 // Inference, one wave per 16 interface cells (lane r + 16 q: cell r, part q).
 // The four lanes of a cell gather its 27 stencil values, derive the same
 // canonical frame and each build eight network inputs; the wave runs the
 // network; group 0 holds the outputs, which every lane of the cell maps back
 // to the mesh frame.  With tol < 1 the residual test follows: the plane of
-// the network normal that leaves the cell's own alpha in its cube is found by
-// multisection of the plane constant D over the vertex range (the liquid
-// fraction falls monotonically as D grows; each round the four lanes
-// evaluate four interior points and the bracket shrinks 5x), and the
-// residual is the mean |fraction - alpha| over the 26 neighbour cubes
+// the network normal that leaves the cell's own alpha in its cube (the
+// inverse of the cube fraction, CubePlane::offset), and the residual is the
+// mean |fraction - alpha| over the 26 neighbour cubes
 // lo + e h of the lattice (lane q takes the neighbours L = q, q+4, ...).
 // Cells without a full uniform 3x3x3 stencil (lat[27c] < 0), degenerate
 // outputs and cells above tol are listed for the least-squares normal.
-__global__ void __launch_bounds__(64 * INFER_WAVES, 2) k_infer(const int* lat, const double* alpha, const int* mixed,
-                                                             const int* nMixedP, const double* C, const double* V,
-                                                             const uint16_t* wbG, const float* pqG, double tol,
-                                                             double* ifN, unsigned char* learned, double* resid,
-                                                             int* lsList, int* nLs, int* nFb) {
+__global__ void __launch_bounds__(64 * INFER_WAVES, 2) k_infer(const int* lat, LatStride ls, int nC, const double* alpha,
+                                                             const int* mixed, const int* nMixedP, const double* C,
+                                                             const double* V, const uint16_t* wbG, const float* pqG,
+                                                             double tol, double* ifN, unsigned char* learned,
+                                                             double* resid, int* lsList, int* nLs, int* nFb) {
     // weights and biases of the network, staged once per workgroup
     __shared__ __attribute__((aligned(16))) uint16_t wb[B_W2T];           // W1b, W2b (W2T is not needed)
     __shared__ __attribute__((aligned(16))) uint16_t w3[NOUTP * NH];
@@ -624,14 +621,31 @@ __global__ void __launch_bounds__(64 * INFER_WAVES, 2) k_infer(const int* lat, c
         // alphas, in flight together; the cell's centre and volume with them
         constexpr int NJ = (NIN + 3) / 4;
         int sl[NJ];
+        double av[NJ];
+        bool valid;
+        if (ls.affine) {
+            // every full stencil is c + off[L]: the alpha loads go out with
+            // the load of the validity entry (indices clamped into the mesh;
+            // values of cells without a full stencil are not used)
+            const int e0 = lat[(long)c * NIN];
 #pragma unroll
-        for (int j = 0; j < NJ; ++j) sl[j] = (q + 4 * j < NIN) ? lat[(long)c * NIN + q + 4 * j] : -1;
+            for (int j = 0; j < NJ; ++j) {
+                const int L = q + 4 * j;
+                sl[j] = min(max(c + ls.off[L < NIN ? L : 0], 0), nC - 1);
+                av[j] = alpha[sl[j]];
+            }
+            valid = live && e0 >= 0;
+#pragma unroll
+            for (int j = 0; j < NJ; ++j) av[j] = valid ? av[j] : 0.0;
+        } else {
+#pragma unroll
+            for (int j = 0; j < NJ; ++j) sl[j] = (q + 4 * j < NIN) ? lat[(long)c * NIN + q + 4 * j] : -1;
+            valid = live && __shfl(sl[0], r) >= 0;  // lattice entry 0 is held by lane r (q = 0)
+#pragma unroll
+            for (int j = 0; j < NJ; ++j) av[j] = (valid && sl[j] >= 0) ? alpha[sl[j]] : 0.0;
+        }
         const double cV = V[c];
         const double cCx = C[3L * c], cCy = C[3L * c + 1], cCz = C[3L * c + 2];
-        const bool valid = live && __shfl(sl[0], r) >= 0;  // lattice entry 0 is held by lane r (q = 0)
-        double av[NJ];
-#pragma unroll
-        for (int j = 0; j < NJ; ++j) av[j] = (valid && sl[j] >= 0) ? alpha[sl[j]] : 0.0;
 #pragma unroll
         for (int j = 0; j < NJ; ++j)
             if (q + 4 * j < NIN) rw[r][q + 4 * j] = av[j];
@@ -662,25 +676,8 @@ __global__ void __launch_bounds__(64 * INFER_WAVES, 2) k_infer(const int* lat, c
             const double lox = live ? cCx - 0.5 * h : 0.0, loy = live ? cCy - 0.5 * h : 0.0,
                          loz = live ? cCz - 0.5 * h : 0.0;
             const double a = rw[r][13];
-            const double base = -(nx * lox + ny * loy + nz * loz);
-            const double sx = nx * h, sy = ny * h, sz = nz * h;
-            const double tmin = fmin(sx, 0.0) + fmin(sy, 0.0) + fmin(sz, 0.0);
-            const double tmax = fmax(sx, 0.0) + fmax(sy, 0.0) + fmax(sz, 0.0);
             const CubePlane cp(nx, ny, nz, h);
-            double dlo = base - tmax, dhi = base - tmin;  // fraction 1 at dlo, 0 at dhi
-            for (int round = 0; round < RESID_ROUNDS; ++round) {
-                const double w = (dhi - dlo) / 5.0;
-                const bool above = cp.fraction(dlo + w * (double)(q + 1), lox, loy, loz) > a;
-                // the cell's lanes are r, r+16, r+32, r+48: count those left of the crossing
-                const unsigned long long bal = __ballot(above);
-                const int k = (int)((bal >> r) & 1ull) + (int)((bal >> (r + 16)) & 1ull) +
-                              (int)((bal >> (r + 32)) & 1ull) + (int)((bal >> (r + 48)) & 1ull);
-                const double newLo = k > 0 ? dlo + w * (double)k : dlo;
-                const double newHi = k < 4 ? dlo + w * (double)(k + 1) : dhi;
-                dlo = newLo;
-                dhi = newHi;
-            }
-            const double d = 0.5 * (dlo + dhi);
+            const double d = cp.offset(a, lox, loy, loz);
             double part = 0.0;
 #pragma unroll
             for (int j = 0; j < NJ; ++j) {
@@ -763,6 +760,36 @@ __global__ void __launch_bounds__(256) k_lattice(DevMesh m, const unsigned char*
         for (int k = 0; k < NIN; ++k) L[k] = -1;
 }
 
+// Is the lattice map affine (lat[27c + L] = c + off[L] for every cell with a
+// full stencil, one off for all)?  probe: [0] lowest cell with a full
+// stencil, [1] affine flag, [2 + L] off[L] of that cell.
+__global__ void k_lat_probe_init(int* probe) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        probe[0] = 0x7fffffff;
+        probe[1] = 1;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_lat_probe_first(const int* lat, int nC, int* probe) {
+    const int c = blockIdx.x * 256 + threadIdx.x;
+    if (c < nC && lat[(long)c * NIN] >= 0) atomicMin(&probe[0], c);
+}
+
+__global__ void k_lat_probe_off(const int* lat, int* probe) {
+    const int L = threadIdx.x;
+    const int c0 = probe[0];
+    if (L < NIN) probe[2 + L] = c0 == 0x7fffffff ? 0 : lat[(long)c0 * NIN + L] - c0;
+    if (L == 0 && c0 == 0x7fffffff) probe[1] = 0;
+}
+
+__global__ void __launch_bounds__(256) k_lat_probe_check(const int* lat, int nC, int* probe) {
+    const int c = blockIdx.x * 256 + threadIdx.x;
+    if (c >= nC || lat[(long)c * NIN] < 0) return;
+    bool ok = true;
+    for (int L = 0; L < NIN; ++L) ok &= lat[(long)c * NIN + L] == c + probe[2 + L];
+    if (!ok) atomicAnd(&probe[1], 0);
+}
+
 __global__ void k_zero_ints(int* p, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0;
@@ -778,6 +805,13 @@ void launch_cube_fractions(const double* in, long k, double* out, hipStream_t s)
 void launch_lattice(const DevMesh& m, unsigned char* cubeScratch, int* lat, hipStream_t s) {
     k_cube_flags<<<cdiv(m.nC, 256), 256, 0, s>>>(m, cubeScratch);
     k_lattice<<<cdiv(m.nC, 256), 256, 0, s>>>(m, cubeScratch, lat);
+}
+
+void launch_lattice_probe(const int* lat, int nC, int* probe, hipStream_t s) {
+    k_lat_probe_init<<<1, 64, 0, s>>>(probe);
+    k_lat_probe_first<<<cdiv(nC, 256), 256, 0, s>>>(lat, nC, probe);
+    k_lat_probe_off<<<1, 64, 0, s>>>(lat, probe);
+    k_lat_probe_check<<<cdiv(nC, 256), 256, 0, s>>>(lat, nC, probe);
 }
 
 void launch_zero_ints(int* p, int n, hipStream_t s) { k_zero_ints<<<cdiv(n, 256), 256, 0, s>>>(p, n); }
@@ -812,9 +846,9 @@ void launch_sync_bf16(const float* pq, uint16_t* wb, hipStream_t s) {
     k_sync_bf16<<<cdiv(N_PADDED, 256), 256, 0, s>>>(pq, wb);
 }
 
-void launch_infer(const int* lat, const double* alpha, const int* mixed, const int* nMixed, const double* C,
-                  const double* V, const uint16_t* wb, const float* pq, double tol, double* ifN, unsigned char* learned,
-                  double* resid, int* lsList, int* nLs, int* nFb, int nC, hipStream_t s) {
+void launch_infer(const int* lat, const LatStride& ls, const double* alpha, const int* mixed, const int* nMixed,
+                  const double* C, const double* V, const uint16_t* wb, const float* pq, double tol, double* ifN,
+                  unsigned char* learned, double* resid, int* lsList, int* nLs, int* nFb, int nC, hipStream_t s) {
     // workgroups (each stages the 45 KB of weights once): knob SVP_INFER_GRID
     static const int maxGrid = [] {
         const char* e = std::getenv("SVP_INFER_GRID");
@@ -822,7 +856,7 @@ void launch_infer(const int* lat, const double* alpha, const int* mixed, const i
         return (g >= 1 && g <= 4096) ? g : 512;
     }();
     k_infer<<<std::max(1, std::min(maxGrid, cdiv(nC, 16 * INFER_WAVES))), 64 * INFER_WAVES, 0, s>>>(
-        lat, alpha, mixed, nMixed, C, V, wb, pq, tol, ifN, learned, resid, lsList, nLs, nFb);
+        lat, ls, nC, alpha, mixed, nMixed, C, V, wb, pq, tol, ifN, learned, resid, lsList, nLs, nFb);
 }
 
 

@@ -27,13 +27,21 @@ void launch_reduce(const float* partial, int nWG, float* grad, hipStream_t s);
 void launch_adam(const float* partial, int nWG, float* pq, float* m1, float* m2, uint16_t* wb, float lr, float b1,
                  float b2, float eps, float c1, float c2, hipStream_t s);
 void launch_sync_bf16(const float* pq, uint16_t* wb, hipStream_t s);
-void launch_infer(const int* lat, const double* alpha, const int* mixed, const int* nMixed, const double* C,
-                  const double* V, const uint16_t* wb, const float* pq, double tol, double* ifN, unsigned char* learned,
-                  double* resid, int* lsList, int* nLs, int* nFb, int nC, hipStream_t s);
+// lattice map of the form lat[27c + L] = c + off[L] on every cell with a full
+// stencil (a lattice numbered in axis order): affine = 1 and the offsets
+struct LatStride {
+    int affine;
+    int off[27];
+};
+void launch_infer(const int* lat, const LatStride& ls, const double* alpha, const int* mixed, const int* nMixed,
+                  const double* C, const double* V, const uint16_t* wb, const float* pq, double tol, double* ifN,
+                  unsigned char* learned, double* resid, int* lsList, int* nLs, int* nFb, int nC, hipStream_t s);
 
 void launch_cube_fractions(const double* in, long k, double* out, hipStream_t s);
 void launch_lattice(const DevMesh& m, unsigned char* cubeScratch, int* lat, hipStream_t s);
 void launch_zero_ints(int* p, int n, hipStream_t s);
+// probe[2 + 27]: see LatStride (probe[1] = affine, probe[2 + L] = off[L])
+void launch_lattice_probe(const int* lat, int nC, int* probe, hipStream_t s);
 
 }  // namespace nm
 }  // namespace svp

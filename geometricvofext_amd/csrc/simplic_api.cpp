@@ -177,7 +177,7 @@ public:
         stage(STAGE_NORMAL, [&] {
             nm::launch_zero_ints(learned_.counts, 2, stream_);
             // the residual test runs inside the inference kernel (tol < 1)
-            nm::launch_infer(learned_.lat, st_.alpha, st_.mixed, st_.nMixed, m_.C, m_.V, learned_.wb, learned_.pq,
+            nm::launch_infer(learned_.lat, learned_.stride, st_.alpha, st_.mixed, st_.nMixed, m_.C, m_.V, learned_.wb, learned_.pq,
                              learned_.tol, st_.ifN, learned_.learned, learned_.resid, learned_.lsList, nLs, nFb,
                              m_.nC, stream_);
             launch_ls_normal(m_, st_, stream_, learned_.lsList, nLs);
@@ -201,8 +201,17 @@ public:
             DBuf cube;
             cube.alloc<unsigned char>(m_.nC);
             nm::launch_lattice(m_, (unsigned char*)cube.p, learned_.lat, stream_);
+            DBuf probe;
+            probe.alloc<int>(2 + 27);
+            nm::launch_lattice_probe(learned_.lat, m_.nC, (int*)probe.p, stream_);
             HIPCHECK(hipGetLastError());
+            int pr[2 + 27];
+            HIPCHECK(hipMemcpyAsync(pr, probe.p, sizeof pr, hipMemcpyDeviceToHost, stream_));
             HIPCHECK(hipStreamSynchronize(stream_));
+            learned_.stride.affine = pr[1];
+            for (int L = 0; L < 27; ++L) learned_.stride.off[L] = pr[2 + L];
+            if (const char* e = std::getenv("SVP_LATTICE_AFFINE"))
+                if (std::atoi(e) == 0) learned_.stride.affine = 0;
         }
         HIPCHECK(hipMemcpyAsync(learned_.pq, nmod.pq.p, sizeof(float) * nm::N_PADDED, hipMemcpyDeviceToDevice,
                                 stream_));
@@ -628,6 +637,7 @@ public:
         bool active = false, ready = false;
         double tol = 0.0;
         int* lat = nullptr;          // 27 per cell, -1 without a uniform stencil
+        nm::LatStride stride{};      // the map as c + off[L] where that holds for every cell
         float* pq = nullptr;         // network parameters (padded FP32)
         uint16_t* wb = nullptr;      // BF16 weights
         unsigned char* learned = nullptr;
@@ -908,7 +918,9 @@ private:
         st_.Ub = dalloc<double>(3L * std::max(nB_, 1));
         st_.dVf = dalloc<double>(nF);
         st_.alphaPhi = dalloc<double>(nF);
-        st_.corr = dalloc<double>(nF);
+        // bounding arrays double-buffered by pass parity: corrections, active
+        // flags, writer indices and correctedFaces keys
+        st_.corr = dalloc<double>(2L * nF);
         st_.mixedFlag = dalloc<unsigned char>(nC);
         st_.nMixed = dalloc<int>(1);
         st_.mixed = dalloc<int>(nC);
@@ -926,16 +938,16 @@ private:
         st_.candCount = dalloc<int>(1);
         st_.candBits = dalloc<unsigned long long>((nC + 63) / 64);
         st_.cand = dalloc<int>(2L * nC);
-        st_.active = dalloc<unsigned char>(nC);
+        st_.active = dalloc<unsigned char>(2L * nC);
         st_.pend = dalloc<int>(nC);
         st_.queue = dalloc<int>(2L * nC);
-        st_.actIdx = dalloc<int>(nC);
-        HIPCHECK(hipMemsetAsync(st_.actIdx, 0xFF, sizeof(int) * nC, stream_));
+        st_.actIdx = dalloc<int>(2L * nC);
+        HIPCHECK(hipMemsetAsync(st_.actIdx, 0xFF, sizeof(int) * 2 * nC, stream_));
         // correctedFaces order keys are cell * (max faces) + rank in 31 bits
         if ((long long)nC * (maxCellFaces_ <= NS_CELL_FACES ? NS_CELL_FACES : MAX_CELL_FACES) >= 0x7FFFFFFFll)
             throw SvpError(SVP_E_UNSUPPORTED, "mesh too large for 31-bit bounding order keys");
-        st_.corrPos = dalloc<int>(nF);
-        HIPCHECK(hipMemsetAsync(st_.corrPos, 0xFF, sizeof(int) * nF, stream_));
+        st_.corrPos = dalloc<int>(2L * nF);
+        HIPCHECK(hipMemsetAsync(st_.corrPos, 0xFF, sizeof(int) * 2 * nF, stream_));
         st_.touched = dalloc<int>(nC);
         st_.touchedFlag = dalloc<int>(nC);
         st_.boundStats = dalloc<int>(4 * MAX_BOUND_PASSES);
@@ -1049,7 +1061,7 @@ public:
                 for (int half = 0; half < 2; ++half) {
                     DevState v = h.st_;
                     if (a) std::swap(v.alpha, v.alphaOld);
-                    v.cand = h.st_.cand + (long)((S + half) & 1) * nC;
+                    v = bound_pass_view(h.m_, v, S + half);
                     b.st[(a << 1) | half] = v;
                 }
             b.detBlocks = detect_list_blocks(nC);
@@ -1082,9 +1094,9 @@ public:
         dims_.fsd = 64;
         dims_.fluxList = 8;
         dims_.fluxFaces = 64;
-        dims_.prep = 8;
         dims_.flow = 4;
         dims_.apply = 8;
+        if (const char* e = std::getenv("SVP_BATCH_GRAPH")) useGraph_ = std::atoi(e) != 0;
         HIPCHECK(hipStreamSynchronize(stream_));
     }
 
@@ -1092,10 +1104,8 @@ public:
         (void)hipSetDevice(device_);
         (void)hipStreamSynchronize(stream_);
         for (size_t k = 0; k < c_.size(); ++k) c_[k]->stream_ = ownStream_[k];
-        for (auto& e : pending_) {
-            (void)hipEventDestroy(e.first);
-            (void)hipEventDestroy(e.second);
-        }
+        dropGraph();
+        for (hipEvent_t e : allEvents_) (void)hipEventDestroy(e);
         for (int r = 0; r < RING; ++r) {
             if (ring_[r]) (void)hipHostFree(ring_[r]);
             if (ringEv_[r]) (void)hipEventDestroy(ringEv_[r]);
@@ -1115,11 +1125,25 @@ public:
         upload(dt, nullptr);
         hipEvent_t a = event(), b = event();
         HIPCHECK(hipEventRecord(a, stream_));
-        for (int sg = STAGE_DETECT; sg <= STAGE_SNAPCLIP; ++sg) {
-            profiled(sg, [&] {
-                launch_batch_stage(sg, (const BatchCase*)bc_.p, (const BatchStepInfo*)info_.p, n, maxC_, maxF_,
-                                   maxDet_, maxList_, dims_, params_, G_, MF_, triQuad_, stream_);
-            });
+        if (profMask_ || !useGraph_) {
+            for (int sg = STAGE_DETECT; sg <= STAGE_SNAPCLIP; ++sg)
+                profiled(sg, [&] { launchStage(sg); });
+        } else {
+            // the step's launches do not depend on dt or the buffer
+            // parities (both are read from info_ on the device), so one
+            // captured graph replays every step
+            if (!graph_) {
+                hipGraph_t g = nullptr;
+                HIPCHECK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+                for (int sg = STAGE_DETECT; sg <= STAGE_SNAPCLIP; ++sg) launchStage(sg);
+                const hipError_t launched = hipGetLastError();
+                HIPCHECK(hipStreamEndCapture(stream_, &g));
+                HIPCHECK(launched);
+                const hipError_t made = hipGraphInstantiate(&graph_, g, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(g);
+                HIPCHECK(made);
+            }
+            HIPCHECK(hipGraphLaunch(graph_, stream_));
         }
         HIPCHECK(hipEventRecord(b, stream_));
         HIPCHECK(hipGetLastError());
@@ -1132,6 +1156,16 @@ public:
             c_[k]->afterBatchStep(dt[k], nPasses);
         }
         steps_++;
+    }
+
+    void launchStage(int sg) {
+        launch_batch_stage(sg, (const BatchCase*)bc_.p, (const BatchStepInfo*)info_.p, size(), maxC_, maxF_,
+                           maxDet_, maxList_, dims_, params_, G_, MF_, triQuad_, stream_);
+    }
+
+    void dropGraph() {
+        if (graph_) (void)hipGraphExecDestroy(graph_);
+        graph_ = nullptr;
     }
 
     // harness: max over each case's cells of sum|phi|/V and of the same for
@@ -1209,6 +1243,7 @@ public:
     // shapes; only the stage times do.
     void tune(const double* dt, int32_t* chosen) {
         if (!dt) throw SvpError(SVP_E_ARG, "svp_batch_tune: null dt");
+        dropGraph();
         const int n = size();
         // the fields every trial starts from
         std::vector<DBuf> saved(n);
@@ -1237,7 +1272,6 @@ public:
                                    {STAGE_IFLUX, &dims_.fluxFaces, {16, 32, 64, 128, 256}},
                                    {STAGE_IFLUX, &dims_.fluxList, {2, 4, 8, 16, 32}},
                                    {STAGE_BOUND, &dims_.flow, {1, 2, 4, 8}},
-                                   {STAGE_BOUND, &dims_.prep, {2, 4, 8, 16, 32}},
                                    {STAGE_BOUND, &dims_.apply, {2, 4, 8, 16, 32}}};
         trial(STAGE_DETECT);  // warm-up
         for (Knob& kb : knobs) {
@@ -1261,23 +1295,22 @@ public:
             chosen[1] = dims_.fsd;
             chosen[2] = dims_.fluxList;
             chosen[3] = dims_.fluxFaces;
-            chosen[4] = dims_.prep;
-            chosen[5] = dims_.flow;
-            chosen[6] = dims_.apply;
+            chosen[4] = dims_.flow;
+            chosen[5] = dims_.apply;
         }
     }
 
     void setDims(const int32_t* d) {
         if (!d) throw SvpError(SVP_E_ARG, "svp_batch_set_shapes: null");
-        for (int k = 0; k < 7; ++k)
+        for (int k = 0; k < 6; ++k)
             if (d[k] < 1 || d[k] > 65535) throw SvpError(SVP_E_ARG, "svp_batch_set_shapes: 1..65535 per stage");
+        dropGraph();
         dims_.ls = d[0];
         dims_.fsd = d[1];
         dims_.fluxList = d[2];
         dims_.fluxFaces = d[3];
-        dims_.prep = d[4];
-        dims_.flow = d[5];
-        dims_.apply = d[6];
+        dims_.flow = d[4];
+        dims_.apply = d[5];
     }
 
     int device_ = 0;
@@ -1293,6 +1326,10 @@ private:
     long maxN_ = 0;
     bool haveFlow_ = true;
     BatchDims dims_{};
+    // the step's launches as one graph (SVP_BATCH_GRAPH=0: launched one by
+    // one); rebuilt after the launch shapes change
+    bool useGraph_ = true;
+    hipGraphExec_t graph_ = nullptr;
     DBuf bc_, info_, factors_, courant_;
     char* ring_[RING] = {};
     hipEvent_t ringEv_[RING] = {};

@@ -2018,12 +2018,14 @@ __device__ inline bool face_downwind_of(const DevMesh& m, const double* phi, int
 // limitFlux (advectionTemplates.C:118-215), one pass = the kernels below.
 // The pass runs only while cells lie outside [-aTol, 1+aTol] (bound count of
 // the update kernel, kept current by the apply kernel):
-//   list              ascending candidates from the bitmap (one launch), and
-//                     the clearing of what the previous pass left
+//   list              ascending candidates from the bitmap (one launch);
+//                     each block then runs its candidates' first
+//                     redistribution loop, which decides whether a cell
+//                     writes corrections and which faces open
+//                     correctedFaces, and writes the record of each writing
+//                     cell; last, the clearing of what the previous pass left
+//                     in the other half of the double buffers
 //                     (dVfCorrectionValues = 0 for its active cells)
-//   prep              every candidate's first redistribution loop, which
-//                     decides whether it writes corrections and which faces
-//                     open correctedFaces; the record of each writing cell
 //   flow              all remaining boundFlux work in dependency order in
 //                     one launch, and the correctedFaces order keys and
 //                     touched cells (the default), or
@@ -2032,7 +2034,7 @@ __device__ inline bool face_downwind_of(const DevMesh& m, const double* phi, int
 //   apply             corrections applied per cell in correctedFaces order,
 //                     dVf of the corrected faces, bitmap and bound count,
 //                     pass statistics
-// bcount: [0] candidates, [1] corrected faces (prep), [2] touched cells,
+// bcount: [0] candidates, [1] corrected faces (list), [2] touched cells,
 // [3+r] frontier size of round r (r <= st.gridRounds; [3] = active cells of
 // the dataflow pass), [BND_CTRL-1] rounds, then the dataflow ticket and
 // completion count.
@@ -2256,9 +2258,12 @@ __device__ inline bool oob_any(const DevState& st) {
     return n != 0;
 }
 
-__device__ __attribute__((always_inline)) inline void k_bnd_list_body(const DevMesh& m, const DevState& st, int nWords, int nBlocks, const int* candPrev) {
+template <int MF>
+__device__ __attribute__((always_inline)) inline void k_bnd_list_body(const DevMesh& m, const DevState& st, int nWords,
+                                                                      int nBlocks, const BndPrev& prev, double dt,
+                                                                      const double* Sp, const double* Su) {
     __shared__ int lds[16];
-    __shared__ int sTicket, sBase;
+    __shared__ int sTicket, sBase, sTotal;
     __shared__ unsigned sEpoch;
     if (threadIdx.x == 0) sTicket = take_ticket(st.bndTicket, nBlocks, sEpoch);
     __syncthreads();
@@ -2271,17 +2276,19 @@ __device__ __attribute__((always_inline)) inline void k_bnd_list_body(const DevM
     auto clearPrevious = [&]() {
         const int nPrev = st.bcount[BND_PREV_COUNT];
         for (int i = b * blockDim.x + threadIdx.x; i < nPrev; i += nBlocks * blockDim.x) {
-            if (!st.active[i]) continue;
-            const int c = candPrev[i];
+            if (!prev.active[i]) continue;
+            const int c = prev.cand[i];
             for (int k = m.cOff[c]; k < m.cOff[c + 1]; ++k) {
                 const int f = m.cFaces[k];
-                st.corr[f] = 0.0;
-                st.corrPos[f] = -1;
+                prev.corr[f] = 0.0;
+                prev.corrPos[f] = -1;
             }
-            st.actIdx[c] = -1;
+            prev.actIdx[c] = -1;
         }
     };
-    if (b == 0 && threadIdx.x >= 1 && threadIdx.x <= BND_PROGRESS) st.bcount[threadIdx.x] = 0;
+    // the counters the later kernels of the pass count into (the corrected
+    // faces of [1] are counted here, so the previous apply cleared them)
+    if (b == 0 && threadIdx.x >= 2 && threadIdx.x <= BND_PROGRESS) st.bcount[threadIdx.x] = 0;
     if (!oob_any(st)) {
         if (b == 0 && threadIdx.x == 0) st.bcount[0] = 0;
         clearPrevious();
@@ -2299,6 +2306,7 @@ __device__ __attribute__((always_inline)) inline void k_bnd_list_body(const DevM
         if (threadIdx.x == 0) {
             if (b == nBlocks - 1) st.bcount[0] = (int)(excl + total);
             sBase = (int)excl;
+            sTotal = (int)total;
         }
     }
     __syncthreads();
@@ -2329,18 +2337,11 @@ __device__ __attribute__((always_inline)) inline void k_bnd_list_body(const DevM
         base += tot;
         __syncthreads();
     }
-    clearPrevious();
-}
-
-__global__ void __launch_bounds__(256) k_bnd_list(DevMesh m, DevState st, int nWords, int nBlocks, const int* candPrev) {
-    k_bnd_list_body(m, st, nWords, nBlocks, candPrev);
-}
-
-template <int MF>
-__device__ __attribute__((always_inline)) inline void k_bnd_prep_body(const DevMesh& m, const DevState& st, double dt, const double* Sp, const double* Su) {
-    const int nCand = st.bcount[0];
+    // every candidate of the block gets its record (iteration 0 of boundFlux
+    // done) and, when it writes corrections, its candidate index and the
+    // CORR_PENDING marks of the faces it will write
     int nOpen = 0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nCand; i += gridDim.x * blockDim.x) {
+    for (int i = sBase + threadIdx.x; i < sBase + sTotal; i += blockDim.x) {
         const int c = st.cand[i];
         const int n = bound_record<MF>(m, st, c, dt, Sp, Su, ((BndRec<MF>*)st.bndRec)[i]);
         st.active[i] = n > 0 ? 1 : 0;
@@ -2350,12 +2351,15 @@ __device__ __attribute__((always_inline)) inline void k_bnd_prep_body(const DevM
     // correctedFaces length of the pass (statistics)
     for (int o = 32; o > 0; o >>= 1) nOpen += __shfl_xor(nOpen, o);
     if ((threadIdx.x & 63) == 0 && nOpen) atomicAdd(&st.bcount[1], nOpen);
+    clearPrevious();
 }
 
 template <int MF>
-__global__ void __launch_bounds__(256) k_bnd_prep(DevMesh m, DevState st, double dt, const double* Sp, const double* Su) {
-    k_bnd_prep_body<MF>(m, st, dt, Sp, Su);
+__global__ void __launch_bounds__(256) k_bnd_list(DevMesh m, DevState st, int nWords, int nBlocks, BndPrev prev,
+                                                  double dt, const double* Sp, const double* Su) {
+    k_bnd_list_body<MF>(m, st, nWords, nBlocks, prev, dt, Sp, Su);
 }
+
 
 // correctedFaces keys and the cells they touch (both schedules), for candidate
 // c whose record is written.  correctedFaces lists the first-loop faces of
@@ -2781,6 +2785,9 @@ __device__ __attribute__((always_inline)) inline void k_bnd_apply_body(const Dev
             st.boundTotals[4] += 1ull;
             *st.candCount = nCand;
         }
+        // the corrected-face count of the next pass starts at zero (its list
+        // kernel counts into it)
+        st.bcount[1] = 0;
     }
     for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < nT + nCand; idx += gridDim.x * blockDim.x) {
         if (idx >= nT) {
@@ -2868,14 +2875,13 @@ __global__ void __launch_bounds__(256) k_bnd_apply(DevMesh m, DevState st, int p
 
 template <int MF>
 void launch_bound_pass(const DevMesh& m, const DevState& st, int pass, double dt, const double* Sp,
-                       const double* Su, const int* candPrev, hipStream_t s) {
+                       const double* Su, const BndPrev& prev, hipStream_t s) {
     const int nWords = (m.nC + 63) / 64;
     // candidate-sized work: a modest grid-stride grid launches faster than
     // one thread per cell of the mesh
     const int grid = std::max(1, std::min(st.bndGrid, cdiv(m.nC, 256)));
     const int nbw = cdiv(nWords, st.bndListWords);
-    k_bnd_list<<<nbw, 256, 0, s>>>(m, st, nWords, nbw, candPrev);
-    k_bnd_prep<MF><<<grid, 256, 0, s>>>(m, st, dt, Sp, Su);
+    k_bnd_list<MF><<<nbw, 256, 0, s>>>(m, st, nWords, nbw, prev, dt, Sp, Su);
     if (st.bndFlow) {
         k_bnd_flow<MF><<<std::max(1, std::min(st.bndFlow, cdiv(m.nC, flow_chunk<MF>()))), flow_threads<MF>(), 0, s>>>(
             m, st, dt);
@@ -2887,19 +2893,34 @@ void launch_bound_pass(const DevMesh& m, const DevState& st, int pass, double dt
     k_bnd_apply<MF><<<grid, 256, 0, s>>>(m, st, pass);
 }
 
-// The candidate list alternates between the two halves of st.cand by the
-// handle's running pass count, so a pass's list kernel can clear what the
-// previous pass left (its candidates are still in the other half).
+// The state of bounding pass number seq of a handle: its candidate list,
+// active flags, corrections, correctedFaces keys and writer indices are the
+// half (seq & 1) of their double buffers, so the pass's list kernel can clear
+// what the previous pass left in the other half while it writes its own.
+DevState bound_pass_view(const DevMesh& m, const DevState& st, int seq) {
+    DevState v = st;
+    const long h = seq & 1;
+    v.cand = st.cand + h * m.nC;
+    v.active = st.active + h * m.nC;
+    v.actIdx = st.actIdx + h * m.nC;
+    v.corr = st.corr + h * m.nF;
+    v.corrPos = st.corrPos + h * m.nF;
+    return v;
+}
+
+BndPrev bound_prev(const DevState& other) {
+    return BndPrev{other.cand, other.active, other.corr, other.corrPos, other.actIdx};
+}
+
 void launch_bound(const DevMesh& m, const DevState& st, const Params& p, double dt, const double* Sp,
                   const double* Su, int* passSeq, hipStream_t s) {
     const int nPasses = std::min(p.nAlphaBounds, MAX_BOUND_PASSES);
     for (int pass = 0; pass < nPasses; ++pass) {
-        DevState sp = st;
         const int seq = (*passSeq)++;
-        sp.cand = st.cand + (long)(seq & 1) * m.nC;
-        const int* candPrev = st.cand + (long)((seq + 1) & 1) * m.nC;
-        if (st.maxCellFaces <= NS_CELL_FACES) launch_bound_pass<NS_CELL_FACES>(m, sp, pass, dt, Sp, Su, candPrev, s);
-        else launch_bound_pass<MAX_CELL_FACES>(m, sp, pass, dt, Sp, Su, candPrev, s);
+        const DevState sp = bound_pass_view(m, st, seq);
+        const BndPrev prev = bound_prev(bound_pass_view(m, st, seq + 1));
+        if (st.maxCellFaces <= NS_CELL_FACES) launch_bound_pass<NS_CELL_FACES>(m, sp, pass, dt, Sp, Su, prev, s);
+        else launch_bound_pass<MAX_CELL_FACES>(m, sp, pass, dt, Sp, Su, prev, s);
     }
 }
 
@@ -3055,21 +3076,15 @@ __global__ void __launch_bounds__(UPDATE_BLOCK) kb_update(const BatchCase* __res
     k_update_body(c.m, batch_view(c, si, 1, 0), si.dt, nullptr, nullptr);
 }
 
+template <int MF>
 __global__ void __launch_bounds__(256) kb_bnd_list(const BatchCase* __restrict__ bc,
                                                    const BatchStepInfo* __restrict__ info, int pass) {
     const BatchCase& c = bc[blockIdx.y];
     const BatchStepInfo si = info[blockIdx.y];
     if (!(si.dt > 0.0) || (int)blockIdx.x >= c.listBlocks) return;
-    k_bnd_list_body(c.m, batch_view(c, si, 1, pass), c.listWords, c.listBlocks, batch_view(c, si, 1, pass + 1).cand);
-}
-
-template <int MF>
-__global__ void __launch_bounds__(256) kb_bnd_prep(const BatchCase* __restrict__ bc,
-                                                   const BatchStepInfo* __restrict__ info, int pass) {
-    const BatchCase& c = bc[blockIdx.y];
-    const BatchStepInfo si = info[blockIdx.y];
-    if (!(si.dt > 0.0)) return;
-    k_bnd_prep_body<MF>(c.m, batch_view(c, si, 1, pass), si.dt, nullptr, nullptr);
+    const DevState& o = batch_view(c, si, 1, pass + 1);
+    const BndPrev prev{o.cand, o.active, o.corr, o.corrPos, o.actIdx};
+    k_bnd_list_body<MF>(c.m, batch_view(c, si, 1, pass), c.listWords, c.listBlocks, prev, si.dt, nullptr, nullptr);
 }
 
 template <int MF>
@@ -3105,8 +3120,7 @@ template <int MF>
 static void launch_batch_bound(const BatchCase* bc, const BatchStepInfo* info, int n, int maxListBlocks,
                                const BatchDims& d, int nPasses, hipStream_t s) {
     for (int pass = 0; pass < nPasses; ++pass) {
-        kb_bnd_list<<<dim3(maxListBlocks, n), 256, 0, s>>>(bc, info, pass);
-        kb_bnd_prep<MF><<<dim3(d.prep, n), 256, 0, s>>>(bc, info, pass);
+        kb_bnd_list<MF><<<dim3(maxListBlocks, n), 256, 0, s>>>(bc, info, pass);
         kb_bnd_flow<MF><<<dim3(d.flow, n), flow_threads<MF>(), 0, s>>>(bc, info, pass);
         kb_bnd_apply<MF><<<dim3(d.apply, n), 256, 0, s>>>(bc, info, pass);
     }

@@ -100,6 +100,17 @@ struct DevState {
     unsigned long long* nIcTotal;  // [1] running sum of mixed-cell counts
 };
 
+// The other half of the bounding double buffers (cand, active, corr, corrPos,
+// actIdx): what the previous pass left there is cleared by the next pass's
+// candidate-list kernel.
+struct BndPrev {
+    const int* cand;
+    const unsigned char* active;
+    double* corr;
+    int* corrPos;
+    int* actIdx;
+};
+
 
 struct Params {
     double mixedCellTol;
@@ -173,7 +184,7 @@ struct BatchStepInfo {
 
 // grid x dimension of each batched stage (the y dimension is the case count)
 struct BatchDims {
-    int ls, fsd, fluxList, fluxFaces, prep, flow, apply;
+    int ls, fsd, fluxList, fluxFaces, flow, apply;
 };
 
 // launchers (simplic_kernels.hip)
@@ -245,6 +256,9 @@ void launch_rho_phi(int nF, const double* ap, const double* phi, double rho1, do
                     double* out, int mode, hipStream_t s);
 int detect_blocks(int nC);
 int detect_list_blocks(int nC);   // blocks of the one-launch detection
+// the state of bounding pass number seq of a handle: the half (seq & 1) of
+// the double-buffered bounding arrays
+DevState bound_pass_view(const DevMesh& m, const DevState& st, int seq);
 // batched stages (same order and kernels as one handle's reconstruct and
 // advect); G is the plane-placement lane group and MF the bounding record
 // size shared by all cases

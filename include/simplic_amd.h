@@ -265,11 +265,13 @@ typedef struct svp_batch svp_batch;
 int svp_batch_create(svp_handle* const* cases, int32_t n_cases, svp_batch** out);
 int svp_batch_destroy(svp_batch* b);
 /* One step of every case k with dt[k] > 0 (host array of n_cases); a case
- * with dt[k] <= 0 is left as it is (a finished case). */
+ * with dt[k] <= 0 is left as it is (a finished case).  The step's launches
+ * are captured once as a graph and replayed (the time steps travel in device
+ * memory); environment SVP_BATCH_GRAPH=0 launches them one by one. */
 int svp_batch_step(svp_batch* b, const double* dt);
 /* Launch shapes: the x extent per case of the stages whose kernels stride
  * over their work (least-squares normal, plane placement, flux list,
- * interface flux, bounding records, dataflow pass, apply), shapes[7] in that
+ * interface flux, dataflow bounding pass, bounding apply), shapes[6] in that
  * order.  svp_batch_tune times each over a few values on one step from the
  * current fields (the fields are restored after every trial), keeps the
  * fastest and writes them to shapes (may be NULL); results do not depend on

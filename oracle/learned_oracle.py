@@ -17,9 +17,11 @@ reference, so these functions restate this library's own design:
   FP64 (``rounded=False``).  The FP64 backward is pinned by central finite
   differences.
 * ``cube_plane_offset`` / ``residual``: the plane of a normal that holds a
-  cube's own alpha, and its mean |fraction - alpha| over the 26 lattice
-  neighbours.  The offset is pinned against the oracle's findSignedDistance
-  on unit cubes.
+  cube's own alpha (the inverse of the cube fraction: closed forms of
+  Scardovelli & Zaleski on the outer pieces, bracketed Newton on the cubic
+  ones), and its mean |fraction - alpha| over the 26 lattice neighbours.
+  The offset is pinned against the oracle's findSignedDistance on unit
+  cubes and by the round trip through ``plane_cube_fraction``.
 """
 from __future__ import annotations
 
@@ -227,23 +229,90 @@ def mlp_loss_grad(params, x, y, rounded=True):
 # ---------------------------------------------------------------------------
 # residual of a plane over the lattice neighbours
 # ---------------------------------------------------------------------------
-def cube_plane_offset(n, a, lo, h, rounds=16):
+def _area_sorted(m1, m2, m3, a):
+    """d _cf_sorted / da (the cut area), piece by piece."""
+    m12 = m1 + m2
+    if a <= 0.0:
+        return 0.0
+    if a < m1:
+        return a * a / (2.0 * m1 * m2 * m3)
+    base = 6.0 * a - 3.0 * m1
+    if a < m2:
+        return base / (6.0 * m2 * m3)
+    if a < m12 and a < m3:
+        b = a - m2
+        return (base - 3.0 * b * b / m1) / (6.0 * m2 * m3)
+    if m3 < m12:
+        b, c = a - m2, a - m3
+        return (base - 3.0 * (b * b + c * c) / m1) / (6.0 * m2 * m3)
+    return 1.0 / m3
+
+
+def _level_sorted(m1, m2, m3, V):
+    """Inverse of _cf_sorted on [0, 1/2]: closed forms on the first two pieces
+    and the linear one (Scardovelli & Zaleski 2000), Newton's method inside
+    the bracket of a cubic piece, bisecting when a step leaves it."""
+    if V <= 0.0:
+        return 0.0
+    m12 = m1 + m2
+    V1 = _cf_sorted(m1, m2, m3, m1)
+    if V < V1:
+        return math.pow(6.0 * m1 * m2 * m3 * V, 1.0 / 3.0)
+    V2 = _cf_sorted(m1, m2, m3, m2)
+    if V < V2:
+        return 0.5 * m1 + math.sqrt(max(2.0 * m2 * m3 * V - m1 * m1 / 12.0, 0.0))
+    if m12 <= m3:
+        V3 = _cf_sorted(m1, m2, m3, m12)
+        if V >= V3:
+            return m3 * V + 0.5 * m12
+        lo, hi, Vlo, Vhi = m2, m12, V2, V3
+    else:
+        V3 = _cf_sorted(m1, m2, m3, m3)
+        if V < V3:
+            lo, hi, Vlo, Vhi = m2, m3, V2, V3
+        else:
+            lo, hi, Vlo, Vhi = m3, 0.5, V3, _cf_sorted(m1, m2, m3, 0.5)
+    a = lo + (hi - lo) * ((V - Vlo) / (Vhi - Vlo)) if Vhi > Vlo else lo
+    for _ in range(12):
+        f = _cf_sorted(m1, m2, m3, a) - V
+        if f == 0.0:
+            break
+        if f > 0.0:
+            hi = a
+        else:
+            lo = a
+        fp = _area_sorted(m1, m2, m3, a)
+        an = a - f / fp if fp > 0.0 else 0.5 * (lo + hi)
+        if abs(an - a) <= 2e-16:
+            a = an
+            break
+        if not (lo < an < hi):
+            an = 0.5 * (lo + hi)
+        a = an
+    return a
+
+
+def cube_plane_offset(n, a, lo, h):
     """Plane constant D of the cube [lo, lo+h]^3 with liquid fraction a on the
-    side n.x + D < 0, by the inference kernel's multisection: per round 4
-    interior points of the bracket, which shrinks to the interval of the
-    crossing."""
-    base = -(n[0] * lo[0] + n[1] * lo[1] + n[2] * lo[2])
-    sx, sy, sz = n[0] * h, n[1] * h, n[2] * h
-    tmin = min(sx, 0.0) + min(sy, 0.0) + min(sz, 0.0)
-    tmax = max(sx, 0.0) + max(sy, 0.0) + max(sz, 0.0)
-    dlo, dhi = base - tmax, base - tmin
-    for _ in range(rounds):
-        w = (dhi - dlo) / 5.0
-        k = sum(1 for q in range(4) if plane_cube_fraction(n, dlo + w * (q + 1), lo, h) > a)
-        new_lo = dlo + w * k if k > 0 else dlo
-        new_hi = dlo + w * (k + 1) if k < 4 else dhi
-        dlo, dhi = new_lo, new_hi
-    return 0.5 * (dlo + dhi)
+    side n.x + D < 0: the inverse of plane_cube_fraction (reflect the axes
+    with n < 0, normalise and sort the coefficients, invert the sorted cube
+    fraction, undo the reflection)."""
+    m = [n[0] * h, n[1] * h, n[2] * h]
+    neg = [mk if mk < 0.0 else 0.0 for mk in m]
+    m = [abs(mk) for mk in m]
+    s = m[0] + m[1] + m[2]
+    if not s > 0.0 or a <= 0.0:
+        lev = 0.0
+    elif a >= 1.0:
+        lev = 1.0
+    else:
+        sm = sorted(mk / s for mk in m)
+        lev = _level_sorted(*sm, a) if a <= 0.5 else 1.0 - _level_sorted(*sm, 1.0 - a)
+    rhs = lev * s
+    for k in range(3):
+        if neg[k] < 0.0:
+            rhs += neg[k]
+    return -rhs - (n[0] * lo[0] + n[1] * lo[1] + n[2] * lo[2])
 
 
 def residual(n, d, lo, h, alpha27):

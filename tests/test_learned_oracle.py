@@ -163,9 +163,9 @@ def test_residual_of_exact_plane_is_zero():
 
 
 def test_cube_plane_offset_matches_find_signed_distance(unit_cube):
-    """The multisection plane of a cube at a given alpha (16 rounds, bracket
-    width 5^-16 of the vertex range) is the plane the oracle's
-    findSignedDistance places in the same cube (cutCell.C:611-799)."""
+    """The plane of a cube at a given alpha (inverse of the cube fraction) is
+    the plane the oracle's findSignedDistance places in the same cube
+    (cutCell.C:611-799)."""
     rng = np.random.default_rng(8)
     for _ in range(100):
         n = rng.normal(size=3)
@@ -175,3 +175,40 @@ def test_cube_plane_offset_matches_find_signed_distance(unit_cube):
         assert abs(LO.plane_cube_fraction(n, d, (0.0, 0.0, 0.0), 1.0) - a) < 1e-9
         st, out = unit_cube.find_signed_distance(0, a, n)
         assert st == 0 and abs(out[0] - d) < 1e-10, (out[0], d)
+
+
+def test_cube_plane_offset_round_trip():
+    """Every piece of the inverse (both closed forms, the linear piece, both
+    cubic pieces) and the degenerate normals (one or two zero components,
+    equal components): the fraction of the returned plane is alpha."""
+    rng = np.random.default_rng(12)
+    pieces = set()
+    for t in range(3000):
+        n = rng.normal(size=3)
+        if t % 5 == 1:
+            n[t % 3] = 0.0
+        if t % 5 == 2:
+            n[:] = 0.0
+            n[t % 3] = 1.0
+        if t % 5 == 3:
+            n[:] = rng.choice([-1.0, 1.0], 3)
+        n /= np.linalg.norm(n)
+        h = rng.uniform(0.1, 2.0)
+        lo = rng.uniform(-1, 1, 3)
+        a = rng.uniform(0.0, 1.0) if t % 7 else rng.choice([1e-9, 0.5, 1.0 - 1e-9])
+        d = LO.cube_plane_offset(n, a, lo, h)
+        assert abs(LO.plane_cube_fraction(n, d, lo, h) - a) < 1e-13, (n, a)
+        m = sorted(abs(n) / np.abs(n).sum())
+        v = min(a, 1.0 - a)
+        lev = LO._level_sorted(*m, v)
+        if lev < m[0]:
+            pieces.add("cube root")
+        elif lev < m[1]:
+            pieces.add("square root")
+        elif lev < min(m[0] + m[1], m[2]):
+            pieces.add("first cubic")
+        elif m[2] < m[0] + m[1]:
+            pieces.add("second cubic")
+        else:
+            pieces.add("linear")
+    assert pieces == {"cube root", "square root", "first cubic", "second cubic", "linear"}, pieces
