@@ -60,7 +60,7 @@ LEARNED_WORKLOAD = dict(kind="hex", N=128, perturb=0.0, seed=0, shapes_seed=2024
 # finer plain meshes of the learned-normal case, to find the plain path's
 # cost at the learned path's error
 EQUAL_ERROR_MESHES = (144, 160)
-LEARNED_TRAINING = dict(samples=2 << 20, depth=4, epochs=20, batch=4096, lr0=3e-3, lr1=3e-5, seed=11)
+LEARNED_TRAINING = dict(samples=4 << 20, depth=4, epochs=20, batch=4096, lr0=3e-3, lr1=3e-5, seed=11)
 
 
 def algorithmic_bytes(stage, n_cells, n_faces, n_ic, cell_kind):

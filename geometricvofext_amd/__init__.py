@@ -471,7 +471,8 @@ class CaseBatch:
     its own time step; every case's results equal its handle stepping alone.
     Close the batch before using the handles on their own again."""
 
-    SHAPE_NAMES = ("normal", "findSignedDistance", "fluxList", "interfaceFlux", "boundFlow", "boundApply")
+    SHAPE_NAMES = ("normal", "findSignedDistance", "fluxList", "interfaceFlux", "boundRecords", "boundFlow",
+                   "boundApply")
 
     def __init__(self, solvers):
         self.solvers = list(solvers)

@@ -140,7 +140,10 @@ __device__ inline double shape_gq(V3 q) { return q.x * q.x + q.y * q.y + q.z * q
 __device__ inline double shape_g(const Shape& s, V3 x) { return shape_gq(shape_q(s, x)); }
 
 // volume fraction of {f < 0} for f linear on a tet with vertex values v[4]:
-// CDF of a linear function over a simplex, sum_i (-v_i)_+^3 / prod_j (v_j - v_i)
+// CDF of a linear function over a simplex, sum_i (-v_i)_+^3 / prod_j (v_j - v_i).
+// Summed over all four vertices the terms give 1 (the third divided
+// difference of (-x)^3), so with three negative values the fraction is one
+// minus the positive vertex's term.
 __device__ inline double linear_tet_fraction(double v0, double v1, double v2, double v3) {
     double v[4] = {v0, v1, v2, v3};
     int nneg = 0;
@@ -165,7 +168,7 @@ __device__ inline double linear_tet_fraction(double v0, double v1, double v2, do
         const double t = -v[i];
         sum += (t * t * t) / den;
     }
-    const double frac = useNeg ? sum : 1.0 + sum;
+    const double frac = useNeg ? sum : 1.0 - sum;
     return fmin(1.0, fmax(0.0, frac));
 }
 

@@ -94,6 +94,10 @@ public:
     solveVofEqu(const svp_mesh& mesh, const svp_dict& dict, int device) : dict_(dict), device_(device) {
         HIPCHECK(hipSetDevice(device));
         HIPCHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+        HIPCHECK(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking));
+        HIPCHECK(hipEventCreateWithFlags(&reconStart_, hipEventDisableTiming));
+        HIPCHECK(hipEventCreateWithFlags(&upwindDone_, hipEventDisableTiming));
+        if (const char* e = std::getenv("SVP_OVERLAP_UPWIND")) overlapUpwind_ = std::atoi(e) != 0;
         params_.mixedCellTol = dict.mixed_cell_tol;
         params_.nAlphaBounds = dict.n_alpha_bounds;
         params_.snapTol = dict.snap_tol;
@@ -113,6 +117,16 @@ public:
         allocState();
         launch_geometry(m_, stream_);
         HIPCHECK(hipGetLastError());
+        // face flux with four lanes per face where most faces are warped
+        // (fan-integrated); SVP_FLUX_FAN=0/1 forces it off/on
+        if (m_.triQuadFaces && st_.un0Fused) {
+            int nWarped = 0;
+            launch_count_warped(m_, nonfinite_, stream_);
+            HIPCHECK(hipMemcpyAsync(&nWarped, nonfinite_, sizeof(int), hipMemcpyDeviceToHost, stream_));
+            HIPCHECK(hipStreamSynchronize(stream_));
+            st_.fluxFan = 2L * nWarped > (long)m_.nF ? 1 : 0;
+            if (const char* e = std::getenv("SVP_FLUX_FAN")) st_.fluxFan = std::atoi(e) != 0 ? 1 : 0;
+        }
         HIPCHECK(hipStreamSynchronize(stream_));
     }
 
@@ -123,6 +137,10 @@ public:
             (void)hipEventDestroy(e.first);
             (void)hipEventDestroy(e.second);
         }
+        if (side_) (void)hipStreamSynchronize(side_);
+        if (reconStart_) (void)hipEventDestroy(reconStart_);
+        if (upwindDone_) (void)hipEventDestroy(upwindDone_);
+        if (side_) (void)hipStreamDestroy(side_);
         if (stream_) (void)hipStreamDestroy(stream_);
     }
 
@@ -130,6 +148,8 @@ public:
     void reconstruct() {
         hipEvent_t a = event(), b = event();
         HIPCHECK(hipEventRecord(a, stream_));
+        HIPCHECK(hipEventRecord(reconStart_, stream_));
+        reconCall_ = calls_;
         stage(STAGE_DETECT, [&] { launch_detect(m_, st_, params_, stream_); });
         if (learned_.active) {
             reconstructLearned();
@@ -152,10 +172,25 @@ public:
     void advect(double dt, const double* Sp, const double* Su) {
         if (!(dt > 0.0) || !std::isfinite(dt)) throw SvpError(SVP_E_ARG, "advect: dt must be positive");
         if (!reconstructed_) reconstruct();
+        // The upwind face volumes read only alpha.oldTime() and phi, which
+        // the reconstruct does not change, and nothing in the reconstruct
+        // reads what the upwind kernel writes (dVf, the flux and bound
+        // counters).  When this advect directly follows its reconstruct (no
+        // call on the handle in between that could change or read those; a
+        // reconstruct inside the same call, as in the sub-cycles, counts),
+        // the upwind kernel runs on a second stream from the reconstruct's
+        // start, beside the plane placement, and the flux stage waits for it.
+        const bool overlap = overlapUpwind_ && profMask_ == 0 && calls_ - reconCall_ <= 1;
+        std::swap(st_.alpha, st_.alphaOld);
+        if (overlap) {
+            HIPCHECK(hipStreamWaitEvent(side_, reconStart_, 0));
+            launch_advect_stage(STAGE_UPWIND, m_, st_, params_, dt, Sp, Su, &bndPassSeq_, side_);
+            HIPCHECK(hipEventRecord(upwindDone_, side_));
+        }
         hipEvent_t a = event(), b = event();
         HIPCHECK(hipEventRecord(a, stream_));
-        std::swap(st_.alpha, st_.alphaOld);
-        for (int sg = STAGE_UPWIND; sg <= STAGE_SNAPCLIP; ++sg)
+        if (overlap) HIPCHECK(hipStreamWaitEvent(stream_, upwindDone_, 0));
+        for (int sg = overlap ? STAGE_UPWIND + 1 : STAGE_UPWIND; sg <= STAGE_SNAPCLIP; ++sg)
             stage(sg, [&] { launch_advect_stage(sg, m_, st_, params_, dt, Sp, Su, &bndPassSeq_, stream_); });
         HIPCHECK(hipEventRecord(b, stream_));
         HIPCHECK(hipGetLastError());
@@ -624,6 +659,11 @@ public:
     int device_ = 0;
     int nB_ = 0;
     hipStream_t stream_ = nullptr;
+    // second stream of the upwind kernel, and the events that order it
+    hipStream_t side_ = nullptr;
+    hipEvent_t reconStart_ = nullptr, upwindDone_ = nullptr;
+    bool overlapUpwind_ = true;
+    long calls_ = 0, reconCall_ = -10;
     double lastDt_ = 0.0;
     // harness base flow
     double* U0_ = nullptr;
@@ -944,7 +984,7 @@ private:
         st_.actIdx = dalloc<int>(2L * nC);
         HIPCHECK(hipMemsetAsync(st_.actIdx, 0xFF, sizeof(int) * 2 * nC, stream_));
         // correctedFaces order keys are cell * (max faces) + rank in 31 bits
-        if ((long long)nC * (maxCellFaces_ <= NS_CELL_FACES ? NS_CELL_FACES : MAX_CELL_FACES) >= 0x7FFFFFFFll)
+        if ((long long)nC * bound_faces(maxCellFaces_) >= 0x7FFFFFFFll)
             throw SvpError(SVP_E_UNSUPPORTED, "mesh too large for 31-bit bounding order keys");
         st_.corrPos = dalloc<int>(2L * nF);
         HIPCHECK(hipMemsetAsync(st_.corrPos, 0xFF, sizeof(int) * 2 * nF, stream_));
@@ -987,8 +1027,7 @@ private:
             const int g = std::atoi(e);
             if (g >= 0 && g <= 4096) st_.bndFlow = g;
         }
-        st_.bndRec = maxCellFaces_ <= NS_CELL_FACES ? (void*)dalloc<BndRec<NS_CELL_FACES>>(nC)
-                                                    : (void*)dalloc<BndRec<MAX_CELL_FACES>>(nC);
+        st_.bndRec = (void*)dalloc<unsigned char>(bound_record_bytes(maxCellFaces_) * (size_t)nC);
         st_.un0Fused = (!dict_.split_warped_face && maxCellFaces_ <= 8 && m_.maxCellPoints <= NS_CELL_POINTS &&
                         m_.triQuadFaces) ? 1 : 0;
         st_.fsdKeep = rdfNormal_ ? 1 : 0;
@@ -1030,7 +1069,7 @@ public:
         HIPCHECK(hipSetDevice(device_));
         params_ = f.params_;
         G_ = f.batchGroup();
-        MF_ = f.st_.maxCellFaces <= NS_CELL_FACES ? NS_CELL_FACES : MAX_CELL_FACES;
+        MF_ = bound_faces(f.st_.maxCellFaces);
         triQuad_ = f.m_.triQuadFaces;
         for (solveVofEqu* h : c_) {
             std::string why;
@@ -1040,7 +1079,7 @@ public:
             if (q.mixedCellTol != params_.mixedCellTol || q.nAlphaBounds != params_.nAlphaBounds ||
                 q.snapTol != params_.snapTol || q.clip != params_.clip)
                 throw SvpError(SVP_E_ARG, "svp_batch_create: the cases' alpha dictionaries differ");
-            if (h->batchGroup() != G_ || (h->st_.maxCellFaces <= NS_CELL_FACES ? NS_CELL_FACES : MAX_CELL_FACES) != MF_ ||
+            if (h->batchGroup() != G_ || bound_faces(h->st_.maxCellFaces) != MF_ ||
                 h->m_.triQuadFaces != triQuad_)
                 throw SvpError(SVP_E_ARG, "svp_batch_create: the cases' cell types differ (hexahedra and "
                                           "tetrahedra go to separate batches)");
@@ -1094,6 +1133,7 @@ public:
         dims_.fsd = 64;
         dims_.fluxList = 8;
         dims_.fluxFaces = 64;
+        dims_.prep = 8;
         dims_.flow = 4;
         dims_.apply = 8;
         if (const char* e = std::getenv("SVP_BATCH_GRAPH")) useGraph_ = std::atoi(e) != 0;
@@ -1272,6 +1312,7 @@ public:
                                    {STAGE_IFLUX, &dims_.fluxFaces, {16, 32, 64, 128, 256}},
                                    {STAGE_IFLUX, &dims_.fluxList, {2, 4, 8, 16, 32}},
                                    {STAGE_BOUND, &dims_.flow, {1, 2, 4, 8}},
+                                   {STAGE_BOUND, &dims_.prep, {2, 4, 8, 16, 32}},
                                    {STAGE_BOUND, &dims_.apply, {2, 4, 8, 16, 32}}};
         trial(STAGE_DETECT);  // warm-up
         for (Knob& kb : knobs) {
@@ -1295,22 +1336,24 @@ public:
             chosen[1] = dims_.fsd;
             chosen[2] = dims_.fluxList;
             chosen[3] = dims_.fluxFaces;
-            chosen[4] = dims_.flow;
-            chosen[5] = dims_.apply;
+            chosen[4] = dims_.prep;
+            chosen[5] = dims_.flow;
+            chosen[6] = dims_.apply;
         }
     }
 
     void setDims(const int32_t* d) {
         if (!d) throw SvpError(SVP_E_ARG, "svp_batch_set_shapes: null");
-        for (int k = 0; k < 6; ++k)
+        for (int k = 0; k < 7; ++k)
             if (d[k] < 1 || d[k] > 65535) throw SvpError(SVP_E_ARG, "svp_batch_set_shapes: 1..65535 per stage");
         dropGraph();
         dims_.ls = d[0];
         dims_.fsd = d[1];
         dims_.fluxList = d[2];
         dims_.fluxFaces = d[3];
-        dims_.flow = d[4];
-        dims_.apply = d[5];
+        dims_.prep = d[4];
+        dims_.flow = d[5];
+        dims_.apply = d[6];
     }
 
     int device_ = 0;
@@ -1415,9 +1458,12 @@ static caseBatch& B(svp_batch* b) {
     return *b->b;
 }
 
+// every entry point that takes a handle passes here once: the call count
+// tells advect whether anything ran between it and its reconstruct
 static solveVofEqu& H(svp_handle* h) {
     if (!h || !h->s) throw SvpError(SVP_E_ARG, "null handle");
     HIPCHECK(hipSetDevice(h->s->device_));
+    h->s->calls_++;
     return *h->s;
 }
 

@@ -1827,6 +1827,119 @@ __device__ __attribute__((always_inline)) inline double tiff_regs(const DevMesh&
     return dVf;
 }
 
+// timeIntegratedFlux of the listed (cut cell, downwind face) pairs with four
+// lanes per face (faces of 3 or 4 points).  On a warped face, which the
+// reference integrates as the fan of triangles about the face centre
+// (cutFace.C:316-345 with a time step, 369-386 without), lane j integrates
+// triangle j (points j and j+1) and the group's first lane adds the
+// triangle volumes in fan order, the reference's sum; a flat face (one
+// timeIntegratedArea of the whole face) runs on the first lane alone.  Every
+// triangle's volume is the expression tiff_regs evaluates, so the face
+// volumes are identical to k_interface_flux_faces.
+constexpr int FAN_LANES = 4;
+
+__device__ __attribute__((always_inline)) inline void k_flux_fan_body(const DevMesh& m, const DevState& st, double dtStep, int slots) {
+    const long nWork = (long)*st.fluxCount;
+    const int lane = threadIdx.x & 63, j = lane & (FAN_LANES - 1), base = lane & ~(FAN_LANES - 1);
+    constexpr int PER_WAVE = 64 / FAN_LANES;
+    // one wave per workgroup; the loop bound is uniform across the wave, so
+    // every lane reaches the shuffles
+    for (long w0 = (long)blockIdx.x * PER_WAVE; w0 < nWork; w0 += (long)gridDim.x * PER_WAVE) {
+        const long wi = w0 + lane / FAN_LANES;
+        const bool live = wi < nWork;
+        int f = 0, i = 0, fsz = 3, s0 = 0;
+        double ph = 0.0;
+        if (live) {
+            const int w = st.fluxList[wi];
+            i = w / slots;
+            const int k = w % slots;
+            f = m.cFaces[m.cOff[st.mixed[i]] + k];
+            ph = st.phi[f];
+            s0 = m.fOff[f];
+            fsz = m.fOff[f + 1] - s0;
+        }
+        const double dt = dtStep;
+        const bool zero = !live || fabs(ph) <= SVP_TSMALL;
+        const bool flat = live && m.flat[f] > (1.0 - SVP_TSMALL);
+        const bool fan = !zero && !flat;
+        double term = 0.0;
+        bool ok = true;
+        if (!zero) {
+            const V3 n = ld3(st.ifN, i);
+            const double d = st.ifD[i], Un0 = st.un0[i];
+            const double magSf = m.magSf[f];
+            if (fan && j < fsz) {
+                const V3 fc = ld3(m.Cf, f);
+                const V3 pa = ld3(m.pts, m.fPts[s0 + j]);
+                const V3 pb = ld3(m.pts, m.fPts[s0 + (j + 1 == fsz ? 0 : j + 1)]);
+                const V3 tri[3] = {fc, pa, pb};
+                const double magSfTri = mag(triAreaNormal(tri[0], tri[1], tri[2]));
+                const double phiTri = ph * magSfTri / magSf;
+                if (fabs(Un0 * dt) > SVP_TSMALL) {
+                    auto arrival = [&](V3 q) {
+                        const double t = (dot(q, n) + d) / Un0;
+                        return fabs(t) < SVP_TSMALL ? 0.0 : t;
+                    };
+                    const double tt[3] = {arrival(fc), arrival(pa), arrival(pb)};
+                    term = phiTri / magSfTri * tia_regs<3>(tri, tt, n, d, Un0, dt, magSfTri, ok);
+                } else {
+                    FaceCut fcut;
+                    ok &= calcSubFaceN<3, false>(tri, n, d, fcut);
+                    const double alphafTri = mag(fcut.area) / magSfTri;
+                    term = (phiTri * dt * alphafTri);
+                }
+            } else if (!fan && j == 0) {
+                if (fsz == 4) {
+                    V3 p[4];
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) p[q] = ld3(m.pts, m.fPts[s0 + q]);
+                    term = tiff_regs<4>(m, f, p, n, d, Un0, dt, ph, magSf, ok);
+                } else {
+                    V3 p[3];
+#pragma unroll
+                    for (int q = 0; q < 3; ++q) p[q] = ld3(m.pts, m.fPts[s0 + q]);
+                    term = tiff_regs<3>(m, f, p, n, d, Un0, dt, ph, magSf, ok);
+                }
+            }
+        }
+        double v = term;
+        if (fan) {
+            v = 0.0;
+#pragma unroll
+            for (int t = 0; t < FAN_LANES; ++t) {
+                const double x = __shfl(term, base + t);
+                if (t < fsz) v += x;
+            }
+        } else {
+#pragma unroll
+            for (int t = 0; t < FAN_LANES; ++t) (void)__shfl(term, base + t);
+        }
+        if (live && j == 0) st.dVf[f] = zero ? 0.0 : v;
+        if (!ok) atomicOr(st.errFlag, ERR_TOO_MANY_CROSSINGS);
+    }
+}
+
+__global__ void __launch_bounds__(64, SVP_FLUX_WAVES) k_flux_fan(DevMesh m, DevState st, double dtStep, int slots) {
+    k_flux_fan_body(m, st, dtStep, slots);
+}
+
+// faces whose flatness is below 1 - tSmall (the fan-integrated ones)
+__global__ void __launch_bounds__(256) k_count_warped(DevMesh m, int* count) {
+    const int f = blockIdx.x * 256 + threadIdx.x;
+    const bool w = f < m.nF && !(m.flat[f] > (1.0 - SVP_TSMALL));
+    const unsigned long long b = __ballot(w);
+    if ((threadIdx.x & 63) == 0 && b) atomicAdd(count, __popcll(b));
+}
+
+__global__ void k_zero_one(int* p) {
+    if (threadIdx.x == 0) *p = 0;
+}
+
+void launch_count_warped(const DevMesh& m, int* count, hipStream_t s) {
+    k_zero_one<<<1, 64, 0, s>>>(count);
+    k_count_warped<<<cdiv(m.nF, 256), 256, 0, s>>>(m, count);
+}
+
 // Un0 = U(interface centre) . n of every cut cell (advection.C:126)
 __global__ void __launch_bounds__(64) k_un0(DevMesh m, DevState st) {
     const int nMixed = *st.nMixed;
@@ -2018,14 +2131,13 @@ __device__ inline bool face_downwind_of(const DevMesh& m, const double* phi, int
 // limitFlux (advectionTemplates.C:118-215), one pass = the kernels below.
 // The pass runs only while cells lie outside [-aTol, 1+aTol] (bound count of
 // the update kernel, kept current by the apply kernel):
-//   list              ascending candidates from the bitmap (one launch);
-//                     each block then runs its candidates' first
-//                     redistribution loop, which decides whether a cell
-//                     writes corrections and which faces open
-//                     correctedFaces, and writes the record of each writing
-//                     cell; last, the clearing of what the previous pass left
-//                     in the other half of the double buffers
+//   list              ascending candidates from the bitmap (one launch),
+//                     then the clearing of what the previous pass left in
+//                     the other half of the double buffers
 //                     (dVfCorrectionValues = 0 for its active cells)
+//   prep              every candidate's first redistribution loop, which
+//                     decides whether it writes corrections and which faces
+//                     open correctedFaces; the record of each writing cell
 //   flow              all remaining boundFlux work in dependency order in
 //                     one launch, and the correctedFaces order keys and
 //                     touched cells (the default), or
@@ -2034,7 +2146,7 @@ __device__ inline bool face_downwind_of(const DevMesh& m, const double* phi, int
 //   apply             corrections applied per cell in correctedFaces order,
 //                     dVf of the corrected faces, bitmap and bound count,
 //                     pass statistics
-// bcount: [0] candidates, [1] corrected faces (list), [2] touched cells,
+// bcount: [0] candidates, [1] corrected faces (prep), [2] touched cells,
 // [3+r] frontier size of round r (r <= st.gridRounds; [3] = active cells of
 // the dataflow pass), [BND_CTRL-1] rounds, then the dataflow ticket and
 // completion count.
@@ -2258,10 +2370,8 @@ __device__ inline bool oob_any(const DevState& st) {
     return n != 0;
 }
 
-template <int MF>
 __device__ __attribute__((always_inline)) inline void k_bnd_list_body(const DevMesh& m, const DevState& st, int nWords,
-                                                                      int nBlocks, const BndPrev& prev, double dt,
-                                                                      const double* Sp, const double* Su) {
+                                                                      int nBlocks, const BndPrev& prev) {
     __shared__ int lds[16];
     __shared__ int sTicket, sBase, sTotal;
     __shared__ unsigned sEpoch;
@@ -2337,11 +2447,24 @@ __device__ __attribute__((always_inline)) inline void k_bnd_list_body(const DevM
         base += tot;
         __syncthreads();
     }
-    // every candidate of the block gets its record (iteration 0 of boundFlux
-    // done) and, when it writes corrections, its candidate index and the
-    // CORR_PENDING marks of the faces it will write
+    clearPrevious();
+}
+
+__global__ void __launch_bounds__(256) k_bnd_list(DevMesh m, DevState st, int nWords, int nBlocks, BndPrev prev) {
+    k_bnd_list_body(m, st, nWords, nBlocks, prev);
+}
+
+// every candidate's record (iteration 0 of boundFlux done) and, when it
+// writes corrections, its candidate index and the CORR_PENDING marks of the
+// faces it will write; one candidate per thread over the whole grid (the
+// candidates cluster in a few list blocks, so the list kernel's blocks would
+// build them unevenly)
+template <int MF>
+__device__ __attribute__((always_inline)) inline void k_bnd_prep_body(const DevMesh& m, const DevState& st, double dt,
+                                                                      const double* Sp, const double* Su) {
+    const int nCand = st.bcount[0];
     int nOpen = 0;
-    for (int i = sBase + threadIdx.x; i < sBase + sTotal; i += blockDim.x) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nCand; i += gridDim.x * blockDim.x) {
         const int c = st.cand[i];
         const int n = bound_record<MF>(m, st, c, dt, Sp, Su, ((BndRec<MF>*)st.bndRec)[i]);
         st.active[i] = n > 0 ? 1 : 0;
@@ -2351,13 +2474,11 @@ __device__ __attribute__((always_inline)) inline void k_bnd_list_body(const DevM
     // correctedFaces length of the pass (statistics)
     for (int o = 32; o > 0; o >>= 1) nOpen += __shfl_xor(nOpen, o);
     if ((threadIdx.x & 63) == 0 && nOpen) atomicAdd(&st.bcount[1], nOpen);
-    clearPrevious();
 }
 
 template <int MF>
-__global__ void __launch_bounds__(256) k_bnd_list(DevMesh m, DevState st, int nWords, int nBlocks, BndPrev prev,
-                                                  double dt, const double* Sp, const double* Su) {
-    k_bnd_list_body<MF>(m, st, nWords, nBlocks, prev, dt, Sp, Su);
+__global__ void __launch_bounds__(256) k_bnd_prep(DevMesh m, DevState st, double dt, const double* Sp, const double* Su) {
+    k_bnd_prep_body<MF>(m, st, dt, Sp, Su);
 }
 
 
@@ -2881,7 +3002,8 @@ void launch_bound_pass(const DevMesh& m, const DevState& st, int pass, double dt
     // one thread per cell of the mesh
     const int grid = std::max(1, std::min(st.bndGrid, cdiv(m.nC, 256)));
     const int nbw = cdiv(nWords, st.bndListWords);
-    k_bnd_list<MF><<<nbw, 256, 0, s>>>(m, st, nWords, nbw, prev, dt, Sp, Su);
+    k_bnd_list<<<nbw, 256, 0, s>>>(m, st, nWords, nbw, prev);
+    k_bnd_prep<MF><<<grid, 256, 0, s>>>(m, st, dt, Sp, Su);
     if (st.bndFlow) {
         k_bnd_flow<MF><<<std::max(1, std::min(st.bndFlow, cdiv(m.nC, flow_chunk<MF>()))), flow_threads<MF>(), 0, s>>>(
             m, st, dt);
@@ -2912,6 +3034,22 @@ BndPrev bound_prev(const DevState& other) {
     return BndPrev{other.cand, other.active, other.corr, other.corrPos, other.actIdx};
 }
 
+int bound_faces(int maxCellFaces) {
+    if (maxCellFaces <= 4) return 4;
+    if (maxCellFaces <= 6) return 6;
+    if (maxCellFaces <= NS_CELL_FACES) return NS_CELL_FACES;
+    return MAX_CELL_FACES;
+}
+
+size_t bound_record_bytes(int maxCellFaces) {
+    switch (bound_faces(maxCellFaces)) {
+        case 4: return sizeof(BndRec<4>);
+        case 6: return sizeof(BndRec<6>);
+        case NS_CELL_FACES: return sizeof(BndRec<NS_CELL_FACES>);
+        default: return sizeof(BndRec<MAX_CELL_FACES>);
+    }
+}
+
 void launch_bound(const DevMesh& m, const DevState& st, const Params& p, double dt, const double* Sp,
                   const double* Su, int* passSeq, hipStream_t s) {
     const int nPasses = std::min(p.nAlphaBounds, MAX_BOUND_PASSES);
@@ -2919,8 +3057,12 @@ void launch_bound(const DevMesh& m, const DevState& st, const Params& p, double 
         const int seq = (*passSeq)++;
         const DevState sp = bound_pass_view(m, st, seq);
         const BndPrev prev = bound_prev(bound_pass_view(m, st, seq + 1));
-        if (st.maxCellFaces <= NS_CELL_FACES) launch_bound_pass<NS_CELL_FACES>(m, sp, pass, dt, Sp, Su, prev, s);
-        else launch_bound_pass<MAX_CELL_FACES>(m, sp, pass, dt, Sp, Su, prev, s);
+        switch (bound_faces(st.maxCellFaces)) {
+            case 4: launch_bound_pass<4>(m, sp, pass, dt, Sp, Su, prev, s); break;
+            case 6: launch_bound_pass<6>(m, sp, pass, dt, Sp, Su, prev, s); break;
+            case NS_CELL_FACES: launch_bound_pass<NS_CELL_FACES>(m, sp, pass, dt, Sp, Su, prev, s); break;
+            default: launch_bound_pass<MAX_CELL_FACES>(m, sp, pass, dt, Sp, Su, prev, s); break;
+        }
     }
 }
 
@@ -2952,7 +3094,8 @@ void launch_advect_stage(int stage, const DevMesh& m, const DevState& st, const 
             if (!st.un0Fused) k_un0<<<512, 64, 0, s>>>(m, st);
             if (st.maxCellFaces > 4) {
                 k_flux_list<<<1024, 256, 0, s>>>(m, st, st.maxCellFaces);
-                if (m.triQuadFaces) k_interface_flux_faces<true, true><<<2048, 64, 0, s>>>(m, st, dt, st.maxCellFaces);
+                if (m.triQuadFaces && st.fluxFan) k_flux_fan<<<4096, 64, 0, s>>>(m, st, dt, st.maxCellFaces);
+                else if (m.triQuadFaces) k_interface_flux_faces<true, true><<<2048, 64, 0, s>>>(m, st, dt, st.maxCellFaces);
                 else k_interface_flux_faces<false, true><<<2048, 64, 0, s>>>(m, st, dt, st.maxCellFaces);
             } else {
                 if (m.triQuadFaces) k_interface_flux_faces<true, false><<<2048, 64, 0, s>>>(m, st, dt, st.maxCellFaces);
@@ -3076,7 +3219,6 @@ __global__ void __launch_bounds__(UPDATE_BLOCK) kb_update(const BatchCase* __res
     k_update_body(c.m, batch_view(c, si, 1, 0), si.dt, nullptr, nullptr);
 }
 
-template <int MF>
 __global__ void __launch_bounds__(256) kb_bnd_list(const BatchCase* __restrict__ bc,
                                                    const BatchStepInfo* __restrict__ info, int pass) {
     const BatchCase& c = bc[blockIdx.y];
@@ -3084,7 +3226,16 @@ __global__ void __launch_bounds__(256) kb_bnd_list(const BatchCase* __restrict__
     if (!(si.dt > 0.0) || (int)blockIdx.x >= c.listBlocks) return;
     const DevState& o = batch_view(c, si, 1, pass + 1);
     const BndPrev prev{o.cand, o.active, o.corr, o.corrPos, o.actIdx};
-    k_bnd_list_body<MF>(c.m, batch_view(c, si, 1, pass), c.listWords, c.listBlocks, prev, si.dt, nullptr, nullptr);
+    k_bnd_list_body(c.m, batch_view(c, si, 1, pass), c.listWords, c.listBlocks, prev);
+}
+
+template <int MF>
+__global__ void __launch_bounds__(256) kb_bnd_prep(const BatchCase* __restrict__ bc,
+                                                   const BatchStepInfo* __restrict__ info, int pass) {
+    const BatchCase& c = bc[blockIdx.y];
+    const BatchStepInfo si = info[blockIdx.y];
+    if (!(si.dt > 0.0)) return;
+    k_bnd_prep_body<MF>(c.m, batch_view(c, si, 1, pass), si.dt, nullptr, nullptr);
 }
 
 template <int MF>
@@ -3120,7 +3271,8 @@ template <int MF>
 static void launch_batch_bound(const BatchCase* bc, const BatchStepInfo* info, int n, int maxListBlocks,
                                const BatchDims& d, int nPasses, hipStream_t s) {
     for (int pass = 0; pass < nPasses; ++pass) {
-        kb_bnd_list<MF><<<dim3(maxListBlocks, n), 256, 0, s>>>(bc, info, pass);
+        kb_bnd_list<<<dim3(maxListBlocks, n), 256, 0, s>>>(bc, info, pass);
+        kb_bnd_prep<MF><<<dim3(d.prep, n), 256, 0, s>>>(bc, info, pass);
         kb_bnd_flow<MF><<<dim3(d.flow, n), flow_threads<MF>(), 0, s>>>(bc, info, pass);
         kb_bnd_apply<MF><<<dim3(d.apply, n), 256, 0, s>>>(bc, info, pass);
     }
@@ -3159,10 +3311,12 @@ void launch_batch_stage(int stage, const BatchCase* bc, const BatchStepInfo* inf
             break;
         case STAGE_BOUND: {
             const int nPasses = std::min(p.nAlphaBounds, MAX_BOUND_PASSES);
-            if (MF <= NS_CELL_FACES)
-                launch_batch_bound<NS_CELL_FACES>(bc, info, n, maxListBlocks, d, nPasses, s);
-            else
-                launch_batch_bound<MAX_CELL_FACES>(bc, info, n, maxListBlocks, d, nPasses, s);
+            switch (MF) {
+                case 4: launch_batch_bound<4>(bc, info, n, maxListBlocks, d, nPasses, s); break;
+                case 6: launch_batch_bound<6>(bc, info, n, maxListBlocks, d, nPasses, s); break;
+                case NS_CELL_FACES: launch_batch_bound<NS_CELL_FACES>(bc, info, n, maxListBlocks, d, nPasses, s); break;
+                default: launch_batch_bound<MAX_CELL_FACES>(bc, info, n, maxListBlocks, d, nPasses, s); break;
+            }
             break;
         }
         case STAGE_SNAPCLIP:

@@ -88,6 +88,7 @@ struct DevState {
     unsigned long long* detTicket;  // detection block tickets (grows by the block count per launch)
     unsigned long long* bndTicket;  // candidate-list block tickets (the same)
     int un0Fused;       // plane placement by 8-lane groups, which also computes Un0
+    int fluxFan;        // face flux with four lanes per face (most faces warped)
     int fsdKeep;        // 1: where findSignedDistance writes no d/C/S they keep their values (isoRDF)
     int gridRounds;     // bounding rounds run as grid-wide kernels before the one-workgroup tail
     int bndGrid;        // workgroups of the grid-stride bounding kernels
@@ -184,7 +185,7 @@ struct BatchStepInfo {
 
 // grid x dimension of each batched stage (the y dimension is the case count)
 struct BatchDims {
-    int ls, fsd, fluxList, fluxFaces, flow, apply;
+    int ls, fsd, fluxList, fluxFaces, prep, flow, apply;
 };
 
 // launchers (simplic_kernels.hip)
@@ -256,6 +257,11 @@ void launch_rho_phi(int nF, const double* ap, const double* phi, double rho1, do
                     double* out, int mode, hipStream_t s);
 int detect_blocks(int nC);
 int detect_list_blocks(int nC);   // blocks of the one-launch detection
+// face slots of the bounding records and kernels for cells of at most
+// maxCellFaces faces (4, 6, 8 or 24), and the bytes of one record
+int bound_faces(int maxCellFaces);
+size_t bound_record_bytes(int maxCellFaces);
+void launch_count_warped(const DevMesh& m, int* count, hipStream_t s);  // faces below flatness 1 - tSmall
 // the state of bounding pass number seq of a handle: the half (seq & 1) of
 // the double-buffered bounding arrays
 DevState bound_pass_view(const DevMesh& m, const DevState& st, int seq);

@@ -271,8 +271,8 @@ int svp_batch_destroy(svp_batch* b);
 int svp_batch_step(svp_batch* b, const double* dt);
 /* Launch shapes: the x extent per case of the stages whose kernels stride
  * over their work (least-squares normal, plane placement, flux list,
- * interface flux, dataflow bounding pass, bounding apply), shapes[6] in that
- * order.  svp_batch_tune times each over a few values on one step from the
+ * interface flux, bounding records, dataflow bounding pass, bounding apply),
+ * shapes[7] in that order.  svp_batch_tune times each over a few values on one step from the
  * current fields (the fields are restored after every trial), keeps the
  * fastest and writes them to shapes (may be NULL); results do not depend on
  * the shapes.  svp_batch_set_shapes sets them directly. */
