@@ -113,7 +113,7 @@ STAGE_KERNELS = {
     "normal": ("k_ls_normal", "k_grad_normal", "k_rdf_"),
     "findSignedDistance": ("k_fsd_group", "k_find_signed_distance"),
     "upwind": ("k_upwind",),
-    "interfaceFlux": ("k_interface_flux", "k_un0", "k_flux_list"),
+    "interfaceFlux": ("k_interface_flux", "k_un0", "k_flux_list", "k_flux_fan"),
     "update": ("k_update",),
     "bounding": ("k_bnd_",),
     "snapClip": ("k_snap_clip",),
@@ -123,13 +123,22 @@ COPY_PROBE_BYTES = 256 << 20
 
 def copy_probe():
     """One device-to-device copy of COPY_PROBE_BYTES: the reference kernel
-    whose FETCH_SIZE / WRITE_SIZE readings calibrate the counters."""
-    import torch
-    a = torch.ones(COPY_PROBE_BYTES // 8, dtype=torch.float64, device="cuda")
-    b = torch.empty_like(a)
-    torch.cuda.synchronize()
-    b.copy_(a)
-    torch.cuda.synchronize()
+    whose FETCH_SIZE / WRITE_SIZE readings calibrate the counters (through
+    the HIP runtime directly, so the probe does not depend on torch)."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so.7")
+    hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    hip.hipMemset.argtypes = [C.c_void_p, C.c_int, C.c_size_t]
+    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    hip.hipFree.argtypes = [C.c_void_p]
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), COPY_PROBE_BYTES) == 0 and hip.hipMalloc(C.byref(b), COPY_PROBE_BYTES) == 0
+    assert hip.hipMemset(a, 1, COPY_PROBE_BYTES) == 0
+    assert hip.hipDeviceSynchronize() == 0
+    assert hip.hipMemcpy(b, a, COPY_PROBE_BYTES, 3) == 0    # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
     print(f"copied {COPY_PROBE_BYTES} bytes", flush=True)
 
 

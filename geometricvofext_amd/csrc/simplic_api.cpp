@@ -94,10 +94,6 @@ public:
     solveVofEqu(const svp_mesh& mesh, const svp_dict& dict, int device) : dict_(dict), device_(device) {
         HIPCHECK(hipSetDevice(device));
         HIPCHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
-        HIPCHECK(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking));
-        HIPCHECK(hipEventCreateWithFlags(&reconStart_, hipEventDisableTiming));
-        HIPCHECK(hipEventCreateWithFlags(&upwindDone_, hipEventDisableTiming));
-        if (const char* e = std::getenv("SVP_OVERLAP_UPWIND")) overlapUpwind_ = std::atoi(e) != 0;
         params_.mixedCellTol = dict.mixed_cell_tol;
         params_.nAlphaBounds = dict.n_alpha_bounds;
         params_.snapTol = dict.snap_tol;
@@ -137,10 +133,6 @@ public:
             (void)hipEventDestroy(e.first);
             (void)hipEventDestroy(e.second);
         }
-        if (side_) (void)hipStreamSynchronize(side_);
-        if (reconStart_) (void)hipEventDestroy(reconStart_);
-        if (upwindDone_) (void)hipEventDestroy(upwindDone_);
-        if (side_) (void)hipStreamDestroy(side_);
         if (stream_) (void)hipStreamDestroy(stream_);
     }
 
@@ -148,8 +140,7 @@ public:
     void reconstruct() {
         hipEvent_t a = event(), b = event();
         HIPCHECK(hipEventRecord(a, stream_));
-        HIPCHECK(hipEventRecord(reconStart_, stream_));
-        reconCall_ = calls_;
+        st_.epoch = nextEpoch();
         stage(STAGE_DETECT, [&] { launch_detect(m_, st_, params_, stream_); });
         if (learned_.active) {
             reconstructLearned();
@@ -172,25 +163,10 @@ public:
     void advect(double dt, const double* Sp, const double* Su) {
         if (!(dt > 0.0) || !std::isfinite(dt)) throw SvpError(SVP_E_ARG, "advect: dt must be positive");
         if (!reconstructed_) reconstruct();
-        // The upwind face volumes read only alpha.oldTime() and phi, which
-        // the reconstruct does not change, and nothing in the reconstruct
-        // reads what the upwind kernel writes (dVf, the flux and bound
-        // counters).  When this advect directly follows its reconstruct (no
-        // call on the handle in between that could change or read those; a
-        // reconstruct inside the same call, as in the sub-cycles, counts),
-        // the upwind kernel runs on a second stream from the reconstruct's
-        // start, beside the plane placement, and the flux stage waits for it.
-        const bool overlap = overlapUpwind_ && profMask_ == 0 && calls_ - reconCall_ <= 1;
-        std::swap(st_.alpha, st_.alphaOld);
-        if (overlap) {
-            HIPCHECK(hipStreamWaitEvent(side_, reconStart_, 0));
-            launch_advect_stage(STAGE_UPWIND, m_, st_, params_, dt, Sp, Su, &bndPassSeq_, side_);
-            HIPCHECK(hipEventRecord(upwindDone_, side_));
-        }
         hipEvent_t a = event(), b = event();
         HIPCHECK(hipEventRecord(a, stream_));
-        if (overlap) HIPCHECK(hipStreamWaitEvent(stream_, upwindDone_, 0));
-        for (int sg = overlap ? STAGE_UPWIND + 1 : STAGE_UPWIND; sg <= STAGE_SNAPCLIP; ++sg)
+        std::swap(st_.alpha, st_.alphaOld);
+        for (int sg = STAGE_UPWIND; sg <= STAGE_SNAPCLIP; ++sg)
             stage(sg, [&] { launch_advect_stage(sg, m_, st_, params_, dt, Sp, Su, &bndPassSeq_, stream_); });
         HIPCHECK(hipEventRecord(b, stream_));
         HIPCHECK(hipGetLastError());
@@ -210,7 +186,7 @@ public:
         int* nLs = learned_.counts;
         int* nFb = learned_.counts + 1;
         stage(STAGE_NORMAL, [&] {
-            nm::launch_zero_ints(learned_.counts, 2, stream_);
+            // the counts were zeroed by the detection kernel (clearAtDetect);
             // the residual test runs inside the inference kernel (tol < 1)
             nm::launch_infer(learned_.lat, learned_.stride, st_.alpha, st_.mixed, st_.nMixed, m_.C, m_.V, learned_.wb, learned_.pq,
                              learned_.tol, st_.ifN, learned_.learned, learned_.resid, learned_.lsList, nLs, nFb,
@@ -232,6 +208,7 @@ public:
             learned_.learned = dalloc<unsigned char>(m_.nC);
             learned_.lsList = dalloc<int>(m_.nC);
             learned_.counts = dalloc<int>(2);
+            st_.clearAtDetect = learned_.counts;
             learned_.resid = dalloc<double>(m_.nC);
             DBuf cube;
             cube.alloc<unsigned char>(m_.nC);
@@ -643,7 +620,18 @@ public:
     }
     int batchGroup() const { return st_.maxCellFaces <= 4 ? 4 : (st_.maxCellFaces <= 6 ? 6 : 8); }
     int passSeq() const { return bndPassSeq_; }
+    // number of the next reconstruct (tags of needBounding); the tags are
+    // cleared before the count wraps
+    int nextEpoch() {
+        if (st_.epoch == 0x7ffffffe) {
+            HIPCHECK(hipMemsetAsync(st_.nbTag, 0, sizeof(int) * m_.nC, stream_));
+            st_.epoch = 0;
+        }
+        return st_.epoch + 1;
+    }
+
     void afterBatchStep(double dt, int nPasses) {
+        st_.epoch = nextEpoch();
         std::swap(st_.alpha, st_.alphaOld);
         bndPassSeq_ += nPasses;
         lastDt_ = dt;
@@ -659,11 +647,6 @@ public:
     int device_ = 0;
     int nB_ = 0;
     hipStream_t stream_ = nullptr;
-    // second stream of the upwind kernel, and the events that order it
-    hipStream_t side_ = nullptr;
-    hipEvent_t reconStart_ = nullptr, upwindDone_ = nullptr;
-    bool overlapUpwind_ = true;
-    long calls_ = 0, reconCall_ = -10;
     double lastDt_ = 0.0;
     // harness base flow
     double* U0_ = nullptr;
@@ -943,6 +926,7 @@ private:
         m_.Sf = dalloc<double>(3L * nF);
         m_.magSf = dalloc<double>(nF);
         m_.flat = dalloc<double>(nF);
+        m_.fullCS = dalloc<double>(6L * nF);
         m_.C = dalloc<double>(3L * nC);
         m_.V = dalloc<double>(nC);
         m_.psW = dalloc<double>(psSrc.size());
@@ -961,7 +945,7 @@ private:
         // bounding arrays double-buffered by pass parity: corrections, active
         // flags, writer indices and correctedFaces keys
         st_.corr = dalloc<double>(2L * nF);
-        st_.mixedFlag = dalloc<unsigned char>(nC);
+        st_.nbTag = dalloc<int>(nC);
         st_.nMixed = dalloc<int>(1);
         st_.mixed = dalloc<int>(nC);
         st_.status = dalloc<int>(nC);
@@ -1401,6 +1385,7 @@ private:
             inf[k].dt = dt ? (dt[k] > 0.0 ? dt[k] : 0.0) : 0.0;
             inf[k].aPar = aPar_[k];
             inf[k].hPar = hPar_[k];
+            inf[k].epoch = c_[k]->nextEpoch();
             if (factors) fac[k] = factors[k];
         }
         HIPCHECK(hipMemcpyAsync(info_.p, inf, sizeof(BatchStepInfo) * n, hipMemcpyHostToDevice, stream_));
@@ -1458,12 +1443,9 @@ static caseBatch& B(svp_batch* b) {
     return *b->b;
 }
 
-// every entry point that takes a handle passes here once: the call count
-// tells advect whether anything ran between it and its reconstruct
 static solveVofEqu& H(svp_handle* h) {
     if (!h || !h->s) throw SvpError(SVP_E_ARG, "null handle");
     HIPCHECK(hipSetDevice(h->s->device_));
-    h->s->calls_++;
     return *h->s;
 }
 

@@ -281,13 +281,28 @@ __device__ inline void rotate_left(T (&a)[N], int r) {
 // IF: record the plane crossings in out.ifPts (the cell cut needs them; the
 // face-flux integrals only need the area).  out may live in LDS: it is then
 // written in place, which keeps a per-lane copy of the record out of scratch.
-template <int N, bool IF = true, class FC>
-__device__ inline bool calcSubFaceN(const V3 (&fPts)[N], V3 normal, double distance, FC& out) {
+// FULL: full holds face::centre and face::areaNormal of these points (6
+// values), which a fully submerged face takes as they are instead of
+// computing them again.  DOTS: dots holds dot(fPts[i], normal), so that a
+// cut adds only the distance (the same values, computed once).
+template <int N, bool IF = true, bool FULL = false, bool DOTS = FULL, class FC>
+__device__ inline bool calcSubFaceN(const V3 (&fPts)[N], V3 normal, double distance, FC& out,
+                                    const double* full = nullptr, const double* dots = nullptr) {
+    auto fullCentre = [&]() {
+        if constexpr (FULL) return v3(full[0], full[1], full[2]);
+        else return faceCentre(fPts, N);
+    };
+    auto fullArea = [&]() {
+        if constexpr (FULL) return v3(full[3], full[4], full[5]);
+        else return faceAreaNormal(fPts, N);
+    };
     double dist[N];
     int nSub = 0, first = -1;
 #pragma unroll
     for (int i = 0; i < N; ++i) {
-        double di = dot(fPts[i], normal) + distance;
+        double di;
+        if constexpr (DOTS) di = dots[i] + distance;
+        else di = dot(fPts[i], normal) + distance;
         if (fabs(di) < SVP_TSMALL) di += sgn(di) * SVP_TSMALL;
         dist[i] = di;
         if (di < 0.0) {
@@ -299,8 +314,8 @@ __device__ inline bool calcSubFaceN(const V3 (&fPts)[N], V3 normal, double dista
     if (nSub == N || nSub == 0) {
         if (nSub == N) {
             out.status = -1;
-            out.centre = faceCentre(fPts, N);
-            out.area = faceAreaNormal(fPts, N);
+            out.centre = fullCentre();
+            out.area = fullArea();
         } else {
             out.status = 1;
             out.centre = VZERO;
@@ -360,8 +375,8 @@ __device__ inline bool calcSubFaceN(const V3 (&fPts)[N], V3 normal, double dista
     out.nIf = nIf > MAX_IF_POINTS ? MAX_IF_POINTS : nIf;
     if (nPts < 3) {
         out.status = -1;
-        out.centre = faceCentre(fPts, N);
-        out.area = faceAreaNormal(fPts, N);
+        out.centre = fullCentre();
+        out.area = fullArea();
         return ok;
     }
     out.status = 0;
@@ -544,6 +559,129 @@ __device__ __attribute__((always_inline)) inline void combineSubCell(const FaceC
         out.VOF = 1.0;
     }
     (void)Cc;
+}
+
+// combineSubCell for a lane group that holds one face per lane (lane l: face
+// l of the cell, records rec[0..nF) in LDS, written by their lanes).  The
+// per-face terms that do not depend on the running sums are computed by the
+// face's own lane: the first interface segment of a cut face (its centre sum,
+// normal and area) and the pyramid volume against cEst; every lane then adds
+// them in the reference's face order, so the sums and the result are those
+// of combineSubCell.  seg/pyr: the group's scratch in LDS (nF entries each).
+struct SegTerm {
+    V3 c, nn;
+    double ar;
+};
+
+__device__ __attribute__((always_inline)) inline void combineSubCellGroup(const FaceCut* fc, SegTerm* seg, double* pyr,
+                                                                           int nF, int l, double Vc, CellCut& out) {
+    bool fullySubmerged = true, fullyEmpty = true;
+    int nSubmergedFaces = 0;
+    for (int f = 0; f < nF; ++f) {
+        if (fc[f].status == 0) {
+            fullySubmerged = false;
+            fullyEmpty = false;
+        } else if (fc[f].status == -1) {
+            fullyEmpty = false;
+            nSubmergedFaces++;
+        } else {
+            fullySubmerged = false;
+        }
+    }
+    out.ifCentre = VZERO;
+    out.ifArea = VZERO;
+    if (!fullySubmerged && !fullyEmpty) {
+        out.status = 0;
+        // calcInterfaceCentreAndArea (cutCell.C:37-100)
+        V3 fCentre = VZERO;
+        int nEdgePoints = 0;
+        for (int f = 0; f < nF; ++f) {
+            if (fc[f].status != 0) continue;
+            for (int k = 0; k < fc[f].nIf; ++k) {
+                fCentre = fCentre + fc[f].ifPts[k];
+                nEdgePoints++;
+            }
+        }
+        if (nEdgePoints > 0) fCentre = fCentre / (double)nEdgePoints;
+        if (l < nF && fc[l].status == 0 && fc[l].nIf >= 2) {
+            const V3 a = fc[l].ifPts[0], b = fc[l].ifPts[1];
+            seg[l].c = a + b + fCentre;
+            const V3 nn = cross(b - a, fCentre - a);
+            seg[l].nn = nn;
+            seg[l].ar = mag(nn);
+        }
+        __builtin_amdgcn_wave_barrier();
+        V3 sumN = VZERO, sumAc = VZERO;
+        double sumA = 0.0;
+        for (int f = 0; f < nF; ++f) {
+            if (fc[f].status != 0) continue;
+            for (int k = 0; k < fc[f].nIf - 1; ++k) {
+                V3 c, nn;
+                double ar;
+                if (k == 0) {
+                    c = seg[f].c;
+                    nn = seg[f].nn;
+                    ar = seg[f].ar;
+                } else {
+                    const V3 a = fc[f].ifPts[k], b = fc[f].ifPts[k + 1];
+                    c = a + b + fCentre;
+                    nn = cross(b - a, fCentre - a);
+                    ar = mag(nn);
+                }
+                sumN = sumN + sgn(dot(nn, sumN)) * nn;
+                sumA += ar;
+                sumAc = sumAc + ar * c;
+            }
+        }
+        if (sumA < SVP_ROOTVSMALL) {
+            out.ifCentre = fCentre;
+            out.ifArea = VZERO;
+        } else {
+            out.ifCentre = (1.0 / 3.0) * sumAc / sumA;
+            out.ifArea = 0.5 * sumN;
+        }
+        if (dot(out.ifArea, out.ifCentre - VZERO) < 0.0) out.ifArea = (-1.0) * out.ifArea;
+        if (mag(out.ifArea) < SVP_TSMALL) {
+            if (nSubmergedFaces == 0) {
+                out.status = 1;
+                out.VOF = 0.0;
+            } else {
+                out.status = -1;
+                out.VOF = 1.0;
+            }
+            return;
+        }
+        // calcSubCellCentreAndVolume (cutCell.C:103-137)
+        V3 sum = VZERO;
+        int cnt = 0;
+        for (int f = 0; f < nF; ++f)
+            if (fc[f].status != 1) {
+                sum = sum + fc[f].centre;
+                cnt++;
+            }
+        sum = sum + out.ifCentre;
+        cnt++;
+        const V3 cEst = sum / (double)cnt;
+        if (l < nF && fc[l].status != 1) pyr[l] = fmax(fabs(dot(fc[l].area, fc[l].centre - cEst)), SVP_VSMALL);
+        __builtin_amdgcn_wave_barrier();
+        double vol = 0.0;
+        for (int f = 0; f < nF; ++f) {
+            if (fc[f].status == 1) continue;
+            vol += pyr[f];
+        }
+        {
+            const double pyr3Vol = fmax(fabs(dot(out.ifArea, out.ifCentre - cEst)), SVP_VSMALL);
+            vol += pyr3Vol;
+        }
+        vol /= 3.0;
+        out.VOF = vol / Vc;
+    } else if (fullyEmpty) {
+        out.status = 1;
+        out.VOF = 0.0;
+    } else {
+        out.status = -1;
+        out.VOF = 1.0;
+    }
 }
 
 // Cut the whole cell sequentially (one lane)

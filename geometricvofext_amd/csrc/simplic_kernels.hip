@@ -27,6 +27,14 @@ __global__ void k_face_geom(DevMesh m) {
     primitiveFaceGeom(p, n, fc, fa);
     st3(m.Cf, f, fc);
     st3(m.Sf, f, fa);
+    // the sub-face of a fully submerged face (cutFace::calcSubFace)
+    const V3 c0 = faceCentre(p, n), a0 = faceAreaNormal(p, n);
+    m.fullCS[6L * f] = c0.x;
+    m.fullCS[6L * f + 1] = c0.y;
+    m.fullCS[6L * f + 2] = c0.z;
+    m.fullCS[6L * f + 3] = a0.x;
+    m.fullCS[6L * f + 4] = a0.y;
+    m.fullCS[6L * f + 5] = a0.z;
     const double ms = mag(fa);
     m.magSf[f] = ms;
     // reconstruction::updateFaceFlatness (reconstruction.C:408-440)
@@ -260,6 +268,7 @@ __device__ __attribute__((always_inline)) inline void k_detect_list_body(const D
     const unsigned epoch = sEpoch;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int c0 = b * DETECT_LIST_CELLS + threadIdx.x;
+    if (b == 0 && threadIdx.x < 2 && st.clearAtDetect) st.clearAtDetect[threadIdx.x] = 0;
     double a[DETECT_LIST_CHUNKS];
 #pragma unroll
     for (int k = 0; k < DETECT_LIST_CHUNKS; ++k) {
@@ -272,10 +281,7 @@ __device__ __attribute__((always_inline)) inline void k_detect_list_body(const D
     for (int k = 0; k < DETECT_LIST_CHUNKS; ++k) {
         const int c = c0 + k * DETECT_LIST_THREADS;
         bool flag = false;
-        if (c < m.nC) {
-            flag = (tol < a[k]) && (a[k] < 1.0 - tol);
-            st.mixedFlag[c] = flag ? 1 : 0;
-        }
+        if (c < m.nC) flag = (tol < a[k]) && (a[k] < 1.0 - tol);
         flags |= flag ? (1u << k) : 0u;
         const unsigned long long bal = __ballot(flag);
         inWave[k] = __popcll(bal & ((1ull << lane) - 1ull));
@@ -1014,9 +1020,9 @@ __global__ void __launch_bounds__(64) k_find_signed_distance(DevMesh m, DevState
 // in LDS.  One wave per workgroup: LDS traffic of a wave is in order.
 
 __device__ __attribute__((always_inline)) inline void fsd_face_cut(int fsz, const V3 (&q4)[4], const V3 (&q3)[3], V3 n, double dist,
-                                    FaceCut& fc, bool& ok) {
-    if (fsz == 4) ok &= calcSubFaceN<4>(q4, n, dist, fc);
-    else ok &= calcSubFaceN<3>(q3, n, dist, fc);
+                                    FaceCut& fc, bool& ok, const double* full, const double (&dots)[4]) {
+    if (fsz == 4) ok &= calcSubFaceN<4, true, true>(q4, n, dist, fc, full, dots);
+    else ok &= calcSubFaceN<3, true, true>(q3, n, dist, fc, full, dots);
 }
 
 // waves per SIMD the register allocation of the interface-cell kernels
@@ -1034,6 +1040,9 @@ __device__ __attribute__((always_inline)) inline void k_fsd_group_body(const Dev
     constexpr int NG = 64 / G;                         // groups per wave; lanes past NG * G idle
     constexpr int PPL = (NS_CELL_POINTS + G - 1) / G;  // cell points per lane
     __shared__ FaceCut rec[NG][G];
+    __shared__ double full[NG][G][6];   // the lane's face as a fully submerged sub-face
+    __shared__ SegTerm seg[NG][G];      // per-face terms of the cut combination
+    __shared__ double pyr[NG][G];
     __shared__ double vraw[NG][NS_CELL_POINTS];
     __shared__ double vsrt[NG][NS_CELL_POINTS];
     const int g = threadIdx.x / G, l = threadIdx.x % G;
@@ -1080,7 +1089,13 @@ __device__ __attribute__((always_inline)) inline void k_fsd_group_body(const Dev
 #pragma unroll
                 for (int k = 0; k < 3; ++k) q3[k] = ld3(m.pts, pl[k]);
             }
+#pragma unroll
+            for (int k = 0; k < 6; ++k) full[g][l][k] = m.fullCS[6L * f + k];
         }
+        // the plane's normal is fixed for the cell: dot(point, n) once
+        double dots[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) dots[k] = (fsz == 4) ? dot(q4[k], normalI) : (k < 3 ? dot(q3[k], normalI) : 0.0);
         // vertex levels -n.x of the cell points, sorted descending by rank
         const int pb = m.cpOff[c], np = m.cpOff[c + 1] - pb;
         // (points past PPL per lane cannot occur: np <= NS_CELL_POINTS = PPL * G)
@@ -1117,9 +1132,9 @@ __device__ __attribute__((always_inline)) inline void k_fsd_group_body(const Dev
         __builtin_amdgcn_wave_barrier();
         bool ok = true;
         auto cut = [&](double dist, CellCut& cc) {
-            if (l < nF) fsd_face_cut(fsz, q4, q3, normalI, dist, rec[g][l], ok);
+            if (l < nF) fsd_face_cut(fsz, q4, q3, normalI, dist, rec[g][l], ok, &full[g][l][0], dots);
             __builtin_amdgcn_wave_barrier();
-            combineSubCell(&rec[g][0], nF, Cc, Vc, cc);
+            combineSubCellGroup(&rec[g][0], &seg[g][0], &pyr[g][0], nF, l, Vc, cc);
             __builtin_amdgcn_wave_barrier();
         };
         // cutCell::findSignedDistance (cutCell.C:611-799) as one loop around a
@@ -1712,9 +1727,14 @@ __device__ inline double time_integrated_face_flux(const DevMesh& m, int f, V3 n
 // times are walked once and every accepted sub-interval gets its Simpson term
 // right away; the additions to tIntArea keep the reference's order (leading
 // uncut segment, trailing uncut segment, then the Simpson terms in time order).
-template <int N>
+// FULL: full points to the face's fully submerged sub-face (fullCS)
+template <int N, bool FULL = false>
 __device__ inline double tia_regs(const V3 (&P)[N], const double (&ptIn)[N], V3 normal, double distance,
-                                  double Un0, double dt, double magSf, bool& ok) {
+                                  double Un0, double dt, double magSf, bool& ok, const double* full = nullptr) {
+    // the plane normal is fixed over the time step: dot(point, n) once
+    double dots[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) dots[k] = dot(P[k], normal);
     double pt[N];
 #pragma unroll
     for (int k = 0; k < N; ++k) pt[k] = ptIn[k];
@@ -1741,16 +1761,16 @@ __device__ inline double tia_regs(const V3 (&P)[N], const double (&ptIn)[N], V3 
     } else {
         prevTime = 0.0;
         tauOld = 0.0;
-        ok &= calcSubFaceN<N, false>(P, normal, distance, fc);
+        ok &= calcSubFaceN<N, false, FULL, true>(P, normal, distance, fc, full, dots);
         subAreaOld = mag(fc.area);
     }
     if (!(lastTime > dt)) tIntArea += magSf * (dt - lastTime) * pos0(Un0);
     const double smallTime = fmax(SVP_TSMALL / fabs(Un0), SVP_TSMALL);
     auto simpson = [&](double tauNew) {
         const double deltaTau = 0.5 * (tauNew - tauOld);
-        ok &= calcSubFaceN<N, false>(P, normal, distance - tauNew * Un0, fc);
+        ok &= calcSubFaceN<N, false, FULL, true>(P, normal, distance - tauNew * Un0, fc, full, dots);
         const double subAreaNew = mag(fc.area);
-        ok &= calcSubFaceN<N, false>(P, normal, distance - (tauOld + deltaTau) * Un0, fc);
+        ok &= calcSubFaceN<N, false, FULL, true>(P, normal, distance - (tauOld + deltaTau) * Un0, fc, full, dots);
         const double subAreaMid = mag(fc.area);
         tIntArea += (deltaTau / 3.0) * (subAreaOld + 4.0 * subAreaMid + subAreaNew);
         subAreaOld = subAreaNew;
@@ -1781,7 +1801,7 @@ __device__ __attribute__((always_inline)) inline double tiff_regs(const DevMesh&
             const double t = (dot(p[i], normal) + distance) / Un0;
             pTimes[i] = fabs(t) < SVP_TSMALL ? 0.0 : t;
         }
-        if (flat) return phi / magSf * tia_regs<N>(p, pTimes, normal, distance, Un0, dt, magSf, ok);
+        if (flat) return phi / magSf * tia_regs<N, true>(p, pTimes, normal, distance, Un0, dt, magSf, ok, &m.fullCS[6L * f]);
         double dVf = 0.0;
         const V3 fc = ld3(m.Cf, f);
         const double t0 = (dot(fc, normal) + distance) / Un0;
@@ -1956,7 +1976,19 @@ __global__ void __launch_bounds__(64) k_un0(DevMesh m, DevState st) {
 // matter: each face is downwind of one cell and gets one volume), one atomic
 // per block and round (a single counter serialises its atomics).  The count
 // is cleared by the upwind kernel.
-__device__ __attribute__((always_inline)) inline void k_flux_list_body(const DevMesh& m, const DevState& st, int slots) {
+// needBounding (advection.C:54-82): each mixed cell of this step's reconstruct
+// tags itself (slot 0) and the neighbour across each internal face (one slot
+// per face) with the step's number, so the update and the bounding apply test
+// it with one load; mixed cells of every status are tagged.
+__device__ inline void tag_need_bounding(const DevMesh& m, const DevState& st, int c, int k, int epoch) {
+    const int fb = m.cOff[c];
+    if (k >= m.cOff[c + 1] - fb) return;
+    if (k == 0) st.nbTag[c] = epoch;
+    const int f = m.cFaces[fb + k];
+    if (f < m.nIF) st.nbTag[m.own[f] == c ? m.nbr[f] : m.own[f]] = epoch;
+}
+
+__device__ __attribute__((always_inline)) inline void k_flux_list_body(const DevMesh& m, const DevState& st, int slots, int epoch) {
     const int nMixed = *st.nMixed;
     const long total = (long)nMixed * slots;
     const int lane = threadIdx.x & 63;
@@ -1970,6 +2002,7 @@ __device__ __attribute__((always_inline)) inline void k_flux_list_body(const Dev
         bool take = false;
         if (w < total) {
             const int i = (int)(w / slots), k = (int)(w % slots);
+            tag_need_bounding(m, st, st.mixed[i], k, epoch);
             if (st.status[i] == 0) {
                 const int c = st.mixed[i];
                 const int fb = m.cOff[c];
@@ -1997,7 +2030,7 @@ __device__ __attribute__((always_inline)) inline void k_flux_list_body(const Dev
 }
 
 __global__ void __launch_bounds__(256) k_flux_list(DevMesh m, DevState st, int slots) {
-    k_flux_list_body(m, st, slots);
+    k_flux_list_body(m, st, slots, st.epoch);
 }
 
 // timeIntegratedFlux with one thread per (cut cell, downwind face): each
@@ -2006,11 +2039,13 @@ __global__ void __launch_bounds__(256) k_flux_list(DevMesh m, DevState st, int s
 // thread and the upwind faces drop out (cheaper for tetrahedra, whose light
 // faces do not repay the list).
 template <bool TRIQUAD, bool LISTED>
-__device__ __attribute__((always_inline)) inline void k_interface_flux_faces_body(const DevMesh& m, const DevState& st, double dtStep, int slots) {
+__device__ __attribute__((always_inline)) inline void k_interface_flux_faces_body(const DevMesh& m, const DevState& st, double dtStep, int slots,
+                                                                                  int epoch) {
     const long nWork = LISTED ? (long)*st.fluxCount : (long)*st.nMixed * slots;
     for (long wi = (long)blockIdx.x * blockDim.x + threadIdx.x; wi < nWork; wi += (long)gridDim.x * blockDim.x) {
         const int w = LISTED ? st.fluxList[wi] : (int)wi;
         const int i = w / slots, k = w % slots;
+        if (!LISTED) tag_need_bounding(m, st, st.mixed[i], k, epoch);
         if (!LISTED && st.status[i] != 0) continue;
         const int c = st.mixed[i];
         const int fb = m.cOff[c];
@@ -2049,24 +2084,15 @@ __device__ __attribute__((always_inline)) inline void k_interface_flux_faces_bod
 
 template <bool TRIQUAD, bool LISTED>
 __global__ void __launch_bounds__(64, SVP_FLUX_WAVES) k_interface_flux_faces(DevMesh m, DevState st, double dtStep, int slots) {
-    k_interface_flux_faces_body<TRIQUAD, LISTED>(m, st, dtStep, slots);
+    k_interface_flux_faces_body<TRIQUAD, LISTED>(m, st, dtStep, slots, st.epoch);
 }
 
 __device__ inline bool loop_oob(double a, double aTol) { return (a - 1.0 > aTol) || (a < -aTol); }
 __device__ inline bool cell_oob(double a, double aTol) { return (a < -aTol) || (a > 1.0 + aTol); }
 
-// needBounding = mixed cells and their face neighbours (advection.C:54-82)
-__device__ inline bool need_bounding(const DevMesh& m, const unsigned char* mixedFlag, int c) {
-    if (mixedFlag[c]) return true;
-    for (int k = m.cOff[c]; k < m.cOff[c + 1]; ++k) {
-        const int f = m.cFaces[k];
-        if (f < m.nIF) {
-            const int o = (m.own[f] == c) ? m.nbr[f] : m.own[f];
-            if (mixedFlag[o]) return true;
-        }
-    }
-    return false;
-}
+// needBounding = mixed cells and their face neighbours (advection.C:54-82):
+// tagged with the reconstruct's number by the detection kernel
+__device__ inline bool need_bounding(const DevState& st, int c, int epoch) { return st.nbTag[c] == epoch; }
 
 // Conservative update (advectionTemplates.C:399-404) with fvc::surfaceIntegrate
 // summing the cell's faces in ascending face order.  Also counts cells outside
@@ -2075,7 +2101,8 @@ __device__ inline bool need_bounding(const DevMesh& m, const unsigned char* mixe
 // per wave.
 constexpr int UPDATE_BLOCK = 256;
 
-__device__ __attribute__((always_inline)) inline void k_update_body(const DevMesh& m, const DevState& st, double dtStep, const double* Sp, const double* Su) {
+__device__ __attribute__((always_inline)) inline void k_update_body(const DevMesh& m, const DevState& st, double dtStep, const double* Sp, const double* Su,
+                                                                    int epoch) {
     const double aTol = 100.0 * SVP_SMALL;
     const int c = blockIdx.x * UPDATE_BLOCK + threadIdx.x;
     bool cand = false, oob = false;
@@ -2100,7 +2127,7 @@ __device__ __attribute__((always_inline)) inline void k_update_body(const DevMes
         const double a = num / den;
         st.alpha[c] = a;
         oob = loop_oob(a, aTol);
-        cand = cell_oob(a, aTol) && need_bounding(m, st.mixedFlag, c);
+        cand = cell_oob(a, aTol) && need_bounding(st, c, epoch);
     }
     const unsigned long long bits = __ballot(cand);
     const unsigned long long oobBits = __ballot(oob);
@@ -2113,7 +2140,7 @@ __device__ __attribute__((always_inline)) inline void k_update_body(const DevMes
 }
 
 __global__ void __launch_bounds__(UPDATE_BLOCK) k_update(DevMesh m, DevState st, double dtStep, const double* Sp, const double* Su) {
-    k_update_body(m, st, dtStep, Sp, Su);
+    k_update_body(m, st, dtStep, Sp, Su, st.epoch);
 }
 
 __device__ inline double face_value(const DevMesh& m, const double* f, int facei) {
@@ -2882,11 +2909,13 @@ __global__ void __launch_bounds__(flow_threads<MF>()) k_bnd_flow(DevMesh m, DevS
     k_bnd_flow_body<MF>(m, st, dt);
 }
 
+
 // alpha of every touched cell in correctedFaces order (advectionTemplates.C:164-192)
 // and the dVf of every corrected face (the first-loop faces of the active
 // candidates)
 template <int MF>
-__device__ __attribute__((always_inline)) inline void k_bnd_apply_body(const DevMesh& m, const DevState& st, int pass) {
+__device__ __attribute__((always_inline)) inline void k_bnd_apply_body(const DevMesh& m, const DevState& st, int pass,
+                                                                       int epoch) {
     const double aTol = 100.0 * SVP_SMALL;
     const int nT = st.bcount[2], nCand = st.bcount[0];
     // pass statistics, and the candidate count for the next list's clearing
@@ -2978,7 +3007,7 @@ __device__ __attribute__((always_inline)) inline void k_bnd_apply_body(const Dev
         st.alpha[c] = a;
         const int dOob = (int)loop_oob(a, aTol) - (int)loop_oob(before, aTol);
         if (dOob) atomicAdd(&st.oobCount[OOB_STRIDE * ((c >> 8) & (OOB_SLOTS - 1))], dOob);
-        const bool candNew = cell_oob(a, aTol) && need_bounding(m, st.mixedFlag, c);
+        const bool candNew = cell_oob(a, aTol) && need_bounding(st, c, epoch);
         const unsigned long long bit = 1ull << (c & 63);
         const bool candOld = (st.candBits[c >> 6] & bit) != 0ull;
         if (candNew != candOld) {
@@ -2991,7 +3020,7 @@ __device__ __attribute__((always_inline)) inline void k_bnd_apply_body(const Dev
 
 template <int MF>
 __global__ void __launch_bounds__(256) k_bnd_apply(DevMesh m, DevState st, int pass) {
-    k_bnd_apply_body<MF>(m, st, pass);
+    k_bnd_apply_body<MF>(m, st, pass, st.epoch);
 }
 
 template <int MF>
@@ -3005,8 +3034,8 @@ void launch_bound_pass(const DevMesh& m, const DevState& st, int pass, double dt
     k_bnd_list<<<nbw, 256, 0, s>>>(m, st, nWords, nbw, prev);
     k_bnd_prep<MF><<<grid, 256, 0, s>>>(m, st, dt, Sp, Su);
     if (st.bndFlow) {
-        k_bnd_flow<MF><<<std::max(1, std::min(st.bndFlow, cdiv(m.nC, flow_chunk<MF>()))), flow_threads<MF>(), 0, s>>>(
-            m, st, dt);
+        k_bnd_flow<MF><<<std::max(1, std::min(st.bndFlow, cdiv(m.nC, flow_chunk<MF>()))), flow_threads<MF>(), 0,
+                         s>>>(m, st, dt);
     } else {
         k_bnd_scatter<MF><<<grid, 256, 0, s>>>(m, st);
         for (int r = 0; r < st.gridRounds; ++r) k_bnd_round<MF><<<grid, 256, 0, s>>>(m, st, r, dt);
@@ -3198,7 +3227,7 @@ __global__ void __launch_bounds__(256) kb_flux_list(const BatchCase* __restrict_
     const BatchStepInfo si = info[blockIdx.y];
     if (!(si.dt > 0.0)) return;
     const DevState& st = batch_view(c, si, 1, 0);
-    k_flux_list_body(c.m, st, st.maxCellFaces);
+    k_flux_list_body(c.m, st, st.maxCellFaces, si.epoch);
 }
 
 template <bool TRIQUAD, bool LISTED>
@@ -3208,7 +3237,7 @@ __global__ void __launch_bounds__(64, SVP_FLUX_WAVES) kb_interface_flux_faces(co
     const BatchStepInfo si = info[blockIdx.y];
     if (!(si.dt > 0.0)) return;
     const DevState& st = batch_view(c, si, 1, 0);
-    k_interface_flux_faces_body<TRIQUAD, LISTED>(c.m, st, si.dt, st.maxCellFaces);
+    k_interface_flux_faces_body<TRIQUAD, LISTED>(c.m, st, si.dt, st.maxCellFaces, si.epoch);
 }
 
 __global__ void __launch_bounds__(UPDATE_BLOCK) kb_update(const BatchCase* __restrict__ bc,
@@ -3216,7 +3245,7 @@ __global__ void __launch_bounds__(UPDATE_BLOCK) kb_update(const BatchCase* __res
     const BatchCase& c = bc[blockIdx.y];
     const BatchStepInfo si = info[blockIdx.y];
     if (!(si.dt > 0.0)) return;
-    k_update_body(c.m, batch_view(c, si, 1, 0), si.dt, nullptr, nullptr);
+    k_update_body(c.m, batch_view(c, si, 1, 0), si.dt, nullptr, nullptr, si.epoch);
 }
 
 __global__ void __launch_bounds__(256) kb_bnd_list(const BatchCase* __restrict__ bc,
@@ -3253,7 +3282,7 @@ __global__ void __launch_bounds__(256) kb_bnd_apply(const BatchCase* __restrict_
     const BatchCase& c = bc[blockIdx.y];
     const BatchStepInfo si = info[blockIdx.y];
     if (!(si.dt > 0.0)) return;
-    k_bnd_apply_body<MF>(c.m, batch_view(c, si, 1, pass), pass);
+    k_bnd_apply_body<MF>(c.m, batch_view(c, si, 1, pass), pass, si.epoch);
 }
 
 __global__ void kb_snap_clip(const BatchCase* __restrict__ bc, const BatchStepInfo* __restrict__ info,

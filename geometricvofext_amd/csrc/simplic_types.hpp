@@ -38,6 +38,7 @@ struct DevMesh {
     double* Sf;      // 3*nF
     double* magSf;   // nF
     double* flat;    // nF
+    double* fullCS;  // 6*nF: face::centre and face::areaNormal of the stored face (a fully submerged sub-face)
     double* C;       // 3*nC
     double* V;       // nC
     double* psW;     // point weights, psOff CSR
@@ -55,7 +56,9 @@ struct DevState {
     double* dVf;        // nF
     double* alphaPhi;   // nF
     double* corr;       // nF, bounding corrections (kept zero between passes)
-    unsigned char* mixedFlag;   // nC
+    int* clearAtDetect; // [2] or null: counters the detection kernel zeroes (learned-normal counts)
+    int* nbTag;         // nC: number of the last step that found the cell mixed or next to a mixed cell
+    int epoch;          // number of the current reconstruct (1, 2, ...)
     int* nMixed;        // [1]
     int* mixed;         // capacity nC
     int* status;
@@ -181,6 +184,7 @@ struct BatchStepInfo {
     double dt;
     int aPar;
     int hPar;
+    int epoch;   // the case's reconstruct number of this step
 };
 
 // grid x dimension of each batched stage (the y dimension is the case count)

@@ -223,41 +223,6 @@ def test_parity_perturbed_hex_many_chunks(oracle_lib):
     assert exact
 
 
-@pytest.mark.parametrize("kind", ["hex", "tet"])
-def test_parity_upwind_beside_reconstruct(oracle_lib, monkeypatch, kind):
-    """advect directly after reconstruct runs the upwind kernel on a second
-    stream beside the plane placement: every step bit-identical to the
-    oracle and to a handle that runs it in order (SVP_OVERLAP_UPWIND=0)."""
-    import geometricvofext_amd as g
-    mesh = g.hex_box(20, perturb=0.2, seed=13) if kind == "hex" else g.tet_box(8, perturb=0.1, seed=5)
-    gpu, om, ors = _pair(mesh, oracle_lib, nAlphaBounds=3, clip=False)
-    monkeypatch.setenv("SVP_OVERLAP_UPWIND", "0")
-    serial = g.SolveVofEqu(mesh, dict(nAlphaBounds=3, clip=False))
-    monkeypatch.delenv("SVP_OVERLAP_UPWIND")
-    a0 = ramp_alpha(om.C, [((0.4, 0.45, 0.4), 0.2)], 1.5 / 20)
-    U0, phi0 = gpu.deformation_field()
-    state = {"phi": np.zeros_like(phi0), "alpha": a0}
-    for h in (gpu, ors, serial):
-        h.set_alpha(a0)
-
-    def co_fn():
-        c = om.courant(state["phi"], state["alpha"])
-        return c[0], c[1]
-
-    _say("parity", mesh.name, "upwind beside the reconstruct")
-    for step, (t, dt, uf) in enumerate(deformation_steps(10, co_fn, period=6.0)):
-        state["phi"] = phi0 * uf
-        for h in (gpu, ors, serial):
-            h.set_flow(phi0 * uf, U0 * uf)
-            h.reconstruct()
-            h.advect(dt)
-        ga, oa, sa = gpu.alpha(), ors.alpha(), serial.alpha()
-        assert bits_equal(ga, oa), f"step {step}: alpha off by {max_abs(ga, oa)}"
-        assert bits_equal(ga, sa)
-        assert bits_equal(gpu.alpha_phi(), ors.alpha_phi())
-        state["alpha"] = oa
-
-
 def test_parity_kahn_schedule(oracle_lib, monkeypatch):
     """The alternative bounding schedule (grid-wide Kahn rounds and the
     one-workgroup tail, SVP_BND_FLOW=0) on the heavy-bounding case."""
