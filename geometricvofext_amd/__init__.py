@@ -123,6 +123,7 @@ def lib():
     L.svp_harness_scale_flow.argtypes = [vp, dbl]
     L.svp_harness_courant.argtypes = [vp, C.POINTER(dbl), C.POINTER(dbl)]
     L.svp_harness_init_alpha.argtypes = [vp, vp, C.c_int, C.c_int]
+    L.svp_harness_init_alpha_surface.argtypes = [vp, vp, i32, i32]
     L.svp_profile_set.argtypes = [vp, C.c_int]
     L.svp_profile_get.argtypes = [vp, vp, vp, C.c_int]
     i64, u64 = C.c_int64, C.c_uint64
@@ -417,6 +418,15 @@ class SolveVofEqu:
                                             np.asarray(R, float).reshape(9)]))
         arr = np.ascontiguousarray(np.stack(rows), dtype=np.float64)
         _check(lib().svp_harness_init_alpha(self.h, arr.ctypes.data, len(rows), int(max_depth)))
+
+    def init_alpha_surface(self, tris, mode="set"):
+        """alpha from a closed outward-oriented triangulated surface ((n, 3, 3)
+        points, e.g. surface.read_stl): each cell's exact inside fraction
+        set, added or subtracted (the setVofField utility's modes), then
+        clipped to [0, 1]."""
+        t = np.ascontiguousarray(tris, dtype=np.float64).reshape(-1, 9)
+        m = {"set": 0, "add": 1, "subtract": 2}[mode]
+        _check(lib().svp_harness_init_alpha_surface(self.h, t.ctypes.data, len(t), m))
 
     STAGES = ("detect", "normal", "findSignedDistance", "upwind", "interfaceFlux", "update",
               "bounding", "snapClip")

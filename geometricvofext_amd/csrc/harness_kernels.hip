@@ -100,6 +100,10 @@ __global__ void __launch_bounds__(256) kb_courant(const BatchCase* __restrict__ 
     k_courant_body(c.m, st.phi, st.alpha, out + 2 * blockIdx.y);
 }
 
+__global__ void k_zero_one_int(int* p) {
+    if (threadIdx.x == 0) *p = 0;
+}
+
 __global__ void k_zero_u64(unsigned long long* p, int n) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0ull;
 }
@@ -261,6 +265,234 @@ __global__ void k_init_alpha(DevMesh m, const Shape* shapes, int nShapes, int ma
 void launch_init_alpha(const DevMesh& m, const void* shapes, int nShapes, int maxDepth, double* alpha,
                        hipStream_t s) {
     k_init_alpha<<<cdiv(m.nC, 128), 128, 0, s>>>(m, (const Shape*)shapes, nShapes, maxDepth, alpha);
+}
+
+// ---------------------------------------------------------------------------
+// Initial alpha from a closed triangulated surface: the volume of each cell
+// inside the surface, exactly (the setVofField utility's CGAL boolean,
+// setAlphaField.H:41-62, without CGAL).  With O any point, the indicator of
+// the enclosed region is the signed sum of the cones tet(O, t) over the
+// surface triangles t (signs from the triangles' orientation), so
+//     vol(cell n inside) = sum_t sign_t * vol(tet(O, t) n cell).
+// The cell is split into the tets (cell centre, face centre, face edge) of
+// its faces; O is the cell centre, and each cone-tet intersection is the cone
+// tet clipped by the cell tet's four planes (a tet cut by a plane leaves a
+// tet or a prism, itself three tets).  Cells whose centre lies farther from
+// every triangle than their farthest point take 1 or 0 from the winding
+// number of their centre (solid angles, van Oosterom & Strackee).
+// ---------------------------------------------------------------------------
+struct Tet4 {
+    V3 p[4];
+};
+
+__device__ inline double tet_abs_vol(const Tet4& t) { return tet_vol(t.p[0], t.p[1], t.p[2], t.p[3]); }
+
+// point-triangle distance (closest point on the triangle; Ericson, Real-Time
+// Collision Detection, 5.1.5)
+__device__ inline double point_tri_dist(V3 p, V3 a, V3 b, V3 c) {
+    const V3 ab = b - a, ac = c - a, ap = p - a;
+    const double d1 = dot(ab, ap), d2 = dot(ac, ap);
+    if (d1 <= 0.0 && d2 <= 0.0) return mag(ap);
+    const V3 bp = p - b;
+    const double d3 = dot(ab, bp), d4 = dot(ac, bp);
+    if (d3 >= 0.0 && d4 <= d3) return mag(bp);
+    const double vc = d1 * d4 - d3 * d2;
+    if (vc <= 0.0 && d1 >= 0.0 && d3 <= 0.0) return mag(ap - (d1 / (d1 - d3)) * ab);
+    const V3 cp = p - c;
+    const double d5 = dot(ab, cp), d6 = dot(ac, cp);
+    if (d6 >= 0.0 && d5 <= d6) return mag(cp);
+    const double vb = d5 * d2 - d1 * d6;
+    if (vb <= 0.0 && d2 >= 0.0 && d6 <= 0.0) return mag(ap - (d2 / (d2 - d6)) * ac);
+    const double va = d3 * d6 - d5 * d4;
+    if (va <= 0.0 && (d4 - d3) >= 0.0 && (d5 - d6) >= 0.0)
+        return mag(bp - ((d4 - d3) / ((d4 - d3) + (d5 - d6))) * (c - b));
+    const double den = 1.0 / (va + vb + vc);
+    return mag(ap - (vb * den) * ab - (vc * den) * ac);
+}
+
+// solid angle of triangle (a, b, c) seen from p (van Oosterom & Strackee 1983)
+__device__ inline double tri_solid_angle(V3 p, V3 a, V3 b, V3 c) {
+    const V3 ra = a - p, rb = b - p, rc = c - p;
+    const double la = mag(ra), lb = mag(rb), lc = mag(rc);
+    const double num = dot(ra, cross(rb, rc));
+    const double den = la * lb * lc + dot(ra, rb) * lc + dot(ra, rc) * lb + dot(rb, rc) * la;
+    return 2.0 * atan2(num, den);
+}
+
+// volume of the cone tet clipped by the four planes of the cell tet T
+__device__ inline double clip_tet_volume(const Tet4& cone, const Tet4& T) {
+    // planes of T: the side of each face that holds the opposite vertex
+    V3 pn[4];
+    double pd[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const V3 a = T.p[(k + 1) & 3], b = T.p[(k + 2) & 3], c = T.p[(k + 3) & 3];
+        V3 n = cross(b - a, c - a);
+        if (dot(n, T.p[k] - a) < 0.0) n = (-1.0) * n;
+        pn[k] = n;
+        pd[k] = dot(n, a);
+    }
+    // depth-first clipping, plane by plane
+    Tet4 stack[16];
+    int level[16];
+    int sp = 0;
+    stack[sp] = cone;
+    level[sp++] = 0;
+    double vol = 0.0;
+    while (sp > 0) {
+        --sp;
+        const Tet4 t = stack[sp];
+        const int lv = level[sp];
+        if (lv == 4) {
+            vol += tet_abs_vol(t);
+            continue;
+        }
+        double sd[4];
+        int in[4], out[4], ni = 0, no = 0;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            sd[i] = dot(pn[lv], t.p[i]) - pd[lv];
+            if (sd[i] >= 0.0) in[ni++] = i;
+            else out[no++] = i;
+        }
+        if (ni == 0) continue;
+        if (sp + 3 > 16) continue;  // cannot happen: at most 3 pieces per level, 4 levels deep
+        auto cut = [&](int i, int o) {
+            const double w = sd[i] / (sd[i] - sd[o]);
+            return t.p[i] + w * (t.p[o] - t.p[i]);
+        };
+        auto push = [&](V3 a, V3 b, V3 c, V3 d) {
+            stack[sp] = Tet4{{a, b, c, d}};
+            level[sp++] = lv + 1;
+        };
+        if (ni == 4) {
+            stack[sp] = t;
+            level[sp++] = lv + 1;
+        } else if (ni == 1) {
+            const int a = in[0];
+            push(t.p[a], cut(a, out[0]), cut(a, out[1]), cut(a, out[2]));
+        } else {
+            // prism with bottom (p0, p1, p2) and top (q0, q1, q2), pi - qi edges:
+            // tets (p0 p1 p2 q0), (p1 p2 q0 q1), (p2 q0 q1 q2)
+            V3 P[3], Q[3];
+            if (ni == 3) {
+                const int d = out[0];
+                for (int k = 0; k < 3; ++k) {
+                    P[k] = t.p[in[k]];
+                    Q[k] = cut(in[k], d);
+                }
+            } else {
+                const int a = in[0], b = in[1];
+                P[0] = t.p[a];
+                P[1] = cut(a, out[0]);
+                P[2] = cut(a, out[1]);
+                Q[0] = t.p[b];
+                Q[1] = cut(b, out[0]);
+                Q[2] = cut(b, out[1]);
+            }
+            push(P[0], P[1], P[2], Q[0]);
+            push(P[1], P[2], Q[0], Q[1]);
+            push(P[2], Q[0], Q[1], Q[2]);
+        }
+    }
+    return vol;
+}
+
+// per cell: 0 / 1 from the winding number for cells away from the surface,
+// else the cell joins the list of cut cells
+__global__ void __launch_bounds__(256) k_surface_classify(DevMesh m, const double* tris, int nT, double* frac,
+                                                          int* cutList, int* nCut) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= m.nC) return;
+    const V3 cc = ld3(m.C, c);
+    double R = 0.0;
+    for (int k = m.cOff[c]; k < m.cOff[c + 1]; ++k) {
+        const int f = m.cFaces[k];
+        for (int q = m.fOff[f]; q < m.fOff[f + 1]; ++q) R = fmax(R, mag(ld3(m.pts, m.fPts[q]) - cc));
+    }
+    double dmin = 1e300, omega = 0.0;
+    for (int t = 0; t < nT; ++t) {
+        const V3 a = ld3(tris, 3 * t), b = ld3(tris, 3 * t + 1), cq = ld3(tris, 3 * t + 2);
+        dmin = fmin(dmin, point_tri_dist(cc, a, b, cq));
+        omega += tri_solid_angle(cc, a, b, cq);
+    }
+    if (dmin > R * (1.0 + 1e-9)) {
+        frac[c] = fabs(omega) > 2.0 * M_PI ? 1.0 : 0.0;   // winding number 1 or 0
+    } else {
+        frac[c] = -1.0;
+        cutList[atomicAdd(nCut, 1)] = c;
+    }
+}
+
+// one workgroup per cut cell: its (cell tet, triangle) pairs over the
+// threads, each thread's sum in a fixed order, then a fixed-order tree
+constexpr int SURF_THREADS = 256;
+
+__global__ void __launch_bounds__(SURF_THREADS) k_surface_cut(DevMesh m, const double* tris, int nT,
+                                                              const int* cutList, const int* nCut, double* frac) {
+    __shared__ double sv[SURF_THREADS], sw[SURF_THREADS];
+    const int n = *nCut;
+    for (int w = blockIdx.x; w < n; w += gridDim.x) {
+        const int c = cutList[w];
+        const V3 cc = ld3(m.C, c);
+        const int f0 = m.cOff[c], f1 = m.cOff[c + 1];
+        // the cell's tets, enumerated as (face slot, edge)
+        int nTet = 0;
+        for (int k = f0; k < f1; ++k) {
+            const int f = m.cFaces[k];
+            nTet += m.fOff[f + 1] - m.fOff[f];
+        }
+        double acc = 0.0, vcell = 0.0;
+        for (long j = threadIdx.x; j < (long)nTet * nT; j += SURF_THREADS) {
+            const int it = (int)(j / nT), t = (int)(j % nT);
+            // cell tet number it
+            int k = f0, e = it;
+            for (; k < f1; ++k) {
+                const int f = m.cFaces[k];
+                const int nf = m.fOff[f + 1] - m.fOff[f];
+                if (e < nf) break;
+                e -= nf;
+            }
+            const int f = m.cFaces[k];
+            const int s0 = m.fOff[f], nf = m.fOff[f + 1] - s0;
+            const Tet4 T{{cc, ld3(m.Cf, f), ld3(m.pts, m.fPts[s0 + e]), ld3(m.pts, m.fPts[s0 + (e + 1) % nf])}};
+            if (t == 0) vcell += tet_abs_vol(T);
+            const V3 a = ld3(tris, 3 * t), b = ld3(tris, 3 * t + 1), cq = ld3(tris, 3 * t + 2);
+            const double sgn6 = dot(a - cc, cross(b - cc, cq - cc));
+            if (sgn6 == 0.0) continue;
+            const double v = clip_tet_volume(Tet4{{cc, a, b, cq}}, T);
+            acc += sgn6 > 0.0 ? v : -v;
+        }
+        sv[threadIdx.x] = acc;
+        sw[threadIdx.x] = vcell;
+        __syncthreads();
+        for (int o = SURF_THREADS / 2; o > 0; o >>= 1) {
+            if ((int)threadIdx.x < o) {
+                sv[threadIdx.x] += sv[threadIdx.x + o];
+                sw[threadIdx.x] += sw[threadIdx.x + o];
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) frac[c] = sw[0] > 0.0 ? sv[0] / sw[0] : 0.0;
+        __syncthreads();
+    }
+}
+
+// alpha = fraction (mode 0), alpha + fraction (1) or alpha - fraction (2),
+// then clipped to [0, 1]
+__global__ void __launch_bounds__(256) k_surface_apply(int nC, const double* frac, int mode, double* alpha) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nC) return;
+    const double v = mode == 0 ? frac[c] : (mode == 1 ? alpha[c] + frac[c] : alpha[c] - frac[c]);
+    alpha[c] = fmin(1.0, fmax(0.0, v));
+}
+
+void launch_init_alpha_surface(const DevMesh& m, const double* tris, int nT, int mode, double* frac, int* cutList,
+                               int* nCut, double* alpha, hipStream_t s) {
+    k_zero_one_int<<<1, 64, 0, s>>>(nCut);
+    k_surface_classify<<<cdiv(m.nC, 256), 256, 0, s>>>(m, tris, nT, frac, cutList, nCut);
+    k_surface_cut<<<2048, SURF_THREADS, 0, s>>>(m, tris, nT, cutList, nCut, frac);
+    k_surface_apply<<<cdiv(m.nC, 256), 256, 0, s>>>(m.nC, frac, mode, alpha);
 }
 
 __global__ void k_count_nonfinite(const double* x, long n, int* count) {

@@ -34,6 +34,8 @@ void launch_courant(const DevMesh& m, const double* phi, const double* alpha, un
 void launch_count_nonfinite(const double* x, long n, int* count, hipStream_t s);
 void launch_init_alpha(const DevMesh& m, const void* shapes, int nShapes, int maxDepth, double* alpha,
                        hipStream_t s);
+void launch_init_alpha_surface(const DevMesh& m, const double* tris, int nT, int mode, double* frac, int* cutList,
+                               int* nCut, double* alpha, hipStream_t s);
 }  // namespace svp
 
 using namespace svp::api;
@@ -1924,6 +1926,26 @@ int svp_harness_scale_flow(svp_handle* h, double factor) {
         svp::launch_scale_flow(s.m_.nC, s.m_.nF, s.nB_, s.U0_, s.phi0_, s.U0b_, factor, s.st_.U, s.st_.phi,
                                s.st_.Ub, s.stream_);
         HIPCHECK(hipGetLastError());
+    });
+}
+
+int svp_harness_init_alpha_surface(svp_handle* h, const double* tris, int32_t n_tris, int32_t mode) {
+    return guarded([&] {
+        solveVofEqu& s = H(h);
+        if (!tris || n_tris <= 0 || mode < 0 || mode > 2)
+            throw SvpError(SVP_E_ARG, "init_alpha_surface: need triangles and mode 0 (set), 1 (add) or 2 (subtract)");
+        for (long k = 0; k < 9L * n_tris; ++k)
+            if (!std::isfinite(tris[k])) throw SvpError(SVP_E_NAN, "init_alpha_surface: non-finite point");
+        const int nC = s.m_.nC;
+        DBuf dt, df, dl, dn;
+        double* dtr = dt.alloc<double>(9L * n_tris);
+        double* frac = df.alloc<double>(nC);
+        int* cut = dl.alloc<int>(nC);
+        int* nCut = dn.alloc<int>(1);
+        HIPCHECK(hipMemcpy(dtr, tris, sizeof(double) * 9 * n_tris, hipMemcpyHostToDevice));
+        svp::launch_init_alpha_surface(s.m_, dtr, n_tris, mode, frac, cut, nCut, s.st_.alpha, s.stream_);
+        HIPCHECK(hipGetLastError());
+        HIPCHECK(hipStreamSynchronize(s.stream_));
     });
 }
 

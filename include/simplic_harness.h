@@ -37,6 +37,15 @@ int svp_harness_scale_flow(svp_handle* h, double factor);
  * function is integrated exactly.  Writes the handle's alpha. */
 int svp_harness_init_alpha(svp_handle* h, const double* shapes, int n_shapes, int max_depth);
 
+/* Initial alpha from a closed, outward-oriented triangulated surface (the
+ * setVofField utility's exact cut, setAlphaField.H:41-62, without CGAL):
+ * tris holds n_tris triangles of 3 points of 3 coordinates.  The inside
+ * volume of each cell is exact: the signed sum over the triangles of the
+ * cone from the cell centre clipped to the cell's (centre, face centre,
+ * edge) tets.  mode 0 sets alpha to it, 1 adds it, 2 subtracts it (the
+ * utility's ADD / SUBTRACT modes); the result is clipped to [0, 1]. */
+int svp_harness_init_alpha_surface(svp_handle* h, const double* tris, int32_t n_tris, int32_t mode);
+
 /* max over cells of sum|phi|/V, and of the same restricted to
  * 0.01 <= alpha <= 0.99 (CourantNo.H, alphaCourantNo.H:95-103), from the
  * current phi and alpha; multiply by 0.5*dt for the Courant numbers.
