@@ -2101,6 +2101,56 @@ __device__ inline bool need_bounding(const DevState& st, int c, int epoch) { ret
 // per wave.
 constexpr int UPDATE_BLOCK = 256;
 
+// fvc::surfaceIntegrate's face sum of dVf for cell c, in ascending face
+// order (neighbour faces subtract, empty-patch faces carry nothing).  With
+// MF > 0, a cell of at most MF faces has all its face labels loaded at once
+// and then all their volumes, so a thread waits on two gathers instead of two
+// per face; larger cells take the face-by-face loop.
+template <int MF>
+__device__ __attribute__((always_inline)) inline double cell_face_sum(const DevMesh& m, const DevState& st, int c) {
+    const int b = m.cOff[c], nf = m.cOff[c + 1] - b;
+    double sI = 0.0;
+    if (MF > 0 && nf <= MF) {
+        constexpr int N = MF > 0 ? MF : 1;
+        int e[N];
+        double v[N];
+        unsigned char em[N];
+#pragma unroll
+        for (int k = 0; k < N; ++k) e[k] = m.cFacesAsc[b + min(k, nf - 1)];
+        // (unconditional loads: a load under a branch would wait for all the
+        // loads before it)
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            const int f = e[k] >= 0 ? e[k] : ~e[k];
+            v[k] = st.dVf[f];
+            em[k] = m.fEmpty[f];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            if (k >= nf) break;
+            if (e[k] >= 0) {
+                if (e[k] >= m.nIF && em[k]) continue;
+                sI = sI + v[k];
+            } else {
+                sI = sI - v[k];
+            }
+        }
+        return sI;
+    }
+    for (int k = b; k < b + nf; ++k) {
+        const int e = m.cFacesAsc[k];
+        if (e >= 0) {
+            if (e >= m.nIF && m.fEmpty[e]) continue;
+            sI = sI + st.dVf[e];
+        } else {
+            sI = sI - st.dVf[~e];
+        }
+    }
+    return sI;
+}
+
+template <int MF>
 __device__ __attribute__((always_inline)) inline void k_update_body(const DevMesh& m, const DevState& st, double dtStep, const double* Sp, const double* Su,
                                                                     int epoch) {
     const double aTol = 100.0 * SVP_SMALL;
@@ -2109,16 +2159,7 @@ __device__ __attribute__((always_inline)) inline void k_update_body(const DevMes
     const double dt = dtStep;
     if (c < m.nC) {
         const double rDeltaT = 1.0 / dt;
-        double sI = 0.0;
-        for (int k = m.cOff[c]; k < m.cOff[c + 1]; ++k) {
-            const int e = m.cFacesAsc[k];
-            if (e >= 0) {
-                if (e >= m.nIF && m.fEmpty[e]) continue;
-                sI = sI + st.dVf[e];
-            } else {
-                sI = sI - st.dVf[~e];
-            }
-        }
+        double sI = cell_face_sum<MF>(m, st, c);
         sI = sI / m.V[c];
         double num = st.alphaOld[c] * rDeltaT;
         if (Su) num = num + Su[c];
@@ -2139,8 +2180,26 @@ __device__ __attribute__((always_inline)) inline void k_update_body(const DevMes
     }
 }
 
+template <int MF>
 __global__ void __launch_bounds__(UPDATE_BLOCK) k_update(DevMesh m, DevState st, double dtStep, const double* Sp, const double* Su) {
-    k_update_body(m, st, dtStep, Sp, Su, st.epoch);
+    k_update_body<MF>(m, st, dtStep, Sp, Su, st.epoch);
+}
+
+// face slots of the update's gathered form: the mesh's largest cell when it
+// has at most 8 faces, else the face-by-face loop (0)
+static int update_faces(int maxCellFaces) {
+    return maxCellFaces <= 4 ? 4 : (maxCellFaces <= 6 ? 6 : (maxCellFaces <= 8 ? 8 : 0));
+}
+
+static void launch_update(const DevMesh& m, const DevState& st, double dt, const double* Sp, const double* Su,
+                          hipStream_t s) {
+    const dim3 grid(cdiv(m.nC, UPDATE_BLOCK));
+    switch (update_faces(st.maxCellFaces)) {
+        case 4: k_update<4><<<grid, UPDATE_BLOCK, 0, s>>>(m, st, dt, Sp, Su); break;
+        case 6: k_update<6><<<grid, UPDATE_BLOCK, 0, s>>>(m, st, dt, Sp, Su); break;
+        case 8: k_update<8><<<grid, UPDATE_BLOCK, 0, s>>>(m, st, dt, Sp, Su); break;
+        default: k_update<0><<<grid, UPDATE_BLOCK, 0, s>>>(m, st, dt, Sp, Su); break;
+    }
 }
 
 __device__ inline double face_value(const DevMesh& m, const double* f, int facei) {
@@ -3132,7 +3191,7 @@ void launch_advect_stage(int stage, const DevMesh& m, const DevState& st, const 
             }
             break;
         case STAGE_UPDATE:
-            k_update<<<cdiv(m.nC, UPDATE_BLOCK), UPDATE_BLOCK, 0, s>>>(m, st, dt, Sp, Su);
+            launch_update(m, st, dt, Sp, Su, s);
             break;
         case STAGE_BOUND:
             launch_bound(m, st, p, dt, Sp, Su, passSeq, s);
@@ -3240,12 +3299,13 @@ __global__ void __launch_bounds__(64, SVP_FLUX_WAVES) kb_interface_flux_faces(co
     k_interface_flux_faces_body<TRIQUAD, LISTED>(c.m, st, si.dt, st.maxCellFaces, si.epoch);
 }
 
+template <int MF>
 __global__ void __launch_bounds__(UPDATE_BLOCK) kb_update(const BatchCase* __restrict__ bc,
                                                           const BatchStepInfo* __restrict__ info) {
     const BatchCase& c = bc[blockIdx.y];
     const BatchStepInfo si = info[blockIdx.y];
     if (!(si.dt > 0.0)) return;
-    k_update_body(c.m, batch_view(c, si, 1, 0), si.dt, nullptr, nullptr, si.epoch);
+    k_update_body<MF>(c.m, batch_view(c, si, 1, 0), si.dt, nullptr, nullptr, si.epoch);
 }
 
 __global__ void __launch_bounds__(256) kb_bnd_list(const BatchCase* __restrict__ bc,
@@ -3336,7 +3396,11 @@ void launch_batch_stage(int stage, const BatchCase* bc, const BatchStepInfo* inf
             }
             break;
         case STAGE_UPDATE:
-            kb_update<<<dim3(cdiv(maxC, UPDATE_BLOCK), n), UPDATE_BLOCK, 0, s>>>(bc, info);
+            // MF: the bounding slots of the largest cell (4, 6, 8 or 24)
+            if (MF == 4) kb_update<4><<<dim3(cdiv(maxC, UPDATE_BLOCK), n), UPDATE_BLOCK, 0, s>>>(bc, info);
+            else if (MF == 6) kb_update<6><<<dim3(cdiv(maxC, UPDATE_BLOCK), n), UPDATE_BLOCK, 0, s>>>(bc, info);
+            else if (MF == 8) kb_update<8><<<dim3(cdiv(maxC, UPDATE_BLOCK), n), UPDATE_BLOCK, 0, s>>>(bc, info);
+            else kb_update<0><<<dim3(cdiv(maxC, UPDATE_BLOCK), n), UPDATE_BLOCK, 0, s>>>(bc, info);
             break;
         case STAGE_BOUND: {
             const int nPasses = std::min(p.nAlphaBounds, MAX_BOUND_PASSES);
