@@ -2474,10 +2474,24 @@ __device__ __attribute__((always_inline)) inline void k_bnd_list_body(const DevM
         for (int i = b * blockDim.x + threadIdx.x; i < nPrev; i += nBlocks * blockDim.x) {
             if (!prev.active[i]) continue;
             const int c = prev.cand[i];
-            for (int k = m.cOff[c]; k < m.cOff[c + 1]; ++k) {
-                const int f = m.cFaces[k];
-                prev.corr[f] = 0.0;
-                prev.corrPos[f] = -1;
+            const int fb = m.cOff[c], nf = m.cOff[c + 1] - fb;
+            if (nf <= 8) {
+                // the face labels loaded together, then the stores
+                int fs[8];
+#pragma unroll
+                for (int k = 0; k < 8; ++k) fs[k] = m.cFaces[fb + min(k, nf - 1)];
+#pragma unroll
+                for (int k = 0; k < 8; ++k)
+                    if (k < nf) {
+                        prev.corr[fs[k]] = 0.0;
+                        prev.corrPos[fs[k]] = -1;
+                    }
+            } else {
+                for (int k = fb; k < fb + nf; ++k) {
+                    const int f = m.cFaces[k];
+                    prev.corr[f] = 0.0;
+                    prev.corrPos[f] = -1;
+                }
             }
             prev.actIdx[c] = -1;
         }
@@ -3004,12 +3018,25 @@ __device__ __attribute__((always_inline)) inline void k_bnd_apply_body(const Dev
             if (!st.active[i]) continue;
             const BndRec<MF>& R = ((const BndRec<MF>*)st.bndRec)[i];
             const unsigned int first = R.first;
+            // every slot's loads unconditional (the record's slots past the
+            // cell's faces repeat its last face), then the writes: the
+            // faces of the first redistribution take dVf + correction
+            // (face_value of both; empty-patch faces keep theirs)
+            int fid[MF];
+            double dv[MF], cr[MF];
+            unsigned char em[MF];
+#pragma unroll
+            for (int k = 0; k < MF; ++k) fid[k] = R.fid[k];
 #pragma unroll
             for (int k = 0; k < MF; ++k) {
-                if (!((first >> k) & 1u)) continue;
-                const int f = R.fid[k];
-                if (f < m.nIF || !m.fEmpty[f]) st.dVf[f] = face_value(m, st.dVf, f) + face_value(m, st.corr, f);
+                dv[k] = st.dVf[fid[k]];
+                cr[k] = st.corr[fid[k]];
+                em[k] = m.fEmpty[fid[k]];
             }
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int k = 0; k < MF; ++k)
+                if (((first >> k) & 1u) && (fid[k] < m.nIF || !em[k])) st.dVf[fid[k]] = dv[k] + cr[k];
             continue;
         }
         const int c = st.touched[idx];
