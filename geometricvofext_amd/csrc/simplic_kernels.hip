@@ -331,13 +331,13 @@ void launch_detect(const DevMesh& m, const DevState& st, const Params& p, hipStr
 // (normal equations, Cholesky decomposition and inverse of OpenFOAM's
 // scalarMatrices, x = inv(A) B).
 // ---------------------------------------------------------------------------
+template <int SB = 8>
 __device__ inline V3 ls_gradient(const DevMesh& m, const double* alpha, int c) {
     double A[4][4] = {};
     double B[4] = {};
     const V3 cc = ld3(m.C, c);
-    // stencil cells in batches of 8: the labels, then their centres and
+    // stencil cells in batches of SB: the labels, then their centres and
     // alphas, are loaded together; the sums still run in stencil order
-    constexpr int SB = 8;
     const int b = m.stOff[c], e = m.stOff[c + 1];
     for (int k0 = b; k0 < e; k0 += SB) {
         // every load is unconditional (past the end: the last entry again)
@@ -428,7 +428,9 @@ __device__ __attribute__((always_inline)) inline void k_ls_normal_body(const Dev
     for (int wi = blockIdx.x * blockDim.x + threadIdx.x; wi < nWork; wi += gridDim.x * blockDim.x) {
         const int i = wl ? wl[wi] : wi;
         const int c = st.mixed[i];
-        V3 n = -ls_gradient(m, st.alpha, c);
+        // (a 3x3x3 stencil of 27 cells in one batch: one wave per 64 cells
+        // has the registers to spare)
+        V3 n = -ls_gradient<28>(m, st.alpha, c);
         n = n / (mag(n) + SVP_SMALL);
         st3(st.ifN, i, n);
     }
@@ -2001,16 +2003,25 @@ __device__ __attribute__((always_inline)) inline void k_flux_list_body(const Dev
         const long w = b0 + threadIdx.x;
         bool take = false;
         if (w < total) {
+            // the pair's cell, face and the face's cells loaded once (the
+            // needBounding tags are stores, after which reloads would wait)
             const int i = (int)(w / slots), k = (int)(w % slots);
-            tag_need_bounding(m, st, st.mixed[i], k, epoch);
-            if (st.status[i] == 0) {
-                const int c = st.mixed[i];
-                const int fb = m.cOff[c];
-                if (k < m.cOff[c + 1] - fb) {
-                    const int f = m.cFaces[fb + k];
-                    const double ph = st.phi[f];
-                    if (f < m.nIF) take = (m.own[f] == c) ? (ph >= 0.0) : (ph < 0.0);
-                    else take = !m.fEmpty[f] && (ph >= 0.0);
+            const int c = st.mixed[i];
+            const int status = st.status[i];
+            const int fb = m.cOff[c], nf = m.cOff[c + 1] - fb;
+            if (k < nf) {
+                const int f = m.cFaces[fb + k];
+                const int fo = m.own[f];
+                const int fn = f < m.nIF ? m.nbr[f] : -1;
+                const double ph = st.phi[f];
+                const unsigned char em = m.fEmpty[f];
+                // needBounding: the cell (slot 0) and its neighbour across
+                // each internal face
+                if (k == 0) st.nbTag[c] = epoch;
+                if (f < m.nIF) st.nbTag[fo == c ? fn : fo] = epoch;
+                if (status == 0) {
+                    if (f < m.nIF) take = (fo == c) ? (ph >= 0.0) : (ph < 0.0);
+                    else take = !em && (ph >= 0.0);
                 }
             }
         }
