@@ -2269,12 +2269,15 @@ template <int MF>
 __device__ inline int bound_record(const DevMesh& m, const DevState& st, int c, double dt, const double* Sp,
                                    const double* Su, BndRec<MF>& R) {
     const double aTol = 100.0 * SVP_SMALL;
+    // the cell's own values in one batch (the face labels wait on cOff only)
     const double ac = st.alpha[c];
     const double Vi = m.V[c];
+    const double aOld = st.alphaOld[c];
+    const double suC = Su ? Su[c] : 0.0, spC = Sp ? Sp[c] : 0.0;
+    const int b = m.cOff[c], nf = m.cOff[c + 1] - b;
     const double alphaOvershoot = pos0(ac - 1.0) * (ac - 1.0) + neg0(ac) * ac;
     if (fabs(alphaOvershoot) < aTol) return 0;
     const double fluidToPassOn = alphaOvershoot * Vi;
-    const int b = m.cOff[c], nf = m.cOff[c + 1] - b;
     int fid[MF], oth[MF];
     double phf[MF], dvf[MF], smax[MF];
     unsigned int flg[(MF + 7) / 8];
@@ -2352,10 +2355,10 @@ __device__ inline int bound_record(const DevMesh& m, const DevState& st, int c, 
     for (int k = 0; k < MF; ++k)
         if (k < nf) nfd = (bits(k) & 1u) ? nfd + dvf[k] : nfd - dvf[k];
     const double rDeltaT = 1.0 / dt;
-    double num = st.alphaOld[c] * rDeltaT;
-    if (Su) num = num + Su[c];
+    double num = aOld * rDeltaT;
+    if (Su) num = num + suC;
     R.num0 = num - nfd / Vi * rDeltaT;
-    R.den = Sp ? (rDeltaT - Sp[c]) : rDeltaT;
+    R.den = Sp ? (rDeltaT - spC) : rDeltaT;
 #pragma unroll
     for (int k = 0; k < MF; ++k) {
         R.fid[k] = fid[k];
