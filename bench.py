@@ -24,6 +24,7 @@ import argparse
 import json
 import os
 import platform
+import re
 import sys
 import time
 
@@ -168,7 +169,8 @@ def pmc_summary(copy_fetch_dir, copy_write_dir, run_fetch_dir, run_write_dir):
     f_fac = COPY_PROBE_BYTES / (1024.0 * probe_kib(copy_fetch_dir))
     w_fac = COPY_PROBE_BYTES / (1024.0 * probe_kib(copy_write_dir))
     fetch, write = _counter_sums(run_fetch_dir), _counter_sums(run_write_dir)
-    steps = max(v[0] for k, v in fetch.items() if k.endswith("k_update"))
+    # one update launch per step (k_update, or k_update<MF> by cell size)
+    steps = max(v[0] for k, v in fetch.items() if re.search(r"k_update(<\d+>)?$", k))
     out = {"fetch_factor": f_fac, "write_factor": w_fac, "steps": steps, "stages": {}, "kernels": {}}
     for k in sorted(set(fetch) | set(write)):
         fb = 1024.0 * f_fac * fetch.get(k, [0, 0.0])[1] / steps
