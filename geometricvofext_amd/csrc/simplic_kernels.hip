@@ -1054,6 +1054,7 @@ __device__ __attribute__((always_inline)) inline void k_fsd_group_body(const Dev
         const int i = wl ? wl[wi] : wi;
         const int c = st.mixed[i];
         const int fb = m.cOff[c], nF = m.cOff[c + 1] - fb;
+        const int pb = m.cpOff[c], np = m.cpOff[c + 1] - pb;
         const V3 normalI = ld3(st.ifN, i);
         const double alphaI = st.alpha[c];
         const V3 Cc = ld3(m.C, c);
@@ -1072,38 +1073,48 @@ __device__ __attribute__((always_inline)) inline void k_fsd_group_body(const Dev
             }
             continue;
         }
-        // this lane's face in registers (faces of 3 or 4 points)
+        // The lane's face (faces of 3 or 4 points) and its two cell points,
+        // every load unconditional (slots past the cell's faces or points,
+        // or past a triangle's third point, repeat the last one): the face
+        // chain and the point chain then go out together, where loads under
+        // a lane condition would each wait for all the loads before them.
+        const int fL = m.cFaces[fb + min(l, nF - 1)];
+        const int cpA = m.cPts[pb + min(l, np - 1)], cpB = m.cPts[pb + min(l + G, np - 1)];
+        const int s0 = m.fOff[fL], fszL = m.fOff[fL + 1] - s0;
+        int plL[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) plL[k] = m.fPts[s0 + min(k, fszL - 1)];
+        V3 qq[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) qq[k] = ld3(m.pts, plL[k]);
+        const V3 ptA = ld3(m.pts, cpA), ptB = ld3(m.pts, cpB);
+        double fcs[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) fcs[k] = m.fullCS[6L * fL + k];
         int fsz = 0, myFace = -1;
         V3 q4[4], q3[3];
         int pl[4] = {0, 0, 0, 0};
         if (l < nF) {
-            const int f = m.cFaces[fb + l];
-            myFace = f;
-            const int s0 = m.fOff[f];
-            fsz = m.fOff[f + 1] - s0;
+            myFace = fL;
+            fsz = fszL;
 #pragma unroll
-            for (int k = 0; k < 4; ++k)
-                if (k < fsz) pl[k] = m.fPts[s0 + k];
-            if (fsz == 4) {
+            for (int k = 0; k < 4; ++k) pl[k] = k < fsz ? plL[k] : 0;
 #pragma unroll
-                for (int k = 0; k < 4; ++k) q4[k] = ld3(m.pts, pl[k]);
-            } else {
+            for (int k = 0; k < 4; ++k) q4[k] = qq[k];
 #pragma unroll
-                for (int k = 0; k < 3; ++k) q3[k] = ld3(m.pts, pl[k]);
-            }
+            for (int k = 0; k < 3; ++k) q3[k] = qq[k];
 #pragma unroll
-            for (int k = 0; k < 6; ++k) full[g][l][k] = m.fullCS[6L * f + k];
+            for (int k = 0; k < 6; ++k) full[g][l][k] = fcs[k];
         }
         // the plane's normal is fixed for the cell: dot(point, n) once
         double dots[4];
 #pragma unroll
         for (int k = 0; k < 4; ++k) dots[k] = (fsz == 4) ? dot(q4[k], normalI) : (k < 3 ? dot(q3[k], normalI) : 0.0);
         // vertex levels -n.x of the cell points, sorted descending by rank
-        const int pb = m.cpOff[c], np = m.cpOff[c + 1] - pb;
         // (points past PPL per lane cannot occur: np <= NS_CELL_POINTS = PPL * G)
         double va = 0.0, vb = 0.0;
-        if (l < np) va = -dot(normalI, ld3(m.pts, m.cPts[pb + l]));
-        if (l + G < np) vb = -dot(normalI, ld3(m.pts, m.cPts[pb + l + G]));
+        if (l < np) va = -dot(normalI, ptA);
+        if (l + G < np) vb = -dot(normalI, ptB);
         if (l < np) vraw[g][l] = va;
         if (l + G < np) vraw[g][l + G] = vb;
         if constexpr (PPL > 2) {
