@@ -1021,6 +1021,43 @@ __global__ void __launch_bounds__(64) k_find_signed_distance(DevMesh m, DevState
 // Newton steps as the one-lane version.  The vertex levels are sorted by rank
 // in LDS.  One wave per workgroup: LDS traffic of a wave is in order.
 
+// volPointInterpolation value at a point: the weighted sum of its source
+// values in source order.  A point of up to PU_SLOTS sources (every interior
+// point of a hexahedral mesh) loads all source labels and weights, then all
+// their values, before the sum, instead of waiting on two loads per source.
+constexpr int PU_SLOTS = 8;
+__device__ inline V3 point_u(const DevMesh& m, const DevState& st, int p) {
+    const int o0 = m.psOff[p], n = m.psOff[p + 1] - o0;
+    V3 t = VZERO;
+    if (n >= 1 && n <= PU_SLOTS) {
+        int src[PU_SLOTS];
+        double w[PU_SLOTS];
+        V3 u[PU_SLOTS];
+#pragma unroll
+        for (int k = 0; k < PU_SLOTS; ++k) {
+            src[k] = m.psSrc[o0 + min(k, n - 1)];
+            w[k] = m.psW[o0 + min(k, n - 1)];
+        }
+#pragma unroll
+        for (int k = 0; k < PU_SLOTS; ++k) {
+            // a source is a cell (>= 0) or a boundary face (~face)
+            const double* base = src[k] >= 0 ? st.U : st.Ub;
+            const int idx = src[k] >= 0 ? src[k] : (~src[k]) - m.nIF;
+            u[k] = ld3(base, idx);
+        }
+#pragma unroll
+        for (int k = 0; k < PU_SLOTS; ++k)
+            if (k < n) t = t + w[k] * u[k];
+        return t;
+    }
+    for (int k = o0; k < o0 + n; ++k) {
+        const int s = m.psSrc[k];
+        const V3 u = (s >= 0) ? ld3(st.U, s) : ld3(st.Ub, (~s) - m.nIF);
+        t = t + m.psW[k] * u;
+    }
+    return t;
+}
+
 __device__ __attribute__((always_inline)) inline void fsd_face_cut(int fsz, const V3 (&q4)[4], const V3 (&q3)[3], V3 n, double dist,
                                     FaceCut& fc, bool& ok, const double* full, const double (&dots)[4]) {
     if (fsz == 4) ok &= calcSubFaceN<4, true, true>(q4, n, dist, fc, full, dots);
@@ -1310,14 +1347,7 @@ __device__ __attribute__((always_inline)) inline void k_fsd_group_body(const Dev
             for (int q = 0; q < 3; ++q) pp[q] = __shfl(foundMask ? fp[q] : bp[q], g * G + src);
             // point values: lane q of the group evaluates point q
             V3 pu = VZERO;
-            if (l < 3) {
-                const int p = (l == 0) ? pp[0] : (l == 1 ? pp[1] : pp[2]);
-                for (int k = m.psOff[p]; k < m.psOff[p + 1]; ++k) {
-                    const int s2 = m.psSrc[k];
-                    const V3 u = (s2 >= 0) ? ld3(st.U, s2) : ld3(st.Ub, (~s2) - m.nIF);
-                    pu = pu + m.psW[k] * u;
-                }
-            }
+            if (l < 3) pu = point_u(m, st, (l == 0) ? pp[0] : (l == 1 ? pp[1] : pp[2]));
             V3 pv[3];
 #pragma unroll
             for (int q = 0; q < 3; ++q)
@@ -1563,16 +1593,6 @@ __global__ void __launch_bounds__(256) k_upwind(DevMesh m, DevState st, double d
     k_upwind_body(m, st, dtStep);
 }
 
-// volPointInterpolation value at a point
-__device__ inline V3 point_u(const DevMesh& m, const DevState& st, int p) {
-    V3 t = VZERO;
-    for (int k = m.psOff[p]; k < m.psOff[p + 1]; ++k) {
-        const int s = m.psSrc[k];
-        const V3 u = (s >= 0) ? ld3(st.U, s) : ld3(st.Ub, (~s) - m.nIF);
-        t = t + m.psW[k] * u;
-    }
-    return t;
-}
 
 // interpolationCellPoint<vector>::interpolate(position, cell) (advection.C:91,126):
 // barycentric weights in the tets (cell centre, face point 0, fan triangle).
