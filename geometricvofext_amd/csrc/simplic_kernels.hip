@@ -2012,14 +2012,8 @@ __global__ void __launch_bounds__(64) k_un0(DevMesh m, DevState st) {
 // needBounding (advection.C:54-82): each mixed cell of this step's reconstruct
 // tags itself (slot 0) and the neighbour across each internal face (one slot
 // per face) with the step's number, so the update and the bounding apply test
-// it with one load; mixed cells of every status are tagged.
-__device__ inline void tag_need_bounding(const DevMesh& m, const DevState& st, int c, int k, int epoch) {
-    const int fb = m.cOff[c];
-    if (k >= m.cOff[c + 1] - fb) return;
-    if (k == 0) st.nbTag[c] = epoch;
-    const int f = m.cFaces[fb + k];
-    if (f < m.nIF) st.nbTag[m.own[f] == c ? m.nbr[f] : m.own[f]] = epoch;
-}
+// it with one load; mixed cells of every status are tagged (here, or by the
+// unlisted flux kernel on tetrahedra).
 
 __device__ __attribute__((always_inline)) inline void k_flux_list_body(const DevMesh& m, const DevState& st, int slots, int epoch) {
     const int nMixed = *st.nMixed;
@@ -2087,8 +2081,8 @@ __device__ __attribute__((always_inline)) inline void k_interface_flux_faces_bod
     for (long wi = (long)blockIdx.x * blockDim.x + threadIdx.x; wi < nWork; wi += (long)gridDim.x * blockDim.x) {
         const int w = LISTED ? st.fluxList[wi] : (int)wi;
         const int i = w / slots, k = w % slots;
-        if (!LISTED) tag_need_bounding(m, st, st.mixed[i], k, epoch);
-        if (!LISTED && st.status[i] != 0) continue;
+        // the pair's cell and face loaded once, before the needBounding
+        // stores (after which reloads of them would wait)
         const int c = st.mixed[i];
         const int fb = m.cOff[c];
         if (!LISTED && k >= m.cOff[c + 1] - fb) continue;
@@ -2096,9 +2090,18 @@ __device__ __attribute__((always_inline)) inline void k_interface_flux_faces_bod
         const int f = m.cFaces[fb + k];
         const double ph = st.phi[f];
         if (!LISTED) {
+            const int status = st.status[i];
+            const int fo = m.own[f];
+            const int fn = f < m.nIF ? m.nbr[f] : -1;
+            const unsigned char em = m.fEmpty[f];
+            // needBounding: the cell (slot 0) and its neighbour across each
+            // internal face, for every status
+            if (k == 0) st.nbTag[c] = epoch;
+            if (f < m.nIF) st.nbTag[fo == c ? fn : fo] = epoch;
+            if (status != 0) continue;
             bool down;
-            if (f < m.nIF) down = (m.own[f] == c) ? (ph >= 0.0) : (ph < 0.0);
-            else down = !m.fEmpty[f] && (ph >= 0.0);
+            if (f < m.nIF) down = (fo == c) ? (ph >= 0.0) : (ph < 0.0);
+            else down = !em && (ph >= 0.0);
             if (!down) continue;
         }
         const V3 n = ld3(st.ifN, i);
