@@ -874,6 +874,7 @@ private:
             st.push_back(c);
             st.insert(st.end(), tmp.begin(), tmp.end());
             stOff[c + 1] = (int)st.size();
+            m_.maxStencil = std::max(m_.maxStencil, stOff[c + 1] - stOff[c]);
         }
         std::vector<int> cpOff(nC + 1, 0), cPtsFlat;
         cPtsFlat.reserve((size_t)nC * 8);

@@ -422,26 +422,30 @@ __device__ inline V3 ls_gradient(const DevMesh& m, const double* alpha, int c) {
     return v3(x[1], x[2], x[3]);
 }
 
-// wl: optional list of mixed-cell positions (*wn entries) instead of all of them
+// wl: optional list of mixed-cell positions (*wn entries) instead of all of
+// them.  SB: stencil cells loaded per batch (one wave per 64 cells has the
+// registers to spare): 28 takes a 3x3x3 hexahedral stencil in one batch, 36
+// the 15-71 cells of a tetrahedral one in two.
+template <int SB = 28>
 __device__ __attribute__((always_inline)) inline void k_ls_normal_body(const DevMesh& m, const DevState& st, const int* wl, const int* wn) {
     const int nWork = wl ? *wn : *st.nMixed;
     for (int wi = blockIdx.x * blockDim.x + threadIdx.x; wi < nWork; wi += gridDim.x * blockDim.x) {
         const int i = wl ? wl[wi] : wi;
         const int c = st.mixed[i];
-        // (a 3x3x3 stencil of 27 cells in one batch: one wave per 64 cells
-        // has the registers to spare)
-        V3 n = -ls_gradient<28>(m, st.alpha, c);
+        V3 n = -ls_gradient<SB>(m, st.alpha, c);
         n = n / (mag(n) + SVP_SMALL);
         st3(st.ifN, i, n);
     }
 }
 
+template <int SB>
 __global__ void __launch_bounds__(64, 1) k_ls_normal(DevMesh m, DevState st, const int* wl, const int* wn) {
-    k_ls_normal_body(m, st, wl, wn);
+    k_ls_normal_body<SB>(m, st, wl, wn);
 }
 
 void launch_ls_normal(const DevMesh& m, const DevState& st, hipStream_t s, const int* wl, const int* wn) {
-    k_ls_normal<<<1024, 64, 0, s>>>(m, st, wl, wn);
+    if (m.maxStencil > 28) k_ls_normal<36><<<1024, 64, 0, s>>>(m, st, wl, wn);
+    else k_ls_normal<28><<<1024, 64, 0, s>>>(m, st, wl, wn);
 }
 
 // calcInterfaceNFromRegAlphaGrad (reconstruction.C:74-82): n = -grad(alpha) /

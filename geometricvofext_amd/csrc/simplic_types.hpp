@@ -30,6 +30,7 @@ struct DevMesh {
     // cell-point-cell stencil: the cell first, then the other cells ascending
     const int* stOff;
     const int* st;
+    int maxStencil;         // largest stencil (cells)
     // volPointInterpolation sources per point: cell c >= 0 or boundary face ~f
     const int* psOff;
     const int* psSrc;
