@@ -50,6 +50,22 @@ def test_pmc_summary_scales_by_the_copy_probe(tmp_path):
     json.dumps(out)
 
 
+def test_pmc_summary_counts_steps_by_the_templated_update(tmp_path):
+    """The update kernel is instantiated per cell size (k_update<6> on
+    hexahedra): one launch per step still numbers the steps, and no other
+    kernel whose name merely contains k_update counts."""
+    probe_kib = bench.COPY_PROBE_BYTES / 1024
+    _write_run(tmp_path / "cf", [("copy_kernel", probe_kib)], "FETCH_SIZE")
+    _write_run(tmp_path / "cw", [("copy_kernel", probe_kib)], "WRITE_SIZE")
+    rows = [("void svp::k_update<6>(svp::DevMesh, svp::DevState)", 8.0)] * 3
+    rows += [("svp::k_update_extra(svp::DevMesh)", 1.0)] * 7
+    _write_run(tmp_path / "rf", rows, "FETCH_SIZE")
+    _write_run(tmp_path / "rw", rows, "WRITE_SIZE")
+    out = bench.pmc_summary(str(tmp_path / "cf"), str(tmp_path / "cw"), str(tmp_path / "rf"), str(tmp_path / "rw"))
+    assert out["steps"] == 3
+    assert out["stages"]["update"]["fetch_bytes"] == pytest.approx(1024 * (3 * 8.0 + 7 * 1.0) / 3)
+
+
 def test_traffic_of_reads_the_committed_profile():
     path = os.path.join(bench.ROOT, "profiles", "hbm_traffic_hex128p.json")
     if not os.path.exists(path):
