@@ -97,7 +97,12 @@ def make_batch_case(spec, alpha_dict):
     deformation, none for the rotation)."""
     import geometricvofext_amd as g
     from geometricvofext_amd import harness
-    mesh = g.hex_box(spec["N"])
+    # (optional spec keys: "kind" "tet" for N^3 x 6 Kuhn tetrahedra, and
+    # "perturb" / "seed" for displaced interior vertices)
+    if spec.get("kind", "hex") == "tet":
+        mesh = g.tet_box(spec["N"], perturb=spec.get("perturb", 0.0), seed=spec.get("seed", 0))
+    else:
+        mesh = g.hex_box(spec["N"], perturb=spec.get("perturb", 0.0), seed=spec.get("seed", 0))
     s = g.SolveVofEqu(mesh, dict(alpha_dict))
     s.init_alpha(spec["shapes"], max_depth=4)
     a0 = s.alpha()
