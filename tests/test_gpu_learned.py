@@ -318,3 +318,40 @@ def test_learned_ellipsoid_deformation_to_end_time(g, trained, oracle_lib):
         s.close()
     assert abs(res["LS"][1]) < 1e-12 and abs(res["learned"][1]) < 1e-12
     assert res["learned"][0] <= res["LS"][0]
+
+
+def test_learned_on_non_lattice_mesh_is_the_plain_path(g, trained):
+    """On a perturbed hexahedral mesh no cell has a uniform 3x3x3 lattice
+    stencil, so every interface cell takes the least-squares normal: the
+    learned handle must then step exactly as the plain one, bit for bit."""
+    from geometricvofext_amd import harness
+    m, _, _ = trained
+    mesh = g.hex_box(14, perturb=0.15, seed=4)
+    base = dict(nAlphaBounds=3, snapTol=0.0, clip=False)
+    plain = g.SolveVofEqu(mesh, dict(base, orientationMethod="LS"))
+    learned = g.SolveVofEqu(mesh, dict(base, orientationMethod="learned"))
+    learned.set_normal_model(m)
+    shapes = [((0.45, 0.5, 0.45), 0.22)]
+    plain.init_alpha(shapes, max_depth=4)
+    learned.set_alpha(plain.alpha())
+    U0, phi0 = plain.deformation_field()
+    for s in (plain, learned):
+        s.upload_base_flow(U0, phi0)
+    rt = harness.RunTime(end_time=0.2)
+    n = 0
+    while rt.run():
+        co, aco = plain.courant()
+        rt.set_delta_t(0.5 * co * rt.delta_t, 0.5 * aco * rt.delta_t)
+        rt.increment()
+        uf = harness.u_factor(rt.value, rt.delta_t, 6.0)
+        for s in (plain, learned):
+            s.scale_flow(uf)
+            s.reconstruct()
+        nmix, kept, fb, lsonly = learned.learned_stats()
+        assert kept == 0 and fb == 0 and lsonly == nmix
+        for s in (plain, learned):
+            s.advect(rt.delta_t)
+        assert np.array_equal(plain.alpha(), learned.alpha()), f"step {n}"
+        n += 1
+    assert n > 10
+    print(f"non-lattice mesh: {n} steps, learned handle bit-identical to the plain path")
