@@ -237,6 +237,23 @@ def test_parity_kahn_schedule(oracle_lib, monkeypatch):
     assert exact
 
 
+@pytest.mark.parametrize("schedule", ["dataflow", "kahn"])
+def test_parity_tets_production_dictionary(oracle_lib, monkeypatch, schedule):
+    """Perturbed tetrahedra (four-face bounding records) with the tutorials'
+    dictionary (snapTol 1e-12, clip true), under either bounding schedule."""
+    import geometricvofext_amd as g
+    if schedule == "kahn":
+        monkeypatch.setenv("SVP_BND_FLOW", "0")
+    mesh = g.tet_box(10, perturb=0.1, seed=21)
+    om = oracle_lib.OracleMesh(mesh)
+    a0 = ramp_alpha(om.C, [((0.45, 0.4, 0.5), 0.22)], 1.5 / 10)
+    _say("parity", mesh.name, schedule, "snapTol 1e-12 clip")
+    exact, gpu, _ = _run_parity(mesh, oracle_lib, a0, 8, nAlphaBounds=3, snapTol=1e-12, clip=True)
+    a = gpu.alpha()
+    assert a.min() >= 0.0 and a.max() <= 1.0
+    assert exact
+
+
 def test_parity_alphagrad_perturbed_hex(oracle_lib):
     """orientationMethod alphaGrad (Gauss-linear gradient normal) on the
     heavy-bounding perturbed hex case."""
