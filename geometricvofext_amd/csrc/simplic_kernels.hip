@@ -2520,10 +2520,11 @@ __device__ __attribute__((always_inline)) inline void k_bnd_list_body(const DevM
     // what the previous pass (of this step or the last one) left: the
     // corrections and correctedFaces keys on the faces of its active cells
     // and their candidate index, back to 0 / -1 before this pass's records.
-    // Done last, off the look-back chain.
-    auto clearPrevious = [&]() {
+    // Done by the waves that wait while wave 0 looks back (thread t of nt
+    // clearing threads per block), off the look-back chain.
+    auto clearPrevious = [&](int t, int nt) {
         const int nPrev = st.bcount[BND_PREV_COUNT];
-        for (int i = b * blockDim.x + threadIdx.x; i < nPrev; i += nBlocks * blockDim.x) {
+        for (int i = b * nt + t; i < nPrev; i += nBlocks * nt) {
             if (!prev.active[i]) continue;
             const int c = prev.cand[i];
             const int fb = m.cOff[c], nf = m.cOff[c + 1] - fb;
@@ -2553,7 +2554,7 @@ __device__ __attribute__((always_inline)) inline void k_bnd_list_body(const DevM
     if (b == 0 && threadIdx.x >= 2 && threadIdx.x <= BND_PROGRESS) st.bcount[threadIdx.x] = 0;
     if (!oob_any(st)) {
         if (b == 0 && threadIdx.x == 0) st.bcount[0] = 0;
-        clearPrevious();
+        clearPrevious(threadIdx.x, blockDim.x);
         return;
     }
     const int wBeg = b * st.bndListWords, wEnd = min(nWords, wBeg + st.bndListWords);
@@ -2570,6 +2571,8 @@ __device__ __attribute__((always_inline)) inline void k_bnd_list_body(const DevM
             sBase = (int)excl;
             sTotal = (int)total;
         }
+    } else {
+        clearPrevious(threadIdx.x - 64, blockDim.x - 64);
     }
     __syncthreads();
     int base = sBase;
@@ -2599,7 +2602,6 @@ __device__ __attribute__((always_inline)) inline void k_bnd_list_body(const DevM
         base += tot;
         __syncthreads();
     }
-    clearPrevious();
 }
 
 __global__ void __launch_bounds__(256) k_bnd_list(DevMesh m, DevState st, int nWords, int nBlocks, BndPrev prev) {
