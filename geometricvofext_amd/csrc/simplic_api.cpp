@@ -777,8 +777,11 @@ private:
             freeEvents_.pop_back();
             return e;
         }
+        // timing only (read after a stream synchronize): without the
+        // system-scope fence a record does not write back and invalidate the
+        // L2 between the kernels of a step
         hipEvent_t e;
-        HIPCHECK(hipEventCreate(&e));
+        HIPCHECK(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
         events_.push_back({e, e});
         return e;
     }
@@ -1402,8 +1405,8 @@ private:
             free_.pop_back();
             return e;
         }
-        hipEvent_t e;
-        HIPCHECK(hipEventCreate(&e));
+        hipEvent_t e;  // timing only, as the handle's step events
+        HIPCHECK(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
         allEvents_.push_back(e);
         return e;
     }
