@@ -125,6 +125,11 @@ int detect_blocks(int nC) { return cdiv(nC, DETECT_BLOCK); }
 constexpr unsigned long long LOOK_AGG = 1ull << 30, LOOK_PREFIX = 2ull << 30;
 constexpr int LOOK_SPIN_LIMIT = 1 << 22;
 
+// where a gathered empty-patch flag of face f is read: f itself for a
+// boundary face; for an interior face (never empty, and its flag unused) the
+// first boundary face's, clamped into the array when there is none
+__device__ inline int empty_slot(const DevMesh& m, int f) { return min(max(f, m.nIF), m.nF - 1); }
+
 // The block's ticket (thread 0 only).  The counter only grows, by nBlocks per
 // launch (every launch of a kernel on one counter has the same grid, and every
 // block takes one ticket), so ticket / nBlocks numbers the launch and
@@ -516,7 +521,7 @@ __global__ void __launch_bounds__(64, 1) k_grad_normal(DevMesh m, DevState st) {
                 const bool in = f < m.nIF;
                 sf[j] = ld3(m.Sf, f);
                 w[j] = m.linW[in ? f : 0];
-                em[j] = m.fEmpty[f];
+                em[j] = m.fEmpty[empty_slot(m, f)];
                 // the other cell's alpha (the cell's own on boundary faces)
                 const int o = ev[j] >= 0 ? (in ? m.nbr[f] : c) : m.own[f];
                 ao[j] = st.alpha[o];
@@ -2043,7 +2048,7 @@ __device__ __attribute__((always_inline)) inline void k_flux_list_body(const Dev
                 const int fo = m.own[f];
                 const int fn = f < m.nIF ? m.nbr[f] : -1;
                 const double ph = st.phi[f];
-                const unsigned char em = m.fEmpty[f];
+                const unsigned char em = m.fEmpty[empty_slot(m, f)];
                 // needBounding: the cell (slot 0) and its neighbour across
                 // each internal face
                 if (k == 0) st.nbTag[c] = epoch;
@@ -2097,7 +2102,7 @@ __device__ __attribute__((always_inline)) inline void k_interface_flux_faces_bod
             const int status = st.status[i];
             const int fo = m.own[f];
             const int fn = f < m.nIF ? m.nbr[f] : -1;
-            const unsigned char em = m.fEmpty[f];
+            const unsigned char em = m.fEmpty[empty_slot(m, f)];
             // needBounding: the cell (slot 0) and its neighbour across each
             // internal face, for every status
             if (k == 0) st.nbTag[c] = epoch;
@@ -2167,12 +2172,14 @@ __device__ __attribute__((always_inline)) inline double cell_face_sum(const DevM
 #pragma unroll
         for (int k = 0; k < N; ++k) e[k] = m.cFacesAsc[b + min(k, nf - 1)];
         // (unconditional loads: a load under a branch would wait for all the
-        // loads before it)
+        // loads before it).  Only boundary faces can be empty: an interior
+        // face's flag load reads the first boundary face's byte, one line
+        // that every thread shares, instead of a line per face.
 #pragma unroll
         for (int k = 0; k < N; ++k) {
             const int f = e[k] >= 0 ? e[k] : ~e[k];
             v[k] = st.dVf[f];
-            em[k] = m.fEmpty[f];
+            em[k] = m.fEmpty[empty_slot(m, f)];
         }
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
@@ -2338,7 +2345,7 @@ __device__ inline int bound_record(const DevMesh& m, const DevState& st, int c, 
         fn[k] = m.nbr[f];
         fph[k] = st.phi[f];
         fdv[k] = st.dVf[f];
-        fem[k] = m.fEmpty[f];
+        fem[k] = m.fEmpty[empty_slot(m, f)];
     }
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
@@ -3085,7 +3092,7 @@ __device__ __attribute__((always_inline)) inline void k_bnd_apply_body(const Dev
             for (int k = 0; k < MF; ++k) {
                 dv[k] = st.dVf[fid[k]];
                 cr[k] = st.corr[fid[k]];
-                em[k] = m.fEmpty[fid[k]];
+                em[k] = m.fEmpty[empty_slot(m, fid[k])];
             }
             __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
@@ -3107,7 +3114,7 @@ __device__ __attribute__((always_inline)) inline void k_bnd_apply_body(const Dev
             kpos[k] = st.corrPos[fid[k]];
             kval[k] = st.corr[fid[k]];
             kon[k] = m.own[fid[k]];
-            kem[k] = m.fEmpty[fid[k]];
+            kem[k] = m.fEmpty[empty_slot(m, fid[k])];
         }
         const double before = st.alpha[c];
         const double Vc = m.V[c];
