@@ -34,3 +34,15 @@ def max_abs(a, b):
     a = np.asarray(a, dtype=np.float64)
     b = np.asarray(b, dtype=np.float64)
     return 0.0 if a.size == 0 else float(np.max(np.abs(a - b)))
+
+
+def empty_z_box(N, perturb, seed):
+    """hex_box whose z patches (top, bottom) are empty, as the front and back
+    of a 2-D case."""
+    from geometricvofext_amd import mesh as M
+    mesh = M.hex_box(N, perturb=perturb, seed=seed)
+    mesh.patch_type = np.array(mesh.patch_type, dtype=np.int32)
+    for p, name in enumerate(mesh.patch_names):
+        if name in ("top", "bottom"):
+            mesh.patch_type[p] = M.PATCH_EMPTY
+    return mesh

@@ -535,6 +535,26 @@ def test_parity_rdf_split_warped_faces(oracle_lib):
     assert exact
 
 
+@pytest.mark.parametrize("method", ["LS", "alphaGrad"])
+def test_parity_empty_patches(oracle_lib, method):
+    """Empty patches carry nothing (advectionTemplates.C:65-70, 104-109): the
+    z patches of a perturbed hex box made empty, with a uniform flow whose
+    flux through them is not zero, liquid against the empty bottom and
+    against a wall, and bounding on.  Every kernel that reads a face's empty
+    flag (upwind, face fluxes, bounding records and apply, update, the
+    alphaGrad normal) meets set and unset ones."""
+    from tests.helpers import empty_z_box
+    mesh = empty_z_box(16, 0.15, 5)
+    om = oracle_lib.OracleMesh(mesh)
+    U = np.array([0.3, -0.2, 0.25])
+    U0, phi0 = np.tile(U, (mesh.n_cells, 1)), om.Sf @ U
+    a0 = ramp_alpha(om.C, [((0.5, 0.45, 0.08), 0.22), ((0.88, 0.5, 0.6), 0.18)], 1.5 / 16)
+    _say("parity empty patches", method, mesh.name)
+    exact, _, _ = _run_parity(mesh, oracle_lib, a0, 8, flow=(U0, phi0), nAlphaBounds=3, clip=False,
+                              orientationMethod=method)
+    assert exact
+
+
 def test_entry_point_errors():
     """Argument errors of the coupling and export entry points come back as
     codes (the reference would abort): no sub-cycles, partial PLIC outputs, a

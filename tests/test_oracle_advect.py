@@ -6,7 +6,7 @@ import pytest
 
 from geometricvofext_amd import harness as H
 from geometricvofext_amd import mesh as M
-from tests.helpers import ramp_alpha
+from tests.helpers import empty_z_box, ramp_alpha
 
 
 def test_ls_gradient_exact_for_linear_fields(oracle_lib):
@@ -141,3 +141,46 @@ def test_explicit_source(oracle_lib, small_case):
     a = s.alpha()
     far = a0 == 0.0
     assert np.allclose(a[far], 0.5 * dt / (1.0 / dt) * (1.0 / dt), rtol=1e-12)
+
+
+def _empty_face_mask(mesh):
+    empty = np.zeros(mesh.n_faces, dtype=bool)
+    for start, size, t in zip(mesh.patch_start, mesh.patch_size, mesh.patch_type):
+        if t == M.PATCH_EMPTY:
+            empty[int(start):int(start + size)] = True
+    return empty
+
+
+def test_empty_patches_carry_nothing(oracle_lib):
+    """faceValue / setFaceValue on empty patches (advectionTemplates.C:65-70,
+    104-109): with the z patches empty and a uniform flow that has a flux
+    through them, the face volumes there stay zero and one step's volume
+    change is the net outflow through the other boundary faces alone; with
+    walls in their place the same step moves a different volume."""
+    U = np.array([0.3, -0.2, 0.25])
+    shapes = [((0.5, 0.45, 0.08), 0.22), ((0.88, 0.5, 0.6), 0.18)]
+    dt = 0.01
+    moved = {}
+    for kind in ("empty", "wall"):
+        mesh = empty_z_box(10, 0.15, 5) if kind == "empty" else M.hex_box(10, perturb=0.15, seed=5)
+        om = oracle_lib.OracleMesh(mesh)
+        phi = om.Sf @ U
+        a0 = ramp_alpha(om.C, shapes, 1.5 / 10)
+        s = oracle_lib.OracleSolver(om, nAlphaBounds=0, clip=False)
+        s.set_alpha(a0)
+        s.set_flow(phi, np.tile(U, (mesh.n_cells, 1)))
+        s.reconstruct()
+        s.advect(dt)
+        dv = s.alpha_phi() * dt
+        empty = _empty_face_mask(mesh)
+        nif = mesh.n_internal_faces
+        if kind == "empty":
+            assert empty.sum() == 2 * 10 * 10
+            # the case puts liquid against the empty bottom, with flux through it
+            assert np.any(a0[np.asarray(mesh.owner)[empty]] > 0.5) and np.all(phi[empty] != 0.0)
+            assert np.all(dv[empty] == 0.0)
+        out = dv[nif:][~empty[nif:]].sum()
+        change = float(np.sum(om.V * (s.alpha() - a0)))
+        assert abs(change + out) <= 1e-13 * max(abs(out), 1e-3)
+        moved[kind] = change
+    assert abs(moved["empty"] - moved["wall"]) > 1e-6
